@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark: LDDMM surface registrations per second on one MI355X.
+
+Workload (default): the single large surface pair -- template icosphere subdivision 6
+(40,962 vertices, 81,920 triangles), target = template displaced by a seeded sum of 20
+Gaussian RBFs (width 0.4, amplitudes N(0, 0.1^2)); Gaussian deformation kernel sigma 0.25,
+Gaussian currents kernel sigma 0.1, sigma_err 0.1, T = 20 Euler steps, up to 100 conjugate-
+gradient iterations, momenta initialised to zero.  One step = one whole registration
+(SurfaceMatching + optimizeMatching) with the template and target already on the device.
+
+Printed JSON line: registrations/s, per-registration latency, the live roofline of the
+dominant kernel (the fused Euler flow step: Gaussian K(x,x)a over 40,962 x 40,962 pairs, 16
+flops per pair), its energy error against the FP64 form, and the FP64 CPU oracle timed on a
+bounded sample of the same sums on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (AMD published)
+FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp32c", choices=["fp32c", "fp64"])
+    ap.add_argument("--workload", default="surface-s6", choices=["surface-s6", "surface-s3"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-reps", type=int, default=20)
+    return ap.parse_args()
+
+
+def make_workload(name):
+    from registration_amd import synthetic
+    return synthetic.surface_pair_config2() if name == "surface-s6" else synthetic.surface_pair_config1()
+
+
+def registration(cfg, prec):
+    from registration_amd import kernelFunctions as kfun, surfaceMatching as sm, surfaces
+    p = sm.SurfaceMatchingParam(timeStep=1.0 / cfg["T"],
+                                KparDiff=kfun.Kernel('gauss', sigma=cfg["sigmaKernel"], precision=prec),
+                                KparDist=kfun.Kernel('gauss', sigma=cfg["sigmaDist"], precision=prec),
+                                sigmaError=cfg["sigmaError"], errorType='current')
+    F, V = cfg["template"]
+    _, Y = cfg["target"]
+    return sm.SurfaceMatching(Template=surfaces.Surface(FV=(F, V)), Target=surfaces.Surface(FV=(F, Y)),
+                              param=p, maxIter=cfg["maxIter"], testGradient=False, verb=False)
+
+
+def flow_step_roofline(cfg, prec, reps):
+    """Average duration of the fused flow-step launch at the workload's size, timed with HIP
+    events on the stream the launches go to."""
+    import numpy as np
+    import torch
+    from registration_amd import _lib, runtime as rt
+    from registration_amd.kernelFunctions import Kernel
+    F, V = cfg["template"]
+    N = V.shape[0]
+    rng = np.random.default_rng(0)
+    x = torch.as_tensor(V, device="cuda")
+    a = torch.as_tensor(1e-2 * rng.standard_normal(V.shape), device="cuda")
+    out = torch.empty_like(x)
+    tot = torch.empty(1, dtype=torch.float64, device="cuda")
+    K = Kernel('gauss', sigma=cfg["sigmaKernel"], precision=prec)
+    plan = rt.Plan(_lib.OP_FLOW_STEP, K.prec_code, 3,
+                   [rt.seg(y=x, w0=a, out0=out, total=tot, ny=N, sigma=K.sigma, dt=0.05,
+                           kernel=K.kernel_code, order=K.order)])
+    plan.run()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        plan.run()
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    pairs = N * N
+    flops = FLOPS_PER_PAIR["apply_k_gauss_d3_m3"] * pairs
+    tflops = flops / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "k_pairs<float,gauss,3,3> + finalize (Euler flow step)",
+            "launch_ms": round(ms, 4), "pairs_per_launch": pairs, "flops_per_pair": 16,
+            "pairs_per_s": pairs / (ms * 1e-3)}
+
+
+def cpu_baseline(cfg, pair_counts, seconds_target=10.0):
+    """FP64 CPU oracle (C/OpenMP restatement of the reference's Fortran sums) timed on a row
+    sample of the workload's two sum kinds; registrations/s extrapolated with the pair counts
+    of the GPU run."""
+    import numpy as np
+    from oracle import lddmm_oracle as orc
+    from registration_amd import _lib
+    F, V = cfg["template"]
+    N = V.shape[0]
+    rng = np.random.default_rng(1)
+    a = 1e-2 * rng.standard_normal(V.shape)
+    rows = min(N, 8192)
+    t0 = time.perf_counter()
+    orc.apply_k(V[:rows], V, a, cfg["sigmaKernel"], 10)
+    tk = time.perf_counter() - t0
+    k_rate = rows * N / tk
+    a1 = rng.standard_normal((3, rows, 3))
+    a2 = rng.standard_normal((3, N, 3))
+    t0 = time.perf_counter()
+    orc.apply_diff_kt(V[:rows], V, a1, a2, cfg["sigmaKernel"], 10)
+    td = time.perf_counter() - t0
+    d_rate = rows * N / td
+    k_pairs = sum(v for op, v in pair_counts.items() if op != _lib.OP_COVECTOR_STEP)
+    d_pairs = pair_counts.get(_lib.OP_COVECTOR_STEP, 0)
+    secs = k_pairs / k_rate + d_pairs / d_rate
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / secs, "unit": "registrations/s", "cores": threads, "kind": "port",
+            "sample": f"oracle applyK and applyDiffKT(na=3), {rows} x {N} pairs each "
+                      f"({tk + td:.1f} s); extrapolated to the GPU run's {k_pairs + d_pairs:.3e} pairs",
+            "extrapolated": True, "pairs_per_s_applyk": k_rate, "pairs_per_s_diffkt": d_rate,
+            "seconds_per_registration": secs}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (one registration "
+              "pair per GPU; see DESIGN.md for how the path shards)", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+    from registration_amd import _lib, runtime as rt
+    _lib.load()
+    torch.cuda.set_device(0)
+    cfg = make_workload(args.workload)
+
+    for _ in range(args.warmup):
+        m = registration(cfg, args.precision)
+        m.optimizeMatching()
+    torch.cuda.synchronize()
+
+    rt.COUNT_PAIRS = True
+    rt.pair_counts.clear()
+    lat = []
+    iters = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        m = registration(cfg, args.precision)
+        m.optimizeMatching()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+        iters.append(len(m.history) - 1)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rt.COUNT_PAIRS = False
+    counts = dict(rt.pair_counts)
+    pairs_per_reg = sum(counts.values()) / args.steps
+
+    # Energy error of the final state against the FP64 form of the same objective.
+    from registration_amd import kernelFunctions as kfun
+    final_obj = m.obj
+    m64 = registration(cfg, "fp64")
+    m64.at = m.at.clone()
+    obj64 = m64.objectiveFun()
+    energy_err = abs(final_obj - obj64) / abs(obj64)
+
+    roof = flow_step_roofline(cfg, args.precision, args.roofline_reps)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(cfg, {k: v / args.steps for k, v in counts.items()})
+
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "registrations/s (plain single-pair GPU path)",
+        "value": args.steps / wall,
+        "unit": "registrations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32c" if args.precision == "fp32c" else "f64",
+        "data": "synthetic (seeded icosphere + RBF target)",
+        "config": {"workload": "single large surface pair: icosphere s6 (40962 V / 81920 F), "
+                               "sigma_V 0.25, sigma_W 0.1, T 20, <=100 CG iterations"
+                   if args.workload == "surface-s6" else "surface pair icosphere s3, T 10, <=50 CG",
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "cg_iterations": iters,
+        "stop_reason": m.stop_reason,
+        "final_energy": final_obj,
+        "energy_rel_err_vs_fp64": energy_err,
+        "pairs_per_registration": pairs_per_reg,
+        "pair_rate_overall": pairs_per_reg / (wall / args.steps),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

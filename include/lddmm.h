@@ -1,0 +1,218 @@
+/*
+ * lddmm.h -- C ABI of the MI355X (gfx950) point-set LDDMM kernel library
+ * (liblddmm_hip.so, built from registration_amd/csrc/lddmm_kernels.hip).
+ *
+ * All arrays are device pointers to row-major float64 data unless stated
+ * otherwise: a point set x(n, d) is x[k*d + c].  Every entry point is
+ * stream-ordered on the hipStream_t passed as `stream` (NULL = the null
+ * stream), returns LDDMM_OK or a negative error code, and never allocates
+ * device memory: scratch space comes in through (work, work_bytes), sized by
+ * the matching *_workspace_bytes query.
+ *
+ * Precision (`prec`):
+ *   LDDMM_PREC_FP64   pairwise arithmetic in FP64 (the plain FP64-equivalent form);
+ *   LDDMM_PREC_FP32C  pairwise arithmetic in FP32 with Kahan-compensated
+ *                     accumulation across source tiles; per-chunk partials and
+ *                     all totals in FP64.
+ *
+ * Kernel selection (`kernel`, `order`) follows kernelFunctions.KernelSpec
+ * (py-lddmm/kernelFunctions.py:238-280): LDDMM_KERNEL_GAUSS is
+ * K(r) = exp(-r^2 / (2 sigma^2)); LDDMM_KERNEL_LAPLACIAN of order 0..4 is
+ * P_order(r/sigma) exp(-r/sigma) with the float32-rounded coefficient tables of
+ * py-lddmm/kernelFunctions.f90:25-26,85-86,461.
+ */
+#ifndef LDDMM_H
+#define LDDMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LDDMM_OK = 0,
+  LDDMM_EINVAL = -1,     /* bad size, dimension, kernel or order */
+  LDDMM_EWORKSPACE = -2, /* work_bytes smaller than the query returned */
+  LDDMM_EHIP = -3,       /* a HIP runtime call failed (see lddmm_last_hip_error) */
+  LDDMM_EUNSUPPORTED = -4
+};
+
+enum { LDDMM_KERNEL_GAUSS = 0, LDDMM_KERNEL_LAPLACIAN = 1 };
+enum { LDDMM_PREC_FP64 = 0, LDDMM_PREC_FP32C = 1 };
+
+/* Operation codes of the segmented pair-sum engine (lddmm_segments_run). */
+enum {
+  /* out0[k,:] = sum_l K(y_k, x_l) w0[l,:]                        (applyK) */
+  LDDMM_OP_APPLY_K = 0,
+  /* out0[k,:] = sum_l 2 K'(r^2) (y_k - x_l) sum_j <tw_j[k], sw_j[l]>
+   * over j < na                                             (applyDiffKT) */
+  LDDMM_OP_DIFF_KT = 1,
+  /* one explicit Euler step of the landmark flow, targets = sources = y:
+   *   v = K(y, y) w0;  out0 = y + dt v;  out1 = v (if non-NULL);
+   *   rowdot[k] = <w0[k], v[k]>;  total[0] = sum_k rowdot[k]   */
+  LDDMM_OP_FLOW_STEP = 2,
+  /* one backward step of the Hamiltonian covector, targets = sources = y,
+   * a = w0, p = w1:
+   *   out0 = p + dt sum_l 2K'(r^2)(y_k - y_l)[<p_k - 2 lambda a_k, a_l> + <a_k, p_l>]
+   * with lambda = coef                                   */
+  LDDMM_OP_COVECTOR_STEP = 3,
+  /* RKHS products: u = K(y, y) w0; total[j] = sum_k <g_j[k], u[k]>, j < ndot */
+  LDDMM_OP_KDOT = 4,
+  /* currents data term of a deformed shape (c1, cr1) = (y, w0) against a
+   * target (c2, cr2) = (x1, w1):
+   *   total[0] = <cr1, K(c1,c1) cr1> - 2 <cr1, K(c1,c2) cr2>   (currentNormDef)
+   * and, when out0/out1 are non-NULL (gradient form):
+   *   out0 = z1 = K(c1,c1) cr1 - K(c1,c2) cr2
+   *   out1 = coef * [applyDiffKT(c1,[cr1],[cr1]) - applyDiffKT(c2,[cr1],[cr2],c1)] */
+  LDDMM_OP_CURRENTS = 5,
+  /* out0[k,:] = sum_l 2<w0_k, w1_l>(2K''<dx,dp>dx + K' dp), p = w2, dx = y_k - y_l,
+   * dp = p_k - p_l                                     (applyDDiffK11and12) */
+  LDDMM_OP_DDIFF_K11_12 = 6,
+  /* total[0] = <w0, K(y, y) w0>  (currentNorm0 when w0 = surfel) */
+  LDDMM_OP_KNORM = 7
+};
+
+/*
+ * One segment of a pair-sum launch: one independent kernel sum (one
+ * registration, one time step, or one call).  The caller fills the fields
+ * marked [in]; lddmm_segments_plan fills the fields marked [plan].
+ * Pointers that an operation does not use may be NULL.
+ */
+typedef struct lddmm_seg {
+  const double *y;     /* [in] targets (ny, d)                                  */
+  const double *x0;    /* [in] sources of set 0 (nx0, d); NULL means y          */
+  const double *x1;    /* [in] sources of set 1 (nx1, d) (currents target)      */
+  const double *w0;    /* [in] weights, meaning per operation (see op codes)    */
+  const double *w1;    /* [in]                                                   */
+  const double *w2;    /* [in]                                                   */
+  const double *tw[4]; /* [in] DIFF_KT target-side weights a1[j] (ny, m)        */
+  const double *sw[4]; /* [in] DIFF_KT source-side weights a2[j] (nx0, m)       */
+  const double *g[2];  /* [in] KDOT second operands (ny, m)                     */
+  double *out0;        /* [in] outputs, meaning per operation                   */
+  double *out1;        /* [in]                                                   */
+  double *rowdot;      /* [in] optional per-row FP64 products (ny)              */
+  double *total;       /* [in] FP64 scalar outputs (1..2 per segment)          */
+  int64_t ny, nx0, nx1;/* [in] row counts                                        */
+  double sigma;        /* [in] kernel width                                      */
+  double dt;           /* [in] time step (FLOW_STEP, COVECTOR_STEP)              */
+  double coef;         /* [in] lambda (COVECTOR_STEP) or gradient factor (CURRENTS) */
+  int32_t kernel;      /* [in] LDDMM_KERNEL_*                                    */
+  int32_t order;       /* [in] Laplacian order 0..4 (ignored for the Gaussian)   */
+  int32_t m;           /* [in] weight columns (APPLY_K, KDOT, DIFF_KT)           */
+  int32_t na;          /* [in] DIFF_KT pair count (1..4) / KDOT ndot (1..2)      */
+  /* [plan] scheduling, filled by lddmm_segments_plan */
+  int64_t item0;       /* first pair-sum work item of this segment              */
+  int64_t part0;       /* offset (doubles) of this segment's chunk partials     */
+  int64_t fin0;        /* first finalize block of this segment                  */
+  int64_t red0;        /* offset (doubles) of its per-block reduction partials  */
+  int32_t row_tiles;   /* target tiles of LDDMM_ROWS_PER_ITEM rows              */
+  int32_t chunks0;     /* source chunks over set 0                              */
+  int32_t chunks1;     /* source chunks over set 1                              */
+  int32_t chunk_len;   /* sources per chunk (multiple of the LDS tile)          */
+  int32_t fin_blocks;  /* finalize blocks                                       */
+  int32_t pad_;
+} lddmm_seg_t;
+
+/* Plan a segmented launch: fills the [plan] fields of segs[0..nseg) and
+ * returns the number of pair-sum work items, finalize blocks and the
+ * workspace size (descriptor head + chunk partials + reduction partials).
+ * Host-only; touches no device memory.  Validates every segment (d, m, na,
+ * kernel, order, sizes); all segments of one launch share the operation, the
+ * kernel family, d, m and na. */
+int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, int nseg,
+                        int64_t *n_items, int64_t *n_fin_blocks,
+                        size_t *work_bytes);
+
+/* Run a planned segmented launch.  segs_host is the planned host array (its
+ * first entry fixes the launch shape).  segs_dev is a device copy of it, or
+ * NULL to have the library copy segs_host into the head of `work` on `stream`.
+ * Results are deterministic: each row sums its sources in a fixed tile order,
+ * chunk partials are combined in chunk order and totals are reduced in a fixed
+ * tree; a segment's chunking depends only on its own sizes, so its result is
+ * the same alone and inside any batch. */
+int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *segs_host,
+                       const lddmm_seg_t *segs_dev, int nseg, int64_t n_items,
+                       int64_t n_fin_blocks, void *work, size_t work_bytes,
+                       void *stream);
+
+/* ---- one-call forms of the reference's f2py kernel sums ----------------
+ * These plan one segment and copy its descriptor into the head of `work`. */
+
+/* kernelFunctions.f90:1-54 applyk (via Kernel.applyK, kernelFunctions.py:310-369):
+ * out(ny, m) = sum_l K(|y_k - x_l|) w(l, :).  y: (ny, d), x: (nx, d), w: (nx, m). */
+size_t lddmm_apply_k_workspace_bytes(int64_t ny, int64_t nx, int d, int m);
+int lddmm_apply_k(const double *y, const double *x, const double *w,
+                  int64_t ny, int64_t nx, int d, int m, int kernel, int order,
+                  double sigma, int prec, double *out, void *work,
+                  size_t work_bytes, void *stream);
+
+/* kernelFunctions.f90:56-114 applykdifft (via Kernel.applyDiffKT,
+ * kernelFunctions.py:413-449): a1 (na, ny, m), a2 (na, nx, m) -> out (ny, d). */
+size_t lddmm_apply_diff_kt_workspace_bytes(int64_t ny, int64_t nx, int d,
+                                           int m, int na);
+int lddmm_apply_diff_kt(const double *y, const double *x, const double *a1,
+                        const double *a2, int64_t ny, int64_t nx, int d, int m,
+                        int na, int kernel, int order, double sigma, int prec,
+                        double *out, void *work, size_t work_bytes,
+                        void *stream);
+
+/* kernelFunctions.f90:430-496 applykdiff11and12 (via
+ * Kernel.applyDDiffK11and12, kernelFunctions.py:452-454):
+ * x, a1, a2, p (n, d) -> out (n, d). */
+size_t lddmm_apply_ddiff_k11and12_workspace_bytes(int64_t n, int d);
+int lddmm_apply_ddiff_k11and12(const double *x, const double *a1,
+                               const double *a2, const double *p, int64_t n,
+                               int d, int kernel, int order, double sigma,
+                               int prec, double *out, void *work,
+                               size_t work_bytes, void *stream);
+
+/* ---- geometry of the currents data term -------------------------------- */
+
+/* surfaces.Surface.updateVertices (py-lddmm/surfaces.py:69-75):
+ * centers(F,3) = (x_f0 + x_f1 + x_f2)/3, surfel(F,3) = (x_f1-x_f0) x (x_f2-x_f0).
+ * faces: int32 (F, 3). */
+int lddmm_surface_geometry(const double *vertices, const int32_t *faces,
+                           int64_t nf, double *centers, double *surfel,
+                           void *stream);
+
+/* curves.Curve.updateVertices (py-lddmm/curves.py:67-73):
+ * centers(E,d) = (x_e0 + x_e1)/2, lenel(E,d) = x_e1 - x_e0.  edges: int32 (E, 2). */
+int lddmm_curve_geometry(const double *vertices, const int32_t *edges,
+                         int64_t ne, int d, double *centers, double *lenel,
+                         void *stream);
+
+/* Vertex scatter of surfaces.currentNormGradient (surfaces.py:1069-1089) as a
+ * deterministic gather: px[v] = scale * sum over the incidences (f, j) of v, in
+ * the order given by the CSR (incid_ptr (nv+1), incid (3F) holding f*3+j), of
+ *   dz[f] - (x_{f,j+2} - x_{f,j+1}) x z1[f].
+ * With incidences sorted by (j, f) the sum order equals the reference's. */
+int lddmm_surface_vertex_gradient(const double *vertices, const int32_t *faces,
+                                  const double *z1, const double *dz,
+                                  const int32_t *incid_ptr,
+                                  const int32_t *incid, int64_t nv,
+                                  double scale, double *px, void *stream);
+
+/* Vertex scatter of curves.currentNormGradient (curves.py:604-613):
+ * px[v] = scale * sum over incidences (e, j) of v of dz[e] + (2j-1) z1[e]. */
+int lddmm_curve_vertex_gradient(const double *z1, const double *dz,
+                                const int32_t *incid_ptr, const int32_t *incid,
+                                int64_t nv, int d, double scale, double *px,
+                                void *stream);
+
+/* ---- small helpers ------------------------------------------------------ */
+
+/* Deterministic FP64 sum of x[0..n) into *out (single fixed reduction tree). */
+int lddmm_sum_f64(const double *x, int64_t n, double *out, void *stream);
+
+/* Last HIP error code seen by the library (0 if none). */
+int lddmm_last_hip_error(void);
+/* Library version string. */
+const char *lddmm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LDDMM_H */

@@ -1,0 +1,675 @@
+"""FP64 CPU oracle of py-lddmm's point-set LDDMM hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker the GPU product path
+(registration_amd/) is compared against.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import it; the product package never does.
+
+It restates, in Python 3 / NumPy float64, the executed reference path:
+  Kernel                  py-lddmm/kernelFunctions.py:238-454 (sums in oracle/kernel_sums.c,
+                          restating kernelFunctions.f90:1-114,430-496)
+  landmark flow/adjoint   py-lddmm/pointEvolution.py:599-817
+  surfaces / curves       py-lddmm/surfaces.py:61-75,711-717,1009-1089; curves.py:58-73,539-613
+  SurfaceMatching         py-lddmm/surfaceMatching.py:19-484 (affine='none', symmetric=False)
+  CurveMatching           py-lddmm/curveMatching.py:19-352 (affine='none')
+  cg                      py-lddmm/conjugateGradient.py:33-217
+Python-2 idioms are restated with their Python-2 meaning (``x == None`` on arrays -> ``is None``).
+File output (VTK), logging text and the saveRate branches are left out: they do not feed
+back into the optimisation.
+
+With ``sums='reference'`` the kernel sums call the reference's own Fortran subroutines
+compiled by oracle/Makefile into oracle/_ref/libkff_ref.so (present only where that build ran).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle_sums.so")
+_REF = os.path.join(_HERE, "_ref", "libkff_ref.so")
+
+_P = ctypes.c_void_p
+_lib = None
+_ref = None
+
+
+def build():
+    """Compile oracle/kernel_sums.c (gcc) into oracle/liboracle_sums.so."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = ctypes.CDLL(_LIB)
+        i64, i, d = ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        _lib.oracle_apply_k.argtypes = [_P, _P, _P, i64, i64, i, i, i, d, _P]
+        _lib.oracle_apply_diff_kt.argtypes = [_P, _P, _P, _P, i64, i64, i, i, i, i, d, _P]
+        _lib.oracle_apply_ddiff_k11and12.argtypes = [_P, _P, _P, _P, i64, i, i, d, _P]
+        for f in (_lib.oracle_apply_k, _lib.oracle_apply_diff_kt, _lib.oracle_apply_ddiff_k11and12):
+            f.restype = ctypes.c_int
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(_REF)
+
+
+def ref():
+    """The reference's Fortran kernel sums (column-major, all arguments by reference)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref/libkff_ref.so is not built (make -C oracle ref)")
+        _ref = ctypes.CDLL(_REF)
+    return _ref
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+# --------------------------------------------------------------------------------------
+# Kernel sums
+# --------------------------------------------------------------------------------------
+def apply_k(y, x, w, sigma, order, sums="oracle"):
+    """f = sum_l K(|y_k - x_l|) w_l (kernelFunctions.f90:1-54)."""
+    y, x, w = _c(y), _c(x), _c(w)
+    if w.ndim == 1:
+        w = w[:, None]
+    ny, d = y.shape
+    nx, m = w.shape
+    if sums == "reference":
+        f = np.zeros((ny, m), order="F")
+        ci = lambda v: ctypes.byref(ctypes.c_int(v))
+        ref().applyk_(_p(np.asfortranarray(y)), _p(np.asfortranarray(x)), _p(np.asfortranarray(w)),
+                      ctypes.byref(ctypes.c_double(sigma)), ci(order), ci(ny), ci(nx), ci(d), ci(m),
+                      _p(f))
+        return np.ascontiguousarray(f)
+    f = np.empty((ny, m))
+    rc = lib().oracle_apply_k(_p(y), _p(x), _p(w), ny, nx, d, m, order, sigma, _p(f))
+    if rc:
+        raise ValueError("oracle_apply_k: invalid arguments")
+    return f
+
+
+def apply_diff_kt(y, x, a1, a2, sigma, order, sums="oracle"):
+    """f = sum_l 2K'(r^2)(y_k - x_l) sum_{j,c} a1[j,k,c] a2[j,l,c] (kernelFunctions.f90:56-114)."""
+    y, x, a1, a2 = _c(y), _c(x), _c(a1), _c(a2)
+    ny, d = y.shape
+    na, _, m = a1.shape
+    nx = x.shape[0]
+    if sums == "reference":
+        f = np.zeros((ny, d), order="F")
+        ci = lambda v: ctypes.byref(ctypes.c_int(v))
+        ref().applykdifft_(_p(np.asfortranarray(y)), _p(np.asfortranarray(x)),
+                           _p(np.asfortranarray(a1)), _p(np.asfortranarray(a2)),
+                           ctypes.byref(ctypes.c_double(sigma)), ci(order), ci(ny), ci(nx), ci(d),
+                           ci(m), ci(na), _p(f))
+        return np.ascontiguousarray(f)
+    f = np.empty((ny, d))
+    rc = lib().oracle_apply_diff_kt(_p(y), _p(x), _p(a1), _p(a2), ny, nx, d, m, na, order, sigma,
+                                    _p(f))
+    if rc:
+        raise ValueError("oracle_apply_diff_kt: invalid arguments")
+    return f
+
+
+def apply_ddiff_k11and12(x, a1, a2, p, sigma, order, sums="oracle"):
+    """kernelFunctions.f90:430-496."""
+    x, a1, a2, p = _c(x), _c(a1), _c(a2), _c(p)
+    n, d = x.shape
+    if sums == "reference":
+        f = np.zeros((n, d), order="F")
+        ci = lambda v: ctypes.byref(ctypes.c_int(v))
+        ref().applykdiff11and12_(_p(np.asfortranarray(x)), _p(np.asfortranarray(a1)),
+                                 _p(np.asfortranarray(a2)), _p(np.asfortranarray(p)),
+                                 ctypes.byref(ctypes.c_double(sigma)), ci(order), ci(n), ci(d), _p(f))
+        return np.ascontiguousarray(f)
+    f = np.empty((n, d))
+    rc = lib().oracle_apply_ddiff_k11and12(_p(x), _p(a1), _p(a2), _p(p), n, d, order, sigma, _p(f))
+    if rc:
+        raise ValueError("oracle_apply_ddiff_k11and12: invalid arguments")
+    return f
+
+
+class Kernel:
+    """kernelFunctions.Kernel restated (kernelFunctions.py:238-454), affine='none' only."""
+
+    def __init__(self, name='gauss', affine='none', sigma=6.5, order=10, w1=1.0, w2=1.0, dim=3,
+                 center=(0, 0, 0), weight=1.0, sums="oracle"):
+        self.name, self.sigma, self.order, self.affine = name, sigma, order, affine
+        self.sums = sums
+        if name == 'laplacian':
+            if self.order > 4:
+                self.order = 3
+        elif name == 'gauss':
+            self.order = 10
+        else:
+            self.name = 'none'
+        if affine != 'none':
+            raise NotImplementedError("oracle restates affine='none' only")
+
+    def applyK(self, x, a, firstVar=None):
+        if self.name == 'none':
+            return np.zeros((x.shape[0], a.shape[1]))
+        y = x if firstVar is None else firstVar
+        return apply_k(y, x, a, self.sigma, self.order, self.sums)
+
+    def applyDiffKT(self, x, a1, a2, firstVar=None):
+        y = x if firstVar is None else firstVar
+        if self.name == 'none':
+            return np.zeros(y.shape)
+        return apply_diff_kt(y, x, np.asarray(a1), np.asarray(a2), self.sigma, self.order, self.sums)
+
+    def applyDDiffK11and12(self, x, n, a, p):
+        return apply_ddiff_k11and12(x, n, a, p, self.sigma, self.order, self.sums)
+
+
+# --------------------------------------------------------------------------------------
+# Landmark flow and adjoint (pointEvolution.py:599-817), affine=None or A=0 arrays
+# --------------------------------------------------------------------------------------
+def landmarkEPDiff(T, x0, a0, KparDiff):
+    """pointEvolution.py:599-658 (no affine, no extras)."""
+    N, dim = x0.shape
+    timeStep = 1.0 / T
+    at = np.zeros([T, N, dim])
+    xt = np.zeros([T + 1, N, dim])
+    xt[0] = x0
+    at[0] = a0
+    for k in range(T):
+        z = xt[k]
+        a = at[k]
+        xt[k + 1] = z + timeStep * KparDiff.applyK(z, a)
+        if k < T - 1:
+            at[k + 1] = a - timeStep * KparDiff.applyDiffKT(z, a[np.newaxis], a[np.newaxis])
+    return xt, at
+
+
+def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, withPointSet=None,
+                                 withNormals=None):
+    """pointEvolution.py:666-730 (Jacobian option omitted)."""
+    N, dim = x0.shape
+    M = at.shape[0] + 1
+    timeStep = 1.0 / (M - 1)
+    xt = np.zeros([M, N, dim])
+    xt[0] = x0
+    yt = nt = None
+    if withPointSet is not None:
+        yt = np.zeros([M, withPointSet.shape[0], dim])
+        yt[0] = withPointSet
+    if withNormals is not None:
+        nt = np.zeros([M, N, dim])
+        nt[0] = withNormals
+    for k in range(M - 1):
+        z = xt[k]
+        a = at[k]
+        xt[k + 1] = z + timeStep * KparDiff.applyK(z, a)
+        if affine is not None:
+            A, b = affine
+            xt[k + 1] += timeStep * (np.dot(z, A[k].T) + b[k])
+        if yt is not None:
+            zy = yt[k]
+            yt[k + 1] = zy + timeStep * KparDiff.applyK(z, a, firstVar=zy)
+            if affine is not None:
+                yt[k + 1] += timeStep * (np.dot(zy, affine[0][k].T) + affine[1][k])
+        if nt is not None:
+            zn = nt[k]
+            nt[k + 1] = zn - timeStep * KparDiff.applyDiffKT(z, zn[np.newaxis], a[np.newaxis])
+            if affine is not None:
+                nt[k + 1] += timeStep * np.dot(zn, affine[0][k])
+    if yt is None and nt is None:
+        return xt
+    out = [xt]
+    if yt is not None:
+        out.append(yt)
+    if nt is not None:
+        out.append(nt)
+    return out
+
+
+def landmarkHamiltonianCovector(x0, at, px1, KparDiff, regweight, affine=None):
+    """pointEvolution.py:735-765."""
+    N, dim = x0.shape
+    M = at.shape[0]
+    timeStep = 1.0 / M
+    xt = landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=affine)
+    pxt = np.zeros([M + 1, N, dim])
+    pxt[M] = px1
+    for t in range(M):
+        px = pxt[M - t]
+        z = xt[M - t - 1]
+        a = at[M - t - 1]
+        a1 = np.concatenate((px[np.newaxis], a[np.newaxis], -2 * regweight * a[np.newaxis]))
+        a2 = np.concatenate((a[np.newaxis], px[np.newaxis], a[np.newaxis]))
+        zpx = KparDiff.applyDiffKT(z, a1, a2)
+        pxt[M - t - 1] = px + timeStep * zpx
+        if affine is not None:
+            pxt[M - t - 1] += timeStep * np.dot(px, affine[0][M - t - 1])
+    return pxt, xt
+
+
+def landmarkHamiltonianGradient(x0, at, px1, KparDiff, regweight, getCovector=False, affine=None):
+    """pointEvolution.py:793-817 (affine=None)."""
+    pxt, xt = landmarkHamiltonianCovector(x0, at, px1, KparDiff, regweight, affine=affine)
+    dat = np.zeros(at.shape)
+    for k in range(at.shape[0]):
+        dat[k] = 2 * regweight * at[k] - pxt[k + 1]
+    if getCovector:
+        return dat, xt, pxt
+    return dat, xt
+
+
+# --------------------------------------------------------------------------------------
+# Surfaces and curves (geometry and currents data term)
+# --------------------------------------------------------------------------------------
+class Surface:
+    """surfaces.Surface restated for the FV / copy constructors (surfaces.py:24-75,261-263,711-717)."""
+
+    def __init__(self, surf=None, FV=None):
+        if surf is None:
+            self.vertices = np.copy(FV[1]).astype(np.float64)
+            self.faces = np.int_(FV[0])
+        else:
+            self.vertices = np.copy(surf.vertices)
+            self.faces = np.copy(surf.faces)
+        self.computeCentersAreas()
+
+    def computeCentersAreas(self):
+        x1 = self.vertices[self.faces[:, 0]]
+        x2 = self.vertices[self.faces[:, 1]]
+        x3 = self.vertices[self.faces[:, 2]]
+        self.centers = (x1 + x2 + x3) / 3
+        self.surfel = np.cross(x2 - x1, x3 - x1)
+
+    def updateVertices(self, x0):
+        self.vertices = np.copy(x0)
+        self.computeCentersAreas()
+
+    def flipFaces(self):
+        self.faces = self.faces[:, [0, 2, 1]]
+        self.computeCentersAreas()
+
+    def surfVolume(self):
+        f, v = self.faces, self.vertices
+        z = 0
+        for c in f:
+            z += np.linalg.det(v[c[:], :]) / 6
+        return z
+
+
+def surf_currentNorm0(fv1, K):
+    return np.multiply(fv1.surfel, K.applyK(fv1.centers, fv1.surfel)).sum()
+
+
+def surf_currentNormDef(fvDef, fv1, K):
+    c1, cr1, c2, cr2 = fvDef.centers, fvDef.surfel, fv1.centers, fv1.surfel
+    return (np.multiply(cr1, K.applyK(c1, cr1)).sum()
+            - 2 * np.multiply(cr1, K.applyK(c2, cr2, firstVar=c1)).sum())
+
+
+def surf_currentNormGradient(fvDef, fv1, K):
+    """surfaces.py:1040-1089."""
+    xDef = fvDef.vertices
+    c1, cr1, c2, cr2 = fvDef.centers, fvDef.surfel, fv1.centers, fv1.surfel
+    dim = c1.shape[1]
+    z1 = K.applyK(c1, cr1) - K.applyK(c2, cr2, firstVar=c1)
+    dz1 = (1. / 3.) * (K.applyDiffKT(c1, cr1[np.newaxis], cr1[np.newaxis])
+                       - K.applyDiffKT(c2, cr1[np.newaxis], cr2[np.newaxis], firstVar=c1))
+    xDef1 = xDef[fvDef.faces[:, 0]]
+    xDef2 = xDef[fvDef.faces[:, 1]]
+    xDef3 = xDef[fvDef.faces[:, 2]]
+    px = np.zeros([xDef.shape[0], dim])
+    for I, crs in ((fvDef.faces[:, 0], np.cross(xDef3 - xDef2, z1)),
+                   (fvDef.faces[:, 1], np.cross(xDef1 - xDef3, z1)),
+                   (fvDef.faces[:, 2], np.cross(xDef2 - xDef1, z1))):
+        for k in range(I.size):
+            px[I[k]] = px[I[k]] + dz1[k] - crs[k]
+    return 2 * px
+
+
+class Curve:
+    """curves.Curve restated for the pointSet / FV / copy constructors (curves.py:15-73)."""
+
+    def __init__(self, curve=None, FV=None, pointSet=None, isOpen=False):
+        if curve is not None:
+            self.vertices = np.copy(curve.vertices)
+            self.faces = np.copy(curve.faces)
+        elif FV is not None:
+            self.vertices = np.copy(FV[1]).astype(np.float64)
+            self.faces = np.int_(FV[0])
+        else:
+            n = pointSet.shape[0]
+            self.vertices = np.copy(pointSet).astype(np.float64)
+            self.faces = np.int_(np.zeros([n, 2]))
+            for k in range(n - 1):
+                self.faces[k] = (k, k + 1)
+            if not isOpen:
+                self.faces[n - 1] = (n - 1, 0)
+        self.computeCentersLengths()
+
+    def computeCentersLengths(self):
+        x1 = self.vertices[self.faces[:, 0]]
+        x2 = self.vertices[self.faces[:, 1]]
+        self.centers = (x1 + x2) / 2
+        self.lenel = x2 - x1
+
+    def updateVertices(self, x0):
+        self.vertices = np.copy(x0)
+        self.computeCentersLengths()
+
+
+def curve_currentNorm0(fv1, K):
+    return np.multiply(fv1.lenel, K.applyK(fv1.centers, fv1.lenel)).sum()
+
+
+def curve_currentNormDef(fvDef, fv1, K):
+    c1, cr1, c2, cr2 = fvDef.centers, fvDef.lenel, fv1.centers, fv1.lenel
+    return (np.multiply(cr1, K.applyK(c1, cr1)).sum()
+            - 2 * np.multiply(cr1, K.applyK(c2, cr2, firstVar=c1)).sum())
+
+
+def curve_currentNormGradient(fvDef, fv1, K):
+    """curves.py:566-613."""
+    xDef = fvDef.vertices
+    c1, cr1, c2, cr2 = fvDef.centers, fvDef.lenel, fv1.centers, fv1.lenel
+    dim = c1.shape[1]
+    z1 = K.applyK(c1, cr1) - K.applyK(c2, cr2, firstVar=c1)
+    dz1 = .5 * (K.applyDiffKT(c1, cr1[np.newaxis], cr1[np.newaxis])
+                - K.applyDiffKT(c2, cr1[np.newaxis], cr2[np.newaxis], firstVar=c1))
+    px = np.zeros([xDef.shape[0], dim])
+    I = fvDef.faces[:, 0]
+    for k in range(I.size):
+        px[I[k]] = px[I[k]] + dz1[k] - z1[k]
+    I = fvDef.faces[:, 1]
+    for k in range(I.size):
+        px[I[k]] = px[I[k]] + dz1[k] + z1[k]
+    return 2 * px
+
+
+# --------------------------------------------------------------------------------------
+# Matching drivers and the CG optimiser
+# --------------------------------------------------------------------------------------
+class Direction:
+    def __init__(self):
+        self.diff = []
+        self.aff = []
+        self.initx = []
+
+
+class _Matching:
+    """Shared body of SurfaceMatching (surfaceMatching.py:74-484) and CurveMatching
+    (curveMatching.py:58-352) for affine='none', symmetric=False."""
+
+    def _setup(self, x0, KparDiff, KparDist, sigmaError, timeStep, regWeight):
+        self.x0 = np.copy(x0)
+        self.npt, self.dim = x0.shape
+        self.KparDiff, self.KparDist = KparDiff, KparDist
+        self.sigmaError = sigmaError
+        self.regweight = regWeight
+        self.Tsize = int(round(1.0 / timeStep))
+        self.at = np.zeros([self.Tsize, self.npt, self.dim])
+        self.atTry = np.zeros_like(self.at)
+        self.xt = np.tile(self.x0, [self.Tsize + 1, 1, 1])
+        self.obj = None
+        self.objTry = None
+        self.gradCoeff = self.npt
+        self.affineDim = 0
+        self.Afft = np.zeros([self.Tsize, 0])
+        self.AfftTry = np.zeros([self.Tsize, 0])
+        self.nobj = 0  # objective evaluations (updateTry + objectiveFun)
+
+    def dataTerm(self, fvDef):
+        return self.fun_obj(fvDef, self.fv1, self.KparDist) / (self.sigmaError ** 2)
+
+    def objectiveFunDef(self, at, withTrajectory=False):
+        timeStep = 1.0 / self.Tsize
+        xt = landmarkDirectEvolutionEuler(self.x0, at, self.KparDiff)
+        obj = 0
+        for t in range(self.Tsize):
+            z = xt[t]
+            a = at[t]
+            ra = self.KparDiff.applyK(z, a)
+            obj = obj + self.regweight * timeStep * np.multiply(a, ra).sum()
+        if withTrajectory:
+            return obj, xt
+        return obj
+
+    def objectiveFun(self):
+        if self.obj is None:
+            self.obj0 = self.fun_obj0(self.fv1, self.KparDist) / (self.sigmaError ** 2)
+            self.obj, self.xt = self.objectiveFunDef(self.at, withTrajectory=True)
+            self.fvDef.updateVertices(self.xt[-1])
+            self.obj += self.obj0 + self.dataTerm(self.fvDef)
+            self.nobj += 1
+        return self.obj
+
+    def getVariable(self):
+        return [self.at, self.Afft]
+
+    def updateTry(self, dir, eps, objRef=None):
+        objTry = self.obj0
+        atTry = self.at - eps * dir.diff
+        foo = self.objectiveFunDef(atTry, withTrajectory=True)
+        objTry += foo[0]
+        ff = self._copy_shape(self.fvDef)
+        ff.updateVertices(foo[1][-1])
+        objTry += self.dataTerm(ff)
+        self.nobj += 1
+        if np.isnan(objTry):
+            return 1e500
+        if (objRef is None) or (objTry < objRef):
+            self.atTry = atTry
+            self.objTry = objTry
+        return objTry
+
+    def endPointGradient(self):
+        return self.fun_objGrad(self.fvDef, self.fv1, self.KparDist) / self.sigmaError ** 2
+
+    def getGradient(self, coeff=1.0):
+        px1 = -self.endPointGradient()
+        foo = landmarkHamiltonianGradient(self.x0, self.at, px1, self.KparDiff, self.regweight)
+        grd = Direction()
+        grd.diff = foo[0] / (coeff * self.Tsize)
+        grd.aff = np.zeros(self.Afft.shape)
+        return grd
+
+    def addProd(self, dir1, dir2, beta):
+        d = Direction()
+        d.diff = dir1.diff + beta * dir2.diff
+        d.aff = dir1.aff + beta * dir2.aff
+        return d
+
+    def copyDir(self, dir0):
+        d = Direction()
+        d.diff = np.copy(dir0.diff)
+        d.aff = np.copy(dir0.aff)
+        return d
+
+    def dotProduct(self, g1, g2):
+        res = np.zeros(len(g2))
+        for t in range(self.Tsize):
+            z = self.xt[t]
+            u = self.KparDiff.applyK(z, g1.diff[t])
+            for ll, gr in enumerate(g2):
+                res[ll] = res[ll] + np.multiply(gr.diff[t], u).sum()
+        return res
+
+    def acceptVarTry(self):
+        self.obj = self.objTry
+        self.at = np.copy(self.atTry)
+        self.Afft = np.copy(self.AfftTry)
+
+    def endOfIteration(self):
+        self.iter += 1
+        obj1, self.xt = self.objectiveFunDef(self.at, withTrajectory=True)
+        self.fvDef.updateVertices(self.xt[-1])
+
+
+class SurfaceMatching(_Matching):
+    """surfaceMatching.SurfaceMatching with errorType='current' (surfaceMatching.py:74-484)."""
+
+    def __init__(self, Template, Target, KparDiff, KparDist, sigmaError=1.0, timeStep=.1,
+                 regWeight=1.0, maxIter=1000):
+        self.fv0 = Surface(surf=Template)
+        self.fv1 = Surface(surf=Target)
+        v0 = self.fv0.surfVolume()
+        v1 = self.fv1.surfVolume()
+        if v0 * v1 < 0:
+            self.fv1.flipFaces()
+        self.fvInit = Surface(surf=self.fv0)
+        self.fvDef = Surface(surf=self.fvInit)
+        self.fun_obj0 = surf_currentNorm0
+        self.fun_obj = surf_currentNormDef
+        self.fun_objGrad = surf_currentNormGradient
+        self._setup(self.fvInit.vertices, KparDiff, KparDist, sigmaError, timeStep, regWeight)
+        self.maxIter = maxIter
+        self.iter = 0
+        self.epsInit = 0.1
+        self.gradLB = 0.001
+
+    @staticmethod
+    def _copy_shape(s):
+        return Surface(surf=s)
+
+    def optimizeMatching(self, cgfun=None):
+        grd = self.getGradient(self.gradCoeff)
+        [grd2] = self.dotProduct(grd, [grd])
+        self.gradEps = max(self.gradLB, np.sqrt(grd2) / 10000)
+        return cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+
+
+class CurveMatching(_Matching):
+    """curveMatching.CurveMatching with errorType='current' (curveMatching.py:58-352)."""
+
+    def __init__(self, Template, Target, KparDiff, KparDist, sigmaError=1.0, timeStep=.1,
+                 regWeight=1.0, maxIter=1000, gradLB=0.001):
+        self.fv0 = Curve(curve=Template)
+        self.fv1 = Curve(curve=Target)
+        self.fvDef = Curve(curve=self.fv0)
+        self.fun_obj0 = curve_currentNorm0
+        self.fun_obj = curve_currentNormDef
+        self.fun_objGrad = curve_currentNormGradient
+        self._setup(self.fv0.vertices, KparDiff, KparDist, sigmaError, timeStep, regWeight)
+        self.maxIter = maxIter
+        self.iter = 0
+        self.epsInit = 10.0
+        self.gradLB = gradLB
+
+    @staticmethod
+    def _copy_shape(s):
+        return Curve(curve=s)
+
+    def optimizeMatching(self):
+        grd = self.getGradient(self.gradCoeff)
+        [grd2] = self.dotProduct(grd, [grd])
+        self.gradEps = max(self.gradLB, np.sqrt(grd2) / 10000)
+        return cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+
+
+def cg(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=10.):
+    """conjugateGradient.cg restated (conjugateGradient.py:33-217).  Returns the per-iteration
+    objective history (the reference returns opt.getVariable(); the history is kept on
+    ``opt.history``)."""
+    gradEps = getattr(opt, 'gradEps', 1.0)
+    cgBurnIn = getattr(opt, 'cgBurnIn', -1)
+    gradCoeff = getattr(opt, 'gradCoeff', 1.0)
+    restartRate = getattr(opt, 'restartRate', 100)
+    eps = epsInit
+    epsMin = 1e-10
+    opt.converged = False
+    obj = opt.objectiveFun()
+    opt.history = [obj]
+    opt.stop_reason = 'maxIter'
+    skipCG = 0
+    for it in range(maxIter):
+        if it % restartRate == 0:
+            skipCG = 1
+        grd = opt.getGradient(gradCoeff)
+        hasDot = hasattr(opt, 'dotProduct')
+        dot = (lambda g, gs: opt.dotProduct(g, gs)) if hasDot else \
+            (lambda g, gs: [np.multiply(g, h).sum() for h in gs])
+        copy = opt.copyDir if hasattr(opt, 'copyDir') else np.copy
+        if it == 0 or it == cgBurnIn:
+            [grdOld2] = dot(grd, [grd])
+            grd2 = grdOld2
+            grdTry = np.sqrt(max(1e-20, grdOld2))
+            oldDir = copy(grd)
+            grdOld = copy(grd)
+            dir0 = copy(grd)
+            beta = 0
+        else:
+            [grd2, grd12] = dot(grd, [grd, grdOld])
+            if skipCG == 0:
+                beta = max(0, (grd2 - grd12) / grdOld2)
+            else:
+                beta = 0
+            grdOld2 = grd2
+            grdTry = np.sqrt(max(1e-20, grd2 + beta * grd12))
+            dir0 = opt.addProd(grd, oldDir, beta) if hasattr(opt, 'addProd') else grd + beta * oldDir
+            oldDir = copy(dir0)
+            grdOld = copy(grd)
+        objTry = opt.updateTry(dir0, eps, obj)
+        noUpdate = 0
+        if objTry > obj:
+            epsSmall = 0.000001 / grdTry
+            objTry0 = opt.updateTry(dir0, epsSmall, obj)
+            if objTry0 > obj:
+                if (skipCG == 1) | (beta < 1e-10):
+                    opt.stop_reason = 'bad direction'
+                    break
+                else:
+                    if verb:
+                        skipCG = 1
+                        noUpdate = 1
+            else:
+                while (objTry > obj) and (eps > epsMin):
+                    eps = eps / 2
+                    objTry = opt.updateTry(dir0, eps, obj)
+        if noUpdate == 0:
+            contt = 1
+            while contt == 1:
+                objTry2 = opt.updateTry(dir0, .5 * eps, objTry)
+                if objTry > objTry2:
+                    eps = eps / 2
+                    objTry = objTry2
+                else:
+                    contt = 0
+            contt = 1
+            while contt == 1:
+                objTry2 = opt.updateTry(dir0, 1.25 * eps, objTry)
+                if objTry > objTry2:
+                    eps *= 1.25
+                    objTry = objTry2
+                else:
+                    contt = 0
+            if np.fabs(obj - objTry) < .000001:
+                if (skipCG == 1) | (beta < 1e-10):
+                    opt.converged = True
+                    opt.stop_reason = 'small variation'
+                    break
+                else:
+                    skipCG = 1
+                    eps = 1.0
+            else:
+                skipCG = 0
+            opt.acceptVarTry()
+            obj = objTry
+            opt.history.append(obj)
+            if np.sqrt(grd2) < gradEps:
+                opt.converged = True
+                opt.stop_reason = 'small gradient'
+                break
+            eps = 100 * eps
+            if hasattr(opt, 'endOfIteration'):
+                opt.endOfIteration()
+    return opt.history

@@ -1,0 +1,1391 @@
+// MI355X (gfx950) kernels for the point-set LDDMM hot path of py-lddmm.
+//
+// One segmented pair-sum engine serves every kernel sum of the path:
+//   applyK / applyDiffKT / applyDDiffK11and12   (py-lddmm/kernelFunctions.f90:1-114,430-496)
+//   the Euler flow step and the covector step    (py-lddmm/pointEvolution.py:694-697,746-760)
+//   the RKHS products of the matching driver      (py-lddmm/surfaceMatching.py:353-377)
+//   the currents norm and its gradient            (py-lddmm/surfaces.py:1009-1089, curves.py:539-613)
+//
+// Work decomposition.  A launch holds nseg independent segments (one kernel
+// sum each: a registration, a time step, a call).  A segment's targets are cut
+// into row tiles of ROWS rows (BLOCK threads x RPT rows per thread, rows held
+// in registers) and its sources into chunks of whole LDS tiles.  One work item
+// = (segment, row tile, source chunk) = one workgroup.  The workgroup stages
+// TS sources at a time into LDS (structure of arrays, all lanes read the same
+// source: LDS broadcast) and every lane accumulates its RPT rows.  Chunk
+// partials go to the workspace in FP64; a finalize kernel combines them in
+// chunk order and applies the operation's epilogue (Euler update, covector
+// update, row products); FP64 totals are reduced in a fixed tree.  The chunk
+// length of a segment depends only on that segment's sizes, so a segment's
+// result is bit-identical alone and inside any batch.
+//
+// Precision.  FP64 mode evaluates every pair in double (exp from the device
+// math library) with the reference's diagonal cut-off.  FP32 mode prescales
+// positions so that the Gaussian is a bare v_exp_f32 of -r'^2, accumulates
+// each KF-source run in FP32 and folds it into a Kahan-compensated (sum, comp)
+// pair; partials leave the kernel as double(sum) - double(comp).
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <type_traits>
+
+#include "lddmm.h"
+
+namespace {
+
+constexpr int BLOCK = 256;       // threads per pair-sum workgroup
+constexpr int TS = 256;          // sources per LDS tile
+constexpr int KF = 32;           // FP32 Kahan fold interval (sources)
+constexpr int FIN_BLOCK = 256;   // rows per finalize workgroup
+constexpr int TOT_BLOCK = 256;   // threads of the totals kernel
+constexpr int64_t ITEMS_PER_SEGMENT = 2048;  // chunking target (per segment)
+constexpr int MAX_NRED = 2;      // FP64 totals per segment
+
+constexpr double LOG2E = 1.4426950408889634074;
+// The reference's cut-off literal 1e-8 is single precision widened to double.
+constexpr double CUTOFF = (double)1e-8f;
+
+int g_last_hip_error = 0;
+
+template <typename R>
+constexpr int rows_per_thread() { return sizeof(R) == 4 ? 2 : 1; }
+
+int rows_per_item(int prec) {
+  return BLOCK * (prec == LDDMM_PREC_FP32C ? rows_per_thread<float>()
+                                           : rows_per_thread<double>());
+}
+
+// Laplacian polynomial tables, float32 literals of kernelFunctions.f90 widened
+// to double.  P_o: c_(o+1,:) (:25-26); Q_o: c1_(o,:) (:85-86); R_o: c2_(o-1,:) (:461).
+__constant__ double LAP_P[5][5] = {
+    {1.0, 0.0, 0.0, 0.0, 0.0},
+    {1.0, 1.0, 0.0, 0.0, 0.0},
+    {1.0, 1.0, (double)(1.0f / 3.0f), 0.0, 0.0},
+    {1.0, 1.0, (double)0.4f, (double)(1.0f / 15.0f), 0.0},
+    {1.0, 1.0, (double)(3.0f / 7.0f), (double)(2.0f / 21.0f), (double)(1.0f / 105.0f)}};
+__constant__ double LAP_Q[5][4] = {
+    {0.0, 0.0, 0.0, 0.0},  // order 0 has no first derivative (rejected on the host)
+    {1.0, 0.0, 0.0, 0.0},
+    {(double)(1.0f / 3.0f), (double)(1.0f / 3.0f), 0.0, 0.0},
+    {(double)(1.0f / 5.0f), (double)(1.0f / 5.0f), (double)(1.0f / 15.0f), 0.0},
+    {(double)(1.0f / 7.0f), (double)(1.0f / 7.0f), (double)(2.0f / 35.0f), (double)(1.0f / 105.0f)}};
+__constant__ double LAP_R[5][3] = {
+    {0.0, 0.0, 0.0},  // orders 0, 1 have no second derivative (rejected on the host)
+    {0.0, 0.0, 0.0},
+    {(double)(1.0f / 3.0f), 0.0, 0.0},
+    {(double)(1.0f / 15.0f), (double)(1.0f / 15.0f), 0.0},
+    {(double)(1.0f / 35.0f), (double)(1.0f / 35.0f), (double)(1.0f / 105.0f)}};
+
+// Coordinate prescale s applied to positions before the pair loop.
+//   FP32 Gaussian:  s = sqrt(log2(e)/2)/sigma, so K = exp2(-|s dx|^2)
+//   FP32 Laplacian: s = 1/sigma, so u = |s dx| = r/sigma
+//   FP64:           s = 1 (positions untouched, the kernel scales r itself)
+__host__ __device__ inline double coord_scale(int prec, int kernel, double sigma) {
+  if (prec != LDDMM_PREC_FP32C) return 1.0;
+  return kernel == LDDMM_KERNEL_GAUSS ? sqrt(0.5 * LOG2E) / sigma : 1.0 / sigma;
+}
+
+// ---------------------------------------------------------------------------
+// Pair evaluation: k0 = K, k1 = -2 sigma^2 K'(r^2), k2 = 4 sigma^4 K''(r^2).
+// ---------------------------------------------------------------------------
+template <typename R, int KT>
+struct KEval;
+
+template <>
+struct KEval<float, LDDMM_KERNEL_GAUSS> {
+  float dummy;
+  __device__ void init(int, double) {}
+  __device__ inline void operator()(float r2, float &k0, float &k1, float &k2) const {
+    float e = __builtin_amdgcn_exp2f(-r2);
+    k0 = e;
+    k1 = e;
+    k2 = e;
+  }
+};
+
+template <>
+struct KEval<float, LDDMM_KERNEL_LAPLACIAN> {
+  float p0, p1, p2, p3, p4, q0, q1, q2, q3, c0, c1, c2;
+  __device__ void init(int order, double) {
+    p0 = (float)LAP_P[order][0]; p1 = (float)LAP_P[order][1]; p2 = (float)LAP_P[order][2];
+    p3 = (float)LAP_P[order][3]; p4 = (float)LAP_P[order][4];
+    q0 = (float)LAP_Q[order][0]; q1 = (float)LAP_Q[order][1]; q2 = (float)LAP_Q[order][2];
+    q3 = (float)LAP_Q[order][3];
+    c0 = (float)LAP_R[order][0]; c1 = (float)LAP_R[order][1]; c2 = (float)LAP_R[order][2];
+  }
+  __device__ inline void operator()(float r2, float &k0, float &k1, float &k2) const {
+    float u = __builtin_amdgcn_sqrtf(r2);
+    float e = __builtin_amdgcn_exp2f(-u * (float)LOG2E);
+    k0 = fmaf(fmaf(fmaf(fmaf(p4, u, p3), u, p2), u, p1), u, p0) * e;
+    k1 = fmaf(fmaf(fmaf(q3, u, q2), u, q1), u, q0) * e;
+    k2 = fmaf(fmaf(c2, u, c1), u, c0) * e;
+  }
+};
+
+template <>
+struct KEval<double, LDDMM_KERNEL_GAUSS> {
+  double inv_s2;
+  __device__ void init(int, double sigma) { inv_s2 = 1.0 / (sigma * sigma); }
+  __device__ inline void operator()(double r2, double &k0, double &k1, double &k2) const {
+    double ut = r2 * inv_s2;
+    double e = exp(-0.5 * ut);
+    double k = ut < CUTOFF ? 1.0 : e;
+    k0 = k;
+    k1 = k;
+    k2 = k;
+  }
+};
+
+template <>
+struct KEval<double, LDDMM_KERNEL_LAPLACIAN> {
+  double inv_s, p0, p1, p2, p3, p4, q0, q1, q2, q3, c0, c1, c2;
+  __device__ void init(int order, double sigma) {
+    inv_s = 1.0 / sigma;
+    p0 = LAP_P[order][0]; p1 = LAP_P[order][1]; p2 = LAP_P[order][2];
+    p3 = LAP_P[order][3]; p4 = LAP_P[order][4];
+    q0 = LAP_Q[order][0]; q1 = LAP_Q[order][1]; q2 = LAP_Q[order][2]; q3 = LAP_Q[order][3];
+    c0 = LAP_R[order][0]; c1 = LAP_R[order][1]; c2 = LAP_R[order][2];
+  }
+  __device__ inline void operator()(double r2, double &k0, double &k1, double &k2) const {
+    double u = sqrt(r2) * inv_s;
+    double e = exp(-u);
+    bool cut = u < CUTOFF;
+    double pk = (((p4 * u + p3) * u + p2) * u + p1) * u + p0;
+    double qk = ((q3 * u + q2) * u + q1) * u + q0;
+    double rk = (c2 * u + c1) * u + c0;
+    k0 = cut ? 1.0 : pk * e;
+    k1 = cut ? q0 : qk * e;
+    k2 = cut ? c0 : rk * e;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Operand resolution per operation (uniform per workgroup).
+// ---------------------------------------------------------------------------
+struct SrcView {
+  const double *pos;
+  const double *w[4];
+  int64_t n;
+  double sign;
+};
+
+struct TgtView {
+  const double *pos;
+  const double *w[4];
+  const double *w0b;  // tw[0] += tcoef * w0b[k] on load (covector)
+  double tcoef;
+};
+
+__device__ inline const double *or_y(const double *p, const lddmm_seg_t *s) {
+  return p ? p : s->y;
+}
+
+__device__ inline SrcView resolve_src(int op, const lddmm_seg_t *s, int set) {
+  SrcView v;
+  v.w[0] = v.w[1] = v.w[2] = v.w[3] = nullptr;
+  v.sign = 1.0;
+  v.n = s->nx0;
+  switch (op) {
+    case LDDMM_OP_APPLY_K:
+      v.pos = or_y(s->x0, s);
+      v.w[0] = s->w0;
+      break;
+    case LDDMM_OP_DIFF_KT:
+      v.pos = or_y(s->x0, s);
+      for (int j = 0; j < 4; ++j) v.w[j] = s->sw[j];
+      break;
+    case LDDMM_OP_COVECTOR_STEP:  // a2 = [a, p]
+      v.pos = s->y;
+      v.w[0] = s->w0;
+      v.w[1] = s->w1;
+      break;
+    case LDDMM_OP_CURRENTS:  // set 0: (c1, cr1); set 1: (c2, -cr2)
+      if (set == 0) {
+        v.pos = s->y;
+        v.w[0] = s->w0;
+      } else {
+        v.pos = s->x1;
+        v.w[0] = s->w1;
+        v.n = s->nx1;
+        v.sign = -1.0;
+      }
+      break;
+    default:  // FLOW_STEP, KDOT, KNORM: targets = sources = y, weights w0
+      v.pos = s->y;
+      v.w[0] = s->w0;
+      break;
+  }
+  return v;
+}
+
+__device__ inline TgtView resolve_tgt(int op, const lddmm_seg_t *s) {
+  TgtView v;
+  v.pos = s->y;
+  v.w[0] = v.w[1] = v.w[2] = v.w[3] = nullptr;
+  v.w0b = nullptr;
+  v.tcoef = 0.0;
+  switch (op) {
+    case LDDMM_OP_DIFF_KT:
+      for (int j = 0; j < 4; ++j) v.w[j] = s->tw[j];
+      break;
+    case LDDMM_OP_COVECTOR_STEP:  // a1 = [p - 2 lambda a, a]
+      v.w[0] = s->w1;
+      v.w0b = s->w0;
+      v.tcoef = -2.0 * s->coef;
+      v.w[1] = s->w0;
+      break;
+    case LDDMM_OP_CURRENTS:  // a1 = [cr1]
+      v.w[0] = s->w0;
+      break;
+    default:
+      break;
+  }
+  return v;
+}
+
+// Row stride of the weight / output arrays of an operation.
+__host__ __device__ inline int weight_ld(int op, const lddmm_seg_t *s, int d) {
+  return (op == LDDMM_OP_APPLY_K || op == LDDMM_OP_DIFF_KT) ? s->m : d;
+}
+
+__device__ inline int find_segment(const lddmm_seg_t *__restrict__ segs, int nseg,
+                                   int64_t key, int which) {
+  // Largest i with start(i) <= key; which: 0 -> item0, 1 -> fin0.
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    int64_t st = which == 0 ? segs[mid].item0 : segs[mid].fin0;
+    if (st <= key) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Pair-sum kernel: K-weighted sums (KSUM) and/or K'-weighted derivative sums
+// with NA target/source weight pairs.  Writes W = KSUM*M + (NA>0)*D doubles per
+// row and chunk.
+// ---------------------------------------------------------------------------
+template <typename R, int KT, int D, int M, bool KSUM, int NA>
+__global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__ segs,
+                                                int nseg, int op, int prec,
+                                                double *__restrict__ part) {
+  constexpr int RPT = rows_per_thread<R>();
+  constexpr int NSW = (NA > (KSUM ? 1 : 0)) ? NA : (KSUM ? 1 : 0);
+  constexpr int W = (KSUM ? M : 0) + (NA > 0 ? D : 0);
+  constexpr bool KAHAN = std::is_same<R, float>::value;
+  constexpr int NKAH = KAHAN ? 1 : 0;
+
+  __shared__ R s_pos[D][TS];
+  __shared__ R s_w[NSW][M][TS];
+
+  const int tid = threadIdx.x;
+  const int64_t item = blockIdx.x;
+  const int si = find_segment(segs, nseg, item, 0);
+  const lddmm_seg_t *sg = segs + si;
+
+  const int64_t ny = sg->ny;
+  const int row_tiles = sg->row_tiles;
+  const int64_t it = item - sg->item0;
+  const int rt = (int)(it % row_tiles);
+  const int cg = (int)(it / row_tiles);
+  const int set = cg < sg->chunks0 ? 0 : 1;
+  const int ch = set ? cg - sg->chunks0 : cg;
+  const int ld = weight_ld(op, sg, D);
+
+  const SrcView src = resolve_src(op, sg, set);
+  const TgtView tgt = resolve_tgt(op, sg);
+  const double scale = coord_scale(prec, KT, sg->sigma);
+
+  KEval<R, KT> kev;
+  kev.init(sg->order, sg->sigma);
+
+  // Target rows held in registers.
+  R ty[RPT][D];
+  R tw[RPT][NA > 0 ? NA : 1][M];
+  int64_t rows[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = (int64_t)rt * (BLOCK * RPT) + r * BLOCK + tid;
+    rows[r] = k;
+    int64_t kk = k < ny ? k : ny - 1;
+#pragma unroll
+    for (int c = 0; c < D; ++c) ty[r][c] = (R)(tgt.pos[kk * D + c] * scale);
+    if constexpr (NA > 0) {
+#pragma unroll
+      for (int j = 0; j < NA; ++j) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+          double v = tgt.w[j][kk * ld + c];
+          if (j == 0 && tgt.w0b) v += tgt.tcoef * tgt.w0b[kk * ld + c];
+          tw[r][j][c] = (R)v;
+        }
+      }
+    }
+  }
+
+  // Accumulators: running sum (s), Kahan compensation (c).
+  R sK[RPT][KSUM ? M : 1], cK[RPT][KSUM ? M : 1];
+  R sD[RPT][D], cD[RPT][D];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+    for (int c = 0; c < (KSUM ? M : 1); ++c) { sK[r][c] = 0; cK[r][c] = 0; }
+#pragma unroll
+    for (int c = 0; c < D; ++c) { sD[r][c] = 0; cD[r][c] = 0; }
+  }
+
+  const int64_t l0 = (int64_t)ch * sg->chunk_len;
+  int64_t l1 = l0 + sg->chunk_len;
+  if (l1 > src.n) l1 = src.n;
+
+  for (int64_t base = l0; base < l1; base += TS) {
+    __syncthreads();
+    {
+      int64_t l = base + tid;
+      if (l < l1) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) s_pos[c][tid] = (R)(src.pos[l * D + c] * scale);
+#pragma unroll
+        for (int j = 0; j < NSW; ++j)
+#pragma unroll
+          for (int c = 0; c < M; ++c) s_w[j][c][tid] = (R)(src.sign * src.w[j][l * ld + c]);
+      } else {
+        // Padding sources carry zero weight: their terms are exact zeros.
+#pragma unroll
+        for (int c = 0; c < D; ++c) s_pos[c][tid] = (R)0;
+#pragma unroll
+        for (int j = 0; j < NSW; ++j)
+#pragma unroll
+          for (int c = 0; c < M; ++c) s_w[j][c][tid] = (R)0;
+      }
+    }
+    __syncthreads();
+
+    for (int j0 = 0; j0 < TS; j0 += KF) {
+      R tK[RPT][KSUM ? M : 1], tD[RPT][D];
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int c = 0; c < (KSUM ? M : 1); ++c) tK[r][c] = 0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) tD[r][c] = 0;
+      }
+#pragma unroll 8
+      for (int j = j0; j < j0 + KF; ++j) {
+        R sx[D], sw[NSW][M];
+#pragma unroll
+        for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j];
+#pragma unroll
+        for (int q = 0; q < NSW; ++q)
+#pragma unroll
+          for (int c = 0; c < M; ++c) sw[q][c] = s_w[q][c][j];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          R dx[D];
+          R r2 = 0;
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            dx[c] = ty[r][c] - sx[c];
+            r2 = dx[c] * dx[c] + r2;
+          }
+          R k0, k1, k2;
+          kev(r2, k0, k1, k2);
+          if constexpr (KSUM) {
+#pragma unroll
+            for (int c = 0; c < M; ++c) tK[r][c] = k0 * sw[0][c] + tK[r][c];
+          }
+          if constexpr (NA > 0) {
+            R wsum = 0;
+#pragma unroll
+            for (int q = 0; q < NA; ++q)
+#pragma unroll
+              for (int c = 0; c < M; ++c) wsum = tw[r][q][c] * sw[q][c] + wsum;
+            R f = k1 * wsum;
+#pragma unroll
+            for (int c = 0; c < D; ++c) tD[r][c] = f * dx[c] + tD[r][c];
+          }
+        }
+      }
+      // Fold the run into the compensated row sums.
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        if constexpr (KSUM) {
+#pragma unroll
+          for (int c = 0; c < M; ++c) {
+            if constexpr (KAHAN) {
+              R y = tK[r][c] - cK[r][c];
+              R t = sK[r][c] + y;
+              cK[r][c] = (t - sK[r][c]) - y;
+              sK[r][c] = t;
+            } else {
+              sK[r][c] += tK[r][c];
+            }
+          }
+        }
+        if constexpr (NA > 0) {
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            if constexpr (KAHAN) {
+              R y = tD[r][c] - cD[r][c];
+              R t = sD[r][c] + y;
+              cD[r][c] = (t - sD[r][c]) - y;
+              sD[r][c] = t;
+            } else {
+              sD[r][c] += tD[r][c];
+            }
+          }
+        }
+      }
+    }
+  }
+  (void)NKAH;
+
+  // Chunk partials, FP64.
+  double *pp = part + sg->part0 + (int64_t)cg * ny * W;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = rows[r];
+    if (k >= ny) continue;
+    double *o = pp + k * W;
+    int q = 0;
+    if constexpr (KSUM) {
+#pragma unroll
+      for (int c = 0; c < M; ++c) o[q++] = (double)sK[r][c] - (double)cK[r][c];
+    }
+    if constexpr (NA > 0) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) o[q++] = (double)sD[r][c] - (double)cD[r][c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// applyDDiffK11and12 pair kernel: targets = sources = y; a1 = w0, a2 = w1, p = w2.
+//   acc += <a1_k, a2_l> (k2 g <dx', dp> dx' - k1 dp),  out = acc / sigma^2.
+// ---------------------------------------------------------------------------
+template <typename R, int KT, int D>
+__global__ __launch_bounds__(BLOCK) void k_pairs_ddk(const lddmm_seg_t *__restrict__ segs,
+                                                    int nseg, int prec,
+                                                    double *__restrict__ part) {
+  constexpr int RPT = rows_per_thread<R>();
+  constexpr bool KAHAN = std::is_same<R, float>::value;
+  __shared__ R s_pos[D][TS];
+  __shared__ R s_a[D][TS];
+  __shared__ R s_p[D][TS];
+
+  const int tid = threadIdx.x;
+  const int64_t item = blockIdx.x;
+  const int si = find_segment(segs, nseg, item, 0);
+  const lddmm_seg_t *sg = segs + si;
+  const int64_t ny = sg->ny;
+  const int row_tiles = sg->row_tiles;
+  const int64_t it = item - sg->item0;
+  const int rt = (int)(it % row_tiles);
+  const int cg = (int)(it / row_tiles);
+  const double scale = coord_scale(prec, KT, sg->sigma);
+  // g = 1/(s^2 sigma^2) with s the effective coordinate scale (1 in FP64).
+  const R g = (R)(prec == LDDMM_PREC_FP32C ? 1.0 / (scale * scale * sg->sigma * sg->sigma)
+                                           : 1.0 / (sg->sigma * sg->sigma));
+  KEval<R, KT> kev;
+  kev.init(sg->order, sg->sigma);
+
+  R ty[RPT][D], ta[RPT][D], tp[RPT][D];
+  int64_t rows[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = (int64_t)rt * (BLOCK * RPT) + r * BLOCK + tid;
+    rows[r] = k;
+    int64_t kk = k < ny ? k : ny - 1;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      ty[r][c] = (R)(sg->y[kk * D + c] * scale);
+      ta[r][c] = (R)sg->w0[kk * D + c];
+      tp[r][c] = (R)sg->w2[kk * D + c];
+    }
+  }
+  R sD[RPT][D], cD[RPT][D];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int c = 0; c < D; ++c) { sD[r][c] = 0; cD[r][c] = 0; }
+
+  const int64_t l0 = (int64_t)cg * sg->chunk_len;
+  int64_t l1 = l0 + sg->chunk_len;
+  if (l1 > sg->ny) l1 = sg->ny;
+  for (int64_t base = l0; base < l1; base += TS) {
+    __syncthreads();
+    {
+      int64_t l = base + tid;
+      bool ok = l < l1;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        s_pos[c][tid] = ok ? (R)(sg->y[l * D + c] * scale) : (R)0;
+        s_a[c][tid] = ok ? (R)sg->w1[l * D + c] : (R)0;
+        s_p[c][tid] = ok ? (R)sg->w2[l * D + c] : (R)0;
+      }
+    }
+    __syncthreads();
+    for (int j0 = 0; j0 < TS; j0 += KF) {
+      R tD[RPT][D];
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) tD[r][c] = 0;
+#pragma unroll 4
+      for (int j = j0; j < j0 + KF; ++j) {
+        R sx[D], sa[D], sp[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) { sx[c] = s_pos[c][j]; sa[c] = s_a[c][j]; sp[c] = s_p[c][j]; }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          R dx[D], dp[D];
+          R r2 = 0, xp = 0, aa = 0;
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            dx[c] = ty[r][c] - sx[c];
+            dp[c] = tp[r][c] - sp[c];
+            r2 = dx[c] * dx[c] + r2;
+            xp = dx[c] * dp[c] + xp;
+            aa = ta[r][c] * sa[c] + aa;
+          }
+          R k0, k1, k2;
+          kev(r2, k0, k1, k2);
+          R f2 = aa * k2 * g * xp;
+          R f1 = aa * k1;
+#pragma unroll
+          for (int c = 0; c < D; ++c) tD[r][c] = f2 * dx[c] - f1 * dp[c] + tD[r][c];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          if constexpr (KAHAN) {
+            R y = tD[r][c] - cD[r][c];
+            R t = sD[r][c] + y;
+            cD[r][c] = (t - sD[r][c]) - y;
+            sD[r][c] = t;
+          } else {
+            sD[r][c] += tD[r][c];
+          }
+        }
+    }
+  }
+  double *pp = part + sg->part0 + (int64_t)cg * ny * D;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = rows[r];
+    if (k >= ny) continue;
+#pragma unroll
+    for (int c = 0; c < D; ++c) pp[k * D + c] = (double)sD[r][c] - (double)cD[r][c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Finalize: combine chunk partials in chunk order and apply the epilogue.
+// ---------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double *sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = FIN_BLOCK / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+template <int OP, int D, int M, bool KSUM, int NA>
+__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__restrict__ segs,
+                                                       int nseg, int prec, int kernel,
+                                                       const double *__restrict__ part,
+                                                       double *__restrict__ red) {
+  constexpr int W = (KSUM ? M : 0) + (NA > 0 ? D : 0);
+  __shared__ double sh[FIN_BLOCK];
+  const int tid = threadIdx.x;
+  const int64_t fb = blockIdx.x;
+  const int si = find_segment(segs, nseg, fb, 1);
+  const lddmm_seg_t *sg = segs + si;
+  const int b = (int)(fb - sg->fin0);
+  const int64_t ny = sg->ny;
+  const int64_t k = (int64_t)b * FIN_BLOCK + tid;
+  const bool valid = k < ny;
+  const int64_t kk = valid ? k : 0;
+  const int nch = sg->chunks0 + sg->chunks1;
+  const double *pp = part + sg->part0;
+
+  double accK0[KSUM ? M : 1], accK1[KSUM ? M : 1], accD[D];
+#pragma unroll
+  for (int c = 0; c < (KSUM ? M : 1); ++c) { accK0[c] = 0; accK1[c] = 0; }
+#pragma unroll
+  for (int c = 0; c < D; ++c) accD[c] = 0;
+  if (valid) {
+    for (int cgi = 0; cgi < nch; ++cgi) {
+      const double *o = pp + ((int64_t)cgi * ny + kk) * W;
+      int q = 0;
+      if constexpr (KSUM) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+          if (cgi < sg->chunks0) accK0[c] += o[q + c]; else accK1[c] += o[q + c];
+        }
+        q += M;
+      }
+      if constexpr (NA > 0) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) accD[c] += o[q + c];
+      }
+    }
+  }
+  const double sigma = sg->sigma;
+  const double s = coord_scale(prec, kernel, sigma);
+  // sum_l 2 K'(r^2) (y_k - x_l) w  =  -acc / (sigma^2 s)
+  const double dscale = -1.0 / (sigma * sigma * s);
+  double rd0 = 0.0, rd1 = 0.0;
+
+  if constexpr (OP == LDDMM_OP_APPLY_K) {
+    if (valid) {
+      const int ld = sg->m;
+#pragma unroll
+      for (int c = 0; c < M; ++c) sg->out0[k * ld + c] = accK0[c];
+    }
+  } else if constexpr (OP == LDDMM_OP_DIFF_KT) {
+    if (valid) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) sg->out0[k * D + c] = dscale * accD[c];
+    }
+  } else if constexpr (OP == LDDMM_OP_FLOW_STEP) {
+    if (valid) {
+      const double dt = sg->dt;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double v = accK0[c];
+        sg->out0[k * D + c] = sg->y[k * D + c] + dt * v;
+        if (sg->out1) sg->out1[k * D + c] = v;
+        rd0 += sg->w0[k * D + c] * v;
+      }
+      if (sg->rowdot) sg->rowdot[k] = rd0;
+    }
+  } else if constexpr (OP == LDDMM_OP_COVECTOR_STEP) {
+    if (valid) {
+      const double dt = sg->dt;
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        sg->out0[k * D + c] = sg->w1[k * D + c] + dt * (dscale * accD[c]);
+    }
+  } else if constexpr (OP == LDDMM_OP_KDOT) {
+    if (valid) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        rd0 += sg->g[0][k * D + c] * accK0[c];
+        if (sg->na > 1) rd1 += sg->g[1][k * D + c] * accK0[c];
+      }
+    }
+  } else if constexpr (OP == LDDMM_OP_KNORM) {
+    if (valid) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) rd0 += sg->w0[k * D + c] * accK0[c];
+      if (sg->rowdot) sg->rowdot[k] = rd0;
+    }
+  } else if constexpr (OP == LDDMM_OP_CURRENTS) {
+    if (valid) {
+      // accK1 already carries the sign of set 1: accK1 = -K(c1,c2) cr2.
+      double e0 = 0.0, e1 = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double w = sg->w0[k * D + c];
+        e0 += w * accK0[c];
+        e1 += w * accK1[c];
+      }
+      rd0 = e0 + 2.0 * e1;
+      if (sg->rowdot) sg->rowdot[k] = rd0;
+      if (sg->out0) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) sg->out0[k * D + c] = accK0[c] + accK1[c];
+      }
+      if constexpr (NA > 0) {
+        if (sg->out1) {
+#pragma unroll
+          for (int c = 0; c < D; ++c) sg->out1[k * D + c] = sg->coef * (dscale * accD[c]);
+        }
+      }
+    }
+  } else if constexpr (OP == LDDMM_OP_DDIFF_K11_12) {
+    if (valid) {
+      const double inv = 1.0 / (sigma * sigma);
+#pragma unroll
+      for (int c = 0; c < D; ++c) sg->out0[k * D + c] = inv * accD[c];
+    }
+  }
+
+  constexpr int NRED = (OP == LDDMM_OP_FLOW_STEP || OP == LDDMM_OP_KNORM ||
+                        OP == LDDMM_OP_CURRENTS) ? 1 : (OP == LDDMM_OP_KDOT ? 2 : 0);
+  if constexpr (NRED > 0) {
+    double t0 = block_sum(rd0, sh);
+    double t1 = 0.0;
+    if constexpr (NRED > 1) t1 = block_sum(rd1, sh);
+    if (tid == 0) {
+      red[sg->red0 + b] = t0;
+      if constexpr (NRED > 1) red[sg->red0 + sg->fin_blocks + b] = t1;
+    }
+  }
+}
+
+// Per-segment FP64 totals from the finalize blocks' partials (fixed tree).
+__global__ __launch_bounds__(TOT_BLOCK) void k_totals(const lddmm_seg_t *__restrict__ segs,
+                                                     int nred, const double *__restrict__ red) {
+  __shared__ double sh[TOT_BLOCK];
+  const lddmm_seg_t *sg = segs + blockIdx.x;
+  const int tid = threadIdx.x;
+  for (int j = 0; j < nred; ++j) {
+    double v = 0.0;
+    for (int b = tid; b < sg->fin_blocks; b += TOT_BLOCK) v += red[sg->red0 + (int64_t)j * sg->fin_blocks + b];
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = TOT_BLOCK / 2; s > 0; s >>= 1) {
+      if (tid < s) sh[tid] += sh[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0 && sg->total) sg->total[j] = sh[0];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Geometry and vertex gathers.
+// ---------------------------------------------------------------------------
+// Geometry and gathers evaluate each expression in the reference's NumPy order without
+// FMA contraction, so their FP64 results are the reference's bit for bit.
+__global__ void k_surface_geometry(const double *__restrict__ v, const int32_t *__restrict__ f,
+                                   int64_t nf, double *__restrict__ cen, double *__restrict__ nrm) {
+#pragma clang fp contract(off)
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const double *a = v + 3 * (int64_t)f[3 * i + 0];
+  const double *b = v + 3 * (int64_t)f[3 * i + 1];
+  const double *c = v + 3 * (int64_t)f[3 * i + 2];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) cen[3 * i + q] = (a[q] + b[q] + c[q]) / 3.0;
+  double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  nrm[3 * i + 0] = e1[1] * e2[2] - e1[2] * e2[1];
+  nrm[3 * i + 1] = e1[2] * e2[0] - e1[0] * e2[2];
+  nrm[3 * i + 2] = e1[0] * e2[1] - e1[1] * e2[0];
+}
+
+__global__ void k_curve_geometry(const double *__restrict__ v, const int32_t *__restrict__ e,
+                                 int64_t ne, int d, double *__restrict__ cen,
+                                 double *__restrict__ len) {
+#pragma clang fp contract(off)
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const double *a = v + (int64_t)d * e[2 * i + 0];
+  const double *b = v + (int64_t)d * e[2 * i + 1];
+  for (int q = 0; q < d; ++q) {
+    cen[d * i + q] = (a[q] + b[q]) / 2.0;
+    len[d * i + q] = b[q] - a[q];
+  }
+}
+
+__global__ void k_surface_vertex_gradient(const double *__restrict__ v, const int32_t *__restrict__ f,
+                                          const double *__restrict__ z1, const double *__restrict__ dz,
+                                          const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
+                                          int64_t nv, double scale, double *__restrict__ px) {
+#pragma clang fp contract(off)
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) {
+    int32_t code = inc[q];
+    int32_t fi = code / 3, j = code - 3 * fi;
+    const double *xa = v + 3 * (int64_t)f[3 * fi + (j + 2) % 3];
+    const double *xb = v + 3 * (int64_t)f[3 * fi + (j + 1) % 3];
+    double e[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+    const double *z = z1 + 3 * (int64_t)fi;
+    double cr[3] = {e[1] * z[2] - e[2] * z[1], e[2] * z[0] - e[0] * z[2], e[0] * z[1] - e[1] * z[0]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] = (acc[c] + dz[3 * (int64_t)fi + c]) - cr[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) px[3 * i + c] = scale * acc[c];
+}
+
+__global__ void k_curve_vertex_gradient(const double *__restrict__ z1, const double *__restrict__ dz,
+                                        const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
+                                        int64_t nv, int d, double scale, double *__restrict__ px) {
+#pragma clang fp contract(off)
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  double acc[8];
+  for (int c = 0; c < d; ++c) acc[c] = 0.0;
+  for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) {
+    int32_t code = inc[q];
+    int32_t ei = code / 2, j = code - 2 * ei;
+    for (int c = 0; c < d; ++c) {
+      double z = z1[(int64_t)d * ei + c];
+      double t = acc[c] + dz[(int64_t)d * ei + c];
+      acc[c] = j ? t + z : t - z;
+    }
+  }
+  for (int c = 0; c < d; ++c) px[(int64_t)d * i + c] = scale * acc[c];
+}
+
+__global__ __launch_bounds__(1024) void k_sum_f64(const double *__restrict__ x, int64_t n,
+                                                 double *__restrict__ out) {
+  __shared__ double sh[1024];
+  const int tid = threadIdx.x;
+  double v = 0.0;
+  for (int64_t i = tid; i < n; i += 1024) v += x[i];
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) *out = sh[0];
+}
+
+// Descriptor upload through kernel arguments: a by-value copy of up to STAMP
+// segments is written to device memory.  Fully stream-ordered; no host buffer
+// has to outlive the call.
+constexpr int STAMP = 4;
+struct SegPack {
+  lddmm_seg_t s[STAMP];
+};
+__global__ void k_stamp(SegPack pack, int n, lddmm_seg_t *__restrict__ dst) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = pack.s[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host side: planning, dispatch.
+// ---------------------------------------------------------------------------
+int hip_check(hipError_t e) {
+  if (e != hipSuccess) {
+    g_last_hip_error = (int)e;
+    return LDDMM_EHIP;
+  }
+  return LDDMM_OK;
+}
+
+int op_nred(int op) {
+  switch (op) {
+    case LDDMM_OP_FLOW_STEP:
+    case LDDMM_OP_KNORM:
+    case LDDMM_OP_CURRENTS:
+      return 1;
+    case LDDMM_OP_KDOT:
+      return 2;
+    default:
+      return 0;
+  }
+}
+
+bool op_valid(int op) { return op >= LDDMM_OP_APPLY_K && op <= LDDMM_OP_KNORM; }
+
+// Launch-uniform shape of a planned operation.
+struct Shape {
+  int M, NA;
+  bool KSUM;
+  int W;
+};
+
+int op_shape(int op, int d, const lddmm_seg_t &s0, Shape *sh) {
+  switch (op) {
+    case LDDMM_OP_APPLY_K:
+      if (s0.m < 1 || s0.m > 3) return LDDMM_EINVAL;
+      *sh = {s0.m, 0, true, s0.m};
+      return LDDMM_OK;
+    case LDDMM_OP_DIFF_KT:
+      if (!(s0.m == 1 || s0.m == d) || s0.na < 1 || s0.na > 3) return LDDMM_EINVAL;
+      *sh = {s0.m, s0.na, false, d};
+      return LDDMM_OK;
+    case LDDMM_OP_FLOW_STEP:
+    case LDDMM_OP_KDOT:
+    case LDDMM_OP_KNORM:
+      *sh = {d, 0, true, d};
+      return LDDMM_OK;
+    case LDDMM_OP_COVECTOR_STEP:
+      *sh = {d, 2, false, d};
+      return LDDMM_OK;
+    case LDDMM_OP_CURRENTS: {
+      bool grad = s0.out1 != nullptr;
+      *sh = {d, grad ? 1 : 0, true, grad ? 2 * d : d};
+      return LDDMM_OK;
+    }
+    case LDDMM_OP_DDIFF_K11_12:
+      *sh = {d, 0, false, d};
+      return LDDMM_OK;
+  }
+  return LDDMM_EINVAL;
+}
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Descriptor head of the workspace (a device copy of the segment array).
+size_t desc_bytes(int nseg) {
+  size_t b = (size_t)nseg * sizeof(lddmm_seg_t);
+  return (b + 255) & ~(size_t)255;
+}
+
+}  // namespace
+
+extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, int nseg,
+                                   int64_t *n_items, int64_t *n_fin_blocks,
+                                   size_t *work_bytes) {
+  if (!op_valid(op) || nseg < 1 || !segs || (d != 2 && d != 3)) return LDDMM_EINVAL;
+  if (prec != LDDMM_PREC_FP64 && prec != LDDMM_PREC_FP32C) return LDDMM_EINVAL;
+  Shape shp;
+  int rc = op_shape(op, d, segs[0], &shp);
+  if (rc) return rc;
+  const int rows = rows_per_item(prec);
+  const int nred = op_nred(op);
+  const int kern = segs[0].kernel;
+  int64_t item = 0, part = 0, fin = 0, redo = 0;
+  for (int i = 0; i < nseg; ++i) {
+    lddmm_seg_t &s = segs[i];
+    if (s.kernel != kern) return LDDMM_EINVAL;  // one kernel family per launch
+    if (s.kernel != LDDMM_KERNEL_GAUSS && s.kernel != LDDMM_KERNEL_LAPLACIAN) return LDDMM_EINVAL;
+    if (s.kernel == LDDMM_KERNEL_LAPLACIAN) {
+      if (s.order < 0 || s.order > 4) return LDDMM_EINVAL;
+      bool deriv = op == LDDMM_OP_DIFF_KT || op == LDDMM_OP_COVECTOR_STEP ||
+                   (op == LDDMM_OP_CURRENTS && s.out1);
+      if (deriv && s.order < 1) return LDDMM_EINVAL;
+      if (op == LDDMM_OP_DDIFF_K11_12 && s.order < 2) return LDDMM_EINVAL;
+    }
+    if (!(s.sigma > 0.0) || !isfinite(s.sigma)) return LDDMM_EINVAL;
+    if (s.ny < 1 || !s.y) return LDDMM_EINVAL;
+    Shape si;
+    if (op_shape(op, d, s, &si) || si.M != shp.M || si.NA != shp.NA || si.W != shp.W)
+      return LDDMM_EINVAL;
+    int64_t nx0 = s.nx0, nx1 = 0;
+    switch (op) {
+      case LDDMM_OP_APPLY_K:
+        if (!s.w0 || !s.out0) return LDDMM_EINVAL;
+        if (!s.x0) nx0 = s.ny;
+        break;
+      case LDDMM_OP_DIFF_KT:
+        if (!s.out0) return LDDMM_EINVAL;
+        for (int j = 0; j < s.na; ++j)
+          if (!s.tw[j] || !s.sw[j]) return LDDMM_EINVAL;
+        if (!s.x0) nx0 = s.ny;
+        break;
+      case LDDMM_OP_FLOW_STEP:
+        if (!s.w0 || !s.out0) return LDDMM_EINVAL;
+        nx0 = s.ny;
+        break;
+      case LDDMM_OP_COVECTOR_STEP:
+        if (!s.w0 || !s.w1 || !s.out0) return LDDMM_EINVAL;
+        nx0 = s.ny;
+        break;
+      case LDDMM_OP_KDOT:
+        if (!s.w0 || s.na < 1 || s.na > 2 || !s.g[0] || (s.na > 1 && !s.g[1])) return LDDMM_EINVAL;
+        nx0 = s.ny;
+        break;
+      case LDDMM_OP_KNORM:
+        if (!s.w0) return LDDMM_EINVAL;
+        nx0 = s.ny;
+        break;
+      case LDDMM_OP_CURRENTS:
+        if (!s.w0 || !s.x1 || !s.w1 || s.nx1 < 1) return LDDMM_EINVAL;
+        nx0 = s.ny;
+        nx1 = s.nx1;
+        break;
+      case LDDMM_OP_DDIFF_K11_12:
+        if (!s.w0 || !s.w1 || !s.w2 || !s.out0) return LDDMM_EINVAL;
+        nx0 = s.ny;
+        break;
+    }
+    if (nx0 < 1) return LDDMM_EINVAL;
+    s.nx0 = nx0;
+    if (op != LDDMM_OP_CURRENTS) s.nx1 = 0;
+    const int64_t row_tiles = ceil_div(s.ny, rows);
+    const int64_t src_tiles = ceil_div(nx0, TS) + (nx1 ? ceil_div(nx1, TS) : 0);
+    int64_t chunk_tiles = ceil_div(src_tiles * row_tiles, ITEMS_PER_SEGMENT);
+    if (chunk_tiles < 1) chunk_tiles = 1;
+    const int64_t chunk_len = chunk_tiles * TS;
+    const int64_t c0 = ceil_div(nx0, chunk_len), c1 = nx1 ? ceil_div(nx1, chunk_len) : 0;
+    if (row_tiles > INT32_MAX || chunk_len > INT32_MAX) return LDDMM_EINVAL;
+    s.row_tiles = (int32_t)row_tiles;
+    s.chunk_len = (int32_t)chunk_len;
+    s.chunks0 = (int32_t)c0;
+    s.chunks1 = (int32_t)c1;
+    s.item0 = item;
+    item += row_tiles * (c0 + c1);
+    s.part0 = part;
+    part += s.ny * (c0 + c1) * shp.W;
+    s.fin_blocks = (int32_t)ceil_div(s.ny, FIN_BLOCK);
+    s.fin0 = fin;
+    fin += s.fin_blocks;
+    s.red0 = redo;
+    redo += (int64_t)s.fin_blocks * (nred > 0 ? nred : 0);
+    s.pad_ = 0;
+  }
+  if (n_items) *n_items = item;
+  if (n_fin_blocks) *n_fin_blocks = fin;
+  if (work_bytes) *work_bytes = desc_bytes(nseg) + (size_t)(part + redo) * sizeof(double);
+  return LDDMM_OK;
+}
+
+namespace {
+
+template <typename R, int KT, int D, int M, bool KSUM, int NA>
+void launch_pairs(const lddmm_seg_t *segs, int nseg, int op, int prec, int64_t n_items,
+                  double *part, hipStream_t st) {
+  hipLaunchKernelGGL((k_pairs<R, KT, D, M, KSUM, NA>), dim3((unsigned)n_items), dim3(BLOCK), 0,
+                     st, segs, nseg, op, prec, part);
+}
+
+template <int D, int M, bool KSUM, int NA>
+int dispatch_pairs(int prec, int kern, const lddmm_seg_t *segs, int nseg, int op,
+                   int64_t n_items, double *part, hipStream_t st) {
+  if (prec == LDDMM_PREC_FP32C) {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      launch_pairs<float, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st);
+    else
+      launch_pairs<float, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st);
+  } else {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      launch_pairs<double, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st);
+    else
+      launch_pairs<double, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st);
+  }
+  return LDDMM_OK;
+}
+
+template <int D>
+int dispatch_pairs_ddk(int prec, int kern, const lddmm_seg_t *segs, int nseg, int64_t n_items,
+                       double *part, hipStream_t st) {
+  dim3 g((unsigned)n_items), b(BLOCK);
+  if (prec == LDDMM_PREC_FP32C) {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      hipLaunchKernelGGL((k_pairs_ddk<float, LDDMM_KERNEL_GAUSS, D>), g, b, 0, st, segs, nseg, prec, part);
+    else
+      hipLaunchKernelGGL((k_pairs_ddk<float, LDDMM_KERNEL_LAPLACIAN, D>), g, b, 0, st, segs, nseg, prec, part);
+  } else {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      hipLaunchKernelGGL((k_pairs_ddk<double, LDDMM_KERNEL_GAUSS, D>), g, b, 0, st, segs, nseg, prec, part);
+    else
+      hipLaunchKernelGGL((k_pairs_ddk<double, LDDMM_KERNEL_LAPLACIAN, D>), g, b, 0, st, segs, nseg, prec, part);
+  }
+  return LDDMM_OK;
+}
+
+template <int OP, int D, int M, bool KSUM, int NA>
+void launch_fin(const lddmm_seg_t *segs, int nseg, int prec, int kern, int64_t n_fin,
+                const double *part, double *red, hipStream_t st) {
+  hipLaunchKernelGGL((k_finalize<OP, D, M, KSUM, NA>), dim3((unsigned)n_fin), dim3(FIN_BLOCK), 0, st,
+                     segs, nseg, prec, kern, part, red);
+}
+
+template <int D>
+int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, int nseg,
+          int64_t n_items, int64_t n_fin, double *part, double *red, hipStream_t st) {
+  switch (op) {
+    case LDDMM_OP_APPLY_K:
+      switch (sh.M) {
+        case 1:
+          dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+          launch_fin<LDDMM_OP_APPLY_K, D, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+          break;
+        case 2:
+          dispatch_pairs<D, 2, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+          launch_fin<LDDMM_OP_APPLY_K, D, 2, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+          break;
+        case 3:
+          dispatch_pairs<D, 3, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+          launch_fin<LDDMM_OP_APPLY_K, D, 3, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+          break;
+        default:
+          return LDDMM_EINVAL;
+      }
+      return LDDMM_OK;
+    case LDDMM_OP_DIFF_KT: {
+#define LDDMM_DKT(MM, NN)                                                                  \
+  if (sh.M == MM && sh.NA == NN) {                                                         \
+    dispatch_pairs<D, MM, false, NN>(prec, kern, segs, nseg, op, n_items, part, st);       \
+    launch_fin<LDDMM_OP_DIFF_KT, D, MM, false, NN>(segs, nseg, prec, kern, n_fin, part, red, st); \
+    return LDDMM_OK;                                                                       \
+  }
+      LDDMM_DKT(1, 1) LDDMM_DKT(1, 2) LDDMM_DKT(1, 3)
+      LDDMM_DKT(D, 1) LDDMM_DKT(D, 2) LDDMM_DKT(D, 3)
+#undef LDDMM_DKT
+      return LDDMM_EINVAL;
+    }
+    case LDDMM_OP_FLOW_STEP:
+      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+      launch_fin<LDDMM_OP_FLOW_STEP, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+      return LDDMM_OK;
+    case LDDMM_OP_KDOT:
+      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+      launch_fin<LDDMM_OP_KDOT, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+      return LDDMM_OK;
+    case LDDMM_OP_KNORM:
+      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+      launch_fin<LDDMM_OP_KNORM, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+      return LDDMM_OK;
+    case LDDMM_OP_COVECTOR_STEP:
+      dispatch_pairs<D, D, false, 2>(prec, kern, segs, nseg, op, n_items, part, st);
+      launch_fin<LDDMM_OP_COVECTOR_STEP, D, D, false, 2>(segs, nseg, prec, kern, n_fin, part, red, st);
+      return LDDMM_OK;
+    case LDDMM_OP_CURRENTS:
+      if (sh.NA > 0) {
+        dispatch_pairs<D, D, true, 1>(prec, kern, segs, nseg, op, n_items, part, st);
+        launch_fin<LDDMM_OP_CURRENTS, D, D, true, 1>(segs, nseg, prec, kern, n_fin, part, red, st);
+      } else {
+        dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+        launch_fin<LDDMM_OP_CURRENTS, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+      }
+      return LDDMM_OK;
+    case LDDMM_OP_DDIFF_K11_12:
+      dispatch_pairs_ddk<D>(prec, kern, segs, nseg, n_items, part, st);
+      launch_fin<LDDMM_OP_DDIFF_K11_12, D, D, false, 1>(segs, nseg, prec, kern, n_fin, part, red, st);
+      return LDDMM_OK;
+  }
+  return LDDMM_EINVAL;
+}
+
+}  // namespace
+
+extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *segs_host,
+                                  const lddmm_seg_t *segs_dev, int nseg, int64_t n_items,
+                                  int64_t n_fin_blocks, void *work, size_t work_bytes,
+                                  void *stream) {
+  if (!op_valid(op) || nseg < 1 || !segs_host || (d != 2 && d != 3) || !work) return LDDMM_EINVAL;
+  if (prec != LDDMM_PREC_FP64 && prec != LDDMM_PREC_FP32C) return LDDMM_EINVAL;
+  if (n_items < 1 || n_items > INT32_MAX || n_fin_blocks < 1 || n_fin_blocks > INT32_MAX)
+    return LDDMM_EINVAL;
+  const lddmm_seg_t &last = segs_host[nseg - 1];
+  Shape sh;
+  int rc = op_shape(op, d, segs_host[0], &sh);
+  if (rc) return rc;
+  const int64_t parts = last.part0 + last.ny * (int64_t)(last.chunks0 + last.chunks1) * sh.W;
+  const int nred = op_nred(op);
+  const int64_t reds = last.red0 + (int64_t)last.fin_blocks * nred;
+  const size_t need = desc_bytes(nseg) + (size_t)(parts + reds) * sizeof(double);
+  if (work_bytes < need) return LDDMM_EWORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  char *wb = (char *)work;
+  const lddmm_seg_t *segs = segs_dev;
+  if (!segs) {
+    lddmm_seg_t *dst = (lddmm_seg_t *)wb;
+    for (int i0 = 0; i0 < nseg; i0 += STAMP) {
+      SegPack pack;
+      memset(&pack, 0, sizeof(pack));
+      int n = nseg - i0 < STAMP ? nseg - i0 : STAMP;
+      for (int i = 0; i < n; ++i) pack.s[i] = segs_host[i0 + i];
+      hipLaunchKernelGGL(k_stamp, dim3(1), dim3(STAMP), 0, st, pack, n, dst + i0);
+    }
+    segs = dst;
+  }
+  double *part = (double *)(wb + desc_bytes(nseg));
+  double *red = part + parts;
+  const int kern = segs_host[0].kernel;
+  rc = d == 2 ? run_d<2>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st)
+              : run_d<3>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st);
+  if (rc) return rc;
+  if (nred > 0)
+    hipLaunchKernelGGL(k_totals, dim3((unsigned)nseg), dim3(TOT_BLOCK), 0, st, segs, nred, red);
+  return hip_check(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// One-call forms of the reference's kernel sums.
+// ---------------------------------------------------------------------------
+namespace {
+
+lddmm_seg_t blank_seg() {
+  lddmm_seg_t s;
+  memset(&s, 0, sizeof(s));
+  return s;
+}
+
+int plan_one(int op, int prec, int d, lddmm_seg_t *s, int64_t *items, int64_t *fins,
+             size_t *bytes) {
+  return lddmm_segments_plan(op, prec, d, s, 1, items, fins, bytes);
+}
+
+// Workspace of a one-call form, whatever the precision: plan both and take the max.
+size_t ws_query(int op, int d, lddmm_seg_t s) {
+  size_t best = 0;
+  for (int prec = 0; prec < 2; ++prec) {
+    lddmm_seg_t t = s;
+    int64_t a, b;
+    size_t wb = 0;
+    if (plan_one(op, prec, d, &t, &a, &b, &wb) == LDDMM_OK && wb > best) best = wb;
+  }
+  return best;
+}
+
+// Dummy non-null device-side pointers for workspace queries (never dereferenced).
+const double *const QP = (const double *)(uintptr_t)256;
+
+}  // namespace
+
+extern "C" size_t lddmm_apply_k_workspace_bytes(int64_t ny, int64_t nx, int d, int m) {
+  if (ny < 1 || nx < 1 || m < 1) return 0;
+  size_t best = 0;
+  for (int c0 = 0; c0 < m; c0 += 3) {
+    lddmm_seg_t s = blank_seg();
+    s.y = QP; s.x0 = QP; s.w0 = QP; s.out0 = (double *)QP;
+    s.ny = ny; s.nx0 = nx; s.sigma = 1.0; s.kernel = LDDMM_KERNEL_GAUSS;
+    s.m = (m - c0) < 3 ? (m - c0) : 3;
+    size_t b = ws_query(LDDMM_OP_APPLY_K, d < 2 ? 2 : d, s);
+    if (b > best) best = b;
+  }
+  return best;
+}
+
+extern "C" int lddmm_apply_k(const double *y, const double *x, const double *w, int64_t ny,
+                             int64_t nx, int d, int m, int kernel, int order, double sigma,
+                             int prec, double *out, void *work, size_t work_bytes,
+                             void *stream) {
+  if (!y || !x || !w || !out || ny < 1 || nx < 1 || m < 1 || d < 2 || d > 3) return LDDMM_EINVAL;
+  // Columns are processed in groups of up to 3: the weight/output row stride
+  // stays m while the kernel's column count is the group width.
+  for (int c0 = 0; c0 < m; c0 += 3) {
+    lddmm_seg_t s = blank_seg();
+    s.y = y; s.x0 = x; s.w0 = w + c0; s.out0 = out + c0;
+    s.ny = ny; s.nx0 = nx; s.sigma = sigma; s.kernel = kernel; s.order = order;
+    s.m = m;
+    // The kernel width of this group travels in `na` (unused by APPLY_K).
+    int width = (m - c0) < 3 ? (m - c0) : 3;
+    lddmm_seg_t plan = s;
+    plan.m = width;  // plan with the group width so the partial width matches
+    int64_t items, fins;
+    size_t need;
+    int rc = plan_one(LDDMM_OP_APPLY_K, prec, d, &plan, &items, &fins, &need);
+    if (rc) return rc;
+    if (need > work_bytes) return LDDMM_EWORKSPACE;
+    // The stored descriptor keeps the full row stride m for addressing.
+    plan.m = m;
+    lddmm_seg_t shape = plan;
+    shape.m = width;
+    // Launch shape from `shape`, device descriptor from `plan`.
+    SegPack pack;
+    memset(&pack, 0, sizeof(pack));
+    pack.s[0] = plan;
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(STAMP), 0, (hipStream_t)stream, pack, 1,
+                       (lddmm_seg_t *)work);
+    rc = lddmm_segments_run(LDDMM_OP_APPLY_K, prec, d, &shape, (const lddmm_seg_t *)work, 1,
+                            items, fins, work, work_bytes, stream);
+    if (rc) return rc;
+  }
+  return LDDMM_OK;
+}
+
+extern "C" size_t lddmm_apply_diff_kt_workspace_bytes(int64_t ny, int64_t nx, int d, int m,
+                                                      int na) {
+  if (ny < 1 || nx < 1 || m < 1 || na < 1) return 0;
+  lddmm_seg_t s = blank_seg();
+  s.y = QP; s.x0 = QP; s.out0 = (double *)QP;
+  for (int j = 0; j < 4; ++j) { s.tw[j] = QP; s.sw[j] = QP; }
+  s.ny = ny; s.nx0 = nx; s.sigma = 1.0; s.kernel = LDDMM_KERNEL_GAUSS;
+  s.m = m; s.na = na < 3 ? na : 3;
+  return ws_query(LDDMM_OP_DIFF_KT, d, s);
+}
+
+extern "C" int lddmm_apply_diff_kt(const double *y, const double *x, const double *a1,
+                                   const double *a2, int64_t ny, int64_t nx, int d, int m,
+                                   int na, int kernel, int order, double sigma, int prec,
+                                   double *out, void *work, size_t work_bytes, void *stream) {
+  if (!y || !x || !a1 || !a2 || !out || ny < 1 || nx < 1 || d < 2 || d > 3) return LDDMM_EINVAL;
+  if (!(m == 1 || m == d) || na < 1 || na > 3) return LDDMM_EUNSUPPORTED;
+  lddmm_seg_t s = blank_seg();
+  s.y = y; s.x0 = x; s.out0 = out;
+  for (int j = 0; j < na; ++j) {
+    s.tw[j] = a1 + (int64_t)j * ny * m;
+    s.sw[j] = a2 + (int64_t)j * nx * m;
+  }
+  s.ny = ny; s.nx0 = nx; s.sigma = sigma; s.kernel = kernel; s.order = order;
+  s.m = m; s.na = na;
+  int64_t items, fins;
+  size_t need;
+  int rc = plan_one(LDDMM_OP_DIFF_KT, prec, d, &s, &items, &fins, &need);
+  if (rc) return rc;
+  if (need > work_bytes) return LDDMM_EWORKSPACE;
+  return lddmm_segments_run(LDDMM_OP_DIFF_KT, prec, d, &s, nullptr, 1, items, fins, work,
+                            work_bytes, stream);
+}
+
+extern "C" size_t lddmm_apply_ddiff_k11and12_workspace_bytes(int64_t n, int d) {
+  if (n < 1) return 0;
+  lddmm_seg_t s = blank_seg();
+  s.y = QP; s.w0 = QP; s.w1 = QP; s.w2 = QP; s.out0 = (double *)QP;
+  s.ny = n; s.sigma = 1.0; s.kernel = LDDMM_KERNEL_GAUSS;
+  return ws_query(LDDMM_OP_DDIFF_K11_12, d, s);
+}
+
+extern "C" int lddmm_apply_ddiff_k11and12(const double *x, const double *a1, const double *a2,
+                                          const double *p, int64_t n, int d, int kernel,
+                                          int order, double sigma, int prec, double *out,
+                                          void *work, size_t work_bytes, void *stream) {
+  if (!x || !a1 || !a2 || !p || !out || n < 1 || d < 2 || d > 3) return LDDMM_EINVAL;
+  lddmm_seg_t s = blank_seg();
+  s.y = x; s.w0 = a1; s.w1 = a2; s.w2 = p; s.out0 = out;
+  s.ny = n; s.sigma = sigma; s.kernel = kernel; s.order = order;
+  int64_t items, fins;
+  size_t need;
+  int rc = plan_one(LDDMM_OP_DDIFF_K11_12, prec, d, &s, &items, &fins, &need);
+  if (rc) return rc;
+  if (need > work_bytes) return LDDMM_EWORKSPACE;
+  return lddmm_segments_run(LDDMM_OP_DDIFF_K11_12, prec, d, &s, nullptr, 1, items, fins, work,
+                            work_bytes, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Geometry entry points.
+// ---------------------------------------------------------------------------
+extern "C" int lddmm_surface_geometry(const double *vertices, const int32_t *faces, int64_t nf,
+                                      double *centers, double *surfel, void *stream) {
+  if (!vertices || !faces || !centers || !surfel || nf < 1) return LDDMM_EINVAL;
+  hipLaunchKernelGGL(k_surface_geometry, dim3((unsigned)ceil_div(nf, 256)), dim3(256), 0,
+                     (hipStream_t)stream, vertices, faces, nf, centers, surfel);
+  return hip_check(hipGetLastError());
+}
+
+extern "C" int lddmm_curve_geometry(const double *vertices, const int32_t *edges, int64_t ne,
+                                    int d, double *centers, double *lenel, void *stream) {
+  if (!vertices || !edges || !centers || !lenel || ne < 1 || d < 1 || d > 8) return LDDMM_EINVAL;
+  hipLaunchKernelGGL(k_curve_geometry, dim3((unsigned)ceil_div(ne, 256)), dim3(256), 0,
+                     (hipStream_t)stream, vertices, edges, ne, d, centers, lenel);
+  return hip_check(hipGetLastError());
+}
+
+extern "C" int lddmm_surface_vertex_gradient(const double *vertices, const int32_t *faces,
+                                             const double *z1, const double *dz,
+                                             const int32_t *incid_ptr, const int32_t *incid,
+                                             int64_t nv, double scale, double *px,
+                                             void *stream) {
+  if (!vertices || !faces || !z1 || !dz || !incid_ptr || !incid || !px || nv < 1)
+    return LDDMM_EINVAL;
+  hipLaunchKernelGGL(k_surface_vertex_gradient, dim3((unsigned)ceil_div(nv, 256)), dim3(256), 0,
+                     (hipStream_t)stream, vertices, faces, z1, dz, incid_ptr, incid, nv, scale,
+                     px);
+  return hip_check(hipGetLastError());
+}
+
+extern "C" int lddmm_curve_vertex_gradient(const double *z1, const double *dz,
+                                           const int32_t *incid_ptr, const int32_t *incid,
+                                           int64_t nv, int d, double scale, double *px,
+                                           void *stream) {
+  if (!z1 || !dz || !incid_ptr || !incid || !px || nv < 1 || d < 1 || d > 8) return LDDMM_EINVAL;
+  hipLaunchKernelGGL(k_curve_vertex_gradient, dim3((unsigned)ceil_div(nv, 256)), dim3(256), 0,
+                     (hipStream_t)stream, z1, dz, incid_ptr, incid, nv, d, scale, px);
+  return hip_check(hipGetLastError());
+}
+
+extern "C" int lddmm_sum_f64(const double *x, int64_t n, double *out, void *stream) {
+  if (!x || !out || n < 1) return LDDMM_EINVAL;
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, (hipStream_t)stream, x, n, out);
+  return hip_check(hipGetLastError());
+}
+
+extern "C" int lddmm_last_hip_error(void) { return g_last_hip_error; }
+
+extern "C" const char *lddmm_version(void) { return "lddmm-mi355x 0.1 (gfx950)"; }

@@ -1,0 +1,132 @@
+"""Curves and the currents data term, drop-in for py-lddmm's curves module.
+
+``Curve`` keeps the reference layout (vertices (N,d), faces = edges (E,2), centers = edge
+midpoints, lenel = edge vectors; py-lddmm/curves.py:15-73) and the constructors
+``Curve(curve=...)``, ``Curve(FV=(edges, vertices))``, ``Curve(pointSet=..., isOpen=...)``.
+``currentNorm0`` / ``currentNormDef`` / ``currentNorm`` / ``currentNormGradient`` keep the
+reference signatures (curves.py:539-613) and run on the GPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import engine
+from . import runtime as rt
+
+
+class Curve:
+    def __init__(self, curve=None, filename=None, FV=None, pointSet=None, isOpen=False):
+        if curve is None:
+            if FV is None:
+                if filename is not None:
+                    raise NotImplementedError("curve file formats are not part of the GPU path")
+                if pointSet is None:
+                    self.vertices = np.empty(0)
+                    self.centers = np.empty(0)
+                    self.faces = np.empty(0)
+                    self.lenel = np.empty(0)
+                else:
+                    n = pointSet.shape[0]
+                    self.vertices = np.copy(pointSet).astype(np.float64)
+                    self.faces = np.int_(np.zeros([n, 2]))
+                    self.faces[:n - 1, 0] = np.arange(n - 1)
+                    self.faces[:n - 1, 1] = np.arange(1, n)
+                    if not isOpen:
+                        self.faces[n - 1, :] = (n - 1, 0)
+                    self.computeCentersLengths()
+            else:
+                self.vertices = np.copy(FV[1]).astype(np.float64)
+                self.faces = np.int_(FV[0])
+                self.computeCentersLengths()
+        else:
+            self.vertices = np.copy(curve.vertices)
+            self.lenel = np.copy(curve.lenel)
+            self.faces = np.copy(curve.faces)
+            self.centers = np.copy(curve.centers)
+
+    def computeCentersLengths(self):
+        xDef1 = self.vertices[self.faces[:, 0], :]
+        xDef2 = self.vertices[self.faces[:, 1], :]
+        self.centers = (xDef1 + xDef2) / 2
+        self.lenel = xDef2 - xDef1
+
+    def updateVertices(self, x0):
+        self.vertices = np.copy(x0)
+        self.computeCentersLengths()
+
+
+class DeviceCurve:
+    """A curve resident on the GPU (vertices, edges, centers, edge vectors, incidence CSR)."""
+
+    def __init__(self, faces_np, vertices):
+        faces_np = np.asarray(faces_np)
+        self.nv, self.d = int(vertices.shape[0]), int(vertices.shape[1])
+        self.faces = rt.as_dev(faces_np.astype(np.int32), dtype=torch.int32)
+        ptr, inc = engine.incidence(faces_np, self.nv)
+        self.ptr = rt.as_dev(ptr, dtype=torch.int32)
+        self.inc = rt.as_dev(inc, dtype=torch.int32)
+        E = faces_np.shape[0]
+        dev = rt.device()
+        self.centers = torch.empty((E, self.d), dtype=torch.float64, device=dev)
+        self.lenel = torch.empty((E, self.d), dtype=torch.float64, device=dev)
+        self.vertices = None
+        self.update(vertices)
+
+    def update(self, vertices):
+        self.vertices = rt.as_dev(vertices)
+        engine.curve_geometry(self.vertices, self.faces, self.centers, self.lenel)
+
+    # uniform names with DeviceSurface
+    @property
+    def surfel(self):
+        return self.lenel
+
+
+def _kernel(K):
+    return engine.DevKernel.of(K)
+
+
+def currentNorm0(fv1, KparDist):
+    """curves.py:539-543."""
+    s = DeviceCurve(fv1.faces, fv1.vertices)
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.knorm([engine.KNormJob(s.centers, s.lenel, _kernel(KparDist), tot)])
+    return float(tot.item())
+
+
+def currentNormDef(fvDef, fv1, KparDist):
+    """curves.py:549-559."""
+    a = DeviceCurve(fvDef.faces, fvDef.vertices)
+    b = DeviceCurve(fv1.faces, fv1.vertices)
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.currents([engine.CurrentsJob(a.centers, a.lenel, b.centers, b.lenel, _kernel(KparDist),
+                                        tot, coef=0.5)])
+    return float(tot.item())
+
+
+def currentNorm(fvDef, fv1, KparDist):
+    return currentNormDef(fvDef, fv1, KparDist) + currentNorm0(fv1, KparDist)
+
+
+def current_gradient_dev(s: DeviceCurve, c2, cr2, K: engine.DevKernel, scale: float,
+                         px: torch.Tensor, total: torch.Tensor | None = None) -> torch.Tensor:
+    E, d = s.centers.shape
+    dev = s.centers.device
+    z1 = torch.empty((E, d), dtype=torch.float64, device=dev)
+    dz = torch.empty((E, d), dtype=torch.float64, device=dev)
+    if total is None:
+        total = torch.empty(1, dtype=torch.float64, device=dev)
+    engine.currents([engine.CurrentsJob(s.centers, s.lenel, c2, cr2, K, total, z1=z1, dz=dz,
+                                        coef=0.5)])
+    engine.curve_vertex_gradient(z1, dz, s.ptr, s.inc, s.nv, scale, px)
+    return px
+
+
+def currentNormGradient(fvDef, fv1, KparDist):
+    """curves.py:566-613."""
+    a = DeviceCurve(fvDef.faces, fvDef.vertices)
+    b = DeviceCurve(fv1.faces, fv1.vertices)
+    px = torch.empty((a.nv, a.d), dtype=torch.float64, device=rt.device())
+    current_gradient_dev(a, b.centers, b.lenel, _kernel(KparDist), 2.0, px)
+    return px.cpu().numpy()

@@ -1,0 +1,275 @@
+"""Device-resident operations of the LDDMM hot path, batched over independent problems.
+
+Every function takes a list of jobs and issues one segmented launch per time step (or one
+launch in total) covering all of them, so a single registration is simply a batch of one and
+a served batch shares the same kernels.  Results of a job never depend on the other jobs of
+its batch: each segment keeps its own chunking and reduction order.
+
+Reference counterparts:
+  shoot      landmarkDirectEvolutionEuler + the energy loop of objectiveFunDef
+             (py-lddmm/pointEvolution.py:666-730, surfaceMatching.py:206-212)
+  covector   landmarkHamiltonianCovector   (pointEvolution.py:735-765)
+  kdot       SurfaceMatching.dotProduct    (surfaceMatching.py:353-377)
+  currents   currentNormDef / currentNormGradient (surfaces.py:1020-1089, curves.py:548-613)
+  knorm      currentNorm0                  (surfaces.py:1009-1016, curves.py:539-543)
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import torch
+
+from . import _lib
+from . import runtime as rt
+
+
+@dataclass(frozen=True)
+class DevKernel:
+    """What a launch needs to know about a Kernel object."""
+
+    kernel: int
+    order: int
+    sigma: float
+    prec: int
+
+    @staticmethod
+    def of(K) -> "DevKernel":
+        if K.name == 'none':
+            raise ValueError("kernel 'none' has no device form")
+        if getattr(K, 'affine', 'none') != 'none':
+            raise NotImplementedError("kernel-level affine terms are not fused into the device path")
+        return DevKernel(K.kernel_code, int(K.order), float(K.sigma), K.prec_code)
+
+    @property
+    def family(self):
+        return (self.kernel, self.prec)
+
+
+def _groups(jobs, key):
+    out = {}
+    for j in jobs:
+        out.setdefault(key(j), []).append(j)
+    return out.values()
+
+
+def _run_steps(op, jobs, d, n_steps, make_seg, step_active):
+    """Run ``n_steps`` dependent launches; launch t covers the jobs active at t.
+
+    Consecutive launches with the same active set share one plan (their sizes are equal), and
+    their descriptor arrays go up in one transfer.
+    """
+    t = 0
+    while t < n_steps:
+        active = [j for j in jobs if step_active(j, t)]
+        t1 = t + 1
+        while t1 < n_steps and all(step_active(j, t1) for j in active) and \
+                sum(1 for j in jobs if step_active(j, t1)) == len(active):
+            t1 += 1
+        if active:
+            plan = rt.Plan(op, active[0].K.prec, d, [make_seg(j, t) for j in active])
+            if t1 - t == 1:
+                plan.run()
+            else:
+                steps = [[make_seg(j, s) for j in active] for s in range(t, t1)]
+                for p in rt.replicate(plan, steps):
+                    p.run()
+        t = t1
+
+
+# --------------------------------------------------------------------------------------
+# Shooting (flow + energy)
+# --------------------------------------------------------------------------------------
+@dataclass
+class FlowJob:
+    x0: torch.Tensor            # (N, d)
+    at: torch.Tensor            # (T, N, d)
+    K: DevKernel
+    xt: torch.Tensor            # (T+1, N, d) output trajectory
+    energy: torch.Tensor        # (T,) output: <a_t, K(x_t) a_t>
+    v: Optional[torch.Tensor] = None  # (T, N, d) optional velocities K(x_t) a_t
+    dt: Optional[float] = None          # time step; 1/T when None
+
+    @property
+    def T(self):
+        return self.at.shape[0]
+
+    @property
+    def step(self):
+        return 1.0 / self.T if self.dt is None else self.dt
+
+
+def shoot(jobs: Sequence[FlowJob]) -> None:
+    """xt[t+1] = xt[t] + dt K(xt[t]) at[t], energy[t] = <at[t], K(xt[t]) at[t]>, dt = 1/T."""
+    for group in _groups(jobs, lambda j: (j.K.family, j.x0.shape[1])):
+        d = group[0].x0.shape[1]
+        for j in group:
+            j.xt[0].copy_(j.x0)
+
+        def make(j, t):
+            return rt.seg(y=j.xt[t], w0=j.at[t], out0=j.xt[t + 1],
+                          out1=(j.v[t] if j.v is not None else None),
+                          total=j.energy[t:t + 1], ny=j.x0.shape[0], sigma=j.K.sigma,
+                          dt=j.step, kernel=j.K.kernel, order=j.K.order)
+
+        _run_steps(_lib.OP_FLOW_STEP, group, d, max(j.T for j in group), make,
+                   lambda j, t: t < j.T)
+
+
+# --------------------------------------------------------------------------------------
+# Covector (adjoint)
+# --------------------------------------------------------------------------------------
+@dataclass
+class CovectorJob:
+    xt: torch.Tensor            # (T+1, N, d) trajectory of at
+    at: torch.Tensor            # (T, N, d)
+    px1: torch.Tensor           # (N, d) end-point covector
+    K: DevKernel
+    regweight: float
+    pxt: torch.Tensor           # (T+1, N, d) output
+    dt: Optional[float] = None  # time step; 1/T when None
+
+    @property
+    def T(self):
+        return self.at.shape[0]
+
+    @property
+    def step(self):
+        return 1.0 / self.T if self.dt is None else self.dt
+
+
+def covector(jobs: Sequence[CovectorJob]) -> None:
+    """pxt[T] = px1; pxt[t] = pxt[t+1] + dt * applyDiffKT(xt[t], [p, a, -2 lam a], [a, p, a])."""
+    for group in _groups(jobs, lambda j: (j.K.family, j.xt.shape[2])):
+        d = group[0].xt.shape[2]
+        for j in group:
+            j.pxt[j.T].copy_(j.px1)
+        maxT = max(j.T for j in group)
+
+        # Backward in time: launch s handles t = T - 1 - s of every job with s < T.
+        def make(j, s):
+            t = j.T - 1 - s
+            return rt.seg(y=j.xt[t], w0=j.at[t], w1=j.pxt[t + 1], out0=j.pxt[t],
+                          ny=j.xt.shape[1], sigma=j.K.sigma, dt=j.step, coef=j.regweight,
+                          kernel=j.K.kernel, order=j.K.order)
+
+        _run_steps(_lib.OP_COVECTOR_STEP, group, d, maxT, make, lambda j, s: s < j.T)
+
+
+# --------------------------------------------------------------------------------------
+# RKHS products over a trajectory
+# --------------------------------------------------------------------------------------
+@dataclass
+class KDotJob:
+    xt: torch.Tensor            # (T+1, N, d) (the first T slices are used)
+    g1: torch.Tensor            # (T, N, d)
+    g2: Sequence[torch.Tensor]  # 1..2 tensors (T, N, d)
+    K: DevKernel
+    out: torch.Tensor           # (T, 2) output totals: out[t, j] = <g2[j][t], K(xt[t]) g1[t]>
+
+
+def kdot(jobs: Sequence[KDotJob]) -> None:
+    for group in _groups(jobs, lambda j: (j.K.family, j.xt.shape[2])):
+        d = group[0].xt.shape[2]
+        segs = []
+        for j in group:
+            T = j.g1.shape[0]
+            g2 = list(j.g2)
+            for t in range(T):
+                segs.append(rt.seg(y=j.xt[t], w0=j.g1[t], g=[g[t] for g in g2],
+                                   total=j.out[t], ny=j.xt.shape[1], sigma=j.K.sigma,
+                                   kernel=j.K.kernel, order=j.K.order, na=len(g2)))
+        rt.launch(_lib.OP_KDOT, group[0].K.prec, d, segs)
+
+
+# --------------------------------------------------------------------------------------
+# Currents data term
+# --------------------------------------------------------------------------------------
+@dataclass
+class CurrentsJob:
+    c1: torch.Tensor            # (F1, d) centers of the deformed shape
+    cr1: torch.Tensor           # (F1, d) surfels / edge vectors of the deformed shape
+    c2: torch.Tensor            # (F2, d) target centers
+    cr2: torch.Tensor           # (F2, d) target surfels
+    K: DevKernel
+    total: torch.Tensor         # (1,) output: currentNormDef
+    z1: Optional[torch.Tensor] = None   # (F1, d) gradient form outputs
+    dz: Optional[torch.Tensor] = None
+    coef: float = 1.0 / 3.0
+
+
+def currents(jobs: Sequence[CurrentsJob]) -> None:
+    for group in _groups(jobs, lambda j: (j.K.family, j.c1.shape[1], j.dz is not None)):
+        d = group[0].c1.shape[1]
+        segs = [rt.seg(y=j.c1, w0=j.cr1, x1=j.c2, w1=j.cr2, out0=j.z1, out1=j.dz, total=j.total,
+                       ny=j.c1.shape[0], nx1=j.c2.shape[0], sigma=j.K.sigma, coef=j.coef,
+                       kernel=j.K.kernel, order=j.K.order) for j in group]
+        rt.launch(_lib.OP_CURRENTS, group[0].K.prec, d, segs)
+
+
+@dataclass
+class KNormJob:
+    c: torch.Tensor
+    cr: torch.Tensor
+    K: DevKernel
+    total: torch.Tensor         # (1,)
+
+
+def knorm(jobs: Sequence[KNormJob]) -> None:
+    for group in _groups(jobs, lambda j: (j.K.family, j.c.shape[1])):
+        d = group[0].c.shape[1]
+        segs = [rt.seg(y=j.c, w0=j.cr, total=j.total, ny=j.c.shape[0], sigma=j.K.sigma,
+                       kernel=j.K.kernel, order=j.K.order) for j in group]
+        rt.launch(_lib.OP_KNORM, group[0].K.prec, d, segs)
+
+
+# --------------------------------------------------------------------------------------
+# Geometry
+# --------------------------------------------------------------------------------------
+def surface_geometry(vertices: torch.Tensor, faces: torch.Tensor, centers: torch.Tensor,
+                     surfel: torch.Tensor) -> None:
+    lib = _lib.load()
+    _lib.check(lib.lddmm_surface_geometry(vertices.data_ptr(), faces.data_ptr(), faces.shape[0],
+                                          centers.data_ptr(), surfel.data_ptr(),
+                                          rt.stream_handle()), "lddmm_surface_geometry")
+
+
+def curve_geometry(vertices, edges, centers, lenel) -> None:
+    lib = _lib.load()
+    _lib.check(lib.lddmm_curve_geometry(vertices.data_ptr(), edges.data_ptr(), edges.shape[0],
+                                        vertices.shape[1], centers.data_ptr(), lenel.data_ptr(),
+                                        rt.stream_handle()), "lddmm_curve_geometry")
+
+
+def incidence(faces_np, nv: int):
+    """Vertex -> (face, corner) CSR sorted by (corner, face), i.e. in the order the reference's
+    scatter loops visit them (surfaces.py:1074-1087, curves.py:604-610).  Host NumPy."""
+    import numpy as np
+    F, k = faces_np.shape
+    corner = np.repeat(np.arange(k)[None, :], F, axis=0)
+    face = np.repeat(np.arange(F)[:, None], k, axis=1)
+    v = faces_np.reshape(-1)
+    code = (face * k + corner).reshape(-1)
+    # Order: by vertex, then corner, then face.
+    order = np.lexsort((face.reshape(-1), corner.reshape(-1), v))
+    ptr = np.zeros(nv + 1, dtype=np.int64)
+    np.add.at(ptr, v + 1, 1)
+    ptr = np.cumsum(ptr)
+    return ptr.astype(np.int32), code[order].astype(np.int32)
+
+
+def surface_vertex_gradient(vertices, faces, z1, dz, ptr, inc, scale, px) -> None:
+    lib = _lib.load()
+    _lib.check(lib.lddmm_surface_vertex_gradient(vertices.data_ptr(), faces.data_ptr(),
+                                                 z1.data_ptr(), dz.data_ptr(), ptr.data_ptr(),
+                                                 inc.data_ptr(), vertices.shape[0], float(scale),
+                                                 px.data_ptr(), rt.stream_handle()),
+               "lddmm_surface_vertex_gradient")
+
+
+def curve_vertex_gradient(z1, dz, ptr, inc, nv, scale, px) -> None:
+    lib = _lib.load()
+    _lib.check(lib.lddmm_curve_vertex_gradient(z1.data_ptr(), dz.data_ptr(), ptr.data_ptr(),
+                                               inc.data_ptr(), nv, z1.shape[1], float(scale),
+                                               px.data_ptr(), rt.stream_handle()),
+               "lddmm_curve_vertex_gradient")

@@ -1,0 +1,265 @@
+"""Device-resident matching state and the batched implementations of the optimiser requests.
+
+``DeviceMatching`` holds one registration on the GPU: template vertices x0, momenta at (T,N,d),
+the trajectory xt of at, the target's currents (c2, cr2), and trial buffers.  The requests
+the CG optimiser issues (objective, line-search trial, gradient, RKHS products) are
+implemented once, for a *list* of registrations, by ``batch_*`` functions that issue one
+segmented launch per kernel-sum stage for the whole list.  A registration's own methods call
+them with a list of one, which is the plain single-pair path; the serving layer calls them
+with every registration waiting on the same request.
+
+Reference: py-lddmm/surfaceMatching.py:187-484 and curveMatching.py:148-352
+(affine='none', symmetric=False, errorType='current').  Differences that do not change any
+result: the trajectory of an accepted trial is kept instead of being recomputed in
+endOfIteration / landmarkHamiltonianCovector (the recomputation is deterministic and
+bit-identical), and the energy sum reuses the flow's K(x_t) a_t instead of a second applyK.
+"""
+from __future__ import annotations
+
+import math
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import engine
+from . import runtime as rt
+
+
+class Direction:
+    """Optimisation direction (surfaceMatching.py:52-56): diff is a (T, N, d) device tensor."""
+
+    def __init__(self):
+        self.diff = []
+        self.aff = []
+        self.initx = []
+
+
+class DeviceMatching:
+    """Common device state of SurfaceMatching / CurveMatching."""
+
+    # subclasses set: coef (1/3 surfaces, 1/2 curves) and _new_shape(vertices)
+    coef = 1.0 / 3.0
+
+    def _init_device(self, x0_np, faces_np, target_centers, target_cr, KparDiff, KparDist,
+                     sigmaError, Tsize, regweight):
+        dev = rt.device()
+        self.npt, self.dim = x0_np.shape
+        self.Tsize = Tsize
+        self.regweight = float(regweight)
+        self.sigmaError = float(sigmaError)
+        self.Kdiff = engine.DevKernel.of(KparDiff)
+        self.Kdist = engine.DevKernel.of(KparDist)
+        if self.Kdiff.prec != self.Kdist.prec:
+            raise ValueError("deformation and data kernels must share one precision")
+        self.x0 = rt.as_dev(x0_np)
+        self.faces_np = np.asarray(faces_np)
+        self.c2 = rt.as_dev(target_centers)
+        self.cr2 = rt.as_dev(target_cr)
+        T, N, d = Tsize, self.npt, self.dim
+        self.at = torch.zeros((T, N, d), dtype=torch.float64, device=dev)
+        self.atTry = torch.zeros_like(self.at)
+        self.xt = self.x0.expand(T + 1, N, d).clone()
+        self._xt_try = torch.empty_like(self.xt)
+        self._xt_scratch = torch.empty_like(self.xt)
+        self.fvDef_dev = self._new_shape(self.x0)
+        self._try_shape = self._new_shape(self.x0)
+        self.obj = None
+        self.objTry = None
+        self.obj0 = None
+        self.gradCoeff = self.npt
+        self.Afft = np.zeros([self.Tsize, 0])
+        self.AfftTry = np.zeros([self.Tsize, 0])
+        self.affineDim = 0
+        self.nobj = 0
+        self.ngrad = 0
+        self.iter = 0
+
+    # ----- reference protocol, single registration -----------------------------------
+    def objectiveFun(self):
+        if self.obj is None:
+            batch_objective([self])
+        return self.obj
+
+    def getVariable(self):
+        return [self.at, self.Afft]
+
+    def updateTry(self, dir, eps, objRef=None):
+        return batch_update_try([(self, dir, eps, objRef)])[0]
+
+    def getGradient(self, coeff=1.0):
+        return batch_gradient([(self, coeff)])[0]
+
+    def dotProduct(self, g1, g2):
+        return batch_dot([(self, g1, g2)])[0]
+
+    def acceptVarTry(self):
+        self.obj = self.objTry
+        self.at = self.atTry
+        self.xt, self._xt_try = self._xt_try, self.xt
+        self.fvDef_dev.update(self.xt[-1])
+
+    def endOfIteration(self):
+        self.iter += 1
+
+    def addProd(self, dir1, dir2, beta):
+        d = Direction()
+        d.diff = dir1.diff + beta * dir2.diff
+        d.aff = dir1.aff + beta * dir2.aff
+        return d
+
+    def copyDir(self, dir0):
+        d = Direction()
+        d.diff = dir0.diff.clone()
+        d.aff = np.copy(dir0.aff)
+        return d
+
+    def randomDir(self):
+        d = Direction()
+        d.diff = torch.randn((self.Tsize, self.npt, self.dim), dtype=torch.float64,
+                             device=self.x0.device)
+        d.aff = np.random.randn(self.Tsize, self.affineDim)
+        return d
+
+    # ----- helpers ---------------------------------------------------------------------
+    def deformed_vertices(self) -> torch.Tensor:
+        return self.xt[-1]
+
+    def _gradient_into(self, shape, scale, px, total=None):
+        raise NotImplementedError
+
+
+# --------------------------------------------------------------------------------------
+# Batched request implementations
+# --------------------------------------------------------------------------------------
+def _readback(buf: torch.Tensor) -> np.ndarray:
+    return buf.cpu().numpy()
+
+
+def batch_objective(opts: Sequence[DeviceMatching]) -> list:
+    """objectiveFun (surfaceMatching.py:233-250): obj0 once, then the energy of at plus the
+    data term of the deformed template."""
+    dev = rt.device()
+    lens = [o.Tsize + 2 for o in opts]
+    scal = torch.zeros(sum(lens), dtype=torch.float64, device=dev)
+    flows, curs, norms, offs = [], [], [], []
+    off = 0
+    for o, L in zip(opts, lens):
+        T = o.Tsize
+        offs.append(off)
+        flows.append(engine.FlowJob(o.x0, o.at, o.Kdiff, o.xt, scal[off:off + T]))
+        off += L
+    engine.shoot(flows)
+    for o, off in zip(opts, offs):
+        T = o.Tsize
+        o.fvDef_dev.update(o.xt[-1])
+        curs.append(engine.CurrentsJob(o.fvDef_dev.centers, o.fvDef_dev.surfel, o.c2, o.cr2,
+                                       o.Kdist, scal[off + T:off + T + 1], coef=o.coef))
+        norms.append(engine.KNormJob(o.c2, o.cr2, o.Kdist, scal[off + T + 1:off + T + 2]))
+    engine.currents(curs)
+    engine.knorm(norms)
+    vals = _readback(scal)
+    out = []
+    for o, off in zip(opts, offs):
+        T = o.Tsize
+        s2 = o.sigmaError ** 2
+        o.obj0 = vals[off + T + 1] / s2
+        obj = 0
+        dt = 1.0 / T
+        for t in range(T):
+            obj = obj + o.regweight * dt * vals[off + t]
+        o.obj = obj + (o.obj0 + vals[off + T] / s2)
+        o.nobj += 1
+        out.append(o.obj)
+    return out
+
+
+def batch_update_try(items) -> list:
+    """updateTry (surfaceMatching.py:255-289) for [(opt, dir, eps, objRef), ...]."""
+    dev = rt.device()
+    lens = [o.Tsize + 1 for (o, _, _, _) in items]
+    scal = torch.zeros(sum(lens), dtype=torch.float64, device=dev)
+    flows, curs, ats, offs = [], [], [], []
+    off = 0
+    for (o, dir, eps, _), L in zip(items, lens):
+        T = o.Tsize
+        atTry = o.at - eps * dir.diff
+        ats.append(atTry)
+        offs.append(off)
+        flows.append(engine.FlowJob(o.x0, atTry, o.Kdiff, o._xt_scratch, scal[off:off + T]))
+        off += L
+    engine.shoot(flows)
+    for (o, _, _, _), off in zip(items, offs):
+        T = o.Tsize
+        o._try_shape.update(o._xt_scratch[-1])
+        curs.append(engine.CurrentsJob(o._try_shape.centers, o._try_shape.surfel, o.c2, o.cr2,
+                                       o.Kdist, scal[off + T:off + T + 1], coef=o.coef))
+    engine.currents(curs)
+    vals = _readback(scal)
+    out = []
+    for (o, dir, eps, objRef), atTry, off in zip(items, ats, offs):
+        T = o.Tsize
+        dt = 1.0 / T
+        objTry = o.obj0
+        objDef = 0
+        for t in range(T):
+            objDef = objDef + o.regweight * dt * vals[off + t]
+        objTry += objDef
+        objTry += vals[off + T] / (o.sigmaError ** 2)
+        o.nobj += 1
+        if math.isnan(objTry):
+            out.append(float('inf'))
+            continue
+        if (objRef is None) or (objTry < objRef):
+            o.atTry = atTry
+            o.objTry = objTry
+            o._xt_try, o._xt_scratch = o._xt_scratch, o._xt_try
+        out.append(objTry)
+    return out
+
+
+def batch_gradient(items) -> list:
+    """getGradient (surfaceMatching.py:301-327): px1 = -grad(data)/sigma^2 at the deformed
+    template, covector along the kept trajectory, dat = 2 lambda a - p, scaled by
+    1/(coeff T)."""
+    dev = rt.device()
+    covs, pxts = [], []
+    for o, coeff in items:
+        px1 = torch.empty((o.npt, o.dim), dtype=torch.float64, device=dev)
+        o._gradient_into(o.fvDef_dev, -2.0 / (o.sigmaError ** 2), px1)
+        pxt = torch.empty_like(o.xt)
+        pxts.append(pxt)
+        covs.append(engine.CovectorJob(o.xt, o.at, px1, o.Kdiff, o.regweight, pxt))
+    engine.covector(covs)
+    out = []
+    for (o, coeff), pxt in zip(items, pxts):
+        g = Direction()
+        dat = 2 * o.regweight * o.at - pxt[1:]
+        g.diff = dat / (coeff * o.Tsize)
+        g.aff = np.zeros(o.Afft.shape)
+        o.ngrad += 1
+        out.append(g)
+    return out
+
+
+def batch_dot(items) -> list:
+    """dotProduct (surfaceMatching.py:353-377): sum_t <g2_t, K(x_t) g1_t> for 1..2 g2's."""
+    dev = rt.device()
+    jobs, bufs = [], []
+    for o, g1, g2 in items:
+        buf = torch.zeros((o.Tsize, 2), dtype=torch.float64, device=dev)
+        bufs.append(buf)
+        jobs.append(engine.KDotJob(o.xt, g1.diff, [g.diff for g in g2], o.Kdiff, buf))
+    engine.kdot(jobs)
+    allv = _readback(torch.cat([b.reshape(-1) for b in bufs])) if bufs else []
+    out = []
+    pos = 0
+    for (o, g1, g2), buf in zip(items, bufs):
+        v = allv[pos:pos + buf.numel()].reshape(o.Tsize, 2)
+        pos += buf.numel()
+        res = np.zeros(len(g2))
+        for t in range(o.Tsize):
+            for ll in range(len(g2)):
+                res[ll] = res[ll] + v[t, ll]
+        out.append(res)
+    return out

@@ -1,0 +1,137 @@
+"""Triangulated surfaces and the currents data term, drop-in for py-lddmm's surfaces module.
+
+``Surface`` keeps the reference's in-memory layout (vertices (N,3) float64, faces (F,3) int,
+centers, surfel = area-weighted normals at twice the area; py-lddmm/surfaces.py:24-75) and
+its constructors ``Surface(surf=...)`` / ``Surface(FV=(faces, vertices))``.  File readers and
+writers (BYU/OFF/VTK/OBJ) are not part of the GPU path.
+
+``currentNorm0``, ``currentNormDef``, ``currentNorm`` and ``currentNormGradient`` keep the
+reference signatures (surfaces.py:1009-1089) and run on the GPU: the two F x F sums of the
+norm and the four of the gradient are fused into one segmented launch, and the per-face
+scatter of the gradient is a deterministic vertex gather in the reference's summation order.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import engine
+from . import runtime as rt
+
+
+class Surface:
+    def __init__(self, surf=None, filename=None, FV=None):
+        if surf is None:
+            if FV is None:
+                if filename is not None:
+                    raise NotImplementedError("surface file formats are not part of the GPU path")
+                self.vertices = np.empty(0)
+                self.centers = np.empty(0)
+                self.faces = np.empty(0)
+                self.surfel = np.empty(0)
+            else:
+                self.vertices = np.copy(FV[1]).astype(np.float64)
+                self.faces = np.int_(FV[0])
+                self.computeCentersAreas()
+        else:
+            self.vertices = np.copy(surf.vertices)
+            self.faces = np.copy(surf.faces)
+            self.surfel = np.copy(surf.surfel)
+            self.centers = np.copy(surf.centers)
+            self.computeCentersAreas()
+
+    def computeCentersAreas(self):
+        xDef1 = self.vertices[self.faces[:, 0], :]
+        xDef2 = self.vertices[self.faces[:, 1], :]
+        xDef3 = self.vertices[self.faces[:, 2], :]
+        self.centers = (xDef1 + xDef2 + xDef3) / 3
+        self.surfel = np.cross(xDef2 - xDef1, xDef3 - xDef1)
+
+    def updateVertices(self, x0):
+        self.vertices = np.copy(x0)
+        self.computeCentersAreas()
+
+    def flipFaces(self):
+        self.faces = self.faces[:, [0, 2, 1]]
+        self.computeCentersAreas()
+
+    def surfVolume(self):
+        """Signed volume sum_f det[x1; x2; x3] / 6 (surfaces.py:711-717)."""
+        v = self.vertices[self.faces]
+        return float(np.linalg.det(v).sum() / 6)
+
+
+class DeviceSurface:
+    """A surface resident on the GPU: vertices, faces, centers, surfels and the
+    vertex-incidence CSR used by the gradient gather."""
+
+    def __init__(self, faces_np, vertices):
+        faces_np = np.asarray(faces_np)
+        self.nv = int(vertices.shape[0])
+        self.faces = rt.as_dev(faces_np.astype(np.int32), dtype=torch.int32)
+        ptr, inc = engine.incidence(faces_np, self.nv)
+        self.ptr = rt.as_dev(ptr, dtype=torch.int32)
+        self.inc = rt.as_dev(inc, dtype=torch.int32)
+        F = faces_np.shape[0]
+        dev = rt.device()
+        self.centers = torch.empty((F, 3), dtype=torch.float64, device=dev)
+        self.surfel = torch.empty((F, 3), dtype=torch.float64, device=dev)
+        self.vertices = None
+        self.update(vertices)
+
+    def update(self, vertices):
+        self.vertices = rt.as_dev(vertices)
+        engine.surface_geometry(self.vertices, self.faces, self.centers, self.surfel)
+
+
+def _kernel(K):
+    return engine.DevKernel.of(K)
+
+
+def _dev_surface(fv) -> DeviceSurface:
+    return DeviceSurface(fv.faces, fv.vertices)
+
+
+def currentNorm0(fv1, KparDist):
+    """<surfel, K(c, c) surfel> of fv1 (surfaces.py:1009-1016)."""
+    s = _dev_surface(fv1)
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.knorm([engine.KNormJob(s.centers, s.surfel, _kernel(KparDist), tot)])
+    return float(tot.item())
+
+
+def currentNormDef(fvDef, fv1, KparDist):
+    """|fvDef|^2 - 2 <fvDef, fv1> in the currents RKHS (surfaces.py:1020-1033)."""
+    a, b = _dev_surface(fvDef), _dev_surface(fv1)
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.currents([engine.CurrentsJob(a.centers, a.surfel, b.centers, b.surfel,
+                                        _kernel(KparDist), tot)])
+    return float(tot.item())
+
+
+def currentNorm(fvDef, fv1, KparDist):
+    return currentNormDef(fvDef, fv1, KparDist) + currentNorm0(fv1, KparDist)
+
+
+def current_gradient_dev(s: DeviceSurface, c2, cr2, K: engine.DevKernel, scale: float,
+                         px: torch.Tensor, total: torch.Tensor | None = None) -> torch.Tensor:
+    """px = scale * gradient of |S - target|^2 / 2 wrt the vertices of s (the reference's
+    ``2*px`` is folded into ``scale``); also writes currentNormDef to ``total``."""
+    F = s.centers.shape[0]
+    dev = s.centers.device
+    z1 = torch.empty((F, 3), dtype=torch.float64, device=dev)
+    dz = torch.empty((F, 3), dtype=torch.float64, device=dev)
+    if total is None:
+        total = torch.empty(1, dtype=torch.float64, device=dev)
+    engine.currents([engine.CurrentsJob(s.centers, s.surfel, c2, cr2, K, total, z1=z1, dz=dz,
+                                        coef=1.0 / 3.0)])
+    engine.surface_vertex_gradient(s.vertices, s.faces, z1, dz, s.ptr, s.inc, scale, px)
+    return px
+
+
+def currentNormGradient(fvDef, fv1, KparDist):
+    """Gradient of currentNorm wrt the vertices of fvDef (surfaces.py:1040-1089)."""
+    a, b = _dev_surface(fvDef), _dev_surface(fv1)
+    px = torch.empty((a.nv, 3), dtype=torch.float64, device=rt.device())
+    current_gradient_dev(a, b.centers, b.surfel, _kernel(KparDist), 2.0, px)
+    return px.cpu().numpy()

@@ -1,0 +1,102 @@
+"""The C ABI library loads, exports every symbol include/lddmm.h declares, and its host-only
+planning behaves (no device calls: these run without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from registration_amd import _lib
+from registration_amd._lib import Seg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lddmm.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lddmm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.SIGNATURES), "ctypes signatures out of sync with the header"
+    assert lib.lddmm_version().startswith(b"lddmm-mi355x")
+
+
+def _plan(op, prec, d, segs):
+    lib = _lib.load()
+    arr = (Seg * len(segs))(*segs)
+    n_items, n_fin, wb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_size_t()
+    rc = lib.lddmm_segments_plan(op, prec, d, arr, len(segs), ctypes.byref(n_items),
+                                 ctypes.byref(n_fin), ctypes.byref(wb))
+    return rc, arr, n_items.value, n_fin.value, wb.value
+
+
+def _seg(ny, **kw):
+    s = Seg()
+    s.y = 4096  # planning never dereferences pointers
+    s.w0 = s.w1 = s.w2 = 4096
+    s.out0 = 4096
+    s.ny = ny
+    s.sigma = kw.pop("sigma", 0.5)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def test_plan_layout_and_chunking():
+    rc, arr, items, fins, wb = _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP32C, 3, [_seg(40962), _seg(642)])
+    assert rc == 0
+    a, b = arr[0], arr[1]
+    assert a.row_tiles == -(-40962 // 512) and b.row_tiles == 2  # 256 threads x 2 rows
+    assert a.chunk_len % 256 == 0 and a.chunks0 * a.chunk_len >= 40962
+    assert (a.chunks0 - 1) * a.chunk_len < 40962
+    assert b.item0 == a.row_tiles * a.chunks0
+    assert items == b.item0 + b.row_tiles * b.chunks0
+    assert a.fin_blocks == -(-40962 // 256) and b.fin0 == a.fin_blocks and fins == a.fin_blocks + b.fin_blocks
+    # The chunking of a segment depends only on its own sizes (result independent of the batch).
+    rc2, alone, *_ = _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP32C, 3, [_seg(642)])
+    assert (alone[0].chunk_len, alone[0].chunks0, alone[0].row_tiles) == (b.chunk_len, b.chunks0, b.row_tiles)
+    assert wb > 0
+
+
+def test_plan_rejects_invalid():
+    assert _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP32C, 4, [_seg(10)])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_FLOW_STEP, 7, 3, [_seg(10)])[0] == _lib.EINVAL
+    # Laplacian order 0 has no derivative; order 1 has no second derivative.
+    lap0 = _seg(10, kernel=_lib.KERNEL_LAPLACIAN, order=0)
+    assert _plan(_lib.OP_COVECTOR_STEP, _lib.PREC_FP64, 3, [lap0])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP64, 3, [lap0])[0] == 0
+    lap1 = _seg(10, kernel=_lib.KERNEL_LAPLACIAN, order=1)
+    assert _plan(_lib.OP_DDIFF_K11_12, _lib.PREC_FP64, 3, [lap1])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP64, 3, [_seg(10, kernel=_lib.KERNEL_LAPLACIAN, order=5)])[0] == _lib.EINVAL
+    # One kernel family per launch.
+    mixed = [_seg(10), _seg(10, kernel=_lib.KERNEL_LAPLACIAN, order=1)]
+    assert _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP64, 3, mixed)[0] == _lib.EINVAL
+    assert _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP64, 3, [_seg(0)])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP64, 3, [_seg(10, sigma=0.0)])[0] == _lib.EINVAL
+    # Currents need a target set.
+    assert _plan(_lib.OP_CURRENTS, _lib.PREC_FP64, 3, [_seg(10)])[0] == _lib.EINVAL
+
+
+def test_workspace_queries():
+    lib = _lib.load()
+    assert lib.lddmm_apply_k_workspace_bytes(1000, 2000, 3, 5) > 1000 * 5 * 8
+    assert lib.lddmm_apply_diff_kt_workspace_bytes(100, 100, 3, 3, 2) > 0
+    assert lib.lddmm_apply_ddiff_k11and12_workspace_bytes(100, 3) > 0
+    assert lib.lddmm_apply_k_workspace_bytes(0, 10, 3, 3) == 0
+
+
+def test_product_never_imports_the_oracle():
+    import registration_amd
+    pkg = os.path.dirname(registration_amd.__file__)
+    for f in os.listdir(pkg):
+        if f.endswith(".py"):
+            src = open(os.path.join(pkg, f)).read()
+            assert not re.search(r"(import\s+oracle|from\s+oracle|lddmm_oracle|liboracle)", src), f

@@ -1,0 +1,144 @@
+"""GPU kernel sums (liblddmm_hip.so through the Kernel drop-in) against the reference's own
+Fortran outputs (golden fixtures) and the FP64 oracle.
+
+Tolerances: FP64 mode 1e-12 relative to the largest output entry; FP32-compensated mode 2e-6
+(the path's 1e-5 trajectory bound with margin)."""
+import numpy as np
+import pytest
+
+from oracle import lddmm_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp32c": 2e-6}
+
+
+def rel_err(got, ref):
+    return float(np.abs(np.asarray(got) - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+def kernel(order, sigma, prec):
+    from registration_amd import kernelFunctions as kfun
+    if order > 4:
+        return kfun.Kernel(name='gauss', sigma=sigma, precision=prec)
+    return kfun.Kernel(name='laplacian', sigma=sigma, order=order, precision=prec)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_golden_kernel_sums(golden, prec):
+    for name, c in golden.items():
+        d, ny, nx, m, na, order, sigma = c["params"]
+        K = kernel(int(order), float(sigma), prec)
+        print(f"case {name} {prec}", flush=True)
+        f = K.applyK(c["x"], c["w"], firstVar=c["y"])
+        assert rel_err(f, c["applyk"]) < TOL[prec], (name, "applyK")
+        if "applykdifft" in c:
+            g = K.applyDiffKT(c["x"], c["a1"], c["a2"], firstVar=c["y"])
+            assert rel_err(g, c["applykdifft"]) < TOL[prec], (name, "applyDiffKT")
+        if "applykdiff11and12" in c:
+            h = K.applyDDiffK11and12(c["y"], c["dd_a1"], c["dd_a2"], c["dd_p"])
+            assert rel_err(h, c["applykdiff11and12"]) < TOL[prec], (name, "applyDDiffK11and12")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+@pytest.mark.parametrize("ny,nx,d,m", [(1, 1, 3, 3), (1, 700, 3, 1), (257, 1, 2, 2), (513, 1031, 3, 5),
+                                       (1000, 300, 1, 1), (2049, 2049, 3, 3), (3000, 64, 2, 4)])
+def test_apply_k_ragged(prec, ny, nx, d, m):
+    rng = np.random.default_rng(ny * 7 + nx)
+    y, x, w = rng.standard_normal((ny, d)), rng.standard_normal((nx, d)), rng.standard_normal((nx, m))
+    for order, sigma in ((10, 0.6), (1, 0.5), (3, 0.8)):
+        K = kernel(order, sigma, prec)
+        print(f"applyK ny={ny} nx={nx} d={d} m={m} order={order} {prec}", flush=True)
+        got = K.applyK(x, w, firstVar=y)
+        ref = orc.apply_k(y, x, w, sigma, order)
+        assert rel_err(got, ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+@pytest.mark.parametrize("na,m,d", [(1, 3, 3), (2, 3, 3), (3, 3, 3), (1, 1, 3), (3, 2, 2), (4, 3, 3), (5, 2, 2)])
+def test_apply_diff_kt(prec, na, m, d):
+    rng = np.random.default_rng(na * 31 + m)
+    ny, nx = 777, 1300
+    y, x = rng.standard_normal((ny, d)), rng.standard_normal((nx, d))
+    x[:3] = y[:3]
+    a1, a2 = rng.standard_normal((na, ny, m)), rng.standard_normal((na, nx, m))
+    for order, sigma in ((10, 0.5), (1, 0.6), (2, 0.7), (4, 0.4)):
+        K = kernel(order, sigma, prec)
+        print(f"applyDiffKT na={na} m={m} d={d} order={order} {prec}", flush=True)
+        got = K.applyDiffKT(x, a1, a2, firstVar=y)
+        ref = orc.apply_diff_kt(y, x, a1, a2, sigma, order)
+        assert rel_err(got, ref) < TOL[prec] * 5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+@pytest.mark.parametrize("order,sigma,d", [(10, 0.5, 3), (2, 0.6, 3), (3, 0.5, 2), (4, 0.8, 3)])
+def test_apply_ddiff_k11and12(prec, order, sigma, d):
+    rng = np.random.default_rng(order)
+    n = 900
+    x, a1, a2, p = (rng.standard_normal((n, d)) for _ in range(4))
+    K = kernel(order, sigma, prec)
+    print(f"applyDDiffK11and12 order={order} d={d} {prec}", flush=True)
+    got = K.applyDDiffK11and12(x, a1, a2, p)
+    ref = orc.apply_ddiff_k11and12(x, a1, a2, p, sigma, order)
+    assert rel_err(got, ref) < TOL[prec] * 5
+
+
+def test_device_tensors_stay_on_device():
+    import torch
+    from registration_amd import kernelFunctions as kfun
+    x = torch.randn(100, 3, dtype=torch.float64, device="cuda")
+    a = torch.randn(100, 3, dtype=torch.float64, device="cuda")
+    z = kfun.Kernel('gauss', sigma=0.5).applyK(x, a)
+    assert isinstance(z, torch.Tensor) and z.is_cuda and z.dtype == torch.float64
+    ref = orc.apply_k(x.cpu().numpy(), x.cpu().numpy(), a.cpu().numpy(), 0.5, 10)
+    assert rel_err(z.cpu().numpy(), ref) < TOL["fp32c"]
+
+
+def test_kernel_none_and_errors():
+    from registration_amd import kernelFunctions as kfun
+    x = np.random.default_rng(0).standard_normal((10, 3))
+    z = kfun.Kernel(name='foo', sigma=1.0).applyK(x, x)
+    assert np.all(z == 0)
+    with pytest.raises(ValueError):
+        kfun.Kernel('laplacian', sigma=1.0, order=0).applyDiffKT(x, x[None], x[None])
+    with pytest.raises(ValueError):
+        kfun.Kernel('gauss', sigma=1.0).applyK(np.zeros((5, 4)), np.zeros((5, 1)))
+
+
+def test_affine_kernel_terms_match_reference_formula():
+    from registration_amd import kernelFunctions as kfun
+    rng = np.random.default_rng(2)
+    x, a, y = rng.standard_normal((50, 3)), rng.standard_normal((50, 3)), rng.standard_normal((20, 3))
+    c = np.array([0.1, -0.2, 0.3])
+    K = kfun.Kernel('gauss', affine='affine', sigma=0.7, w1=0.3, w2=0.5, center=c, precision='fp64')
+    got = K.applyK(x, a, firstVar=y)
+    xx, yy = x - c, y - c
+    ref = orc.apply_k(y, x, a, 0.7, 10) + 0.3 * yy @ (xx.T @ a) + 0.5 * a.sum(axis=0)
+    assert rel_err(got, ref) < 1e-12
+    Ke = kfun.Kernel('gauss', affine='euclidean', sigma=0.7, w1=0.3, w2=0.5, center=c, precision='fp64')
+    got = Ke.applyK(x, a)
+    ref = orc.apply_k(x, x, a, 0.7, 10) + 0.5 * a.sum(axis=0)
+    for E in Ke.affine_basis:
+        xE = xx @ E.T
+        ref = ref + 0.3 * (xE * a).sum() * xE
+    assert rel_err(got, ref) < 1e-12
+
+
+def test_large_rows_sampled_against_oracle():
+    """Full-size sum (40,962 points, the large-surface deformation kernel) checked on a row
+    sample against the oracle, and FP32-compensated against FP64 on every row."""
+    import torch
+    from registration_amd import synthetic
+    from registration_amd.kernelFunctions import apply_k_dev
+    V, _ = synthetic.icosphere(6)
+    rng = np.random.default_rng(6)
+    a = 1e-2 * rng.standard_normal(V.shape)
+    xd = torch.as_tensor(V, device="cuda")
+    ad = torch.as_tensor(a, device="cuda")
+    z32 = apply_k_dev(xd, xd, ad, kernel(10, 0.25, "fp32c")).cpu().numpy()
+    z64 = apply_k_dev(xd, xd, ad, kernel(10, 0.25, "fp64")).cpu().numpy()
+    rows = rng.choice(V.shape[0], 256, replace=False)
+    ref = orc.apply_k(V[rows], V, a, 0.25, 10)
+    assert rel_err(z64[rows], ref) < 1e-12
+    assert rel_err(z32[rows], ref) < TOL["fp32c"]
+    assert rel_err(z32, z64) < TOL["fp32c"]
