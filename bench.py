@@ -16,6 +16,7 @@ bounded sample of the same sums on this host.
 from __future__ import annotations
 
 import argparse
+import collections
 import json
 import os
 import sys
@@ -37,6 +38,12 @@ def parse():
     ap.add_argument("--workload", default="surface-s6", choices=["surface-s6", "surface-s3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=20)
+    ap.add_argument("--served", type=int, default=32,
+                    help="requests of the served-surfaces stream to serve (0 = skip)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the flow-step launches and a calibration copy (for rocprof passes)")
+    ap.add_argument("--plain-compare", type=int, default=8,
+                    help="of those, how many also run one at a time on the plain path")
     return ap.parse_args()
 
 
@@ -128,6 +135,48 @@ def cpu_baseline(cfg, pair_counts, seconds_target=10.0):
             "seconds_per_registration": secs}
 
 
+def served_leg(n, n_plain, prec):
+    """A prefix of the served-surfaces stream (512 pairs, icosphere s3-s6 log-uniform in
+    vertices, random rotations, RBF targets, sigma_V ~ U[0.2, 0.5], T 10, 30 CG iterations,
+    Poisson arrivals at 50/s) through the continuous-batching service, and its first n_plain
+    requests one at a time through the plain path."""
+    import numpy as np
+    import torch
+    from registration_amd import serving, synthetic
+    specs = synthetic.served_surface_requests(512)[:n]
+    arr = serving.poisson_arrivals(512, 50.0, seed=4)[:n]
+    arr = arr - arr[0]
+    reqs = [serving.Request(rid=i, kind='surface', template=c["template"], target=c["target"],
+                            sigmaKernel=c["sigmaKernel"], sigmaDist=c["sigmaDist"],
+                            sigmaError=c["sigmaError"], T=c["T"], maxIter=c["maxIter"],
+                            arrival=float(arr[i])) for i, c in enumerate(specs)]
+    # plain: one registration at a time, back to back
+    plain = []
+    t0 = time.perf_counter()
+    for r in reqs[:n_plain]:
+        plain.append(serving.plain_run(r, prec))
+    plain_wall = time.perf_counter() - t0
+    srv = serving.Server(hbm_cap_bytes=200e9, precision=prec)
+    for r in reqs:
+        srv.submit(r)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = srv.run()
+    wall = time.perf_counter() - t0
+    lat = np.array([res[i].latency for i in range(n)]) * 1e3
+    same = all(res[i].history == plain[i].history for i in range(n_plain))
+    sizes = collections.Counter(c["subdiv"] for c in specs)
+    return {"requests": n, "wall_s": wall, "registrations_per_s": n / wall,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+            "plain_requests": n_plain, "plain_registrations_per_s": n_plain / plain_wall,
+            "plain_latency_ms": {"p50": float(np.median([p.finished for p in plain]) * 1e3)},
+            "speedup_vs_plain": (n / wall) / (n_plain / plain_wall),
+            "identical_to_plain": same, "rounds": srv.rounds,
+            "subdiv_histogram": {str(k): v for k, v in sorted(sizes.items())},
+            "workload": "first %d requests of the served-surfaces stream (512 pairs, s3-s6, "
+                        "T 10, 30 CG, Poisson 50/s)" % n}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -140,6 +189,17 @@ def main():
     _lib.load()
     torch.cuda.set_device(0)
     cfg = make_workload(args.workload)
+
+    if args.roofline_only:
+        roof = flow_step_roofline(cfg, args.precision, args.roofline_reps)
+        # Calibration copy of a known size: 256 MiB read + 256 MiB written.
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"roofline": roof, "calibration_copy_bytes": 256 << 20}))
+        return 0
 
     for _ in range(args.warmup):
         m = registration(cfg, args.precision)
@@ -178,6 +238,9 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, {k: v / args.steps for k, v in counts.items()})
 
+    served = served_leg(args.served, min(args.plain_compare, args.served), args.precision) \
+        if args.served > 0 else None
+
     lat_ms = np.array(lat) * 1e3
     out = {
         "metric": "registrations/s (plain single-pair GPU path)",
@@ -205,6 +268,7 @@ def main():
         "pair_rate_overall": pairs_per_reg / (wall / args.steps),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "served": served,
     }
     print(json.dumps(out))
     return 0

@@ -126,14 +126,47 @@ struct KEval<float, LDDMM_KERNEL_LAPLACIAN> {
   }
 };
 
+// 2^t in double for t <= 0, written for the pair loops (all VALU, no library call):
+// t = n + f with n = rint(t) (exact: the 1.5 * 2^52 shift rounds to an integer), |f| <= 1/2;
+// 2^f = sum_k (f ln 2)^k / k! truncated after degree 13 (remainder < 4e-18 on |f| <= 1/2,
+// Horner rounding ~2 ulp); 2^n is added to the exponent field.  t is clamped at -1000
+// (2^-1000 ~ 9e-302), which keeps every result a normal double.
+struct Exp2Coef {
+  double c[14];
+  constexpr Exp2Coef() : c() {
+    const double ln2 = 0.69314718055994530942;
+    double v = 1.0;
+    for (int k = 0; k < 14; ++k) {
+      c[k] = v;
+      v = v * ln2 / (double)(k + 1);
+    }
+  }
+};
+constexpr Exp2Coef EXP2C{};
+
+__device__ inline double exp2_nonpos(double t) {
+  t = fmax(t, -1000.0);
+  const double shifter = 6755399441055744.0;  // 1.5 * 2^52
+  const double k = t + shifter;
+  const double n = k - shifter;
+  const double f = t - n;
+  double p = EXP2C.c[13];
+#pragma unroll
+  for (int i = 12; i >= 0; --i) p = fma(p, f, EXP2C.c[i]);
+  const int ni = __double2loint(k);
+  return __hiloint2double(__double2hiint(p) + (ni << 20), __double2loint(p));
+}
+
 template <>
 struct KEval<double, LDDMM_KERNEL_GAUSS> {
-  double inv_s2;
-  __device__ void init(int, double sigma) { inv_s2 = 1.0 / (sigma * sigma); }
+  double h, cut_r2;
+  __device__ void init(int, double sigma) {
+    h = LOG2E / (2.0 * sigma * sigma);  // exp(-r^2/(2 sigma^2)) = 2^(-h r^2)
+    cut_r2 = CUTOFF * sigma * sigma;    // (r/sigma)^2 < 1e-8  <=>  r^2 < 1e-8 sigma^2
+  }
   __device__ inline void operator()(double r2, double &k0, double &k1, double &k2) const {
-    double ut = r2 * inv_s2;
-    double e = exp(-0.5 * ut);
-    double k = ut < CUTOFF ? 1.0 : e;
+    double e = exp2_nonpos(-h * r2);
+    double k = r2 < cut_r2 ? 1.0 : e;
     k0 = k;
     k1 = k;
     k2 = k;
@@ -152,7 +185,7 @@ struct KEval<double, LDDMM_KERNEL_LAPLACIAN> {
   }
   __device__ inline void operator()(double r2, double &k0, double &k1, double &k2) const {
     double u = sqrt(r2) * inv_s;
-    double e = exp(-u);
+    double e = exp2_nonpos(-LOG2E * u);
     bool cut = u < CUTOFF;
     double pk = (((p4 * u + p3) * u + p2) * u + p1) * u + p0;
     double qk = ((q3 * u + q2) * u + q1) * u + q0;

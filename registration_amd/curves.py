@@ -91,7 +91,7 @@ def currentNorm0(fv1, KparDist):
     """curves.py:539-543."""
     s = DeviceCurve(fv1.faces, fv1.vertices)
     tot = torch.empty(1, dtype=torch.float64, device=rt.device())
-    engine.knorm([engine.KNormJob(s.centers, s.lenel, _kernel(KparDist), tot)])
+    engine.knorm([engine.KNormJob(s.centers, s.lenel, _kernel(KparDist).energy(), tot)])
     return float(tot.item())
 
 
@@ -100,8 +100,8 @@ def currentNormDef(fvDef, fv1, KparDist):
     a = DeviceCurve(fvDef.faces, fvDef.vertices)
     b = DeviceCurve(fv1.faces, fv1.vertices)
     tot = torch.empty(1, dtype=torch.float64, device=rt.device())
-    engine.currents([engine.CurrentsJob(a.centers, a.lenel, b.centers, b.lenel, _kernel(KparDist),
-                                        tot, coef=0.5)])
+    engine.currents([engine.CurrentsJob(a.centers, a.lenel, b.centers, b.lenel,
+                                        _kernel(KparDist).energy(), tot, coef=0.5)])
     return float(tot.item())
 
 

@@ -45,6 +45,13 @@ class DevKernel:
     def family(self):
         return (self.kernel, self.prec)
 
+    def energy(self) -> "DevKernel":
+        """The form used for data-term energies: always FP64 pairwise arithmetic.  A currents
+        norm is a small difference of large sums (|S|^2 - 2<S,T> + |T|^2), and per-pair FP32
+        rounding leaves an error near 1e-8 of the large sums, i.e. far above 1e-6 of the
+        objective once the shapes match."""
+        return DevKernel(self.kernel, self.order, self.sigma, _lib.PREC_FP64)
+
 
 def _groups(jobs, key):
     out = {}

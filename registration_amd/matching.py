@@ -9,8 +9,9 @@ them with a list of one, which is the plain single-pair path; the serving layer 
 with every registration waiting on the same request.
 
 Reference: py-lddmm/surfaceMatching.py:187-484 and curveMatching.py:148-352
-(affine='none', symmetric=False, errorType='current').  Differences that do not change any
-result: the trajectory of an accepted trial is kept instead of being recomputed in
+(affine='none', symmetric=False, errorType='current').  Data-term energies (currents norms)
+are always evaluated in FP64 pairwise arithmetic; the flow, the covector, the RKHS products and
+the data-term gradient use the kernels' precision.  Differences that do not change any result: the trajectory of an accepted trial is kept instead of being recomputed in
 endOfIteration / landmarkHamiltonianCovector (the recomputation is deterministic and
 bit-identical), and the energy sum reuses the flow's K(x_t) a_t instead of a second applyK.
 """
@@ -154,8 +155,8 @@ def batch_objective(opts: Sequence[DeviceMatching]) -> list:
         T = o.Tsize
         o.fvDef_dev.update(o.xt[-1])
         curs.append(engine.CurrentsJob(o.fvDef_dev.centers, o.fvDef_dev.surfel, o.c2, o.cr2,
-                                       o.Kdist, scal[off + T:off + T + 1], coef=o.coef))
-        norms.append(engine.KNormJob(o.c2, o.cr2, o.Kdist, scal[off + T + 1:off + T + 2]))
+                                       o.Kdist.energy(), scal[off + T:off + T + 1], coef=o.coef))
+        norms.append(engine.KNormJob(o.c2, o.cr2, o.Kdist.energy(), scal[off + T + 1:off + T + 2]))
     engine.currents(curs)
     engine.knorm(norms)
     vals = _readback(scal)
@@ -193,7 +194,7 @@ def batch_update_try(items) -> list:
         T = o.Tsize
         o._try_shape.update(o._xt_scratch[-1])
         curs.append(engine.CurrentsJob(o._try_shape.centers, o._try_shape.surfel, o.c2, o.cr2,
-                                       o.Kdist, scal[off + T:off + T + 1], coef=o.coef))
+                                       o.Kdist.energy(), scal[off + T:off + T + 1], coef=o.coef))
     engine.currents(curs)
     vals = _readback(scal)
     out = []

@@ -1,0 +1,236 @@
+"""Continuous-batching registration service (no counterpart in the reference, whose only
+multi-registration mode is a sequential loop over target files,
+py-lddmm/surfaceMatchingDirectory.py:46-55).
+
+Requests (surface pairs, curve pairs) arrive over time.  Each admitted request becomes a
+GPU-resident registration (matching.DeviceMatching) whose optimiser runs as a generator
+(conjugateGradient.cg_steps).  One scheduling round advances every active registration by one
+optimiser request: requests of the same kind (line-search trial, gradient, RKHS products,
+objective) are answered together by the batch_* functions, which issue one segmented launch
+per kernel-sum stage for all of them.  Registrations finish at different times; new arrivals
+are admitted as soon as device memory allows (HBM-footprint admission), so the batch is
+refilled continuously.
+
+A registration's result does not depend on its batch neighbours: every segment keeps its own
+chunking and reduction order, so its kernel sums, and therefore its optimiser path, are the
+same alone and in any mix.
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+import numpy as np
+import torch
+
+from . import conjugateGradient as cg
+from . import curveMatching as cm
+from . import curves
+from . import kernelFunctions as kfun
+from . import matching
+from . import surfaceMatching as sm
+from . import surfaces
+
+
+@dataclass
+class Request:
+    """One registration: ``kind`` 'surface' or 'curve'; template/target are (faces, vertices);
+    kernel names/widths as in SurfaceMatchingParam; ``arrival`` in seconds from service start."""
+
+    rid: int
+    kind: str
+    template: tuple
+    target: tuple
+    sigmaKernel: float
+    sigmaDist: float
+    sigmaError: float
+    T: int
+    maxIter: int
+    typeKernel: str = 'gauss'
+    order: int = 10
+    arrival: float = 0.0
+
+
+@dataclass
+class Result:
+    rid: int
+    energy: float
+    iterations: int
+    stop_reason: str
+    arrival: float
+    admitted: float
+    finished: float
+    history: list = field(default_factory=list)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def build_registration(req: Request, precision: str):
+    """The device-resident matching object of a request (same as the plain path builds)."""
+    Kd = kfun.Kernel(name=req.typeKernel, sigma=req.sigmaKernel, order=req.order, precision=precision)
+    Kw = kfun.Kernel(name='gauss', sigma=req.sigmaDist, precision=precision)
+    if req.kind == 'surface':
+        p = sm.SurfaceMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, KparDist=Kw,
+                                    sigmaError=req.sigmaError, errorType='current')
+        return sm.SurfaceMatching(Template=surfaces.Surface(FV=req.template),
+                                  Target=surfaces.Surface(FV=req.target), param=p,
+                                  maxIter=req.maxIter, testGradient=False, verb=False)
+    if req.kind == 'curve':
+        p = cm.CurveMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, KparDist=Kw,
+                                  sigmaError=req.sigmaError, errorType='current')
+        return cm.CurveMatching(Template=curves.Curve(FV=req.template),
+                                Target=curves.Curve(FV=req.target), param=p, maxIter=req.maxIter,
+                                verb=False)
+    raise ValueError(f"unknown request kind {req.kind!r}")
+
+
+def footprint_bytes(req: Request) -> int:
+    """Device memory of one active registration (float64): trajectories (3 buffers), momenta,
+    trial, direction, gradient and CG history (~8 (T,N,d) arrays), covector, and the currents
+    buffers of the deformed shapes."""
+    F, V = req.template
+    N, d = V.shape
+    nf = F.shape[0]
+    T = req.T
+    traj = 3 * (T + 1) * N * d
+    mom = 9 * T * N * d
+    cur = 10 * nf * d
+    return 8 * (traj + mom + cur) + 64 * 1024
+
+
+class _Active:
+    __slots__ = ("req", "opt", "gen", "pending", "admitted")
+
+    def __init__(self, req, opt, admitted):
+        self.req = req
+        self.opt = opt
+        self.gen = opt.cg_steps()
+        self.pending = next(self.gen)
+        self.admitted = admitted
+
+
+class Server:
+    """Continuous-batching service on the current GPU.
+
+    hbm_cap_bytes: admission cap on the summed footprint of active registrations;
+    max_active: cap on concurrently optimised registrations; precision: 'fp32c' or 'fp64'.
+    """
+
+    def __init__(self, hbm_cap_bytes: float = 200e9, max_active: int = 256,
+                 precision: str = 'fp32c', clock: Callable[[], float] = time.perf_counter):
+        self.cap = float(hbm_cap_bytes)
+        self.max_active = int(max_active)
+        self.precision = precision
+        self.clock = clock
+        self.queue: collections.deque = collections.deque()
+        self.active: list = []
+        self.results: dict = {}
+        self.used = 0.0
+        self.rounds = 0
+        self.answered = 0      # optimiser requests answered (summed over rounds)
+        self.max_batch = 0     # most registrations answered in one round
+        self.t0 = None
+
+    def submit(self, req: Request) -> None:
+        if footprint_bytes(req) > self.cap:
+            raise ValueError(f"request {req.rid} needs more device memory than the admission cap")
+        self.queue.append(req)
+
+    # ---------------------------------------------------------------------------------
+    def _now(self) -> float:
+        return self.clock() - self.t0
+
+    def _admit(self) -> None:
+        now = self._now()
+        while self.queue and self.queue[0].arrival <= now and len(self.active) < self.max_active:
+            req = self.queue[0]
+            need = footprint_bytes(req)
+            if self.used + need > self.cap and self.active:
+                break
+            self.queue.popleft()
+            self.used += need
+            self.active.append(_Active(req, build_registration(req, self.precision), now))
+
+    def _finish(self, a: _Active, value) -> None:
+        o = a.opt
+        self.results[a.req.rid] = Result(a.req.rid, float(o.obj), len(o.history) - 1, o.stop_reason,
+                                         a.req.arrival, a.admitted, self._now(), list(o.history))
+        self.used -= footprint_bytes(a.req)
+
+    def step(self) -> None:
+        """One scheduling round: answer every active registration's pending request."""
+        groups = collections.defaultdict(list)
+        for a in self.active:
+            groups[a.pending[0]].append(a)
+        answers = {}
+        if groups.get(cg.TRY):
+            items = [(a.opt, a.pending[1], a.pending[2], a.pending[3]) for a in groups[cg.TRY]]
+            for a, v in zip(groups[cg.TRY], matching.batch_update_try(items)):
+                answers[id(a)] = v
+        if groups.get(cg.GRADIENT):
+            items = [(a.opt, a.pending[1]) for a in groups[cg.GRADIENT]]
+            for a, v in zip(groups[cg.GRADIENT], matching.batch_gradient(items)):
+                answers[id(a)] = v
+        if groups.get(cg.DOT):
+            items = [(a.opt, a.pending[1], a.pending[2]) for a in groups[cg.DOT]]
+            for a, v in zip(groups[cg.DOT], matching.batch_dot(items)):
+                answers[id(a)] = v
+        if groups.get(cg.OBJECTIVE):
+            opts = [a.opt for a in groups[cg.OBJECTIVE]]
+            for a, v in zip(groups[cg.OBJECTIVE], matching.batch_objective(opts)):
+                answers[id(a)] = v
+        for kind in (cg.ACCEPT, cg.END_ITER):
+            for a in groups.get(kind, []):
+                answers[id(a)] = getattr(a.opt, a.pending[0])()
+        still = []
+        for a in self.active:
+            try:
+                a.pending = a.gen.send(answers[id(a)])
+                still.append(a)
+            except StopIteration as stop:
+                self._finish(a, stop.value)
+        self.answered += len(self.active)
+        self.max_batch = max(self.max_batch, len(self.active))
+        self.active = still
+        self.rounds += 1
+
+    def run(self, idle_sleep: float = 1e-4) -> dict:
+        """Serve until every submitted request is finished; returns {rid: Result}."""
+        if self.t0 is None:
+            self.t0 = self.clock()
+        while self.queue or self.active:
+            self._admit()
+            if not self.active:
+                wait = self.queue[0].arrival - self._now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.05) if wait > idle_sleep else idle_sleep)
+                continue
+            self.step()
+        torch.cuda.synchronize()
+        return self.results
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 0, burst_factor: float = 1.0,
+                     burst_prob: float = 0.0) -> np.ndarray:
+    """Arrival times of a Poisson process of ``rate`` requests/s; with burst_prob > 0 each gap
+    is shortened by ``burst_factor`` with that probability (bursty arrivals)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=n)
+    if burst_prob > 0:
+        gaps = np.where(rng.random(n) < burst_prob, gaps / burst_factor, gaps)
+    return np.cumsum(gaps)
+
+
+def plain_run(req: Request, precision: str) -> Result:
+    """One request through the plain single-registration path (reference call pattern)."""
+    t = time.perf_counter()
+    opt = build_registration(req, precision)
+    opt.optimizeMatching()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return Result(req.rid, float(opt.obj), len(opt.history) - 1, opt.stop_reason, 0.0, 0.0,
+                  t1 - t, list(opt.history))

@@ -96,7 +96,7 @@ def currentNorm0(fv1, KparDist):
     """<surfel, K(c, c) surfel> of fv1 (surfaces.py:1009-1016)."""
     s = _dev_surface(fv1)
     tot = torch.empty(1, dtype=torch.float64, device=rt.device())
-    engine.knorm([engine.KNormJob(s.centers, s.surfel, _kernel(KparDist), tot)])
+    engine.knorm([engine.KNormJob(s.centers, s.surfel, _kernel(KparDist).energy(), tot)])
     return float(tot.item())
 
 
@@ -105,7 +105,7 @@ def currentNormDef(fvDef, fv1, KparDist):
     a, b = _dev_surface(fvDef), _dev_surface(fv1)
     tot = torch.empty(1, dtype=torch.float64, device=rt.device())
     engine.currents([engine.CurrentsJob(a.centers, a.surfel, b.centers, b.surfel,
-                                        _kernel(KparDist), tot)])
+                                        _kernel(KparDist).energy(), tot)])
     return float(tot.item())
 
 

@@ -106,6 +106,7 @@ def test_gradient_against_central_differences():
     h = 1e-4
     for s in range(8):
         v = np.random.default_rng(100 + s).standard_normal((T, N, 3))
+        v /= np.linalg.norm(v)  # unit direction: h is the perturbation length
         fd = (J(a + h * v) - J(a - h * v)) / (2 * h)
         # Euclidean gradient of J wrt a_t is dt K(x_t) dat_t
         an = sum((v[t] * K.applyK(xt[t], dat[t])).sum() / T for t in range(T))
