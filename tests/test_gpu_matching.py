@@ -43,16 +43,43 @@ def test_surface_requests_match_oracle(prec, tol):
     assert abs(t1 - t2) < tol * abs(t2)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32c", 1e-6)])
-def test_surface_cg_sequence(prec, tol):
-    g, o = surface_pair(prec, subdiv=2, T=5)
+def test_surface_cg_sequence_fp64():
+    g, o = surface_pair("fp64", subdiv=2, T=5)
     g.maxIter = o.maxIter = 8
     g.optimizeMatching()
     o.optimizeMatching()
     hg, ho = np.array(g.history), np.array(o.history)
+    assert len(hg) == len(ho) and len(hg) >= 5
+    assert np.all(np.abs(hg - ho) <= 1e-9 * np.abs(ho)), (hg, ho)
+
+
+def test_surface_cg_fp32c_energies_and_path():
+    """FP32-compensated run: (a) the energy of every accepted iterate equals the FP64 energy of
+    the same momenta within 1e-6 relative (the precision claim); (b) the iterate path stays
+    within 1e-4 of the FP64 oracle's (FP32 gradients change the search directions slightly,
+    so the paths are not identical)."""
+    import torch
+    g, o = surface_pair("fp32c", subdiv=2, T=5)
+    g.maxIter = o.maxIter = 8
+    stash = []
+    accept = g.acceptVarTry
+
+    def spy():
+        accept()
+        stash.append((g.at.clone(), g.obj))
+    g.acceptVarTry = spy
+    g.optimizeMatching()
+    o.optimizeMatching()
+    g64, _ = surface_pair("fp64", subdiv=2, T=5)
+    for at, e in stash:
+        g64.at = at
+        g64.obj = None
+        e64 = g64.objectiveFun()
+        assert abs(e - e64) <= 1e-6 * abs(e64), (e, e64)
+    hg, ho = np.array(g.history), np.array(o.history)
     n = min(len(hg), len(ho))
     assert n >= 5
-    assert np.all(np.abs(hg[:n] - ho[:n]) <= tol * np.abs(ho[:n])), (hg[:n], ho[:n])
+    assert np.all(np.abs(hg[:n] - ho[:n]) <= 1e-4 * np.abs(ho[:n])), (hg[:n], ho[:n])
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32c", 1e-6)])
