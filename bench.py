@@ -38,12 +38,10 @@ def parse():
     ap.add_argument("--workload", default="surface-s6", choices=["surface-s6", "surface-s3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=20)
-    ap.add_argument("--served", type=int, default=32,
+    ap.add_argument("--served", type=int, default=16,
                     help="requests of the served-surfaces stream to serve (0 = skip)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the flow-step launches and a calibration copy (for rocprof passes)")
-    ap.add_argument("--plain-compare", type=int, default=8,
-                    help="of those, how many also run one at a time on the plain path")
     return ap.parse_args()
 
 
@@ -135,26 +133,15 @@ def cpu_baseline(cfg, pair_counts, seconds_target=10.0):
             "seconds_per_registration": secs}
 
 
-def served_leg(n, n_plain, prec):
-    """A prefix of the served-surfaces stream (512 pairs, icosphere s3-s6 log-uniform in
-    vertices, random rotations, RBF targets, sigma_V ~ U[0.2, 0.5], T 10, 30 CG iterations,
-    Poisson arrivals at 50/s) through the continuous-batching service, and its first n_plain
-    requests one at a time through the plain path."""
+def _served_compare(reqs, prec, label):
+    """The same requests one at a time on the plain path, then together through the
+    continuous-batching service (arrival times as given)."""
     import numpy as np
     import torch
-    from registration_amd import serving, synthetic
-    specs = synthetic.served_surface_requests(512)[:n]
-    arr = serving.poisson_arrivals(512, 50.0, seed=4)[:n]
-    arr = arr - arr[0]
-    reqs = [serving.Request(rid=i, kind='surface', template=c["template"], target=c["target"],
-                            sigmaKernel=c["sigmaKernel"], sigmaDist=c["sigmaDist"],
-                            sigmaError=c["sigmaError"], T=c["T"], maxIter=c["maxIter"],
-                            arrival=float(arr[i])) for i, c in enumerate(specs)]
-    # plain: one registration at a time, back to back
-    plain = []
+    from registration_amd import serving
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for r in reqs[:n_plain]:
-        plain.append(serving.plain_run(r, prec))
+    plain = [serving.plain_run(r, prec) for r in reqs]
     plain_wall = time.perf_counter() - t0
     srv = serving.Server(hbm_cap_bytes=200e9, precision=prec)
     for r in reqs:
@@ -163,18 +150,43 @@ def served_leg(n, n_plain, prec):
     t0 = time.perf_counter()
     res = srv.run()
     wall = time.perf_counter() - t0
-    lat = np.array([res[i].latency for i in range(n)]) * 1e3
-    same = all(res[i].history == plain[i].history for i in range(n_plain))
-    sizes = collections.Counter(c["subdiv"] for c in specs)
-    return {"requests": n, "wall_s": wall, "registrations_per_s": n / wall,
-            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-            "plain_requests": n_plain, "plain_registrations_per_s": n_plain / plain_wall,
-            "plain_latency_ms": {"p50": float(np.median([p.finished for p in plain]) * 1e3)},
-            "speedup_vs_plain": (n / wall) / (n_plain / plain_wall),
-            "identical_to_plain": same, "rounds": srv.rounds,
-            "subdiv_histogram": {str(k): v for k, v in sorted(sizes.items())},
-            "workload": "first %d requests of the served-surfaces stream (512 pairs, s3-s6, "
-                        "T 10, 30 CG, Poisson 50/s)" % n}
+    n = len(reqs)
+    lat = np.array([res[r.rid].latency for r in reqs]) * 1e3
+    plat = np.array([p.finished for p in plain]) * 1e3
+    same = all(res[r.rid].history == p.history for r, p in zip(reqs, plain))
+    return {"workload": label, "requests": n,
+            "served_registrations_per_s": n / wall, "served_wall_s": wall,
+            "served_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+            "plain_registrations_per_s": n / plain_wall, "plain_wall_s": plain_wall,
+            "plain_latency_ms": {"p50": float(np.percentile(plat, 50)), "p99": float(np.percentile(plat, 99))},
+            "speedup_served_vs_plain": plain_wall / wall,
+            "identical_results": same, "rounds": srv.rounds,
+            "mean_batch": srv.answered / max(srv.rounds, 1), "max_batch": srv.max_batch}
+
+
+def served_leg(n, prec):
+    """(1) The first n requests of the served-surfaces stream (512 pairs, icosphere s3-s6
+    log-uniform in vertices, random rotations, RBF targets, sigma_V ~ U[0.2, 0.5], T 10, 30 CG
+    iterations, Poisson arrivals at 50/s); (2) 4n small surface pairs (icosphere s3, otherwise
+    drawn the same way), all arriving at once."""
+    import collections as _c
+    from registration_amd import serving, synthetic
+    specs = synthetic.served_surface_requests(512)[:n]
+    arr = serving.poisson_arrivals(512, 50.0, seed=4)[:n]
+    arr = arr - arr[0]
+    mk = lambda i, c, a: serving.Request(rid=i, kind='surface', template=c["template"], target=c["target"],
+                                         sigmaKernel=c["sigmaKernel"], sigmaDist=c["sigmaDist"],
+                                         sigmaError=c["sigmaError"], T=c["T"], maxIter=c["maxIter"],
+                                         arrival=float(a))
+    mixed = _served_compare([mk(i, c, arr[i]) for i, c in enumerate(specs)], prec,
+                            "first %d requests of the served-surfaces stream (s3-s6, T 10, 30 CG, "
+                            "Poisson 50/s)" % n)
+    mixed["subdiv_histogram"] = {str(k): v for k, v in sorted(_c.Counter(c["subdiv"] for c in specs).items())}
+    small_specs = synthetic.served_surface_requests(4 * n, seed=5, subdivs=(3,))
+    small = _served_compare([mk(i, c, 0.0) for i, c in enumerate(small_specs)], prec,
+                            "%d small surface pairs (icosphere s3, 642 vertices; T 10, 30 CG), "
+                            "all arriving at t=0" % (4 * n))
+    return {"mixed": mixed, "small": small}
 
 
 def main():
@@ -238,8 +250,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, {k: v / args.steps for k, v in counts.items()})
 
-    served = served_leg(args.served, min(args.plain_compare, args.served), args.precision) \
-        if args.served > 0 else None
+    served = served_leg(args.served, args.precision) if args.served > 0 else None
 
     lat_ms = np.array(lat) * 1e3
     out = {
@@ -253,7 +264,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32c" if args.precision == "fp32c" else "f64",
+        "dtype": "f32" if args.precision == "fp32c" else "f64",
         "data": "synthetic (seeded icosphere + RBF target)",
         "config": {"workload": "single large surface pair: icosphere s6 (40962 V / 81920 F), "
                                "sigma_V 0.25, sigma_W 0.1, T 20, <=100 CG iterations"

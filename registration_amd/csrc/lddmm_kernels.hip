@@ -31,6 +31,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <stdlib.h>
+
 #include <type_traits>
 
 #include "lddmm.h"
@@ -38,7 +40,7 @@
 namespace {
 
 constexpr int BLOCK = 256;       // threads per pair-sum workgroup
-constexpr int TS = 256;          // sources per LDS tile
+constexpr int TS = 256;          // sources per LDS tile (= BLOCK: one source per thread)
 constexpr int KF = 32;           // FP32 Kahan fold interval (sources)
 constexpr int FIN_BLOCK = 256;   // rows per finalize workgroup
 constexpr int TOT_BLOCK = 256;   // threads of the totals kernel
@@ -197,6 +199,38 @@ struct KEval<double, LDDMM_KERNEL_LAPLACIAN> {
 };
 
 // ---------------------------------------------------------------------------
+// Tile culling.  A wave skips a staged source tile when the distance between
+// the bounding box of its target rows and the tile's bounding box makes every
+// kernel value of the tile smaller than EPS_K (2^-40 in FP32, 2^-80 in FP64),
+// below the accumulation precision of the row sums.  For the Laplacian the
+// bound uses P, Q, R <= (1+u)^4, and (1+u)^4 e^-u < EPS_K for u > U_CUT.
+// Returns the squared cut radius in the kernel's coordinates (prescaled in
+// FP32, raw in FP64).
+// ---------------------------------------------------------------------------
+__device__ inline double cull_r2(int prec, int kernel, double sigma) {
+  const bool f32 = prec == LDDMM_PREC_FP32C;
+  if (kernel == LDDMM_KERNEL_GAUSS) {
+    // FP32: K = 2^-r'^2 with prescaled r'.  FP64: K = 2^(-h r^2), h = log2(e)/(2 sigma^2).
+    return f32 ? 40.0 : 80.0 * 2.0 * sigma * sigma / LOG2E;
+  }
+  const double u_cut = f32 ? 43.0 : 73.0;  // (1+u)^4 e^-u = 2^-40 at u~42.9, 2^-80 at u~72.7
+  return f32 ? u_cut * u_cut : (u_cut * sigma) * (u_cut * sigma);
+}
+
+template <typename R>
+__device__ inline R wave_min(R v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <typename R>
+__device__ inline R wave_max(R v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
 // Operand resolution per operation (uniform per workgroup).
 // ---------------------------------------------------------------------------
 struct SrcView {
@@ -314,8 +348,11 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
 
   __shared__ R s_pos[D][TS];
   __shared__ R s_w[NSW][M][TS];
+  __shared__ R s_box[TS / 64][2 * D];  // bounding box of each 64-source quarter
 
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   const int64_t item = blockIdx.x;
   const int si = find_segment(segs, nseg, item, 0);
   const lddmm_seg_t *sg = segs + si;
@@ -336,17 +373,23 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   KEval<R, KT> kev;
   kev.init(sg->order, sg->sigma);
 
-  // Target rows held in registers.
+  // Target rows held in registers; each wave owns RPT x 64 consecutive rows.
+  // Positions stay in FP64 (yd, prescaled); FP32 mode rounds them per tile,
+  // after moving the origin to the tile (see the tile loop).
+  double yd[RPT][D];
   R ty[RPT][D];
   R tw[RPT][NA > 0 ? NA : 1][M];
   int64_t rows[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    int64_t k = (int64_t)rt * (BLOCK * RPT) + r * BLOCK + tid;
+    int64_t k = (int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT) + r * 64 + lane;
     rows[r] = k;
     int64_t kk = k < ny ? k : ny - 1;
 #pragma unroll
-    for (int c = 0; c < D; ++c) ty[r][c] = (R)(tgt.pos[kk * D + c] * scale);
+    for (int c = 0; c < D; ++c) {
+      yd[r][c] = tgt.pos[kk * D + c] * scale;
+      ty[r][c] = (R)yd[r][c];
+    }
     if constexpr (NA > 0) {
 #pragma unroll
       for (int j = 0; j < NA; ++j) {
@@ -359,6 +402,19 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
       }
     }
   }
+
+  // Bounding box of this wave's target rows (clamped duplicates are inside it),
+  // FP64, prescaled absolute coordinates.
+  double tlo[D], thi[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    double lo = yd[0][c], hi = yd[0][c];
+#pragma unroll
+    for (int r = 1; r < RPT; ++r) { lo = fmin(lo, yd[r][c]); hi = fmax(hi, yd[r][c]); }
+    tlo[c] = wave_min(lo);
+    thi[c] = wave_max(hi);
+  }
+  const R cut2 = (R)cull_r2(prec, KT, sg->sigma);
 
   // Accumulators: running sum (s), Kahan compensation (c).
   R sK[RPT][KSUM ? M : 1], cK[RPT][KSUM ? M : 1];
@@ -376,12 +432,30 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   if (l1 > src.n) l1 = src.n;
 
   for (int64_t base = l0; base < l1; base += TS) {
+    // Tile origin: FP32 mode evaluates the tile in a frame centred at its first
+    // source, so rounding to FP32 costs relative precision of the local
+    // distances, not of the absolute coordinates.  FP64 keeps the raw frame.
+    double org[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) org[c] = KAHAN ? src.pos[base * D + c] * scale : 0.0;
+    if constexpr (KAHAN) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) ty[r][c] = (R)(yd[r][c] - org[c]);
+    }
     __syncthreads();
     {
       int64_t l = base + tid;
+      R lo[D], hi[D];
       if (l < l1) {
 #pragma unroll
-        for (int c = 0; c < D; ++c) s_pos[c][tid] = (R)(src.pos[l * D + c] * scale);
+        for (int c = 0; c < D; ++c) {
+          R v = (R)(src.pos[l * D + c] * scale - org[c]);
+          s_pos[c][tid] = v;
+          lo[c] = v;
+          hi[c] = v;
+        }
 #pragma unroll
         for (int j = 0; j < NSW; ++j)
 #pragma unroll
@@ -389,16 +463,48 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
       } else {
         // Padding sources carry zero weight: their terms are exact zeros.
 #pragma unroll
-        for (int c = 0; c < D; ++c) s_pos[c][tid] = (R)0;
+        for (int c = 0; c < D; ++c) {
+          s_pos[c][tid] = (R)0;
+          lo[c] = (R)INFINITY;
+          hi[c] = (R)-INFINITY;
+        }
 #pragma unroll
         for (int j = 0; j < NSW; ++j)
 #pragma unroll
           for (int c = 0; c < M; ++c) s_w[j][c][tid] = (R)0;
       }
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        lo[c] = wave_min(lo[c]);
+        hi[c] = wave_max(hi[c]);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          s_box[wave][c] = lo[c];
+          s_box[wave][D + c] = hi[c];
+        }
+      }
     }
     __syncthreads();
+    // Which 64-source quarters of the tile hold a pair above the cut for this
+    // wave (squared distance between the target box and each quarter's box).
+    unsigned live = 0;
+#pragma unroll
+    for (int q = 0; q < TS / 64; ++q) {
+      R d2 = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        R g = fmax(fmax((R)(tlo[c] - org[c]) - s_box[q][D + c], s_box[q][c] - (R)(thi[c] - org[c])),
+                   (R)0);
+        d2 += g * g;
+      }
+      if (!(d2 > cut2)) live |= 1u << q;
+    }
+    if (!live) continue;  // wave-uniform: every pair of this tile is negligible
 
     for (int j0 = 0; j0 < TS; j0 += KF) {
+      if (!((live >> (j0 / 64)) & 1u)) continue;
       R tK[RPT][KSUM ? M : 1], tD[RPT][D];
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
@@ -492,6 +598,142 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
     if constexpr (NA > 0) {
 #pragma unroll
       for (int c = 0; c < D; ++c) o[q++] = (double)sD[r][c] - (double)cD[r][c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 K-weighted sums with the accumulation on the matrix cores.
+//
+// v_mfma_f32_4x4x1f32 runs 16 independent 4x4x1 products per wave: for block
+// b = lane/4, lane 4b+i supplies row i of A, lane 4b+j supplies column j of B,
+// and lane 4b+j's result register v holds D[b][v][j] (layout checked on
+// gfx950).  Each lane owns one target row; per source l it computes
+// A = K(y_lane, x_l) on the VALU and supplies B = w_l[lane % 4] (weight column
+// j, zero for j >= M), so one MFMA per source and row set accumulates
+// D[b][v][j] += K(y_{4b+v}, x_l) w_l[j]: the three accumulation FMAs per pair
+// leave the VALU.  Runs of KF sources accumulate in D and are folded into
+// Kahan (sum, compensation) pairs exactly as in k_pairs.
+// ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int KT, int D, int M>
+__global__ __launch_bounds__(BLOCK) void k_pairs_ksum_mfma(const lddmm_seg_t *__restrict__ segs,
+                                                          int nseg, int op, int prec,
+                                                          double *__restrict__ part) {
+  constexpr int RPT = 2;  // rows per lane in this kernel (independent MFMA chains)
+  constexpr int W = M;
+  static_assert(M >= 1 && M <= 4, "MFMA K-sum handles up to 4 weight columns");
+  __shared__ float s_pos[D][TS];
+  __shared__ float s_wt[4][TS];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int jcol = lane & 3;
+  const int64_t item = blockIdx.x;
+  const int si = find_segment(segs, nseg, item, 0);
+  const lddmm_seg_t *sg = segs + si;
+
+  const int64_t ny = sg->ny;
+  const int row_tiles = sg->row_tiles;
+  const int64_t it = item - sg->item0;
+  const int rt = (int)(it % row_tiles);
+  const int cg = (int)(it / row_tiles);
+  const int set = cg < sg->chunks0 ? 0 : 1;
+  const int ch = set ? cg - sg->chunks0 : cg;
+  const int ld = weight_ld(op, sg, D);
+  const SrcView src = resolve_src(op, sg, set);
+  const double scale = coord_scale(prec, KT, sg->sigma);
+  KEval<float, KT> kev;
+  kev.init(sg->order, sg->sigma);
+
+  // Each wave owns RPT x 64 consecutive target rows.
+  const int64_t wbase = (int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT);
+  float ty[RPT][D];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = wbase + r * 64 + lane;
+    int64_t kk = k < ny ? k : ny - 1;
+#pragma unroll
+    for (int c = 0; c < D; ++c) ty[r][c] = (float)(sg->y[kk * D + c] * scale);
+  }
+  float sK[RPT][4], cK[RPT][4];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { sK[r][v] = 0.f; cK[r][v] = 0.f; }
+
+  const int64_t l0 = (int64_t)ch * sg->chunk_len;
+  int64_t l1 = l0 + sg->chunk_len;
+  if (l1 > src.n) l1 = src.n;
+
+  for (int64_t base = l0; base < l1; base += TS) {
+    __syncthreads();
+    {
+      int64_t l = base + tid;
+      bool ok = l < l1;
+#pragma unroll
+      for (int c = 0; c < D; ++c) s_pos[c][tid] = ok ? (float)(src.pos[l * D + c] * scale) : 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        s_wt[q][tid] = (ok && q < M) ? (float)(src.sign * src.w[0][l * ld + q]) : 0.f;
+    }
+    __syncthreads();
+    for (int j0 = 0; j0 < TS; j0 += KF) {
+      v4f acc[RPT];
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) acc[r] = (v4f){0.f, 0.f, 0.f, 0.f};
+      // Kernel values of 4 sources x RPT rows first, then their MFMAs: the
+      // matrix core never waits on a value the VALU just produced.
+      for (int j = j0; j < j0 + KF; j += 4) {
+        float kv[4][RPT], wb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          float sx[D];
+#pragma unroll
+          for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j + u];
+          wb[u] = s_wt[jcol][j + u];
+#pragma unroll
+          for (int r = 0; r < RPT; ++r) {
+            float r2 = 0.f;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+              float dx = ty[r][c] - sx[c];
+              r2 = dx * dx + r2;
+            }
+            float k1, k2;
+            kev(r2, kv[u][r], k1, k2);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int r = 0; r < RPT; ++r)
+            acc[r] = __builtin_amdgcn_mfma_f32_4x4x1f32(kv[u][r], wb[u], acc[r], 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float y = acc[r][v] - cK[r][v];
+          float t = sK[r][v] + y;
+          cK[r][v] = (t - sK[r][v]) - y;
+          sK[r][v] = t;
+        }
+    }
+  }
+
+  // Lane 4b+j holds column j of rows 4b+v (v = 0..3) of each row set.
+  if (jcol < M) {
+    double *pp = part + sg->part0 + (int64_t)cg * ny * W;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int64_t k = wbase + r * 64 + (lane & ~3) + v;
+        if (k < ny) pp[k * W + jcol] = (double)sK[r][v] - (double)cK[r][v];
+      }
     }
   }
 }
@@ -1067,9 +1309,25 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
 
 namespace {
 
+// LDDMM_MFMA=1 selects the K-sum kernel that accumulates on the matrix cores
+// (same sums up to FP32 accumulation rounding).  It issues ~25% fewer VALU
+// instructions per pair but measured slower on MI355X (MFMA dependency
+// stalls), so the all-VALU kernel is the default.
+bool use_mfma() {
+  const char *e = getenv("LDDMM_MFMA");
+  return e && e[0] == '1';
+}
+
 template <typename R, int KT, int D, int M, bool KSUM, int NA>
 void launch_pairs(const lddmm_seg_t *segs, int nseg, int op, int prec, int64_t n_items,
                   double *part, hipStream_t st) {
+  if constexpr (std::is_same<R, float>::value && KSUM && NA == 0) {
+    if (use_mfma()) {
+      hipLaunchKernelGGL((k_pairs_ksum_mfma<KT, D, M>), dim3((unsigned)n_items), dim3(BLOCK), 0,
+                         st, segs, nseg, op, prec, part);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_pairs<R, KT, D, M, KSUM, NA>), dim3((unsigned)n_items), dim3(BLOCK), 0,
                      st, segs, nseg, op, prec, part);
 }

@@ -11,6 +11,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import conjugateGradient as cg
+from . import engine
 from . import curves
 from . import matchingParam
 from .matching import DeviceMatching, Direction  # noqa: F401
@@ -65,8 +66,8 @@ class CurveMatching(DeviceMatching):
     def _new_shape(self, vertices):
         return curves.DeviceCurve(self.faces_np, vertices)
 
-    def _gradient_into(self, shape, scale, px, total=None):
-        return curves.current_gradient_dev(shape, self.c2, self.cr2, self.Kdist, scale, px, total)
+    def _gather(self, shape, z1, dz, scale, px):
+        engine.curve_vertex_gradient(z1, dz, shape.ptr, shape.inc, shape.nv, scale, px)
 
     def endOptim(self):
         self.fvDef.updateVertices(self.xt[-1].cpu().numpy())

@@ -15,9 +15,10 @@ Reference counterparts:
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import Optional, Sequence
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -60,28 +61,21 @@ def _groups(jobs, key):
     return out.values()
 
 
-def _run_steps(op, jobs, d, n_steps, make_seg, step_active):
-    """Run ``n_steps`` dependent launches; launch t covers the jobs active at t.
-
-    Consecutive launches with the same active set share one plan (their sizes are equal), and
-    their descriptor arrays go up in one transfer.
-    """
+def _runs(jobs, n_steps, active):
+    """Maximal runs [t0, t1) of steps with one active set: (t0, t1, [job indices])."""
     t = 0
     while t < n_steps:
-        active = [j for j in jobs if step_active(j, t)]
+        idx = [i for i, j in enumerate(jobs) if active(j, t)]
         t1 = t + 1
-        while t1 < n_steps and all(step_active(j, t1) for j in active) and \
-                sum(1 for j in jobs if step_active(j, t1)) == len(active):
+        while t1 < n_steps and [i for i, j in enumerate(jobs) if active(j, t1)] == idx:
             t1 += 1
-        if active:
-            plan = rt.Plan(op, active[0].K.prec, d, [make_seg(j, t) for j in active])
-            if t1 - t == 1:
-                plan.run()
-            else:
-                steps = [[make_seg(j, s) for j in active] for s in range(t, t1)]
-                for p in rt.replicate(plan, steps):
-                    p.run()
+        if idx:
+            yield t, t1, idx
         t = t1
+
+
+def _u64(vals):
+    return np.asarray(vals, dtype=np.uint64)
 
 
 # --------------------------------------------------------------------------------------
@@ -96,6 +90,7 @@ class FlowJob:
     energy: torch.Tensor        # (T,) output: <a_t, K(x_t) a_t>
     v: Optional[torch.Tensor] = None  # (T, N, d) optional velocities K(x_t) a_t
     dt: Optional[float] = None          # time step; 1/T when None
+    init: bool = True                   # copy x0 into xt[0] (False: xt[0] already holds x0)
 
     @property
     def T(self):
@@ -108,19 +103,33 @@ class FlowJob:
 
 def shoot(jobs: Sequence[FlowJob]) -> None:
     """xt[t+1] = xt[t] + dt K(xt[t]) at[t], energy[t] = <at[t], K(xt[t]) at[t]>, dt = 1/T."""
+    init = [j for j in jobs if j.init]
+    if init:
+        torch._foreach_copy_([j.xt[0] for j in init], [j.x0 for j in init])
     for group in _groups(jobs, lambda j: (j.K.family, j.x0.shape[1])):
         d = group[0].x0.shape[1]
-        for j in group:
-            j.xt[0].copy_(j.x0)
-
-        def make(j, t):
-            return rt.seg(y=j.xt[t], w0=j.at[t], out0=j.xt[t + 1],
-                          out1=(j.v[t] if j.v is not None else None),
-                          total=j.energy[t:t + 1], ny=j.x0.shape[0], sigma=j.K.sigma,
-                          dt=j.step, kernel=j.K.kernel, order=j.K.order)
-
-        _run_steps(_lib.OP_FLOW_STEP, group, d, max(j.T for j in group), make,
-                   lambda j, t: t < j.T)
+        prec = group[0].K.prec
+        for t0, t1, idx in _runs(group, max(j.T for j in group), lambda j, t: t < j.T):
+            g = [group[i] for i in idx]
+            base = rt.segs(len(g))
+            base["ny"] = [j.xt.shape[1] for j in g]
+            base["sigma"] = [j.K.sigma for j in g]
+            base["dt"] = [j.step for j in g]
+            base["kernel"] = [j.K.kernel for j in g]
+            base["order"] = [j.K.order for j in g]
+            stride = _u64([j.xt.shape[1] * d * 8 for j in g])
+            xt = _u64([j.xt.data_ptr() for j in g])
+            at = _u64([j.at.data_ptr() for j in g])
+            en = _u64([j.energy.data_ptr() for j in g])
+            vv = _u64([rt.ptr(j.v) for j in g])
+            tt = np.arange(t0, t1, dtype=np.uint64)[:, None]
+            steps = np.repeat(base[None, :], t1 - t0, axis=0)
+            steps["y"] = xt + tt * stride
+            steps["w0"] = at + tt * stride
+            steps["out0"] = xt + (tt + 1) * stride
+            steps["out1"] = np.where(vv != 0, vv + tt * stride, 0)
+            steps["total"] = en + tt * 8
+            rt.run_steps(_lib.OP_FLOW_STEP, prec, d, steps)
 
 
 # --------------------------------------------------------------------------------------
@@ -147,20 +156,33 @@ class CovectorJob:
 
 def covector(jobs: Sequence[CovectorJob]) -> None:
     """pxt[T] = px1; pxt[t] = pxt[t+1] + dt * applyDiffKT(xt[t], [p, a, -2 lam a], [a, p, a])."""
+    torch._foreach_copy_([j.pxt[j.T] for j in jobs], [j.px1 for j in jobs])
     for group in _groups(jobs, lambda j: (j.K.family, j.xt.shape[2])):
         d = group[0].xt.shape[2]
-        for j in group:
-            j.pxt[j.T].copy_(j.px1)
-        maxT = max(j.T for j in group)
-
+        prec = group[0].K.prec
         # Backward in time: launch s handles t = T - 1 - s of every job with s < T.
-        def make(j, s):
-            t = j.T - 1 - s
-            return rt.seg(y=j.xt[t], w0=j.at[t], w1=j.pxt[t + 1], out0=j.pxt[t],
-                          ny=j.xt.shape[1], sigma=j.K.sigma, dt=j.step, coef=j.regweight,
-                          kernel=j.K.kernel, order=j.K.order)
-
-        _run_steps(_lib.OP_COVECTOR_STEP, group, d, maxT, make, lambda j, s: s < j.T)
+        for s0, s1, idx in _runs(group, max(j.T for j in group), lambda j, s: s < j.T):
+            g = [group[i] for i in idx]
+            base = rt.segs(len(g))
+            base["ny"] = [j.xt.shape[1] for j in g]
+            base["sigma"] = [j.K.sigma for j in g]
+            base["dt"] = [j.step for j in g]
+            base["coef"] = [j.regweight for j in g]
+            base["kernel"] = [j.K.kernel for j in g]
+            base["order"] = [j.K.order for j in g]
+            stride = _u64([j.xt.shape[1] * d * 8 for j in g])
+            T = _u64([j.T for j in g])
+            xt = _u64([j.xt.data_ptr() for j in g])
+            at = _u64([j.at.data_ptr() for j in g])
+            px = _u64([j.pxt.data_ptr() for j in g])
+            ss = np.arange(s0, s1, dtype=np.uint64)[:, None]
+            tt = T[None, :] - 1 - ss
+            steps = np.repeat(base[None, :], s1 - s0, axis=0)
+            steps["y"] = xt + tt * stride
+            steps["w0"] = at + tt * stride
+            steps["w1"] = px + (tt + 1) * stride
+            steps["out0"] = px + tt * stride
+            rt.run_steps(_lib.OP_COVECTOR_STEP, prec, d, steps)
 
 
 # --------------------------------------------------------------------------------------
@@ -176,17 +198,27 @@ class KDotJob:
 
 
 def kdot(jobs: Sequence[KDotJob]) -> None:
+    """One launch: every time step of every job is a segment."""
     for group in _groups(jobs, lambda j: (j.K.family, j.xt.shape[2])):
         d = group[0].xt.shape[2]
-        segs = []
+        parts = []
         for j in group:
-            T = j.g1.shape[0]
-            g2 = list(j.g2)
-            for t in range(T):
-                segs.append(rt.seg(y=j.xt[t], w0=j.g1[t], g=[g[t] for g in g2],
-                                   total=j.out[t], ny=j.xt.shape[1], sigma=j.K.sigma,
-                                   kernel=j.K.kernel, order=j.K.order, na=len(g2)))
-        rt.launch(_lib.OP_KDOT, group[0].K.prec, d, segs)
+            T, N = j.g1.shape[0], j.xt.shape[1]
+            r = rt.segs(T)
+            r["ny"] = N
+            r["sigma"] = j.K.sigma
+            r["kernel"] = j.K.kernel
+            r["order"] = j.K.order
+            r["na"] = len(j.g2)
+            stride = np.uint64(N * d * 8)
+            tt = np.arange(T, dtype=np.uint64)
+            r["y"] = np.uint64(j.xt.data_ptr()) + tt * stride
+            r["w0"] = np.uint64(j.g1.data_ptr()) + tt * stride
+            for q, g in enumerate(j.g2):
+                r["g"][:, q] = np.uint64(g.data_ptr()) + tt * stride
+            r["total"] = np.uint64(j.out.data_ptr()) + tt * np.uint64(16)
+            parts.append(r)
+        rt.launch(_lib.OP_KDOT, group[0].K.prec, d, np.concatenate(parts))
 
 
 # --------------------------------------------------------------------------------------
@@ -208,10 +240,21 @@ class CurrentsJob:
 def currents(jobs: Sequence[CurrentsJob]) -> None:
     for group in _groups(jobs, lambda j: (j.K.family, j.c1.shape[1], j.dz is not None)):
         d = group[0].c1.shape[1]
-        segs = [rt.seg(y=j.c1, w0=j.cr1, x1=j.c2, w1=j.cr2, out0=j.z1, out1=j.dz, total=j.total,
-                       ny=j.c1.shape[0], nx1=j.c2.shape[0], sigma=j.K.sigma, coef=j.coef,
-                       kernel=j.K.kernel, order=j.K.order) for j in group]
-        rt.launch(_lib.OP_CURRENTS, group[0].K.prec, d, segs)
+        r = rt.segs(len(group))
+        r["y"] = [j.c1.data_ptr() for j in group]
+        r["w0"] = [j.cr1.data_ptr() for j in group]
+        r["x1"] = [j.c2.data_ptr() for j in group]
+        r["w1"] = [j.cr2.data_ptr() for j in group]
+        r["out0"] = [rt.ptr(j.z1) for j in group]
+        r["out1"] = [rt.ptr(j.dz) for j in group]
+        r["total"] = [j.total.data_ptr() for j in group]
+        r["ny"] = [j.c1.shape[0] for j in group]
+        r["nx1"] = [j.c2.shape[0] for j in group]
+        r["sigma"] = [j.K.sigma for j in group]
+        r["coef"] = [j.coef for j in group]
+        r["kernel"] = [j.K.kernel for j in group]
+        r["order"] = [j.K.order for j in group]
+        rt.launch(_lib.OP_CURRENTS, group[0].K.prec, d, r)
 
 
 @dataclass
@@ -225,9 +268,15 @@ class KNormJob:
 def knorm(jobs: Sequence[KNormJob]) -> None:
     for group in _groups(jobs, lambda j: (j.K.family, j.c.shape[1])):
         d = group[0].c.shape[1]
-        segs = [rt.seg(y=j.c, w0=j.cr, total=j.total, ny=j.c.shape[0], sigma=j.K.sigma,
-                       kernel=j.K.kernel, order=j.K.order) for j in group]
-        rt.launch(_lib.OP_KNORM, group[0].K.prec, d, segs)
+        r = rt.segs(len(group))
+        r["y"] = [j.c.data_ptr() for j in group]
+        r["w0"] = [j.cr.data_ptr() for j in group]
+        r["total"] = [j.total.data_ptr() for j in group]
+        r["ny"] = [j.c.shape[0] for j in group]
+        r["sigma"] = [j.K.sigma for j in group]
+        r["kernel"] = [j.K.kernel for j in group]
+        r["order"] = [j.K.order for j in group]
+        rt.launch(_lib.OP_KNORM, group[0].K.prec, d, r)
 
 
 # --------------------------------------------------------------------------------------

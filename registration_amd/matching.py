@@ -60,9 +60,10 @@ class DeviceMatching:
         T, N, d = Tsize, self.npt, self.dim
         self.at = torch.zeros((T, N, d), dtype=torch.float64, device=dev)
         self.atTry = torch.zeros_like(self.at)
+        # Slice 0 of every trajectory buffer is x0 for the whole optimisation.
         self.xt = self.x0.expand(T + 1, N, d).clone()
-        self._xt_try = torch.empty_like(self.xt)
-        self._xt_scratch = torch.empty_like(self.xt)
+        self._xt_try = self.xt.clone()
+        self._xt_scratch = self.xt.clone()
         self.fvDef_dev = self._new_shape(self.x0)
         self._try_shape = self._new_shape(self.x0)
         self.obj = None
@@ -126,7 +127,8 @@ class DeviceMatching:
     def deformed_vertices(self) -> torch.Tensor:
         return self.xt[-1]
 
-    def _gradient_into(self, shape, scale, px, total=None):
+    def _gather(self, shape, z1, dz, scale, px):
+        """px = scale * (vertex gather of the per-face currents gradient)."""
         raise NotImplementedError
 
 
@@ -148,7 +150,7 @@ def batch_objective(opts: Sequence[DeviceMatching]) -> list:
     for o, L in zip(opts, lens):
         T = o.Tsize
         offs.append(off)
-        flows.append(engine.FlowJob(o.x0, o.at, o.Kdiff, o.xt, scal[off:off + T]))
+        flows.append(engine.FlowJob(o.x0, o.at, o.Kdiff, o.xt, scal[off:off + T], init=False))
         off += L
     engine.shoot(flows)
     for o, off in zip(opts, offs):
@@ -180,14 +182,15 @@ def batch_update_try(items) -> list:
     dev = rt.device()
     lens = [o.Tsize + 1 for (o, _, _, _) in items]
     scal = torch.zeros(sum(lens), dtype=torch.float64, device=dev)
-    flows, curs, ats, offs = [], [], [], []
+    # atTry = at - eps dir for every registration (multi-tensor launches)
+    ats = torch._foreach_mul([d.diff for (_, d, _, _) in items], [-float(e) for (_, _, e, _) in items])
+    torch._foreach_add_(ats, [o.at for (o, _, _, _) in items])
+    flows, curs, offs = [], [], []
     off = 0
-    for (o, dir, eps, _), L in zip(items, lens):
+    for (o, _, _, _), atTry, L in zip(items, ats, lens):
         T = o.Tsize
-        atTry = o.at - eps * dir.diff
-        ats.append(atTry)
         offs.append(off)
-        flows.append(engine.FlowJob(o.x0, atTry, o.Kdiff, o._xt_scratch, scal[off:off + T]))
+        flows.append(engine.FlowJob(o.x0, atTry, o.Kdiff, o._xt_scratch, scal[off:off + T], init=False))
         off += L
     engine.shoot(flows)
     for (o, _, _, _), off in zip(items, offs):
@@ -224,19 +227,34 @@ def batch_gradient(items) -> list:
     template, covector along the kept trajectory, dat = 2 lambda a - p, scaled by
     1/(coeff T)."""
     dev = rt.device()
+    # Data-term gradient of every registration: one currents launch, then vertex gathers.
+    curs, px1s, zs = [], [], []
+    for o, _ in items:
+        sh = o.fvDef_dev
+        F, d = sh.centers.shape
+        z1 = torch.empty((F, d), dtype=torch.float64, device=dev)
+        dz = torch.empty((F, d), dtype=torch.float64, device=dev)
+        zs.append((z1, dz))
+        curs.append(engine.CurrentsJob(sh.centers, sh.surfel, o.c2, o.cr2, o.Kdist,
+                                       torch.empty(1, dtype=torch.float64, device=dev),
+                                       z1=z1, dz=dz, coef=o.coef))
+    engine.currents(curs)
     covs, pxts = [], []
-    for o, coeff in items:
+    for (o, _), (z1, dz) in zip(items, zs):
         px1 = torch.empty((o.npt, o.dim), dtype=torch.float64, device=dev)
-        o._gradient_into(o.fvDef_dev, -2.0 / (o.sigmaError ** 2), px1)
+        o._gather(o.fvDef_dev, z1, dz, -2.0 / (o.sigmaError ** 2), px1)
         pxt = torch.empty_like(o.xt)
         pxts.append(pxt)
         covs.append(engine.CovectorJob(o.xt, o.at, px1, o.Kdiff, o.regweight, pxt))
     engine.covector(covs)
+    # dat = 2 lambda a - p_{t+1};  grd = dat / (coeff T)
+    dats = torch._foreach_mul([o.at for o, _ in items], [2 * o.regweight for o, _ in items])
+    torch._foreach_sub_(dats, [p[1:] for p in pxts])
+    torch._foreach_div_(dats, [coeff * o.Tsize for o, coeff in items])
     out = []
-    for (o, coeff), pxt in zip(items, pxts):
+    for (o, coeff), dat in zip(items, dats):
         g = Direction()
-        dat = 2 * o.regweight * o.at - pxt[1:]
-        g.diff = dat / (coeff * o.Tsize)
+        g.diff = dat
         g.aff = np.zeros(o.Afft.shape)
         o.ngrad += 1
         out.append(g)
@@ -246,18 +264,21 @@ def batch_gradient(items) -> list:
 def batch_dot(items) -> list:
     """dotProduct (surfaceMatching.py:353-377): sum_t <g2_t, K(x_t) g1_t> for 1..2 g2's."""
     dev = rt.device()
-    jobs, bufs = [], []
+    total = sum(o.Tsize for o, _, _ in items)
+    buf = torch.zeros((total, 2), dtype=torch.float64, device=dev)
+    jobs = []
+    pos = 0
     for o, g1, g2 in items:
-        buf = torch.zeros((o.Tsize, 2), dtype=torch.float64, device=dev)
-        bufs.append(buf)
-        jobs.append(engine.KDotJob(o.xt, g1.diff, [g.diff for g in g2], o.Kdiff, buf))
+        jobs.append(engine.KDotJob(o.xt, g1.diff, [g.diff for g in g2], o.Kdiff,
+                                   buf[pos:pos + o.Tsize]))
+        pos += o.Tsize
     engine.kdot(jobs)
-    allv = _readback(torch.cat([b.reshape(-1) for b in bufs])) if bufs else []
+    allv = _readback(buf)
     out = []
     pos = 0
-    for (o, g1, g2), buf in zip(items, bufs):
-        v = allv[pos:pos + buf.numel()].reshape(o.Tsize, 2)
-        pos += buf.numel()
+    for o, g1, g2 in items:
+        v = allv[pos:pos + o.Tsize]
+        pos += o.Tsize
         res = np.zeros(len(g2))
         for t in range(o.Tsize):
             for ll in range(len(g2)):

@@ -1,20 +1,30 @@
 """Device plumbing around the kernel library: workspace, streams, descriptor upload, launches.
 
 PyTorch-ROCm provides device memory and streams only; all arithmetic of the kernel sums runs
-in liblddmm_hip.so.  A launch is a list of ``Seg`` descriptors (one independent kernel sum
-each).  ``launch`` plans it on the host (C), uploads the planned descriptors through a
-pinned staging buffer and runs it on the current stream.
+in liblddmm_hip.so.  A launch is an array of ``lddmm_seg_t`` descriptors (one independent
+kernel sum each), held here as a NumPy structured array (``SEG_DT``) so that the descriptors
+of many registrations and time steps are built with vectorised pointer arithmetic.  ``Plan``
+plans a launch on the host (C), uploads the planned descriptors through pinned memory and
+runs it on the current stream; ``run_steps`` plans a sequence of same-size launches once and
+uploads all of their descriptors in one transfer.  Device buffers freed here are reused by
+the caching allocator only by later work on the same stream, after the launches that read
+them.
 """
 from __future__ import annotations
 
 import ctypes
-from typing import Sequence
 
 import numpy as np
 import torch
 
 from . import _lib
 from ._lib import Seg
+
+SEG_DT = np.dtype(Seg)
+SEG_SIZE = SEG_DT.itemsize
+PLAN_FIELDS = ("nx0", "nx1", "item0", "part0", "fin0", "red0", "row_tiles", "chunks0",
+               "chunks1", "chunk_len", "fin_blocks")
+_SEG_PTR = ctypes.POINTER(Seg)
 
 _workspaces: dict = {}
 
@@ -47,10 +57,12 @@ def workspace(nbytes: int) -> torch.Tensor:
     return ws
 
 
-def ptr(t) -> int | None:
+def ptr(t) -> int:
     if t is None:
-        return None
-    return t.data_ptr()
+        return 0
+    if isinstance(t, torch.Tensor):
+        return t.data_ptr()
+    return int(t)
 
 
 def as_dev(a, dtype=torch.float64) -> torch.Tensor:
@@ -61,101 +73,100 @@ def as_dev(a, dtype=torch.float64) -> torch.Tensor:
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev)
 
 
-def seg(**kw) -> Seg:
-    s = Seg()
+def segs(n: int) -> np.ndarray:
+    """n zeroed descriptors."""
+    return np.zeros(n, SEG_DT)
+
+
+def seg(**kw) -> np.ndarray:
+    """One descriptor (a length-1 record array) from keyword fields; tensors become pointers."""
+    r = segs(1)
     for k, v in kw.items():
         if k in ("tw", "sw", "g"):
-            arr = getattr(s, k)
             for i, t in enumerate(v):
-                arr[i] = ptr(t) if isinstance(t, torch.Tensor) else t
-        elif isinstance(v, torch.Tensor):
-            setattr(s, k, v.data_ptr())
+                r[k][0, i] = ptr(t)
+        elif isinstance(v, torch.Tensor) or v is None:
+            r[k] = ptr(v)
         else:
-            setattr(s, k, v)
-    return s
+            r[k] = v
+    return r
 
 
-def upload(arr) -> torch.Tensor:
-    """Copy a ctypes array to the device through pinned memory (async on the current stream)."""
-    nbytes = ctypes.sizeof(arr)
-    host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
-    ctypes.memmove(host.data_ptr(), ctypes.addressof(arr), nbytes)
+def _as_records(s) -> np.ndarray:
+    if isinstance(s, np.ndarray):
+        return np.ascontiguousarray(s.reshape(-1))
+    return np.ascontiguousarray(np.concatenate([np.asarray(x).reshape(-1) for x in s]))
+
+
+def upload(recs: np.ndarray) -> torch.Tensor:
+    """Copy descriptor records to the device through pinned memory (async on the current stream)."""
+    raw = np.ascontiguousarray(recs).view(np.uint8).reshape(-1)
+    host = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True)
+    host.numpy()[:] = raw
     return host.to(device(), non_blocking=True)
+
+
+def _count(op, recs):
+    if COUNT_PAIRS:
+        n = int((recs["ny"].astype(np.int64) * (recs["nx0"] + recs["nx1"])).sum())
+        pair_counts[op] = pair_counts.get(op, 0) + n
 
 
 class Plan:
     """A planned segmented launch (host descriptors + device copy + sizes)."""
 
-    __slots__ = ("op", "prec", "d", "segs", "nseg", "n_items", "n_fin", "work_bytes", "dev")
+    __slots__ = ("op", "prec", "d", "recs", "nseg", "n_items", "n_fin", "work_bytes", "dev")
 
-    def __init__(self, op: int, prec: int, d: int, segs: Sequence[Seg]):
+    def __init__(self, op: int, prec: int, d: int, recs):
         lib = _lib.load()
         self.op, self.prec, self.d = op, prec, d
-        self.nseg = len(segs)
-        self.segs = (Seg * self.nseg)(*segs)
+        self.recs = _as_records(recs)
+        self.nseg = self.recs.shape[0]
         n_items = ctypes.c_int64()
         n_fin = ctypes.c_int64()
         wb = ctypes.c_size_t()
-        _lib.check(lib.lddmm_segments_plan(op, prec, d, self.segs, self.nseg, ctypes.byref(n_items),
-                                           ctypes.byref(n_fin), ctypes.byref(wb)),
+        _lib.check(lib.lddmm_segments_plan(op, prec, d, self.recs.ctypes.data_as(_SEG_PTR), self.nseg,
+                                           ctypes.byref(n_items), ctypes.byref(n_fin), ctypes.byref(wb)),
                    "lddmm_segments_plan")
         self.n_items, self.n_fin, self.work_bytes = n_items.value, n_fin.value, wb.value
         self.dev = None
 
-    def run(self, dev_segs: torch.Tensor | None = None) -> None:
+    def run(self) -> None:
         lib = _lib.load()
         ws = workspace(self.work_bytes)
-        if dev_segs is None:
-            dev_segs = self.dev if self.dev is not None else upload(self.segs)
-        self.dev = dev_segs
-        if COUNT_PAIRS:
-            n = 0
-            for i in range(self.nseg):
-                sg = self.segs[i]
-                n += sg.ny * (sg.nx0 + sg.nx1)
-            pair_counts[self.op] = pair_counts.get(self.op, 0) + n
-        _lib.check(lib.lddmm_segments_run(self.op, self.prec, self.d, self.segs,
-                                          None if dev_segs is None else dev_segs.data_ptr(),
-                                          self.nseg, self.n_items, self.n_fin, ws.data_ptr(),
-                                          ws.numel(), stream_handle()),
+        if self.dev is None:
+            self.dev = upload(self.recs)
+        _count(self.op, self.recs)
+        _lib.check(lib.lddmm_segments_run(self.op, self.prec, self.d, self.recs.ctypes.data_as(_SEG_PTR),
+                                          self.dev.data_ptr(), self.nseg, self.n_items, self.n_fin,
+                                          ws.data_ptr(), ws.numel(), stream_handle()),
                    "lddmm_segments_run")
 
 
-def launch(op: int, prec: int, d: int, segs: Sequence[Seg]) -> Plan:
-    p = Plan(op, prec, d, segs)
+def launch(op: int, prec: int, d: int, recs) -> Plan:
+    p = Plan(op, prec, d, recs)
     p.run()
     return p
 
 
-_PLAN_FIELDS = ("nx0", "nx1", "item0", "part0", "fin0", "red0", "row_tiles", "chunks0",
-                "chunks1", "chunk_len", "fin_blocks")
+def run_steps(op: int, prec: int, d: int, steps: np.ndarray) -> None:
+    """Run launches steps[0], steps[1], ... in order (steps: (S, n) records of equal sizes).
 
-
-def replicate(plan: Plan, steps: Sequence[Sequence[Seg]]) -> list[Plan]:
-    """Plans for a sequence of launches that have the sizes of ``plan``.
-
-    ``steps[t]`` holds the segments of launch t (same sizes as ``plan.segs``, other
-    pointers).  The scheduling fields of a segment depend only on its sizes, so a time loop
-    plans once and copies them; all descriptor arrays go up in one transfer.
+    Scheduling fields depend only on a segment's sizes, so step 0 is planned once and its
+    plan copied to the other steps; every step's descriptors go up in one transfer.
     """
-    n = plan.nseg
-    esz = ctypes.sizeof(Seg)
-    big = (Seg * (n * len(steps)))()
-    for t, segs in enumerate(steps):
-        if len(segs) != n:
-            raise ValueError("replicate: every step needs the planned number of segments")
-        for i, sg in enumerate(segs):
-            ref = plan.segs[i]
-            for f in _PLAN_FIELDS:
-                setattr(sg, f, getattr(ref, f))
-            big[t * n + i] = sg
-    dev = upload(big)
-    out = []
-    for t in range(len(steps)):
-        p = Plan.__new__(Plan)
-        p.op, p.prec, p.d, p.nseg = plan.op, plan.prec, plan.d, n
-        p.segs = (Seg * n).from_buffer(big, t * n * esz)
-        p.n_items, p.n_fin, p.work_bytes = plan.n_items, plan.n_fin, plan.work_bytes
-        p.dev = dev[t * n * esz:(t + 1) * n * esz]
-        out.append(p)
-    return out
+    S, n = steps.shape
+    plan = Plan(op, prec, d, steps[0])
+    for f in PLAN_FIELDS:
+        steps[f][:] = plan.recs[f][None, :]
+    steps = np.ascontiguousarray(steps)
+    dev = upload(steps)
+    base = dev.data_ptr()
+    lib = _lib.load()
+    ws = workspace(plan.work_bytes)
+    for t in range(S):
+        _count(op, steps[t])
+        _lib.check(lib.lddmm_segments_run(op, prec, d, steps[t].ctypes.data_as(_SEG_PTR),
+                                          base + t * n * SEG_SIZE, n, plan.n_items, plan.n_fin,
+                                          ws.data_ptr(), ws.numel(), stream_handle()),
+                   "lddmm_segments_run")

@@ -142,3 +142,42 @@ def test_large_rows_sampled_against_oracle():
     assert rel_err(z64[rows], ref) < 1e-12
     assert rel_err(z32[rows], ref) < TOL["fp32c"]
     assert rel_err(z32, z64) < TOL["fp32c"]
+
+
+def test_mfma_and_valu_ksum_kernels_agree(monkeypatch):
+    """The opt-in FP32 K-sum with its accumulation on the matrix cores (LDDMM_MFMA=1,
+    v_mfma_f32_4x4x1f32) and the
+    default all-VALU kernel is the same sum with the same per-source order."""
+    rng = np.random.default_rng(12)
+    y, x, w = rng.standard_normal((1500, 3)), rng.standard_normal((2100, 3)), rng.standard_normal((2100, 3))
+    for order, sigma in ((10, 0.5), (2, 0.7)):
+        K = kernel(order, sigma, "fp32c")
+        a = K.applyK(x, w, firstVar=y)
+        monkeypatch.setenv("LDDMM_MFMA", "1")
+        b = K.applyK(x, w, firstVar=y)
+        monkeypatch.delenv("LDDMM_MFMA")
+        ref = orc.apply_k(y, x, w, sigma, order)
+        assert rel_err(a, ref) < TOL["fp32c"] and rel_err(b, ref) < TOL["fp32c"]
+        assert rel_err(a, b) < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+@pytest.mark.parametrize("name,order", [("gauss", 10), ("laplacian", 2)])
+def test_tile_culling_matches_full_sums(prec, name, order):
+    """Clustered points with a small kernel width: most (wave, tile) pairs are culled because
+    every kernel value between them is below 2^-40 (FP32) / 2^-80 (FP64); the sums still
+    match the oracle, which evaluates every pair."""
+    from registration_amd import kernelFunctions as kfun
+    rng = np.random.default_rng(21)
+    centers = np.array([[0, 0, 0], [3, 0, 0], [0, 3, 0], [3, 3, 3]], dtype=float)
+    pts = np.concatenate([c + 0.15 * rng.standard_normal((700, 3)) for c in centers])
+    w = rng.standard_normal((pts.shape[0], 3))
+    sigma = 0.05 if name == "gauss" else 0.02
+    K = kfun.Kernel(name=name, sigma=sigma, order=order, precision=prec)
+    got = K.applyK(pts, w)
+    ref = orc.apply_k(pts, pts, w, sigma, K.order)
+    assert rel_err(got, ref) < TOL[prec]
+    a1, a2 = rng.standard_normal((2, pts.shape[0], 3)), rng.standard_normal((2, pts.shape[0], 3))
+    g = K.applyDiffKT(pts, a1, a2)
+    gr = orc.apply_diff_kt(pts, pts, a1, a2, sigma, K.order)
+    assert rel_err(g, gr) < 5 * TOL[prec]

@@ -82,8 +82,7 @@ def test_surface_cg_fp32c_energies_and_path():
     assert np.all(np.abs(hg[:n] - ho[:n]) <= 1e-4 * np.abs(ho[:n])), (hg[:n], ho[:n])
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32c", 1e-6)])
-def test_curve_cg_sequence(prec, tol):
+def curve_pair(prec):
     from registration_amd import curveMatching as cm, curves, kernelFunctions as kfun, synthetic
     n = 256
     X, _ = synthetic.circle(n)
@@ -95,12 +94,39 @@ def test_curve_cg_sequence(prec, tol):
                          maxIter=6, verb=False)
     o = orc.CurveMatching(orc.Curve(pointSet=X), orc.Curve(pointSet=Y), orc.Kernel('laplacian', sigma=0.2, order=1),
                           orc.Kernel('gauss', sigma=0.1), sigmaError=0.1, timeStep=0.2, maxIter=6)
+    return g, o
+
+
+def test_curve_cg_sequence_fp64():
+    g, o = curve_pair("fp64")
     g.optimizeMatching()
     o.optimizeMatching()
     hg, ho = np.array(g.history), np.array(o.history)
+    assert len(hg) == len(ho) and len(hg) >= 4
+    assert np.all(np.abs(hg - ho) <= 1e-9 * np.abs(ho)), (hg, ho)
+
+
+def test_curve_cg_fp32c_energies_and_path():
+    g, o = curve_pair("fp32c")
+    stash = []
+    accept = g.acceptVarTry
+
+    def spy():
+        accept()
+        stash.append((g.at.clone(), g.obj))
+    g.acceptVarTry = spy
+    g.optimizeMatching()
+    o.optimizeMatching()
+    g64, _ = curve_pair("fp64")
+    for at, e in stash:
+        g64.at = at
+        g64.obj = None
+        e64 = g64.objectiveFun()
+        assert abs(e - e64) <= 1e-6 * abs(e64), (e, e64)
+    hg, ho = np.array(g.history), np.array(o.history)
     n = min(len(hg), len(ho))
     assert n >= 4
-    assert np.all(np.abs(hg[:n] - ho[:n]) <= tol * np.abs(ho[:n])), (hg[:n], ho[:n])
+    assert np.all(np.abs(hg[:n] - ho[:n]) <= 1e-4 * np.abs(ho[:n])), (hg[:n], ho[:n])
 
 
 def test_nan_trial_is_rejected():
