@@ -201,6 +201,46 @@ int lddmm_curve_vertex_gradient(const double *z1, const double *dz,
                                 int64_t nv, int d, double scale, double *px,
                                 void *stream);
 
+/* ---- batched geometry and gathers (no counterpart in the reference) -------
+ * One launch updates the face geometry, or gathers the vertex gradients, of
+ * many shapes at once: the serving layer's batched trials and gradients.  The
+ * caller fills blk0 = first block of each entry (running sum of
+ * ceil(count / 256) over the preceding entries) and passes the host array (for
+ * the grid size) and a device copy (read by the kernel). */
+typedef struct lddmm_geom {
+  const double *vertices;  /* (nv, d) */
+  const int32_t *faces;    /* (nf, k): k = 3 triangles (d = 3) or k = 2 edges */
+  double *centers;         /* (nf, d) */
+  double *normals;         /* (nf, d): surfel (triangles) or lenel (edges) */
+  int64_t nf;
+  int32_t k;
+  int32_t d;
+  int64_t blk0;
+} lddmm_geom_t;
+
+/* surfaces.Surface.updateVertices / curves.Curve.updateVertices for n shapes. */
+int lddmm_geometry_batch(const lddmm_geom_t *geoms_host, const lddmm_geom_t *geoms_dev, int n,
+                         void *stream);
+
+typedef struct lddmm_gather {
+  const double *vertices;  /* (nv, d); used by triangles */
+  const int32_t *faces;    /* (nf, k) */
+  const double *z1;        /* (nf, d) */
+  const double *dz;        /* (nf, d) */
+  const int32_t *incid_ptr;/* (nv + 1) */
+  const int32_t *incid;    /* (k nf), entries f*k + j sorted by (vertex, corner, face) */
+  double *px;              /* (nv, d) */
+  int64_t nv;
+  double scale;
+  int32_t k;
+  int32_t d;
+  int64_t blk0;
+} lddmm_gather_t;
+
+/* lddmm_surface_vertex_gradient / lddmm_curve_vertex_gradient for n shapes. */
+int lddmm_vertex_gradient_batch(const lddmm_gather_t *g_host, const lddmm_gather_t *g_dev, int n,
+                                void *stream);
+
 /* ---- small helpers ------------------------------------------------------ */
 
 /* Deterministic FP64 sum of x[0..n) into *out (single fixed reduction tree). */

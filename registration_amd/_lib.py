@@ -61,6 +61,21 @@ class Seg(ctypes.Structure):
     ]
 
 
+class Geom(ctypes.Structure):
+    """Mirror of ``lddmm_geom_t``."""
+
+    _fields_ = [("vertices", _P), ("faces", _P), ("centers", _P), ("normals", _P),
+                ("nf", _I64), ("k", ctypes.c_int32), ("d", ctypes.c_int32), ("blk0", _I64)]
+
+
+class Gather(ctypes.Structure):
+    """Mirror of ``lddmm_gather_t``."""
+
+    _fields_ = [("vertices", _P), ("faces", _P), ("z1", _P), ("dz", _P), ("incid_ptr", _P),
+                ("incid", _P), ("px", _P), ("nv", _I64), ("scale", _D), ("k", ctypes.c_int32),
+                ("d", ctypes.c_int32), ("blk0", _I64)]
+
+
 # Every symbol include/lddmm.h declares, with its ctypes signature.
 SIGNATURES = {
     "lddmm_segments_plan": (_I, [_I, _I, _I, ctypes.POINTER(Seg), _I,
@@ -78,6 +93,8 @@ SIGNATURES = {
     "lddmm_curve_geometry": (_I, [_P, _P, _I64, _I, _P, _P, _P]),
     "lddmm_surface_vertex_gradient": (_I, [_P, _P, _P, _P, _P, _P, _I64, _D, _P, _P]),
     "lddmm_curve_vertex_gradient": (_I, [_P, _P, _P, _P, _I64, _I, _D, _P, _P]),
+    "lddmm_geometry_batch": (_I, [ctypes.POINTER(Geom), _P, _I, _P]),
+    "lddmm_vertex_gradient_batch": (_I, [ctypes.POINTER(Gather), _P, _I, _P]),
     "lddmm_sum_f64": (_I, [_P, _I64, _P, _P]),
     "lddmm_last_hip_error": (_I, []),
     "lddmm_version": (ctypes.c_char_p, []),

@@ -1036,11 +1036,9 @@ __global__ __launch_bounds__(TOT_BLOCK) void k_totals(const lddmm_seg_t *__restr
 // ---------------------------------------------------------------------------
 // Geometry and gathers evaluate each expression in the reference's NumPy order without
 // FMA contraction, so their FP64 results are the reference's bit for bit.
-__global__ void k_surface_geometry(const double *__restrict__ v, const int32_t *__restrict__ f,
-                                   int64_t nf, double *__restrict__ cen, double *__restrict__ nrm) {
+__device__ inline void surface_face(const double *__restrict__ v, const int32_t *__restrict__ f,
+                                    int64_t i, double *__restrict__ cen, double *__restrict__ nrm) {
 #pragma clang fp contract(off)
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nf) return;
   const double *a = v + 3 * (int64_t)f[3 * i + 0];
   const double *b = v + 3 * (int64_t)f[3 * i + 1];
   const double *c = v + 3 * (int64_t)f[3 * i + 2];
@@ -1053,12 +1051,10 @@ __global__ void k_surface_geometry(const double *__restrict__ v, const int32_t *
   nrm[3 * i + 2] = e1[0] * e2[1] - e1[1] * e2[0];
 }
 
-__global__ void k_curve_geometry(const double *__restrict__ v, const int32_t *__restrict__ e,
-                                 int64_t ne, int d, double *__restrict__ cen,
-                                 double *__restrict__ len) {
+__device__ inline void curve_edge(const double *__restrict__ v, const int32_t *__restrict__ e,
+                                  int64_t i, int d, double *__restrict__ cen,
+                                  double *__restrict__ len) {
 #pragma clang fp contract(off)
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ne) return;
   const double *a = v + (int64_t)d * e[2 * i + 0];
   const double *b = v + (int64_t)d * e[2 * i + 1];
   for (int q = 0; q < d; ++q) {
@@ -1067,13 +1063,12 @@ __global__ void k_curve_geometry(const double *__restrict__ v, const int32_t *__
   }
 }
 
-__global__ void k_surface_vertex_gradient(const double *__restrict__ v, const int32_t *__restrict__ f,
-                                          const double *__restrict__ z1, const double *__restrict__ dz,
-                                          const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
-                                          int64_t nv, double scale, double *__restrict__ px) {
+__device__ inline void surface_gather(const double *__restrict__ v, const int32_t *__restrict__ f,
+                                      const double *__restrict__ z1, const double *__restrict__ dz,
+                                      const int32_t *__restrict__ ptr,
+                                      const int32_t *__restrict__ inc, int64_t i, double scale,
+                                      double *__restrict__ px) {
 #pragma clang fp contract(off)
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) {
     int32_t code = inc[q];
@@ -1090,12 +1085,11 @@ __global__ void k_surface_vertex_gradient(const double *__restrict__ v, const in
   for (int c = 0; c < 3; ++c) px[3 * i + c] = scale * acc[c];
 }
 
-__global__ void k_curve_vertex_gradient(const double *__restrict__ z1, const double *__restrict__ dz,
-                                        const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
-                                        int64_t nv, int d, double scale, double *__restrict__ px) {
+__device__ inline void curve_gather(const double *__restrict__ z1, const double *__restrict__ dz,
+                                    const int32_t *__restrict__ ptr,
+                                    const int32_t *__restrict__ inc, int64_t i, int d,
+                                    double scale, double *__restrict__ px) {
 #pragma clang fp contract(off)
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
   double acc[8];
   for (int c = 0; c < d; ++c) acc[c] = 0.0;
   for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) {
@@ -1108,6 +1102,67 @@ __global__ void k_curve_vertex_gradient(const double *__restrict__ z1, const dou
     }
   }
   for (int c = 0; c < d; ++c) px[(int64_t)d * i + c] = scale * acc[c];
+}
+
+__global__ void k_surface_geometry(const double *__restrict__ v, const int32_t *__restrict__ f,
+                                   int64_t nf, double *__restrict__ cen, double *__restrict__ nrm) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nf) surface_face(v, f, i, cen, nrm);
+}
+
+__global__ void k_curve_geometry(const double *__restrict__ v, const int32_t *__restrict__ e,
+                                 int64_t ne, int d, double *__restrict__ cen,
+                                 double *__restrict__ len) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ne) curve_edge(v, e, i, d, cen, len);
+}
+
+__global__ void k_surface_vertex_gradient(const double *__restrict__ v, const int32_t *__restrict__ f,
+                                          const double *__restrict__ z1, const double *__restrict__ dz,
+                                          const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
+                                          int64_t nv, double scale, double *__restrict__ px) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv) surface_gather(v, f, z1, dz, ptr, inc, i, scale, px);
+}
+
+__global__ void k_curve_vertex_gradient(const double *__restrict__ z1, const double *__restrict__ dz,
+                                        const int32_t *__restrict__ ptr, const int32_t *__restrict__ inc,
+                                        int64_t nv, int d, double scale, double *__restrict__ px) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv) curve_gather(z1, dz, ptr, inc, i, d, scale, px);
+}
+
+// Batched forms: block -> entry by binary search on blk0.
+template <typename T>
+__device__ inline int find_blk(const T *__restrict__ e, int n, int64_t b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (e[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_geometry_batch(const lddmm_geom_t *__restrict__ g, int n) {
+  const int64_t b = blockIdx.x;
+  const lddmm_geom_t *e = g + find_blk(g, n, b);
+  const int64_t i = (b - e->blk0) * 256 + threadIdx.x;
+  if (i >= e->nf) return;
+  if (e->k == 3)
+    surface_face(e->vertices, e->faces, i, e->centers, e->normals);
+  else
+    curve_edge(e->vertices, e->faces, i, e->d, e->centers, e->normals);
+}
+
+__global__ __launch_bounds__(256) void k_gather_batch(const lddmm_gather_t *__restrict__ g, int n) {
+  const int64_t b = blockIdx.x;
+  const lddmm_gather_t *e = g + find_blk(g, n, b);
+  const int64_t i = (b - e->blk0) * 256 + threadIdx.x;
+  if (i >= e->nv) return;
+  if (e->k == 3)
+    surface_gather(e->vertices, e->faces, e->z1, e->dz, e->incid_ptr, e->incid, i, e->scale, e->px);
+  else
+    curve_gather(e->z1, e->dz, e->incid_ptr, e->incid, i, e->d, e->scale, e->px);
 }
 
 __global__ __launch_bounds__(1024) void k_sum_f64(const double *__restrict__ x, int64_t n,
@@ -1668,6 +1723,41 @@ extern "C" int lddmm_curve_vertex_gradient(const double *z1, const double *dz,
   if (!z1 || !dz || !incid_ptr || !incid || !px || nv < 1 || d < 1 || d > 8) return LDDMM_EINVAL;
   hipLaunchKernelGGL(k_curve_vertex_gradient, dim3((unsigned)ceil_div(nv, 256)), dim3(256), 0,
                      (hipStream_t)stream, z1, dz, incid_ptr, incid, nv, d, scale, px);
+  return hip_check(hipGetLastError());
+}
+
+extern "C" int lddmm_geometry_batch(const lddmm_geom_t *geoms_host, const lddmm_geom_t *geoms_dev,
+                                    int n, void *stream) {
+  if (!geoms_host || !geoms_dev || n < 1) return LDDMM_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    const lddmm_geom_t &e = geoms_host[i];
+    if (!e.vertices || !e.faces || !e.centers || !e.normals || e.nf < 1) return LDDMM_EINVAL;
+    if (!((e.k == 3 && e.d == 3) || (e.k == 2 && e.d >= 1 && e.d <= 8))) return LDDMM_EINVAL;
+    int64_t expect = i == 0 ? 0 : geoms_host[i - 1].blk0 + ceil_div(geoms_host[i - 1].nf, 256);
+    if (e.blk0 != expect) return LDDMM_EINVAL;
+  }
+  const int64_t blocks = geoms_host[n - 1].blk0 + ceil_div(geoms_host[n - 1].nf, 256);
+  if (blocks > INT32_MAX) return LDDMM_EINVAL;
+  hipLaunchKernelGGL(k_geometry_batch, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     geoms_dev, n);
+  return hip_check(hipGetLastError());
+}
+
+extern "C" int lddmm_vertex_gradient_batch(const lddmm_gather_t *g_host, const lddmm_gather_t *g_dev,
+                                           int n, void *stream) {
+  if (!g_host || !g_dev || n < 1) return LDDMM_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    const lddmm_gather_t &e = g_host[i];
+    if (!e.z1 || !e.dz || !e.incid_ptr || !e.incid || !e.px || e.nv < 1) return LDDMM_EINVAL;
+    if (!((e.k == 3 && e.d == 3 && e.vertices && e.faces) || (e.k == 2 && e.d >= 1 && e.d <= 8)))
+      return LDDMM_EINVAL;
+    int64_t expect = i == 0 ? 0 : g_host[i - 1].blk0 + ceil_div(g_host[i - 1].nv, 256);
+    if (e.blk0 != expect) return LDDMM_EINVAL;
+  }
+  const int64_t blocks = g_host[n - 1].blk0 + ceil_div(g_host[n - 1].nv, 256);
+  if (blocks > INT32_MAX) return LDDMM_EINVAL;
+  hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     g_dev, n);
   return hip_check(hipGetLastError());
 }
 

@@ -11,7 +11,6 @@ from __future__ import annotations
 import numpy as np
 
 from . import conjugateGradient as cg
-from . import engine
 from . import curves
 from . import matchingParam
 from .matching import DeviceMatching, Direction  # noqa: F401
@@ -66,8 +65,6 @@ class CurveMatching(DeviceMatching):
     def _new_shape(self, vertices):
         return curves.DeviceCurve(self.faces_np, vertices)
 
-    def _gather(self, shape, z1, dz, scale, px):
-        engine.curve_vertex_gradient(z1, dz, shape.ptr, shape.inc, shape.nv, scale, px)
 
     def endOptim(self):
         self.fvDef.updateVertices(self.xt[-1].cpu().numpy())

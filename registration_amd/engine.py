@@ -15,6 +15,7 @@ Reference counterparts:
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -61,17 +62,15 @@ def _groups(jobs, key):
     return out.values()
 
 
-def _runs(jobs, n_steps, active):
-    """Maximal runs [t0, t1) of steps with one active set: (t0, t1, [job indices])."""
-    t = 0
-    while t < n_steps:
-        idx = [i for i, j in enumerate(jobs) if active(j, t)]
-        t1 = t + 1
-        while t1 < n_steps and [i for i, j in enumerate(jobs) if active(j, t1)] == idx:
-            t1 += 1
-        if idx:
-            yield t, t1, idx
-        t = t1
+def _runs(lengths, n_steps):
+    """Maximal runs [t0, t1) of steps with one active set, for jobs of ``lengths`` steps
+    (job i is active at steps t < lengths[i]): yields (t0, t1, [job indices])."""
+    lengths = np.asarray(lengths)
+    bounds = sorted(set(int(v) for v in lengths if v < n_steps) | {0, n_steps})
+    for t0, t1 in zip(bounds[:-1], bounds[1:]):
+        idx = np.nonzero(lengths > t0)[0].tolist()
+        if idx and t1 > t0:
+            yield t0, t1, idx
 
 
 def _u64(vals):
@@ -109,7 +108,8 @@ def shoot(jobs: Sequence[FlowJob]) -> None:
     for group in _groups(jobs, lambda j: (j.K.family, j.x0.shape[1])):
         d = group[0].x0.shape[1]
         prec = group[0].K.prec
-        for t0, t1, idx in _runs(group, max(j.T for j in group), lambda j, t: t < j.T):
+        Ts = [j.T for j in group]
+        for t0, t1, idx in _runs(Ts, max(Ts)):
             g = [group[i] for i in idx]
             base = rt.segs(len(g))
             base["ny"] = [j.xt.shape[1] for j in g]
@@ -161,7 +161,8 @@ def covector(jobs: Sequence[CovectorJob]) -> None:
         d = group[0].xt.shape[2]
         prec = group[0].K.prec
         # Backward in time: launch s handles t = T - 1 - s of every job with s < T.
-        for s0, s1, idx in _runs(group, max(j.T for j in group), lambda j, s: s < j.T):
+        Ts = [j.T for j in group]
+        for s0, s1, idx in _runs(Ts, max(Ts)):
             g = [group[i] for i in idx]
             base = rt.segs(len(g))
             base["ny"] = [j.xt.shape[1] for j in g]
@@ -329,3 +330,61 @@ def curve_vertex_gradient(z1, dz, ptr, inc, nv, scale, px) -> None:
                                                inc.data_ptr(), nv, z1.shape[1], float(scale),
                                                px.data_ptr(), rt.stream_handle()),
                "lddmm_curve_vertex_gradient")
+
+
+# --------------------------------------------------------------------------------------
+# Batched geometry and vertex gathers (one launch for many shapes)
+# --------------------------------------------------------------------------------------
+GEOM_DT = np.dtype(_lib.Geom)
+GATHER_DT = np.dtype(_lib.Gather)
+
+
+def _blk0(counts):
+    c = np.asarray([-(-int(n) // 256) for n in counts], dtype=np.int64)
+    return np.concatenate([[0], np.cumsum(c)[:-1]])
+
+
+def geometry_batch(shapes, vertices) -> None:
+    """shape.update(v) for every (shape, v): face centers and normals of many surfaces and
+    curves in one launch."""
+    if not shapes:
+        return
+    lib = _lib.load()
+    r = np.zeros(len(shapes), GEOM_DT)
+    for i, (sh, v) in enumerate(zip(shapes, vertices)):
+        sh.vertices = v
+    r["vertices"] = [sh.vertices.data_ptr() for sh in shapes]
+    r["faces"] = [sh.faces.data_ptr() for sh in shapes]
+    r["centers"] = [sh.centers.data_ptr() for sh in shapes]
+    r["normals"] = [sh.surfel.data_ptr() for sh in shapes]
+    r["nf"] = [sh.faces.shape[0] for sh in shapes]
+    r["k"] = [sh.faces.shape[1] for sh in shapes]
+    r["d"] = [sh.centers.shape[1] for sh in shapes]
+    r["blk0"] = _blk0(r["nf"])
+    dev = rt.upload(r)
+    _lib.check(lib.lddmm_geometry_batch(r.ctypes.data_as(ctypes.POINTER(_lib.Geom)), dev.data_ptr(),
+                                        len(shapes), rt.stream_handle()), "lddmm_geometry_batch")
+
+
+def gather_batch(shapes, z1s, dzs, scales, pxs) -> None:
+    """Vertex gathers of the currents gradient for many shapes in one launch."""
+    if not shapes:
+        return
+    lib = _lib.load()
+    r = np.zeros(len(shapes), GATHER_DT)
+    r["vertices"] = [sh.vertices.data_ptr() for sh in shapes]
+    r["faces"] = [sh.faces.data_ptr() for sh in shapes]
+    r["z1"] = [z.data_ptr() for z in z1s]
+    r["dz"] = [z.data_ptr() for z in dzs]
+    r["incid_ptr"] = [sh.ptr.data_ptr() for sh in shapes]
+    r["incid"] = [sh.inc.data_ptr() for sh in shapes]
+    r["px"] = [p.data_ptr() for p in pxs]
+    r["nv"] = [sh.nv for sh in shapes]
+    r["scale"] = scales
+    r["k"] = [sh.faces.shape[1] for sh in shapes]
+    r["d"] = [sh.centers.shape[1] for sh in shapes]
+    r["blk0"] = _blk0(r["nv"])
+    dev = rt.upload(r)
+    _lib.check(lib.lddmm_vertex_gradient_batch(r.ctypes.data_as(ctypes.POINTER(_lib.Gather)),
+                                               dev.data_ptr(), len(shapes), rt.stream_handle()),
+               "lddmm_vertex_gradient_batch")

@@ -127,9 +127,6 @@ class DeviceMatching:
     def deformed_vertices(self) -> torch.Tensor:
         return self.xt[-1]
 
-    def _gather(self, shape, z1, dz, scale, px):
-        """px = scale * (vertex gather of the per-face currents gradient)."""
-        raise NotImplementedError
 
 
 # --------------------------------------------------------------------------------------
@@ -153,9 +150,9 @@ def batch_objective(opts: Sequence[DeviceMatching]) -> list:
         flows.append(engine.FlowJob(o.x0, o.at, o.Kdiff, o.xt, scal[off:off + T], init=False))
         off += L
     engine.shoot(flows)
+    engine.geometry_batch([o.fvDef_dev for o in opts], [o.xt[-1] for o in opts])
     for o, off in zip(opts, offs):
         T = o.Tsize
-        o.fvDef_dev.update(o.xt[-1])
         curs.append(engine.CurrentsJob(o.fvDef_dev.centers, o.fvDef_dev.surfel, o.c2, o.cr2,
                                        o.Kdist.energy(), scal[off + T:off + T + 1], coef=o.coef))
         norms.append(engine.KNormJob(o.c2, o.cr2, o.Kdist.energy(), scal[off + T + 1:off + T + 2]))
@@ -193,21 +190,22 @@ def batch_update_try(items) -> list:
         flows.append(engine.FlowJob(o.x0, atTry, o.Kdiff, o._xt_scratch, scal[off:off + T], init=False))
         off += L
     engine.shoot(flows)
+    engine.geometry_batch([o._try_shape for (o, _, _, _) in items],
+                          [o._xt_scratch[-1] for (o, _, _, _) in items])
     for (o, _, _, _), off in zip(items, offs):
         T = o.Tsize
-        o._try_shape.update(o._xt_scratch[-1])
         curs.append(engine.CurrentsJob(o._try_shape.centers, o._try_shape.surfel, o.c2, o.cr2,
                                        o.Kdist.energy(), scal[off + T:off + T + 1], coef=o.coef))
     engine.currents(curs)
-    vals = _readback(scal)
+    vals = _readback(scal).tolist()
     out = []
     for (o, dir, eps, objRef), atTry, off in zip(items, ats, offs):
         T = o.Tsize
-        dt = 1.0 / T
+        c = o.regweight * (1.0 / T)
         objTry = o.obj0
         objDef = 0
-        for t in range(T):
-            objDef = objDef + o.regweight * dt * vals[off + t]
+        for e in vals[off:off + T]:
+            objDef = objDef + c * e
         objTry += objDef
         objTry += vals[off + T] / (o.sigmaError ** 2)
         o.nobj += 1
@@ -239,10 +237,11 @@ def batch_gradient(items) -> list:
                                        torch.empty(1, dtype=torch.float64, device=dev),
                                        z1=z1, dz=dz, coef=o.coef))
     engine.currents(curs)
+    px1s = [torch.empty((o.npt, o.dim), dtype=torch.float64, device=dev) for o, _ in items]
+    engine.gather_batch([o.fvDef_dev for o, _ in items], [z for z, _ in zs], [z for _, z in zs],
+                        [-2.0 / (o.sigmaError ** 2) for o, _ in items], px1s)
     covs, pxts = [], []
-    for (o, _), (z1, dz) in zip(items, zs):
-        px1 = torch.empty((o.npt, o.dim), dtype=torch.float64, device=dev)
-        o._gather(o.fvDef_dev, z1, dz, -2.0 / (o.sigmaError ** 2), px1)
+    for (o, _), px1 in zip(items, px1s):
         pxt = torch.empty_like(o.xt)
         pxts.append(pxt)
         covs.append(engine.CovectorJob(o.xt, o.at, px1, o.Kdiff, o.regweight, pxt))

@@ -34,14 +34,21 @@ pair_counts: dict = {}
 COUNT_PAIRS = False
 
 
+_has_gpu = None
+
+
 def device() -> torch.device:
-    if not torch.cuda.is_available():
+    global _has_gpu
+    if _has_gpu is None:
+        _has_gpu = torch.cuda.is_available()
+    if not _has_gpu:
         raise RuntimeError("registration_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
     return torch.device("cuda", torch.cuda.current_device())
 
 
 def stream_handle() -> int:
-    return torch.cuda.current_stream().cuda_stream
+    """Raw handle of the current stream of the current device (what torch launches on)."""
+    return torch._C._cuda_getCurrentRawStream(torch.cuda.current_device())
 
 
 def workspace(nbytes: int) -> torch.Tensor:
@@ -164,9 +171,10 @@ def run_steps(op: int, prec: int, d: int, steps: np.ndarray) -> None:
     base = dev.data_ptr()
     lib = _lib.load()
     ws = workspace(plan.work_bytes)
+    st = stream_handle()
     for t in range(S):
         _count(op, steps[t])
         _lib.check(lib.lddmm_segments_run(op, prec, d, steps[t].ctypes.data_as(_SEG_PTR),
                                           base + t * n * SEG_SIZE, n, plan.n_items, plan.n_fin,
-                                          ws.data_ptr(), ws.numel(), stream_handle()),
+                                          ws.data_ptr(), ws.numel(), st),
                    "lddmm_segments_run")

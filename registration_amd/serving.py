@@ -102,6 +102,9 @@ def footprint_bytes(req: Request) -> int:
     return 8 * (traj + mom + cur) + 64 * 1024
 
 
+_HOST_ONLY = (cg.ACCEPT, cg.END_ITER)
+
+
 class _Active:
     __slots__ = ("req", "opt", "gen", "pending", "admitted")
 
@@ -183,13 +186,16 @@ class Server:
             opts = [a.opt for a in groups[cg.OBJECTIVE]]
             for a, v in zip(groups[cg.OBJECTIVE], matching.batch_objective(opts)):
                 answers[id(a)] = v
-        for kind in (cg.ACCEPT, cg.END_ITER):
+        for kind in _HOST_ONLY:
             for a in groups.get(kind, []):
                 answers[id(a)] = getattr(a.opt, a.pending[0])()
         still = []
         for a in self.active:
             try:
                 a.pending = a.gen.send(answers[id(a)])
+                # Requests without device work are answered at once, not in a later round.
+                while a.pending[0] in _HOST_ONLY:
+                    a.pending = a.gen.send(getattr(a.opt, a.pending[0])())
                 still.append(a)
             except StopIteration as stop:
                 self._finish(a, stop.value)

@@ -14,7 +14,6 @@ import logging
 import numpy as np
 
 from . import conjugateGradient as cg
-from . import engine
 from . import kernelFunctions as kfun
 from . import surfaces
 from .matching import DeviceMatching, Direction  # noqa: F401  (Direction is part of the API)
@@ -87,9 +86,6 @@ class SurfaceMatching(DeviceMatching):
     def _new_shape(self, vertices):
         return surfaces.DeviceSurface(self.faces_np, vertices)
 
-    def _gather(self, shape, z1, dz, scale, px):
-        engine.surface_vertex_gradient(shape.vertices, shape.faces, z1, dz, shape.ptr, shape.inc,
-                                       scale, px)
 
     def endOptim(self):
         self.fvDef.updateVertices(self.xt[-1].cpu().numpy())

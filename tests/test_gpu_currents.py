@@ -77,3 +77,37 @@ def test_gradient_gather_is_bit_exact_in_fp64_order():
     engine.surface_vertex_gradient(dev(V), dev(F.astype(np.int32), torch.int32), dev(z1), dev(dz),
                                    dev(ptr, torch.int32), dev(inc, torch.int32), 2.0, out)
     assert np.array_equal(out.cpu().numpy(), 2 * px)
+
+
+def test_batched_geometry_and_gather_equal_single_shape_kernels():
+    import torch
+    from registration_amd import curves, engine, surfaces, synthetic
+    rng = np.random.default_rng(3)
+    shapes, verts, singles = [], [], []
+    for s in (1, 2, 3):
+        V, F = synthetic.icosphere(s)
+        X = V + 0.05 * rng.standard_normal(V.shape)
+        shapes.append(surfaces.DeviceSurface(F, V))
+        verts.append(torch.as_tensor(X, device="cuda"))
+        singles.append(surfaces.DeviceSurface(F, X))
+    for n in (50, 333):
+        X, _ = synthetic.circle(n)
+        X = X + 0.01 * rng.standard_normal(X.shape)
+        E = synthetic.closed_curve_edges(n)
+        shapes.append(curves.DeviceCurve(E, np.zeros_like(X)))
+        verts.append(torch.as_tensor(X, device="cuda"))
+        singles.append(curves.DeviceCurve(E, X))
+    engine.geometry_batch(shapes, verts)
+    for a, b in zip(shapes, singles):
+        assert torch.equal(a.centers, b.centers) and torch.equal(a.surfel, b.surfel)
+    z1s = [torch.randn_like(sh.centers) for sh in shapes]
+    dzs = [torch.randn_like(sh.centers) for sh in shapes]
+    pxs = [torch.empty((sh.nv, sh.centers.shape[1]), dtype=torch.float64, device="cuda") for sh in shapes]
+    engine.gather_batch(shapes, z1s, dzs, [0.5 + i for i in range(len(shapes))], pxs)
+    for i, sh in enumerate(shapes):
+        ref = torch.empty_like(pxs[i])
+        if sh.faces.shape[1] == 3:
+            engine.surface_vertex_gradient(sh.vertices, sh.faces, z1s[i], dzs[i], sh.ptr, sh.inc, 0.5 + i, ref)
+        else:
+            engine.curve_vertex_gradient(z1s[i], dzs[i], sh.ptr, sh.inc, sh.nv, 0.5 + i, ref)
+        assert torch.equal(pxs[i], ref)
