@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--roofline-reps", type=int, default=20)
     ap.add_argument("--served", type=int, default=16,
                     help="requests of the served-surfaces stream to serve (0 = skip)")
+    ap.add_argument("--mixed-service", type=int, default=5,
+                    help="requests of the mixed-service stream to serve (0 = skip)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the flow-step launches and a calibration copy (for rocprof passes)")
     return ap.parse_args()
@@ -189,6 +191,52 @@ def served_leg(n, prec):
     return {"mixed": mixed, "small": small}
 
 
+def mixed_service_leg(k, prec):
+    """The first k requests of the mixed-service stream (2,000 requests: 40% closed curves,
+    40% surfaces s3-s7, 20% landmark sets 1k-200k points; Gaussian or Laplacian kernels,
+    sigma ~ U[0.1, 0.5], T in {10, 20, 40}; bursty Poisson arrivals at 200/s) through the
+    service, with each result's energy checked against the FP64 form of the same objective
+    at the same final momenta (per class)."""
+    import collections as _c
+    import numpy as np
+    import torch
+    from registration_amd import serving, synthetic
+    specs = synthetic.mixed_service_requests(k)
+    t0a = specs[0]["arrival"]
+    reqs = [serving.Request(rid=i, kind=c["kind"], template=c["template"], target=c["target"],
+                            sigmaKernel=c["sigmaKernel"], sigmaDist=c["sigmaDist"],
+                            sigmaError=c["sigmaError"], T=c["T"], maxIter=c["maxIter"],
+                            typeKernel=c["typeKernel"], order=c["order"], arrival=c["arrival"] - t0a)
+            for i, c in enumerate(specs)]
+    srv = serving.Server(hbm_cap_bytes=200e9, precision=prec, keep_momenta=True)
+    for r in reqs:
+        srv.submit(r)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = srv.run()
+    wall = time.perf_counter() - t0
+    lat = np.array([res[r.rid].latency for r in reqs]) * 1e3
+    errs = _c.defaultdict(list)
+    for r in reqs:
+        m64 = serving.build_registration(r, "fp64")
+        m64.at = res[r.rid].momenta.clone()
+        e64 = m64.objectiveFun()
+        errs[r.kind].append(abs(res[r.rid].energy - e64) / abs(e64))
+        del m64
+    torch.cuda.synchronize()
+    return {"workload": "first %d requests of the mixed-service stream (curves 256-8192 pts, "
+                        "surfaces s3-s7, landmark sets 1k-200k; gauss/laplacian sigma U[0.1,0.5]; "
+                        "T 10/20/40; 30 CG; bursty Poisson 200/s)" % k,
+            "requests": k, "served_registrations_per_s": k / wall, "served_wall_s": wall,
+            "served_latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                  "p99": float(np.percentile(lat, 99))},
+            "classes": {c: {"count": len(v), "max_energy_rel_err_vs_fp64": float(max(v))}
+                        for c, v in sorted(errs.items())},
+            "sizes": [[r.kind, int(r.template[1].shape[0]), r.T, r.typeKernel] for r in reqs],
+            "rounds": srv.rounds, "mean_batch": srv.answered / max(srv.rounds, 1),
+            "max_batch": srv.max_batch}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -251,6 +299,9 @@ def main():
         cpu = cpu_baseline(cfg, {k: v / args.steps for k, v in counts.items()})
 
     served = served_leg(args.served, args.precision) if args.served > 0 else None
+    if args.mixed_service > 0:
+        served = dict(served or {})
+        served["mixed_service"] = mixed_service_leg(args.mixed_service, args.precision)
 
     lat_ms = np.array(lat) * 1e3
     out = {

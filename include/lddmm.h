@@ -241,6 +241,26 @@ typedef struct lddmm_gather {
 int lddmm_vertex_gradient_batch(const lddmm_gather_t *g_host, const lddmm_gather_t *g_dev, int n,
                                 void *stream);
 
+/* ---- landmark data term ---------------------------------------------------
+ * Squared-distance data term of landmark-set matching, D(x) = sum_k |x_k - y_k|^2
+ * over corresponding points (the energy of LandmarkObjective,
+ * diffeoReg/landmarkMatching.h:49-62).  The gradient written is
+ * grad = scale * (x - y); scale = 2 / sigma_err^2 gives dD/dx of D / sigma_err^2
+ * (the reference's LandmarkObjective::gradient, landmarkMatching.h:63-69, uses
+ * (x - y) / N instead, a rescaled direction for its own optimiser).
+ * One workgroup per entry; the FP64 total follows one fixed order per entry,
+ * so it is the same alone and in any batch. */
+typedef struct lddmm_lmk {
+  const double *x;  /* (n) deformed point coordinates, n = points * d */
+  const double *y;  /* (n) target coordinates */
+  double *total;    /* 1 double: sum (x - y)^2, or NULL */
+  double *grad;     /* (n): scale * (x - y), or NULL */
+  int64_t n;
+  double scale;
+} lddmm_lmk_t;
+
+int lddmm_landmark_batch(const lddmm_lmk_t *host, const lddmm_lmk_t *dev, int n, void *stream);
+
 /* ---- small helpers ------------------------------------------------------ */
 
 /* Deterministic FP64 sum of x[0..n) into *out (single fixed reduction tree). */

@@ -576,6 +576,61 @@ class CurveMatching(_Matching):
         return cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
 
 
+class PointSet:
+    """Landmark set: the deformed 'shape' of landmark matching is its point array."""
+
+    def __init__(self, points):
+        self.vertices = np.array(points, dtype=np.float64)
+
+    def updateVertices(self, x0):
+        self.vertices = np.copy(x0)
+
+
+def landmark_norm0(fv1, K):
+    return 0.0
+
+
+def landmark_normDef(fvDef, fv1, K):
+    """sum_k |x_k - y_k|^2, the energy of LandmarkObjective (diffeoReg/landmarkMatching.h:52-62)."""
+    u = fvDef.vertices - fv1.vertices
+    return float((u * u).sum())
+
+
+def landmark_normGradient(fvDef, fv1, K):
+    """d/dx of landmark_normDef: 2 (x - y)."""
+    return 2.0 * (fvDef.vertices - fv1.vertices)
+
+
+class LandmarkMatching(_Matching):
+    """Landmark-set matching: the surface driver's call pattern (surfaceMatching.py:74-484)
+    with the squared-distance data term above; epsInit 0.1 and gradient bound
+    max(gradLB, |g|/10^4) as in SurfaceMatching.optimizeMatching (surfaceMatching.py:474-484)."""
+
+    def __init__(self, Template, Target, KparDiff, sigmaError=1.0, timeStep=.1, regWeight=1.0,
+                 maxIter=1000, gradLB=0.001, epsInit=0.1):
+        self.fv0 = PointSet(Template)
+        self.fv1 = PointSet(Target)
+        self.fvDef = PointSet(Template)
+        self.fun_obj0 = landmark_norm0
+        self.fun_obj = landmark_normDef
+        self.fun_objGrad = landmark_normGradient
+        self._setup(self.fv0.vertices, KparDiff, None, sigmaError, timeStep, regWeight)
+        self.maxIter = maxIter
+        self.iter = 0
+        self.epsInit = epsInit
+        self.gradLB = gradLB
+
+    @staticmethod
+    def _copy_shape(s):
+        return PointSet(s.vertices)
+
+    def optimizeMatching(self):
+        grd = self.getGradient(self.gradCoeff)
+        [grd2] = self.dotProduct(grd, [grd])
+        self.gradEps = max(self.gradLB, np.sqrt(grd2) / 10000)
+        return cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+
+
 def cg(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=10.):
     """conjugateGradient.cg restated (conjugateGradient.py:33-217).  Returns the per-iteration
     objective history (the reference returns opt.getVariable(); the history is kept on

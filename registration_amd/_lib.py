@@ -76,6 +76,12 @@ class Gather(ctypes.Structure):
                 ("d", ctypes.c_int32), ("blk0", _I64)]
 
 
+class Landmark(ctypes.Structure):
+    """Mirror of ``lddmm_lmk_t``."""
+
+    _fields_ = [("x", _P), ("y", _P), ("total", _P), ("grad", _P), ("n", _I64), ("scale", _D)]
+
+
 # Every symbol include/lddmm.h declares, with its ctypes signature.
 SIGNATURES = {
     "lddmm_segments_plan": (_I, [_I, _I, _I, ctypes.POINTER(Seg), _I,
@@ -95,6 +101,7 @@ SIGNATURES = {
     "lddmm_curve_vertex_gradient": (_I, [_P, _P, _P, _P, _I64, _I, _D, _P, _P]),
     "lddmm_geometry_batch": (_I, [ctypes.POINTER(Geom), _P, _I, _P]),
     "lddmm_vertex_gradient_batch": (_I, [ctypes.POINTER(Gather), _P, _I, _P]),
+    "lddmm_landmark_batch": (_I, [ctypes.POINTER(Landmark), _P, _I, _P]),
     "lddmm_sum_f64": (_I, [_P, _I64, _P, _P]),
     "lddmm_last_hip_error": (_I, []),
     "lddmm_version": (ctypes.c_char_p, []),

@@ -84,11 +84,14 @@ __constant__ double LAP_R[5][3] = {
 
 // Coordinate prescale s applied to positions before the pair loop.
 //   FP32 Gaussian:  s = sqrt(log2(e)/2)/sigma, so K = exp2(-|s dx|^2)
-//   FP32 Laplacian: s = 1/sigma, so u = |s dx| = r/sigma
+//   FP32 Laplacian: s = log2(e)/sigma, so |s dx| = log2(e) r/sigma and e^-u = exp2(-|s dx|)
 //   FP64:           s = 1 (positions untouched, the kernel scales r itself)
+// Folding the constants into the FP64 prescale keeps FP32-rounded constants out
+// of the exponents (a rounded log2(e) would bias every kernel value by a fixed
+// relative amount proportional to the exponent).
 __host__ __device__ inline double coord_scale(int prec, int kernel, double sigma) {
   if (prec != LDDMM_PREC_FP32C) return 1.0;
-  return kernel == LDDMM_KERNEL_GAUSS ? sqrt(0.5 * LOG2E) / sigma : 1.0 / sigma;
+  return kernel == LDDMM_KERNEL_GAUSS ? sqrt(0.5 * LOG2E) / sigma : LOG2E / sigma;
 }
 
 // ---------------------------------------------------------------------------
@@ -97,8 +100,10 @@ __host__ __device__ inline double coord_scale(int prec, int kernel, double sigma
 template <typename R, int KT>
 struct KEval;
 
+// R2_0: start value of the squared-distance accumulation in the pair loops.
 template <>
 struct KEval<float, LDDMM_KERNEL_GAUSS> {
+  static constexpr float R2_0 = 0.0f;
   float dummy;
   __device__ void init(int, double) {}
   __device__ inline void operator()(float r2, float &k0, float &k1, float &k2) const {
@@ -111,6 +116,8 @@ struct KEval<float, LDDMM_KERNEL_GAUSS> {
 
 template <>
 struct KEval<float, LDDMM_KERNEL_LAPLACIAN> {
+  static constexpr float R2_0 = 1e-30f;
+  static constexpr float LN2F = 0.69314718055994530942f;
   float p0, p1, p2, p3, p4, q0, q1, q2, q3, c0, c1, c2;
   __device__ void init(int order, double) {
     p0 = (float)LAP_P[order][0]; p1 = (float)LAP_P[order][1]; p2 = (float)LAP_P[order][2];
@@ -119,9 +126,18 @@ struct KEval<float, LDDMM_KERNEL_LAPLACIAN> {
     q3 = (float)LAP_Q[order][3];
     c0 = (float)LAP_R[order][0]; c1 = (float)LAP_R[order][1]; c2 = (float)LAP_R[order][2];
   }
+  // r2 = (log2(e) r / sigma)^2, at least R2_0 > 0.  The distance v = sqrt(r2) is
+  // v_rsq_f32 followed by one Newton correction with an FMA residual: unbiased
+  // and within one ulp.  (v_sqrt_f32 alone is low by 1.5e-8 on average on
+  // gfx950; such a bias shifts every kernel value the same way and accumulates
+  // along a flow.)  R2_0 keeps rsq finite on coincident points.
   __device__ inline void operator()(float r2, float &k0, float &k1, float &k2) const {
-    float u = __builtin_amdgcn_sqrtf(r2);
-    float e = __builtin_amdgcn_exp2f(-u * (float)LOG2E);
+    const float y = __builtin_amdgcn_rsqf(r2);
+    float v = r2 * y;
+    const float t = fmaf(-v, v, r2);
+    v = fmaf(t * y, 0.5f, v);
+    const float e = __builtin_amdgcn_exp2f(-v);
+    const float u = v * LN2F;
     k0 = fmaf(fmaf(fmaf(fmaf(p4, u, p3), u, p2), u, p1), u, p0) * e;
     k1 = fmaf(fmaf(fmaf(q3, u, q2), u, q1), u, q0) * e;
     k2 = fmaf(fmaf(c2, u, c1), u, c0) * e;
@@ -161,6 +177,7 @@ __device__ inline double exp2_nonpos(double t) {
 
 template <>
 struct KEval<double, LDDMM_KERNEL_GAUSS> {
+  static constexpr double R2_0 = 0.0;
   double h, cut_r2;
   __device__ void init(int, double sigma) {
     h = LOG2E / (2.0 * sigma * sigma);  // exp(-r^2/(2 sigma^2)) = 2^(-h r^2)
@@ -177,6 +194,7 @@ struct KEval<double, LDDMM_KERNEL_GAUSS> {
 
 template <>
 struct KEval<double, LDDMM_KERNEL_LAPLACIAN> {
+  static constexpr double R2_0 = 0.0;
   double inv_s, p0, p1, p2, p3, p4, q0, q1, q2, q3, c0, c1, c2;
   __device__ void init(int order, double sigma) {
     inv_s = 1.0 / sigma;
@@ -213,8 +231,9 @@ __device__ inline double cull_r2(int prec, int kernel, double sigma) {
     // FP32: K = 2^-r'^2 with prescaled r'.  FP64: K = 2^(-h r^2), h = log2(e)/(2 sigma^2).
     return f32 ? 40.0 : 80.0 * 2.0 * sigma * sigma / LOG2E;
   }
-  const double u_cut = f32 ? 43.0 : 73.0;  // (1+u)^4 e^-u = 2^-40 at u~42.9, 2^-80 at u~72.7
-  return f32 ? u_cut * u_cut : (u_cut * sigma) * (u_cut * sigma);
+  // (1+u)^4 e^-u = 2^-40 at u~42.9, 2^-80 at u~72.7; FP32 coordinates carry log2(e)/sigma.
+  const double u_cut = f32 ? 43.0 * LOG2E : 73.0 * sigma;
+  return u_cut * u_cut;
 }
 
 template <typename R>
@@ -525,7 +544,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
           R dx[D];
-          R r2 = 0;
+          R r2 = KEval<R, KT>::R2_0;
 #pragma unroll
           for (int c = 0; c < D; ++c) {
             dx[c] = ty[r][c] - sx[c];
@@ -696,7 +715,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ksum_mfma(const lddmm_seg_t *__
           wb[u] = s_wt[jcol][j + u];
 #pragma unroll
           for (int r = 0; r < RPT; ++r) {
-            float r2 = 0.f;
+            float r2 = KEval<float, KT>::R2_0;
 #pragma unroll
             for (int c = 0; c < D; ++c) {
               float dx = ty[r][c] - sx[c];
@@ -818,7 +837,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ddk(const lddmm_seg_t *__restri
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
           R dx[D], dp[D];
-          R r2 = 0, xp = 0, aa = 0;
+          R r2 = KEval<R, KT>::R2_0, xp = 0, aa = 0;
 #pragma unroll
           for (int c = 0; c < D; ++c) {
             dx[c] = ty[r][c] - sx[c];
@@ -1163,6 +1182,29 @@ __global__ __launch_bounds__(256) void k_gather_batch(const lddmm_gather_t *__re
     surface_gather(e->vertices, e->faces, e->z1, e->dz, e->incid_ptr, e->incid, i, e->scale, e->px);
   else
     curve_gather(e->z1, e->dz, e->incid_ptr, e->incid, i, e->d, e->scale, e->px);
+}
+
+// Landmark data term: one workgroup per entry, strided FP64 partials, fixed tree.
+__global__ __launch_bounds__(1024) void k_landmark_batch(const lddmm_lmk_t *__restrict__ g) {
+  __shared__ double sh[1024];
+  const lddmm_lmk_t *e = g + blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t n = e->n;
+  const double scale = e->scale;
+  double v = 0.0;
+  for (int64_t i = tid; i < n; i += 1024) {
+    const double u = e->x[i] - e->y[i];
+    v += u * u;
+    if (e->grad) e->grad[i] = scale * u;
+  }
+  if (!e->total) return;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) *e->total = sh[0];
 }
 
 __global__ __launch_bounds__(1024) void k_sum_f64(const double *__restrict__ x, int64_t n,
@@ -1758,6 +1800,17 @@ extern "C" int lddmm_vertex_gradient_batch(const lddmm_gather_t *g_host, const l
   if (blocks > INT32_MAX) return LDDMM_EINVAL;
   hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                      g_dev, n);
+  return hip_check(hipGetLastError());
+}
+
+extern "C" int lddmm_landmark_batch(const lddmm_lmk_t *host, const lddmm_lmk_t *dev, int n,
+                                    void *stream) {
+  if (!host || !dev || n < 1) return LDDMM_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    const lddmm_lmk_t &e = host[i];
+    if (!e.x || !e.y || e.n < 1 || (!e.total && !e.grad)) return LDDMM_EINVAL;
+  }
+  hipLaunchKernelGGL(k_landmark_batch, dim3((unsigned)n), dim3(1024), 0, (hipStream_t)stream, dev);
   return hip_check(hipGetLastError());
 }
 

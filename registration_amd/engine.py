@@ -388,3 +388,27 @@ def gather_batch(shapes, z1s, dzs, scales, pxs) -> None:
     _lib.check(lib.lddmm_vertex_gradient_batch(r.ctypes.data_as(ctypes.POINTER(_lib.Gather)),
                                                dev.data_ptr(), len(shapes), rt.stream_handle()),
                "lddmm_vertex_gradient_batch")
+
+
+LMK_DT = np.dtype(_lib.Landmark)
+
+
+def landmark_batch(xs, ys, totals, grads=None, scales=None) -> None:
+    """Landmark data terms of many registrations in one launch: totals[i] (a 1-element
+    device view) = sum |x_i - y_i|^2 in FP64; grads[i] (optional) = scales[i] (x_i - y_i)."""
+    if not xs:
+        return
+    lib = _lib.load()
+    n = len(xs)
+    r = np.zeros(n, LMK_DT)
+    r["x"] = [x.data_ptr() for x in xs]
+    r["y"] = [y.data_ptr() for y in ys]
+    r["total"] = [rt.ptr(t) for t in totals]
+    if grads is not None:
+        r["grad"] = [rt.ptr(g) for g in grads]
+        r["scale"] = scales
+    r["n"] = [x.numel() for x in xs]
+    dev = rt.upload(r)
+    _lib.check(lib.lddmm_landmark_batch(r.ctypes.data_as(ctypes.POINTER(_lib.Landmark)),
+                                        dev.data_ptr(), n, rt.stream_handle()),
+               "lddmm_landmark_batch")

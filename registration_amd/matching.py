@@ -9,8 +9,9 @@ them with a list of one, which is the plain single-pair path; the serving layer 
 with every registration waiting on the same request.
 
 Reference: py-lddmm/surfaceMatching.py:187-484 and curveMatching.py:148-352
-(affine='none', symmetric=False, errorType='current').  Data-term energies (currents norms)
-are always evaluated in FP64 pairwise arithmetic; the flow, the covector, the RKHS products and
+(affine='none', symmetric=False, errorType='current'), plus a landmark-set data term
+sum |x_T - y|^2 / sigmaError^2 over corresponding points (landmarkMatching.LandmarkMatching).
+Data-term energies (currents norms) are always evaluated in FP64 pairwise arithmetic; the flow, the covector, the RKHS products and
 the data-term gradient use the kernels' precision.  Differences that do not change any result: the trajectory of an accepted trial is kept instead of being recomputed in
 endOfIteration / landmarkHamiltonianCovector (the recomputation is deterministic and
 bit-identical), and the energy sum reuses the flow's K(x_t) a_t instead of a second applyK.
@@ -27,6 +28,10 @@ from . import engine
 from . import runtime as rt
 
 
+CURRENT = 'current'
+LANDMARK = 'landmark'
+
+
 class Direction:
     """Optimisation direction (surfaceMatching.py:52-56): diff is a (T, N, d) device tensor."""
 
@@ -37,35 +42,49 @@ class Direction:
 
 
 class DeviceMatching:
-    """Common device state of SurfaceMatching / CurveMatching."""
+    """Common device state of SurfaceMatching / CurveMatching / LandmarkMatching."""
 
     # subclasses set: coef (1/3 surfaces, 1/2 curves) and _new_shape(vertices)
     coef = 1.0 / 3.0
 
     def _init_device(self, x0_np, faces_np, target_centers, target_cr, KparDiff, KparDist,
-                     sigmaError, Tsize, regweight):
+                     sigmaError, Tsize, regweight, target_points=None):
+        """Device state.  With ``target_points`` (N, d) the data term is the landmark one,
+        sum |x_T - y|^2 / sigmaError^2 over corresponding points, and the shape arguments
+        are unused; otherwise it is the currents norm against (target_centers, target_cr)."""
         dev = rt.device()
         self.npt, self.dim = x0_np.shape
         self.Tsize = Tsize
         self.regweight = float(regweight)
         self.sigmaError = float(sigmaError)
         self.Kdiff = engine.DevKernel.of(KparDiff)
-        self.Kdist = engine.DevKernel.of(KparDist)
-        if self.Kdiff.prec != self.Kdist.prec:
-            raise ValueError("deformation and data kernels must share one precision")
         self.x0 = rt.as_dev(x0_np)
-        self.faces_np = np.asarray(faces_np)
-        self.c2 = rt.as_dev(target_centers)
-        self.cr2 = rt.as_dev(target_cr)
         T, N, d = Tsize, self.npt, self.dim
+        if target_points is not None:
+            self.data_kind = LANDMARK
+            self.y = rt.as_dev(target_points)
+            if tuple(self.y.shape) != (N, d):
+                raise ValueError("landmark target must have the template's shape (N, d)")
+            self.Kdist = None
+        else:
+            self.data_kind = CURRENT
+            self.Kdist = engine.DevKernel.of(KparDist)
+            if self.Kdiff.prec != self.Kdist.prec:
+                raise ValueError("deformation and data kernels must share one precision")
+            self.faces_np = np.asarray(faces_np)
+            self.c2 = rt.as_dev(target_centers)
+            self.cr2 = rt.as_dev(target_cr)
         self.at = torch.zeros((T, N, d), dtype=torch.float64, device=dev)
         self.atTry = torch.zeros_like(self.at)
         # Slice 0 of every trajectory buffer is x0 for the whole optimisation.
         self.xt = self.x0.expand(T + 1, N, d).clone()
         self._xt_try = self.xt.clone()
         self._xt_scratch = self.xt.clone()
-        self.fvDef_dev = self._new_shape(self.x0)
-        self._try_shape = self._new_shape(self.x0)
+        if self.data_kind == CURRENT:
+            self.fvDef_dev = self._new_shape(self.x0)
+            self._try_shape = self._new_shape(self.x0)
+        else:
+            self.fvDef_dev = self._try_shape = None
         self.obj = None
         self.objTry = None
         self.obj0 = None
@@ -99,7 +118,8 @@ class DeviceMatching:
         self.obj = self.objTry
         self.at = self.atTry
         self.xt, self._xt_try = self._xt_try, self.xt
-        self.fvDef_dev.update(self.xt[-1])
+        if self.fvDef_dev is not None:
+            self.fvDef_dev.update(self.xt[-1])
 
     def endOfIteration(self):
         self.iter += 1
@@ -136,13 +156,30 @@ def _readback(buf: torch.Tensor) -> np.ndarray:
     return buf.cpu().numpy()
 
 
+def _data_energies(opts, vertices, shapes, totals) -> None:
+    """FP64 data-term energies of deformed templates into ``totals`` (1-element device
+    views): currents norm against the target for currents registrations (after updating
+    their face geometry in ``shapes``), squared distance to the target points for landmark
+    registrations.  One launch per stage for the whole list."""
+    cur = [i for i, o in enumerate(opts) if o.data_kind == CURRENT]
+    lmk = [i for i, o in enumerate(opts) if o.data_kind == LANDMARK]
+    if cur:
+        engine.geometry_batch([shapes[i] for i in cur], [vertices[i] for i in cur])
+        engine.currents([engine.CurrentsJob(shapes[i].centers, shapes[i].surfel, opts[i].c2,
+                                            opts[i].cr2, opts[i].Kdist.energy(), totals[i],
+                                            coef=opts[i].coef) for i in cur])
+    if lmk:
+        engine.landmark_batch([vertices[i] for i in lmk], [opts[i].y for i in lmk],
+                              [totals[i] for i in lmk])
+
+
 def batch_objective(opts: Sequence[DeviceMatching]) -> list:
     """objectiveFun (surfaceMatching.py:233-250): obj0 once, then the energy of at plus the
     data term of the deformed template."""
     dev = rt.device()
     lens = [o.Tsize + 2 for o in opts]
     scal = torch.zeros(sum(lens), dtype=torch.float64, device=dev)
-    flows, curs, norms, offs = [], [], [], []
+    flows, norms, offs = [], [], []
     off = 0
     for o, L in zip(opts, lens):
         T = o.Tsize
@@ -150,13 +187,13 @@ def batch_objective(opts: Sequence[DeviceMatching]) -> list:
         flows.append(engine.FlowJob(o.x0, o.at, o.Kdiff, o.xt, scal[off:off + T], init=False))
         off += L
     engine.shoot(flows)
-    engine.geometry_batch([o.fvDef_dev for o in opts], [o.xt[-1] for o in opts])
+    _data_energies(opts, [o.xt[-1] for o in opts], [o.fvDef_dev for o in opts],
+                   [scal[off + o.Tsize:off + o.Tsize + 1] for o, off in zip(opts, offs)])
+    # Target self-term (currentNorm0); zero for the landmark data term.
     for o, off in zip(opts, offs):
-        T = o.Tsize
-        curs.append(engine.CurrentsJob(o.fvDef_dev.centers, o.fvDef_dev.surfel, o.c2, o.cr2,
-                                       o.Kdist.energy(), scal[off + T:off + T + 1], coef=o.coef))
-        norms.append(engine.KNormJob(o.c2, o.cr2, o.Kdist.energy(), scal[off + T + 1:off + T + 2]))
-    engine.currents(curs)
+        if o.data_kind == CURRENT:
+            T = o.Tsize
+            norms.append(engine.KNormJob(o.c2, o.cr2, o.Kdist.energy(), scal[off + T + 1:off + T + 2]))
     engine.knorm(norms)
     vals = _readback(scal)
     out = []
@@ -182,7 +219,7 @@ def batch_update_try(items) -> list:
     # atTry = at - eps dir for every registration (multi-tensor launches)
     ats = torch._foreach_mul([d.diff for (_, d, _, _) in items], [-float(e) for (_, _, e, _) in items])
     torch._foreach_add_(ats, [o.at for (o, _, _, _) in items])
-    flows, curs, offs = [], [], []
+    flows, offs = [], []
     off = 0
     for (o, _, _, _), atTry, L in zip(items, ats, lens):
         T = o.Tsize
@@ -190,13 +227,9 @@ def batch_update_try(items) -> list:
         flows.append(engine.FlowJob(o.x0, atTry, o.Kdiff, o._xt_scratch, scal[off:off + T], init=False))
         off += L
     engine.shoot(flows)
-    engine.geometry_batch([o._try_shape for (o, _, _, _) in items],
-                          [o._xt_scratch[-1] for (o, _, _, _) in items])
-    for (o, _, _, _), off in zip(items, offs):
-        T = o.Tsize
-        curs.append(engine.CurrentsJob(o._try_shape.centers, o._try_shape.surfel, o.c2, o.cr2,
-                                       o.Kdist.energy(), scal[off + T:off + T + 1], coef=o.coef))
-    engine.currents(curs)
+    opts = [o for (o, _, _, _) in items]
+    _data_energies(opts, [o._xt_scratch[-1] for o in opts], [o._try_shape for o in opts],
+                   [scal[off + o.Tsize:off + o.Tsize + 1] for o, off in zip(opts, offs)])
     vals = _readback(scal).tolist()
     out = []
     for (o, dir, eps, objRef), atTry, off in zip(items, ats, offs):
@@ -225,9 +258,11 @@ def batch_gradient(items) -> list:
     template, covector along the kept trajectory, dat = 2 lambda a - p, scaled by
     1/(coeff T)."""
     dev = rt.device()
-    # Data-term gradient of every registration: one currents launch, then vertex gathers.
-    curs, px1s, zs = [], [], []
-    for o, _ in items:
+    px1s = [torch.empty((o.npt, o.dim), dtype=torch.float64, device=dev) for o, _ in items]
+    # Currents data-term gradient: one currents launch, then the vertex gathers.
+    cur = [(o, px1) for (o, _), px1 in zip(items, px1s) if o.data_kind == CURRENT]
+    curs, zs = [], []
+    for o, _ in cur:
         sh = o.fvDef_dev
         F, d = sh.centers.shape
         z1 = torch.empty((F, d), dtype=torch.float64, device=dev)
@@ -236,10 +271,16 @@ def batch_gradient(items) -> list:
         curs.append(engine.CurrentsJob(sh.centers, sh.surfel, o.c2, o.cr2, o.Kdist,
                                        torch.empty(1, dtype=torch.float64, device=dev),
                                        z1=z1, dz=dz, coef=o.coef))
-    engine.currents(curs)
-    px1s = [torch.empty((o.npt, o.dim), dtype=torch.float64, device=dev) for o, _ in items]
-    engine.gather_batch([o.fvDef_dev for o, _ in items], [z for z, _ in zs], [z for _, z in zs],
-                        [-2.0 / (o.sigmaError ** 2) for o, _ in items], px1s)
+    if cur:
+        engine.currents(curs)
+        engine.gather_batch([o.fvDef_dev for o, _ in cur], [z for z, _ in zs], [z for _, z in zs],
+                            [-2.0 / (o.sigmaError ** 2) for o, _ in cur], [p for _, p in cur])
+    # Landmark data term: px1 = -2 (x_T - y) / sigmaError^2.
+    lmk = [(o, px1) for (o, _), px1 in zip(items, px1s) if o.data_kind == LANDMARK]
+    if lmk:
+        engine.landmark_batch([o.xt[-1] for o, _ in lmk], [o.y for o, _ in lmk],
+                              [None] * len(lmk), [p for _, p in lmk],
+                              [-2.0 / (o.sigmaError ** 2) for o, _ in lmk])
     covs, pxts = [], []
     for (o, _), px1 in zip(items, px1s):
         pxt = torch.empty_like(o.xt)

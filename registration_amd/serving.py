@@ -2,7 +2,7 @@
 multi-registration mode is a sequential loop over target files,
 py-lddmm/surfaceMatchingDirectory.py:46-55).
 
-Requests (surface pairs, curve pairs) arrive over time.  Each admitted request becomes a
+Requests (surface pairs, curve pairs, landmark sets) arrive over time.  Each admitted request becomes a
 GPU-resident registration (matching.DeviceMatching) whose optimiser runs as a generator
 (conjugateGradient.cg_steps).  One scheduling round advances every active registration by one
 optimiser request: requests of the same kind (line-search trial, gradient, RKHS products,
@@ -29,6 +29,7 @@ from . import conjugateGradient as cg
 from . import curveMatching as cm
 from . import curves
 from . import kernelFunctions as kfun
+from . import landmarkMatching as lm
 from . import matching
 from . import surfaceMatching as sm
 from . import surfaces
@@ -36,8 +37,10 @@ from . import surfaces
 
 @dataclass
 class Request:
-    """One registration: ``kind`` 'surface' or 'curve'; template/target are (faces, vertices);
-    kernel names/widths as in SurfaceMatchingParam; ``arrival`` in seconds from service start."""
+    """One registration: ``kind`` 'surface', 'curve' or 'landmark'; template/target are
+    (faces, vertices), with faces None for landmark sets (corresponding points); kernel
+    names/widths as in SurfaceMatchingParam (sigmaDist unused by landmark sets); ``arrival``
+    in seconds from service start."""
 
     rid: int
     kind: str
@@ -63,6 +66,7 @@ class Result:
     admitted: float
     finished: float
     history: list = field(default_factory=list)
+    momenta: Optional[torch.Tensor] = None   # final at (T, N, d) when the server keeps it
 
     @property
     def latency(self) -> float:
@@ -79,6 +83,10 @@ def build_registration(req: Request, precision: str):
         return sm.SurfaceMatching(Template=surfaces.Surface(FV=req.template),
                                   Target=surfaces.Surface(FV=req.target), param=p,
                                   maxIter=req.maxIter, testGradient=False, verb=False)
+    if req.kind == 'landmark':
+        p = lm.LandmarkMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, sigmaError=req.sigmaError)
+        return lm.LandmarkMatching(Template=req.template[1], Target=req.target[1], param=p,
+                                   maxIter=req.maxIter, verb=False)
     if req.kind == 'curve':
         p = cm.CurveMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, KparDist=Kw,
                                   sigmaError=req.sigmaError, errorType='current')
@@ -91,14 +99,13 @@ def build_registration(req: Request, precision: str):
 def footprint_bytes(req: Request) -> int:
     """Device memory of one active registration (float64): trajectories (3 buffers), momenta,
     trial, direction, gradient and CG history (~8 (T,N,d) arrays), covector, and the currents
-    buffers of the deformed shapes."""
+    buffers of the deformed shapes (landmark sets: the target and the data gradient)."""
     F, V = req.template
     N, d = V.shape
-    nf = F.shape[0]
     T = req.T
     traj = 3 * (T + 1) * N * d
     mom = 9 * T * N * d
-    cur = 10 * nf * d
+    cur = 10 * F.shape[0] * d if F is not None else 2 * N * d
     return 8 * (traj + mom + cur) + 64 * 1024
 
 
@@ -120,11 +127,14 @@ class Server:
     """Continuous-batching service on the current GPU.
 
     hbm_cap_bytes: admission cap on the summed footprint of active registrations;
-    max_active: cap on concurrently optimised registrations; precision: 'fp32c' or 'fp64'.
+    max_active: cap on concurrently optimised registrations; precision: 'fp32c' or 'fp64';
+    keep_momenta: results carry the final momenta (device tensors) for checking.
     """
 
     def __init__(self, hbm_cap_bytes: float = 200e9, max_active: int = 256,
-                 precision: str = 'fp32c', clock: Callable[[], float] = time.perf_counter):
+                 precision: str = 'fp32c', clock: Callable[[], float] = time.perf_counter,
+                 keep_momenta: bool = False):
+        self.keep_momenta = bool(keep_momenta)
         self.cap = float(hbm_cap_bytes)
         self.max_active = int(max_active)
         self.precision = precision
@@ -161,7 +171,8 @@ class Server:
     def _finish(self, a: _Active, value) -> None:
         o = a.opt
         self.results[a.req.rid] = Result(a.req.rid, float(o.obj), len(o.history) - 1, o.stop_reason,
-                                         a.req.arrival, a.admitted, self._now(), list(o.history))
+                                         a.req.arrival, a.admitted, self._now(), list(o.history),
+                                         o.at if self.keep_momenta else None)
         self.used -= footprint_bytes(a.req)
 
     def step(self) -> None:

@@ -138,3 +138,80 @@ def served_surface_requests(n=512, seed=4, subdivs=(3, 4, 5, 6)):
                         sigmaKernel=float(rng.uniform(0.2, 0.5)), sigmaDist=0.1, sigmaError=0.1,
                         T=10, maxIter=30, subdiv=s))
     return out
+
+
+def _log_uniform_int(rng, lo, hi):
+    return int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+
+
+def landmark_momenta_scale(n, sigma, d=3):
+    """Momentum scale of the landmark targets: 0.1 / sqrt(1 + n sigma^d), so that the summed
+    kernel-weighted momenta (about n sigma^d points within one kernel width of uniform points
+    in the unit ball) displace points by about 0.1 whatever n and sigma."""
+    return 0.1 / np.sqrt(1.0 + n * sigma ** d)
+
+
+def mixed_service_requests(n=2000, seed=5, shoot=None, rate=200.0, burst_factor=5.0,
+                           burst_prob=0.2):
+    """The mixed-service stream: 40% 2D closed curves (256-8192 points, log-uniform; star
+    targets r = 1 + A cos(k theta), k in 3..8, A ~ U[0.05, 0.3]), 40% surfaces (icosphere
+    subdivision 3-7, uniform in subdivision = log-uniform in vertex count; random rotation;
+    RBF targets as for the large surface pair), 20% 3D landmark sets (1,000-200,000 points
+    uniform in the unit ball, log-uniform; the target is x_T of one seeded geodesic shooting
+    from momenta N(0, s^2), s = landmark_momenta_scale).  Per request: Gaussian or Laplacian
+    (order 3) deformation kernel with sigma ~ U[0.1, 0.5], T in {10, 20, 40}, currents kernel
+    width 0.1, sigma_err 0.1, 30 CG iterations.  Arrivals: Poisson at ``rate`` with each gap
+    shortened ``burst_factor`` times with probability ``burst_prob`` (bursts).
+
+    Request i is drawn from its own generator seeded (seed, i), so the first k requests of
+    any longer stream are the stream of length k.  shoot(x0, a0, kernel_name, sigma, T) -> x_T
+    builds the landmark targets; the default uses this package's GPU geodesic shooting
+    (pointEvolution.landmarkEPDiff).  Returns a list of dicts with the Request fields plus
+    'kind', 'size' and 'arrival'."""
+    if shoot is None:
+        shoot = _gpu_shoot
+    meshes = {}
+    out = []
+    t = 0.0
+    for i in range(n):
+        rng = np.random.default_rng((seed, i))
+        gap = rng.exponential(1.0 / rate)
+        t += gap / burst_factor if rng.random() < burst_prob else gap
+        u = rng.random()
+        kernel = 'gauss' if rng.random() < 0.5 else 'laplacian'
+        sigma = float(rng.uniform(0.1, 0.5))
+        T = int(rng.choice([10, 20, 40]))
+        common = dict(sigmaKernel=sigma, sigmaDist=0.1, sigmaError=0.1, T=T, maxIter=30,
+                      typeKernel=kernel, order=10 if kernel == 'gauss' else 3, arrival=t)
+        if u < 0.4:
+            npts = _log_uniform_int(rng, 256, 8192)
+            X, _ = circle(npts)
+            Y = star_curve(npts, int(rng.integers(3, 9)), float(rng.uniform(0.05, 0.3)))
+            E = closed_curve_edges(npts)
+            out.append(dict(kind='curve', template=(E, X), target=(E, Y), size=npts, **common))
+        elif u < 0.8:
+            s = int(rng.integers(3, 8))
+            if s not in meshes:
+                meshes[s] = icosphere(s)
+            V, F = meshes[s]
+            R = random_rotation(rng)
+            Vr = V @ R.T
+            out.append(dict(kind='surface', template=(F, Vr), target=(F, rbf_field(rng, Vr)),
+                            size=V.shape[0], subdiv=s, **common))
+        else:
+            npts = _log_uniform_int(rng, 1000, 200000)
+            X = ball_points(rng, npts)
+            a0 = landmark_momenta_scale(npts, sigma) * rng.standard_normal(X.shape)
+            Y = shoot(X, a0, kernel, sigma, T)
+            out.append(dict(kind='landmark', template=(None, X), target=(None, Y), size=npts,
+                            **common))
+    return out
+
+
+def _gpu_shoot(x0, a0, kernel_name, sigma, T):
+    from . import kernelFunctions as kfun
+    from . import pointEvolution as pe
+    K = kfun.Kernel(name=kernel_name, sigma=sigma, order=10 if kernel_name == 'gauss' else 3,
+                    precision='fp64')
+    xt, _ = pe.landmarkEPDiff(T, x0, a0, K)
+    return np.asarray(xt[-1])

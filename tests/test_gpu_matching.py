@@ -140,3 +140,68 @@ def test_nan_trial_is_rejected():
     before = g.objTry
     assert g.updateTry(d, 0.1, g.obj) == float('inf')
     assert g.objTry is before
+
+
+def landmark_pair(prec, n=300, T=5):
+    from registration_amd import kernelFunctions as kfun, landmarkMatching as lm, synthetic
+    rng = np.random.default_rng(31)
+    X = synthetic.ball_points(rng, n)
+    xt, _ = orc.landmarkEPDiff(T, X, 0.2 * rng.standard_normal(X.shape), orc.Kernel('gauss', sigma=0.3))
+    Y = xt[-1]
+    p = lm.LandmarkMatchingParam(timeStep=1.0 / T, KparDiff=kfun.Kernel('gauss', sigma=0.3, precision=prec),
+                                 sigmaError=0.1)
+    g = lm.LandmarkMatching(Template=X, Target=Y, param=p, maxIter=8, verb=False)
+    o = orc.LandmarkMatching(X, Y, orc.Kernel('gauss', sigma=0.3), sigmaError=0.1, timeStep=1.0 / T,
+                             maxIter=8)
+    return g, o
+
+
+def test_landmark_batch_kernel():
+    import torch
+    from registration_amd import engine
+    rng = np.random.default_rng(3)
+    xs = [rng.standard_normal((n, 3)) for n in (1, 700, 5000)]
+    ys = [rng.standard_normal(x.shape) for x in xs]
+    xd = [torch.as_tensor(x, device="cuda") for x in xs]
+    yd = [torch.as_tensor(y, device="cuda") for y in ys]
+    tot = torch.zeros(3, dtype=torch.float64, device="cuda")
+    grads = [torch.empty_like(x) for x in xd]
+    engine.landmark_batch(xd, yd, [tot[i:i + 1] for i in range(3)], grads, [-2.0, 0.5, 3.0])
+    for i, (x, y, s) in enumerate(zip(xs, ys, (-2.0, 0.5, 3.0))):
+        ref = ((x - y) ** 2).sum()
+        assert abs(tot[i].item() - ref) <= 1e-13 * ref
+        assert np.array_equal(grads[i].cpu().numpy(), s * (x - y))
+    # energy only, batched vs alone: bit-identical
+    alone = torch.zeros(1, dtype=torch.float64, device="cuda")
+    engine.landmark_batch(xd[2:], yd[2:], [alone])
+    assert alone.item() == tot[2].item()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32c", 1e-6)])
+def test_landmark_requests_match_oracle(prec, tol):
+    import torch
+    g, o = landmark_pair(prec)
+    rng = np.random.default_rng(2)
+    a = 0.05 * rng.standard_normal(o.at.shape)
+    g.at = torch.as_tensor(a, device="cuda")
+    o.at = a.copy()
+    assert abs(g.objectiveFun() - o.objectiveFun()) < tol * abs(o.obj)
+    o.endOfIteration()
+    gg, go = g.getGradient(g.gradCoeff), o.getGradient(o.gradCoeff)
+    err = np.abs(gg.diff.cpu().numpy() - go.diff).max() / np.abs(go.diff).max()
+    assert err < 10 * tol
+    [d1] = g.dotProduct(gg, [gg])
+    [r1] = o.dotProduct(go, [go])
+    assert abs(d1 - r1) < 10 * tol * abs(r1)
+    t1 = g.updateTry(gg, 0.05, None)
+    t2 = o.updateTry(go, 0.05, None)
+    assert abs(t1 - t2) < tol * abs(t2)
+
+
+def test_landmark_cg_sequence_fp64():
+    g, o = landmark_pair("fp64")
+    g.optimizeMatching()
+    o.optimizeMatching()
+    hg, ho = np.array(g.history), np.array(o.history)
+    assert len(hg) == len(ho) and len(hg) >= 4
+    assert np.all(np.abs(hg - ho) <= 1e-9 * np.abs(ho)), (hg, ho)

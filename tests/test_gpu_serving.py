@@ -25,6 +25,14 @@ def requests():
                                     sigmaKernel=0.3, sigmaDist=0.1, sigmaError=0.1, T=5,
                                     maxIter=6, typeKernel='laplacian' if i else 'gauss',
                                     order=1 if i else 10))
+    for i, n in enumerate((200, 513)):
+        X = synthetic.ball_points(rng, n)
+        a0 = 0.2 * rng.standard_normal(X.shape)
+        Y = synthetic._gpu_shoot(X, a0, 'gauss', 0.3, 5)
+        reqs.append(serving.Request(rid=20 + i, kind='landmark', template=(None, X),
+                                    target=(None, Y), sigmaKernel=0.3 + 0.1 * i, sigmaDist=0.1,
+                                    sigmaError=0.1, T=5 - i, maxIter=6,
+                                    typeKernel='laplacian' if i else 'gauss', order=3 if i else 10))
     return reqs
 
 

@@ -223,6 +223,53 @@ def test_currents_gradient_fd(kind):
         assert abs(fd - g[i, c]) < 1e-6 * max(1.0, abs(fd))
 
 
+# ---------------------------------------------------------------- landmarks
+def test_landmark_data_term_closed_form_and_fd():
+    rng = np.random.default_rng(13)
+    X, Y = rng.standard_normal((30, 3)), rng.standard_normal((30, 3))
+    A, B = orc.PointSet(X), orc.PointSet(Y)
+    assert orc.landmark_normDef(A, A, None) == 0.0 and orc.landmark_norm0(B, None) == 0.0
+    one = orc.landmark_normDef(orc.PointSet(X[:1]), orc.PointSet(Y[:1]), None)
+    assert abs(one - np.linalg.norm(X[0] - Y[0]) ** 2) < 1e-14 * one
+    g = orc.landmark_normGradient(A, B, None)
+    h = 1e-6
+    for (i, c) in [(0, 0), (5, 2), (29, 1)]:
+        e = np.zeros_like(X)
+        e[i, c] = h
+        fd = (orc.landmark_normDef(orc.PointSet(X + e), B, None)
+              - orc.landmark_normDef(orc.PointSet(X - e), B, None)) / (2 * h)
+        assert abs(fd - g[i, c]) < 1e-7 * max(1.0, abs(fd))
+
+
+def test_landmark_matching_gradient_is_adjoint():
+    """Directional derivative of the landmark objective (central FD) equals the RKHS product
+    of getGradient with the direction: pins the sign and the 2/sigma^2 of p_T."""
+    rng = np.random.default_rng(14)
+    X = synthetic.ball_points(rng, 60)
+    Y = X + 0.1 * rng.standard_normal(X.shape)
+    o = orc.LandmarkMatching(X, Y, orc.Kernel('gauss', sigma=0.4), sigmaError=0.2, timeStep=0.2)
+    o.at = 0.05 * rng.standard_normal(o.at.shape)
+    o.objectiveFun()
+    o.endOfIteration()
+    g = o.getGradient(1.0)
+    d = orc.Direction()
+    d.diff = rng.standard_normal(o.at.shape)
+    h = 1e-5
+    fd = (o.updateTry(d, -h) - o.updateTry(d, h)) / (2 * h)
+    [gd] = o.dotProduct(g, [d])
+    assert abs(fd - gd) < 1e-6 * abs(fd)
+
+
+def test_landmark_matching_reduces_objective():
+    rng = np.random.default_rng(15)
+    X = synthetic.ball_points(rng, 50)
+    K = orc.Kernel('gauss', sigma=0.4)
+    xt, _ = orc.landmarkEPDiff(5, X, 0.3 * rng.standard_normal(X.shape), K)
+    o = orc.LandmarkMatching(X, xt[-1], K, sigmaError=0.1, timeStep=0.2, maxIter=10)
+    h = o.optimizeMatching()
+    assert len(h) >= 3 and h[-1] < 0.2 * h[0]
+
+
 # ---------------------------------------------------------------- optimiser
 class _Quadratic:
     """f(v) = 0.5 v^T A v - b^T v with an SPD A: the optimiser must reach A^{-1} b."""
