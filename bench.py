@@ -27,6 +27,11 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (AMD published)
 FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
+# HBM bytes per fused flow step at 40,962 points (pair kernel + finalize + totals), measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on `bench.py --roofline-only --roofline-reps 3` and
+# scaled by the counters' reading for 256 MiB device copies (FETCH x2.0, WRITE x1.0):
+# 41.0 MB read + 23.6 MB written (profiles/flow_step_s6_fp32c.md).
+FLOW_STEP_TRAFFIC_S6 = 64.6e6
 
 
 def parse():
@@ -96,7 +101,9 @@ def flow_step_roofline(cfg, prec, reps):
     flops = FLOPS_PER_PAIR["apply_k_gauss_d3_m3"] * pairs
     tflops = flops / (ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+            "traffic": FLOW_STEP_TRAFFIC_S6 if N == 40962 else None,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
             "kernel": "k_pairs<float,gauss,3,3> + finalize (Euler flow step)",
             "launch_ms": round(ms, 4), "pairs_per_launch": pairs, "flops_per_pair": 16,
             "pairs_per_s": pairs / (ms * 1e-3)}
