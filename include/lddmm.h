@@ -137,6 +137,18 @@ int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *segs_host,
                        int64_t n_fin_blocks, void *work, size_t work_bytes,
                        void *stream);
 
+/* n_steps launches in order on one stream: launch t runs the nseg descriptors
+ * segs_host[t*nseg .. (t+1)*nseg) (device copy at segs_dev + t*nseg, required).
+ * Every step must have the sizes of step 0, which was planned (n_items,
+ * n_fin_blocks, work_bytes as returned for it) and whose scheduling fields
+ * every step carries.  Used for the T time steps of Euler flows and adjoints,
+ * where step t reads what step t-1 wrote.  No counterpart in the reference
+ * (its time loop calls the kernel sum once per step, pointEvolution.py:691-697). */
+int lddmm_segments_run_steps(int op, int prec, int d, const lddmm_seg_t *segs_host,
+                             const lddmm_seg_t *segs_dev, int n_steps, int nseg,
+                             int64_t n_items, int64_t n_fin_blocks, void *work,
+                             size_t work_bytes, void *stream);
+
 /* ---- one-call forms of the reference's f2py kernel sums ----------------
  * These plan one segment and copy its descriptor into the head of `work`. */
 

@@ -1582,6 +1582,29 @@ extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *se
   return hip_check(hipGetLastError());
 }
 
+extern "C" int lddmm_segments_run_steps(int op, int prec, int d, const lddmm_seg_t *segs_host,
+                                        const lddmm_seg_t *segs_dev, int n_steps, int nseg,
+                                        int64_t n_items, int64_t n_fin_blocks, void *work,
+                                        size_t work_bytes, void *stream) {
+  if (n_steps < 1 || nseg < 1 || !segs_host || !segs_dev) return LDDMM_EINVAL;
+  for (int t = 0; t < n_steps; ++t) {
+    const lddmm_seg_t *h = segs_host + (int64_t)t * nseg;
+    // Steps share step 0's plan: same sizes and scheduling fields, segment by segment.
+    for (int i = 0; t > 0 && i < nseg; ++i) {
+      const lddmm_seg_t &a = h[i], &b = segs_host[i];
+      if (a.ny != b.ny || a.nx0 != b.nx0 || a.nx1 != b.nx1 || a.item0 != b.item0 ||
+          a.part0 != b.part0 || a.fin0 != b.fin0 || a.red0 != b.red0 ||
+          a.row_tiles != b.row_tiles || a.chunks0 != b.chunks0 || a.chunks1 != b.chunks1 ||
+          a.chunk_len != b.chunk_len || a.fin_blocks != b.fin_blocks)
+        return LDDMM_EINVAL;
+    }
+    int rc = lddmm_segments_run(op, prec, d, h, segs_dev + (int64_t)t * nseg, nseg, n_items,
+                                n_fin_blocks, work, work_bytes, stream);
+    if (rc) return rc;
+  }
+  return LDDMM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // One-call forms of the reference's kernel sums.
 // ---------------------------------------------------------------------------

@@ -66,10 +66,10 @@ def _runs(lengths, n_steps):
     """Maximal runs [t0, t1) of steps with one active set, for jobs of ``lengths`` steps
     (job i is active at steps t < lengths[i]): yields (t0, t1, [job indices])."""
     lengths = np.asarray(lengths)
-    bounds = sorted(set(int(v) for v in lengths if v < n_steps) | {0, n_steps})
+    bounds = np.unique(np.concatenate([lengths[lengths < n_steps], [0, n_steps]])).tolist()
     for t0, t1 in zip(bounds[:-1], bounds[1:]):
-        idx = np.nonzero(lengths > t0)[0].tolist()
-        if idx and t1 > t0:
+        idx = np.nonzero(lengths > t0)[0]
+        if idx.size and t1 > t0:
             yield t0, t1, idx
 
 
@@ -100,6 +100,41 @@ class FlowJob:
         return 1.0 / self.T if self.dt is None else self.dt
 
 
+def flow_record(N: int, T: int, K: DevKernel, dt: Optional[float] = None) -> np.ndarray:
+    """The pointer-free part of a registration's flow-step descriptor (one record)."""
+    r = rt.segs(1)
+    r["ny"] = N
+    r["sigma"] = K.sigma
+    r["dt"] = 1.0 / T if dt is None else dt
+    r["kernel"] = K.kernel
+    r["order"] = K.order
+    return r
+
+
+def shoot_records(base, prec, d, T, xt, at, energy, v=None) -> None:
+    """Flow launches for n trajectories of one (kernel family, precision, d) from arrays:
+    ``base`` (n,) flow_record rows; T (n,) step counts; xt, at, energy (n,) device base
+    addresses of the (T+1, N, d) trajectories, (T, N, d) momenta and (T,) energy rows; v
+    optional velocity outputs (0 = none).  Time step t of every trajectory with t < T goes
+    into launch t."""
+    T = np.asarray(T, dtype=np.int64)
+    stride = base["ny"].astype(np.uint64) * np.uint64(d * 8)
+    xt, at, en = _u64(xt), _u64(at), _u64(energy)
+    vv = _u64(v) if v is not None else None
+    for t0, t1, idx in _runs(T, int(T.max())):
+        i = np.asarray(idx)
+        tt = np.arange(t0, t1, dtype=np.uint64)[:, None]
+        steps = np.repeat(base[i][None, :], t1 - t0, axis=0)
+        st = stride[i]
+        steps["y"] = xt[i] + tt * st
+        steps["w0"] = at[i] + tt * st
+        steps["out0"] = xt[i] + (tt + 1) * st
+        if vv is not None:
+            steps["out1"] = np.where(vv[i] != 0, vv[i] + tt * st, 0)
+        steps["total"] = en[i] + tt * np.uint64(8)
+        rt.run_steps(_lib.OP_FLOW_STEP, prec, d, steps)
+
+
 def shoot(jobs: Sequence[FlowJob]) -> None:
     """xt[t+1] = xt[t] + dt K(xt[t]) at[t], energy[t] = <at[t], K(xt[t]) at[t]>, dt = 1/T."""
     init = [j for j in jobs if j.init]
@@ -107,29 +142,10 @@ def shoot(jobs: Sequence[FlowJob]) -> None:
         torch._foreach_copy_([j.xt[0] for j in init], [j.x0 for j in init])
     for group in _groups(jobs, lambda j: (j.K.family, j.x0.shape[1])):
         d = group[0].x0.shape[1]
-        prec = group[0].K.prec
-        Ts = [j.T for j in group]
-        for t0, t1, idx in _runs(Ts, max(Ts)):
-            g = [group[i] for i in idx]
-            base = rt.segs(len(g))
-            base["ny"] = [j.xt.shape[1] for j in g]
-            base["sigma"] = [j.K.sigma for j in g]
-            base["dt"] = [j.step for j in g]
-            base["kernel"] = [j.K.kernel for j in g]
-            base["order"] = [j.K.order for j in g]
-            stride = _u64([j.xt.shape[1] * d * 8 for j in g])
-            xt = _u64([j.xt.data_ptr() for j in g])
-            at = _u64([j.at.data_ptr() for j in g])
-            en = _u64([j.energy.data_ptr() for j in g])
-            vv = _u64([rt.ptr(j.v) for j in g])
-            tt = np.arange(t0, t1, dtype=np.uint64)[:, None]
-            steps = np.repeat(base[None, :], t1 - t0, axis=0)
-            steps["y"] = xt + tt * stride
-            steps["w0"] = at + tt * stride
-            steps["out0"] = xt + (tt + 1) * stride
-            steps["out1"] = np.where(vv != 0, vv + tt * stride, 0)
-            steps["total"] = en + tt * 8
-            rt.run_steps(_lib.OP_FLOW_STEP, prec, d, steps)
+        base = rt.stack([rt.raw(flow_record(j.xt.shape[1], j.T, j.K, j.step)) for j in group])
+        shoot_records(base, group[0].K.prec, d, [j.T for j in group],
+                      [j.xt.data_ptr() for j in group], [j.at.data_ptr() for j in group],
+                      [j.energy.data_ptr() for j in group], [rt.ptr(j.v) for j in group])
 
 
 # --------------------------------------------------------------------------------------
@@ -198,28 +214,39 @@ class KDotJob:
     out: torch.Tensor           # (T, 2) output totals: out[t, j] = <g2[j][t], K(xt[t]) g1[t]>
 
 
+def kdot_records(base, prec, d, T, xt, g1, g2, na, out) -> None:
+    """One launch of RKHS products for n trajectories of one (kernel family, precision, d):
+    ``base`` (n,) flow_record rows; T (n,) step counts; xt, g1 (n,) base addresses; g2 (n, 2)
+    base addresses of the second factors (na[i] of them used); out (n,) addresses of (T, 2)
+    totals.  Every time step of every trajectory is a segment."""
+    T = np.asarray(T, dtype=np.int64)
+    n = len(T)
+    stride = base["ny"].astype(np.uint64) * np.uint64(d * 8)
+    rep = np.repeat(np.arange(n), T)                       # trajectory of each segment
+    first = np.concatenate([[0], np.cumsum(T)[:-1]])
+    tt = (np.arange(int(T.sum())) - first[rep]).astype(np.uint64)
+    r = base[rep].copy()
+    r["dt"] = 0.0
+    r["na"] = np.asarray(na, dtype=np.int32)[rep]
+    st = stride[rep]
+    r["y"] = _u64(xt)[rep] + tt * st
+    r["w0"] = _u64(g1)[rep] + tt * st
+    g2 = _u64(g2).reshape(n, 2)
+    for q in range(2):
+        r["g"][:, q] = np.where(g2[rep, q] != 0, g2[rep, q] + tt * st, 0)
+    r["total"] = _u64(out)[rep] + tt * np.uint64(16)
+    rt.launch(_lib.OP_KDOT, prec, d, r)
+
+
 def kdot(jobs: Sequence[KDotJob]) -> None:
     """One launch: every time step of every job is a segment."""
     for group in _groups(jobs, lambda j: (j.K.family, j.xt.shape[2])):
         d = group[0].xt.shape[2]
-        parts = []
-        for j in group:
-            T, N = j.g1.shape[0], j.xt.shape[1]
-            r = rt.segs(T)
-            r["ny"] = N
-            r["sigma"] = j.K.sigma
-            r["kernel"] = j.K.kernel
-            r["order"] = j.K.order
-            r["na"] = len(j.g2)
-            stride = np.uint64(N * d * 8)
-            tt = np.arange(T, dtype=np.uint64)
-            r["y"] = np.uint64(j.xt.data_ptr()) + tt * stride
-            r["w0"] = np.uint64(j.g1.data_ptr()) + tt * stride
-            for q, g in enumerate(j.g2):
-                r["g"][:, q] = np.uint64(g.data_ptr()) + tt * stride
-            r["total"] = np.uint64(j.out.data_ptr()) + tt * np.uint64(16)
-            parts.append(r)
-        rt.launch(_lib.OP_KDOT, group[0].K.prec, d, np.concatenate(parts))
+        base = rt.stack([rt.raw(flow_record(j.xt.shape[1], j.g1.shape[0], j.K)) for j in group])
+        g2 = [[g.data_ptr() for g in j.g2] + [0] * (2 - len(j.g2)) for j in group]
+        kdot_records(base, group[0].K.prec, d, [j.g1.shape[0] for j in group],
+                     [j.xt.data_ptr() for j in group], [j.g1.data_ptr() for j in group], g2,
+                     [len(j.g2) for j in group], [j.out.data_ptr() for j in group])
 
 
 # --------------------------------------------------------------------------------------
@@ -236,6 +263,30 @@ class CurrentsJob:
     z1: Optional[torch.Tensor] = None   # (F1, d) gradient form outputs
     dz: Optional[torch.Tensor] = None
     coef: float = 1.0 / 3.0
+
+
+def currents_record(c1, cr1, c2, cr2, K: DevKernel, coef: float) -> np.ndarray:
+    """The currents descriptor of (c1, cr1) against (c2, cr2) without its outputs (one record)."""
+    r = rt.segs(1)
+    r["y"] = c1.data_ptr()
+    r["w0"] = cr1.data_ptr()
+    r["x1"] = c2.data_ptr()
+    r["w1"] = cr2.data_ptr()
+    r["ny"] = c1.shape[0]
+    r["nx1"] = c2.shape[0]
+    r["sigma"] = K.sigma
+    r["coef"] = coef
+    r["kernel"] = K.kernel
+    r["order"] = K.order
+    return r
+
+
+def currents_records(recs, prec, d, totals) -> None:
+    """Currents energies of n records of one (kernel family, precision, d) into ``totals``
+    (device addresses), one launch."""
+    r = np.array(recs, copy=True)
+    r["total"] = _u64(totals)
+    rt.launch(_lib.OP_CURRENTS, prec, d, r)
 
 
 def currents(jobs: Sequence[CurrentsJob]) -> None:
@@ -340,7 +391,7 @@ GATHER_DT = np.dtype(_lib.Gather)
 
 
 def _blk0(counts):
-    c = np.asarray([-(-int(n) // 256) for n in counts], dtype=np.int64)
+    c = (np.asarray(counts, dtype=np.int64) + 255) // 256
     return np.concatenate([[0], np.cumsum(c)[:-1]])
 
 
@@ -364,6 +415,29 @@ def geometry_batch(shapes, vertices) -> None:
     dev = rt.upload(r)
     _lib.check(lib.lddmm_geometry_batch(r.ctypes.data_as(ctypes.POINTER(_lib.Geom)), dev.data_ptr(),
                                         len(shapes), rt.stream_handle()), "lddmm_geometry_batch")
+
+
+def geometry_record(sh) -> np.ndarray:
+    """The geometry descriptor of a device shape without its vertex address (one record)."""
+    r = np.zeros(1, GEOM_DT)
+    r["faces"] = sh.faces.data_ptr()
+    r["centers"] = sh.centers.data_ptr()
+    r["normals"] = sh.surfel.data_ptr()
+    r["nf"] = sh.faces.shape[0]
+    r["k"] = sh.faces.shape[1]
+    r["d"] = sh.centers.shape[1]
+    return r
+
+
+def geometry_records(recs, vertices) -> None:
+    """Face geometry of n shapes from their geometry_record rows and vertex addresses."""
+    lib = _lib.load()
+    r = np.array(recs, copy=True)
+    r["vertices"] = _u64(vertices)
+    r["blk0"] = _blk0(r["nf"])
+    dev = rt.upload(r)
+    _lib.check(lib.lddmm_geometry_batch(r.ctypes.data_as(ctypes.POINTER(_lib.Geom)), dev.data_ptr(),
+                                        len(r), rt.stream_handle()), "lddmm_geometry_batch")
 
 
 def gather_batch(shapes, z1s, dzs, scales, pxs) -> None:

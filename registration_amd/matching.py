@@ -85,6 +85,19 @@ class DeviceMatching:
             self._try_shape = self._new_shape(self.x0)
         else:
             self.fvDef_dev = self._try_shape = None
+        # Pointer-free descriptor parts of this registration's launches, built once; the
+        # batched requests fill in the buffer addresses of each call with NumPy.
+        # Kept as raw bytes (runtime.raw / runtime.stack).
+        self._flow_base = rt.raw(engine.flow_record(N, T, self.Kdiff))
+        self._flow_key = (self.Kdiff.kernel, self.Kdiff.prec, d)
+        self._slice_bytes = N * d * 8
+        if self.data_kind == CURRENT:
+            Ke = self.Kdist.energy()
+            self._cur_try = rt.raw(engine.currents_record(self._try_shape.centers,
+                                                          self._try_shape.surfel, self.c2,
+                                                          self.cr2, Ke, self.coef))
+            self._cur_key = (Ke.kernel, Ke.prec, d)
+            self._geom_try = rt.raw(engine.geometry_record(self._try_shape))
         self.obj = None
         self.objTry = None
         self.obj0 = None
@@ -211,28 +224,53 @@ def batch_objective(opts: Sequence[DeviceMatching]) -> list:
     return out
 
 
+def _by_key(keys):
+    """Index arrays of equal keys, in first-appearance order."""
+    groups = {}
+    for i, k in enumerate(keys):
+        groups.setdefault(k, []).append(i)
+    return [np.asarray(v) for v in groups.values()]
+
+
 def batch_update_try(items) -> list:
     """updateTry (surfaceMatching.py:255-289) for [(opt, dir, eps, objRef), ...]."""
     dev = rt.device()
-    lens = [o.Tsize + 1 for (o, _, _, _) in items]
-    scal = torch.zeros(sum(lens), dtype=torch.float64, device=dev)
+    n = len(items)
+    opts = [it[0] for it in items]
+    Ts = np.fromiter((o.Tsize for o in opts), np.int64, n)
+    offs = np.zeros(n, np.int64)
+    np.cumsum(Ts[:-1] + 1, out=offs[1:])
+    scal = torch.zeros(int(offs[-1] + Ts[-1] + 1), dtype=torch.float64, device=dev)
+    sbase = np.uint64(scal.data_ptr())
     # atTry = at - eps dir for every registration (multi-tensor launches)
-    ats = torch._foreach_mul([d.diff for (_, d, _, _) in items], [-float(e) for (_, _, e, _) in items])
-    torch._foreach_add_(ats, [o.at for (o, _, _, _) in items])
-    flows, offs = [], []
-    off = 0
-    for (o, _, _, _), atTry, L in zip(items, ats, lens):
-        T = o.Tsize
-        offs.append(off)
-        flows.append(engine.FlowJob(o.x0, atTry, o.Kdiff, o._xt_scratch, scal[off:off + T], init=False))
-        off += L
-    engine.shoot(flows)
-    opts = [o for (o, _, _, _) in items]
-    _data_energies(opts, [o._xt_scratch[-1] for o in opts], [o._try_shape for o in opts],
-                   [scal[off + o.Tsize:off + o.Tsize + 1] for o, off in zip(opts, offs)])
+    ats = torch._foreach_mul([it[1].diff for it in items], [-float(it[2]) for it in items])
+    torch._foreach_add_(ats, [o.at for o in opts])
+    xt = np.fromiter((o._xt_scratch.data_ptr() for o in opts), np.uint64, n)
+    at = np.fromiter((a.data_ptr() for a in ats), np.uint64, n)
+    slice_bytes = np.fromiter((o._slice_bytes for o in opts), np.uint64, n)
+    energy = sbase + np.uint64(8) * offs.astype(np.uint64)
+    data_total = energy + np.uint64(8) * Ts.astype(np.uint64)
+    final = xt + Ts.astype(np.uint64) * slice_bytes          # address of x_T in each trajectory
+    for idx in _by_key([o._flow_key for o in opts]):
+        o0 = opts[idx[0]]
+        engine.shoot_records(rt.stack([opts[i]._flow_base for i in idx]), o0.Kdiff.prec,
+                             o0.dim, Ts[idx], xt[idx], at[idx], energy[idx])
+    cur = [i for i, o in enumerate(opts) if o.data_kind == CURRENT]
+    if cur:
+        engine.geometry_records(rt.stack([opts[i]._geom_try for i in cur], engine.GEOM_DT),
+                                final[cur])
+        for idx in _by_key([opts[i]._cur_key for i in cur]):
+            g = [cur[i] for i in idx]
+            o0 = opts[g[0]]
+            engine.currents_records(rt.stack([opts[i]._cur_try for i in g]),
+                                    o0._cur_key[1], o0._cur_key[2], data_total[g])
+    lmk = [i for i, o in enumerate(opts) if o.data_kind == LANDMARK]
+    if lmk:
+        engine.landmark_batch([opts[i]._xt_scratch[-1] for i in lmk], [opts[i].y for i in lmk],
+                              [int(data_total[i]) for i in lmk])
     vals = _readback(scal).tolist()
     out = []
-    for (o, dir, eps, objRef), atTry, off in zip(items, ats, offs):
+    for (o, dir, eps, objRef), atTry, off in zip(items, ats, offs.tolist()):
         T = o.Tsize
         c = o.regweight * (1.0 / T)
         objTry = o.obj0
@@ -304,24 +342,28 @@ def batch_gradient(items) -> list:
 def batch_dot(items) -> list:
     """dotProduct (surfaceMatching.py:353-377): sum_t <g2_t, K(x_t) g1_t> for 1..2 g2's."""
     dev = rt.device()
-    total = sum(o.Tsize for o, _, _ in items)
-    buf = torch.zeros((total, 2), dtype=torch.float64, device=dev)
-    jobs = []
-    pos = 0
-    for o, g1, g2 in items:
-        jobs.append(engine.KDotJob(o.xt, g1.diff, [g.diff for g in g2], o.Kdiff,
-                                   buf[pos:pos + o.Tsize]))
-        pos += o.Tsize
-    engine.kdot(jobs)
+    n = len(items)
+    opts = [it[0] for it in items]
+    Ts = np.fromiter((o.Tsize for o in opts), np.int64, n)
+    first = np.zeros(n, np.int64)
+    np.cumsum(Ts[:-1], out=first[1:])
+    buf = torch.zeros((int(Ts.sum()), 2), dtype=torch.float64, device=dev)
+    out_ptr = np.uint64(buf.data_ptr()) + np.uint64(16) * first.astype(np.uint64)
+    xt = np.fromiter((o.xt.data_ptr() for o in opts), np.uint64, n)
+    g1 = np.fromiter((it[1].diff.data_ptr() for it in items), np.uint64, n)
+    na = np.fromiter((len(it[2]) for it in items), np.int32, n)
+    g2 = np.zeros((n, 2), np.uint64)
+    for i, it in enumerate(items):
+        for q, g in enumerate(it[2]):
+            g2[i, q] = g.diff.data_ptr()
+    for idx in _by_key([o._flow_key for o in opts]):
+        o0 = opts[idx[0]]
+        engine.kdot_records(rt.stack([opts[i]._flow_base for i in idx]), o0.Kdiff.prec,
+                            o0.dim, Ts[idx], xt[idx], g1[idx], g2[idx], na[idx], out_ptr[idx])
     allv = _readback(buf)
     out = []
-    pos = 0
-    for o, g1, g2 in items:
-        v = allv[pos:pos + o.Tsize]
-        pos += o.Tsize
-        res = np.zeros(len(g2))
-        for t in range(o.Tsize):
-            for ll in range(len(g2)):
-                res[ll] = res[ll] + v[t, ll]
-        out.append(res)
+    for i in range(n):
+        # sequential sum over t (numpy's accumulate), as the reference's loop
+        v = allv[first[i]:first[i] + Ts[i], :na[i]]
+        out.append(np.cumsum(v, axis=0)[-1].copy())
     return out
