@@ -450,7 +450,20 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   int64_t l1 = l0 + sg->chunk_len;
   if (l1 > src.n) l1 = src.n;
 
+  // Symmetric self-sums.  When only the total <w, K(y, y) w> is asked for (KNORM, and
+  // the self set of an energy-only CURRENTS segment: no per-row outputs), each
+  // unordered pair of TS-blocks is evaluated once: a wave whose rows lie in block
+  // rb takes source tile sb with weight 2 if sb > rb, 1 if sb == rb, 0 if sb < rb.
+  // Tiles start at multiples of TS and a wave's rows never straddle a TS block,
+  // so the weights are wave-uniform and every unordered pair counts twice.
+  const bool sym = set == 0 && !sg->rowdot && !sg->out0 &&
+                   (op == LDDMM_OP_KNORM || (op == LDDMM_OP_CURRENTS && NA == 0));
+  const int64_t wave_blk = ((int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT)) / TS;
+  const int64_t wg_blk = ((int64_t)rt * (BLOCK * RPT)) / TS;
+
   for (int64_t base = l0; base < l1; base += TS) {
+    if (sym && base / TS < wg_blk) continue;  // workgroup-uniform: below every wave's block
+    const int sym_w = !sym ? 1 : (base / TS > wave_blk ? 2 : (base / TS == wave_blk ? 1 : 0));
     // Tile origin: FP32 mode evaluates the tile in a frame centred at its first
     // source, so rounding to FP32 costs relative precision of the local
     // distances, not of the absolute coordinates.  FP64 keeps the raw frame.
@@ -520,7 +533,8 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
       }
       if (!(d2 > cut2)) live |= 1u << q;
     }
-    if (!live) continue;  // wave-uniform: every pair of this tile is negligible
+    if (!live || sym_w == 0) continue;  // wave-uniform: nothing of this tile to add
+    const R wfac = (R)sym_w;
 
     for (int j0 = 0; j0 < TS; j0 += KF) {
       if (!((live >> (j0 / 64)) & 1u)) continue;
@@ -574,6 +588,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
         if constexpr (KSUM) {
 #pragma unroll
           for (int c = 0; c < M; ++c) {
+            tK[r][c] *= wfac;  // exact: 1 or 2
             if constexpr (KAHAN) {
               R y = tK[r][c] - cK[r][c];
               R t = sK[r][c] + y;
