@@ -456,10 +456,18 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   // rb takes source tile sb with weight 2 if sb > rb, 1 if sb == rb, 0 if sb < rb.
   // Tiles start at multiples of TS and a wave's rows never straddle a TS block,
   // so the weights are wave-uniform and every unordered pair counts twice.
-  const bool sym = set == 0 && !sg->rowdot && !sg->out0 &&
-                   (op == LDDMM_OP_KNORM || (op == LDDMM_OP_CURRENTS && NA == 0));
+  const bool energy_only = !sg->rowdot && !sg->out0 &&
+                           (op == LDDMM_OP_KNORM || (op == LDDMM_OP_CURRENTS && NA == 0));
+  const bool sym = energy_only && set == 0;
   const int64_t wave_blk = ((int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT)) / TS;
   const int64_t wg_blk = ((int64_t)rt * (BLOCK * RPT)) / TS;
+  // Mixed precision by magnitude (FP64 Gaussian energy sums only: segments without per-row
+  // outputs, both source sets): runs of 64 sources whose bounding box is so far from the
+  // wave's rows that every kernel value is below 2^-26 are evaluated with an FP32 exponential.
+  constexpr bool MIXED = std::is_same<R, double>::value && KT == LDDMM_KERNEL_GAUSS && KSUM &&
+                         NA == 0;
+  const bool mixed = MIXED && energy_only;
+  const R far2 = MIXED ? (R)(26.0 * 2.0 * sg->sigma * sg->sigma / LOG2E) : (R)0;
 
   for (int64_t base = l0; base < l1; base += TS) {
     if (sym && base / TS < wg_blk) continue;  // workgroup-uniform: below every wave's block
@@ -521,7 +529,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
     __syncthreads();
     // Which 64-source quarters of the tile hold a pair above the cut for this
     // wave (squared distance between the target box and each quarter's box).
-    unsigned live = 0;
+    unsigned live = 0, far = 0;
 #pragma unroll
     for (int q = 0; q < TS / 64; ++q) {
       R d2 = 0;
@@ -532,6 +540,9 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
         d2 += g * g;
       }
       if (!(d2 > cut2)) live |= 1u << q;
+      if constexpr (MIXED) {
+        if (d2 > far2) far |= 1u << q;
+      }
     }
     if (!live || sym_w == 0) continue;  // wave-uniform: nothing of this tile to add
     const R wfac = (R)sym_w;
@@ -545,6 +556,34 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
         for (int c = 0; c < (KSUM ? M : 1); ++c) tK[r][c] = 0;
 #pragma unroll
         for (int c = 0; c < D; ++c) tD[r][c] = 0;
+      }
+      if constexpr (MIXED) {
+        if (mixed && ((far >> (j0 / 64)) & 1u)) {
+          // Every pair of this run has K < 2^-26: v_exp_f32 of the FP64 exponent (relative
+          // error ~4e-6 from rounding an exponent of magnitude < 80 to FP32) changes each
+          // term by less than 1e-13 of a unit kernel value; the products accumulate in FP64.
+#pragma unroll 8
+          for (int j = j0; j < j0 + KF; ++j) {
+            R sx[D], sw[M];
+#pragma unroll
+            for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j];
+#pragma unroll
+            for (int c = 0; c < M; ++c) sw[c] = s_w[0][c][j];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+              R r2 = 0;
+#pragma unroll
+              for (int c = 0; c < D; ++c) {
+                const R dx = ty[r][c] - sx[c];
+                r2 = dx * dx + r2;
+              }
+              const R k = (R)__builtin_amdgcn_exp2f((float)(-kev.h * r2));
+#pragma unroll
+              for (int c = 0; c < M; ++c) tK[r][c] = k * sw[c] + tK[r][c];
+            }
+          }
+          goto fold;
+        }
       }
 #pragma unroll 8
       for (int j = j0; j < j0 + KF; ++j) {
@@ -582,6 +621,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
           }
         }
       }
+    fold:
       // Fold the run into the compensated row sums.
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
