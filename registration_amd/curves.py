@@ -62,10 +62,7 @@ class DeviceCurve:
     def __init__(self, faces_np, vertices):
         faces_np = np.asarray(faces_np)
         self.nv, self.d = int(vertices.shape[0]), int(vertices.shape[1])
-        self.faces = rt.as_dev(faces_np.astype(np.int32), dtype=torch.int32)
-        ptr, inc = engine.incidence(faces_np, self.nv)
-        self.ptr = rt.as_dev(ptr, dtype=torch.int32)
-        self.inc = rt.as_dev(inc, dtype=torch.int32)
+        self.faces, self.ptr, self.inc = engine.topology(faces_np, self.nv)
         E = faces_np.shape[0]
         dev = rt.device()
         self.centers = torch.empty((E, self.d), dtype=torch.float64, device=dev)

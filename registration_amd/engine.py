@@ -170,36 +170,46 @@ class CovectorJob:
         return 1.0 / self.T if self.dt is None else self.dt
 
 
+def covector_record(N: int, T: int, K: DevKernel, regweight: float,
+                    dt: Optional[float] = None) -> np.ndarray:
+    """The pointer-free part of a registration's covector-step descriptor (one record)."""
+    r = flow_record(N, T, K, dt)
+    r["coef"] = regweight
+    return r
+
+
+def covector_records(base, prec, d, T, xt, at, pxt) -> None:
+    """Covector launches for n trajectories of one (kernel family, precision, d) from arrays:
+    ``base`` (n,) covector_record rows; T (n,) step counts; xt, at, pxt (n,) device base
+    addresses of the (T+1, N, d) trajectories, (T, N, d) momenta and (T+1, N, d) covectors,
+    whose slice T already holds the end-point covector.  Launch s handles t = T - 1 - s of
+    every trajectory with s < T."""
+    T = np.asarray(T, dtype=np.int64)
+    stride = base["ny"].astype(np.uint64) * np.uint64(d * 8)
+    xt, at, px = _u64(xt), _u64(at), _u64(pxt)
+    for s0, s1, idx in _runs(T, int(T.max())):
+        i = np.asarray(idx)
+        ss = np.arange(s0, s1, dtype=np.uint64)[:, None]
+        tt = T[i].astype(np.uint64)[None, :] - np.uint64(1) - ss
+        steps = np.repeat(base[i][None, :], s1 - s0, axis=0)
+        st = stride[i]
+        steps["y"] = xt[i] + tt * st
+        steps["w0"] = at[i] + tt * st
+        steps["w1"] = px[i] + (tt + 1) * st
+        steps["out0"] = px[i] + tt * st
+        rt.run_steps(_lib.OP_COVECTOR_STEP, prec, d, steps)
+
+
 def covector(jobs: Sequence[CovectorJob]) -> None:
     """pxt[T] = px1; pxt[t] = pxt[t+1] + dt * applyDiffKT(xt[t], [p, a, -2 lam a], [a, p, a])."""
     torch._foreach_copy_([j.pxt[j.T] for j in jobs], [j.px1 for j in jobs])
     for group in _groups(jobs, lambda j: (j.K.family, j.xt.shape[2])):
         d = group[0].xt.shape[2]
-        prec = group[0].K.prec
-        # Backward in time: launch s handles t = T - 1 - s of every job with s < T.
-        Ts = [j.T for j in group]
-        for s0, s1, idx in _runs(Ts, max(Ts)):
-            g = [group[i] for i in idx]
-            base = rt.segs(len(g))
-            base["ny"] = [j.xt.shape[1] for j in g]
-            base["sigma"] = [j.K.sigma for j in g]
-            base["dt"] = [j.step for j in g]
-            base["coef"] = [j.regweight for j in g]
-            base["kernel"] = [j.K.kernel for j in g]
-            base["order"] = [j.K.order for j in g]
-            stride = _u64([j.xt.shape[1] * d * 8 for j in g])
-            T = _u64([j.T for j in g])
-            xt = _u64([j.xt.data_ptr() for j in g])
-            at = _u64([j.at.data_ptr() for j in g])
-            px = _u64([j.pxt.data_ptr() for j in g])
-            ss = np.arange(s0, s1, dtype=np.uint64)[:, None]
-            tt = T[None, :] - 1 - ss
-            steps = np.repeat(base[None, :], s1 - s0, axis=0)
-            steps["y"] = xt + tt * stride
-            steps["w0"] = at + tt * stride
-            steps["w1"] = px + (tt + 1) * stride
-            steps["out0"] = px + tt * stride
-            rt.run_steps(_lib.OP_COVECTOR_STEP, prec, d, steps)
+        base = rt.stack([rt.raw(covector_record(j.xt.shape[1], j.T, j.K, j.regweight, j.step))
+                         for j in group])
+        covector_records(base, group[0].K.prec, d, [j.T for j in group],
+                         [j.xt.data_ptr() for j in group], [j.at.data_ptr() for j in group],
+                         [j.pxt.data_ptr() for j in group])
 
 
 # --------------------------------------------------------------------------------------
@@ -349,6 +359,34 @@ def curve_geometry(vertices, edges, centers, lenel) -> None:
                                         rt.stream_handle()), "lddmm_curve_geometry")
 
 
+_TOPOLOGY: "collections.OrderedDict" = None
+TOPOLOGY_CACHE = 64
+
+
+def topology(faces_np, nv: int):
+    """Device faces (int32) and vertex-incidence CSR (ptr, inc) of a mesh, shared by every
+    shape with the same faces and vertex count (read-only; the last TOPOLOGY_CACHE distinct
+    topologies are kept)."""
+    global _TOPOLOGY
+    import collections
+    if _TOPOLOGY is None:
+        _TOPOLOGY = collections.OrderedDict()
+    faces_np = np.ascontiguousarray(faces_np)
+    key = (torch.cuda.current_device(), int(nv), faces_np.shape, faces_np.dtype.str,
+           hash(faces_np.tobytes()))
+    hit = _TOPOLOGY.get(key)
+    if hit is not None:
+        _TOPOLOGY.move_to_end(key)
+        return hit
+    ptr, inc = incidence(faces_np, nv)
+    val = (rt.as_dev(faces_np.astype(np.int32), dtype=torch.int32),
+           rt.as_dev(ptr, dtype=torch.int32), rt.as_dev(inc, dtype=torch.int32))
+    _TOPOLOGY[key] = val
+    while len(_TOPOLOGY) > TOPOLOGY_CACHE:
+        _TOPOLOGY.popitem(last=False)
+    return val
+
+
 def incidence(faces_np, nv: int):
     """Vertex -> (face, corner) CSR sorted by (corner, face), i.e. in the order the reference's
     scatter loops visit them (surfaces.py:1074-1087, curves.py:604-610).  Host NumPy."""
@@ -438,6 +476,34 @@ def geometry_records(recs, vertices) -> None:
     dev = rt.upload(r)
     _lib.check(lib.lddmm_geometry_batch(r.ctypes.data_as(ctypes.POINTER(_lib.Geom)), dev.data_ptr(),
                                         len(r), rt.stream_handle()), "lddmm_geometry_batch")
+
+
+def gather_record(sh, z1, dz, scale: float, px_ptr: int) -> np.ndarray:
+    """The vertex-gather descriptor of a device shape without its vertex address (one record)."""
+    r = np.zeros(1, GATHER_DT)
+    r["faces"] = sh.faces.data_ptr()
+    r["z1"] = z1.data_ptr()
+    r["dz"] = dz.data_ptr()
+    r["incid_ptr"] = sh.ptr.data_ptr()
+    r["incid"] = sh.inc.data_ptr()
+    r["px"] = px_ptr
+    r["nv"] = sh.nv
+    r["scale"] = scale
+    r["k"] = sh.faces.shape[1]
+    r["d"] = sh.centers.shape[1]
+    return r
+
+
+def gather_records(recs, vertices) -> None:
+    """Vertex gathers of n shapes from their gather_record rows and vertex addresses."""
+    lib = _lib.load()
+    r = np.array(recs, copy=True)
+    r["vertices"] = _u64(vertices)
+    r["blk0"] = _blk0(r["nv"])
+    dev = rt.upload(r)
+    _lib.check(lib.lddmm_vertex_gradient_batch(r.ctypes.data_as(ctypes.POINTER(_lib.Gather)),
+                                               dev.data_ptr(), len(r), rt.stream_handle()),
+               "lddmm_vertex_gradient_batch")
 
 
 def gather_batch(shapes, z1s, dzs, scales, pxs) -> None:

@@ -24,6 +24,7 @@ from typing import Sequence
 import numpy as np
 import torch
 
+from . import _lib
 from . import engine
 from . import runtime as rt
 
@@ -91,6 +92,14 @@ class DeviceMatching:
         self._flow_base = rt.raw(engine.flow_record(N, T, self.Kdiff))
         self._flow_key = (self.Kdiff.kernel, self.Kdiff.prec, d)
         self._slice_bytes = N * d * 8
+        # Gradient buffers, reused by every gradient request: the covector (T+1, N, d) with
+        # the end-point covector written straight into its slice T, and the currents
+        # gradient outputs.
+        self._pxt = torch.empty_like(self.xt)
+        self._pxt_tail = self._pxt[1:]
+        self._pxT = self._pxt.data_ptr() + T * self._slice_bytes
+        self._cov_base = rt.raw(engine.covector_record(N, T, self.Kdiff, self.regweight))
+        self._def_stale = False
         if self.data_kind == CURRENT:
             Ke = self.Kdist.energy()
             self._cur_try = rt.raw(engine.currents_record(self._try_shape.centers,
@@ -98,6 +107,21 @@ class DeviceMatching:
                                                           self.cr2, Ke, self.coef))
             self._cur_key = (Ke.kernel, Ke.prec, d)
             self._geom_try = rt.raw(engine.geometry_record(self._try_shape))
+            sh = self.fvDef_dev
+            F = sh.centers.shape[0]
+            self._z1 = torch.empty((F, d), dtype=torch.float64, device=dev)
+            self._dz = torch.empty((F, d), dtype=torch.float64, device=dev)
+            self._gtot = torch.empty(1, dtype=torch.float64, device=dev)
+            g = engine.currents_record(sh.centers, sh.surfel, self.c2, self.cr2, self.Kdist,
+                                       self.coef)
+            g["out0"] = self._z1.data_ptr()
+            g["out1"] = self._dz.data_ptr()
+            g["total"] = self._gtot.data_ptr()
+            self._cur_grad = rt.raw(g)
+            self._cur_grad_key = (self.Kdist.kernel, self.Kdist.prec, d)
+            self._gather = rt.raw(engine.gather_record(sh, self._z1, self._dz,
+                                                       -2.0 / (self.sigmaError ** 2), self._pxT))
+            self._geom_def = rt.raw(engine.geometry_record(sh))
         self.obj = None
         self.objTry = None
         self.obj0 = None
@@ -131,8 +155,10 @@ class DeviceMatching:
         self.obj = self.objTry
         self.at = self.atTry
         self.xt, self._xt_try = self._xt_try, self.xt
+        # The deformed shape's geometry follows xt[-1]; it is refreshed (batched over
+        # registrations) before the next request that reads it.
         if self.fvDef_dev is not None:
-            self.fvDef_dev.update(self.xt[-1])
+            self._def_stale = True
 
     def endOfIteration(self):
         self.iter += 1
@@ -165,8 +191,44 @@ class DeviceMatching:
 # --------------------------------------------------------------------------------------
 # Batched request implementations
 # --------------------------------------------------------------------------------------
-def _readback(buf: torch.Tensor) -> np.ndarray:
-    return buf.cpu().numpy()
+class Pending:
+    """Answers of a batched request whose device work is in flight.  The launch's FP64
+    scalars are copied asynchronously to pinned host memory and an event is recorded after
+    the copy; ``result()`` waits for that event only, so work launched afterwards (another
+    group of registrations) keeps the GPU busy while the host turns these scalars into
+    answers."""
+
+    __slots__ = ("_dev", "_host", "_event", "_finish", "_done", "_value")
+
+    def __init__(self, dev_buf: torch.Tensor, finish):
+        self._dev = dev_buf
+        self._host = torch.empty(dev_buf.shape, dtype=dev_buf.dtype, pin_memory=True)
+        self._host.copy_(dev_buf, non_blocking=True)
+        self._event = torch.cuda.Event()
+        self._event.record()
+        self._finish = finish
+        self._done = False
+        self._value = None
+
+    def result(self):
+        if not self._done:
+            self._event.synchronize()
+            self._value = self._finish(self._host.numpy())
+            self._done = True
+            self._dev = self._host = self._event = self._finish = None
+        return self._value
+
+
+class Done:
+    """Answers available at launch time (device tensors)."""
+
+    __slots__ = ("_value",)
+
+    def __init__(self, value):
+        self._value = value
+
+    def result(self):
+        return self._value
 
 
 def _data_energies(opts, vertices, shapes, totals) -> None:
@@ -186,9 +248,26 @@ def _data_energies(opts, vertices, shapes, totals) -> None:
                               [totals[i] for i in lmk])
 
 
+def _refresh_def(opts) -> None:
+    """Face geometry of the deformed shapes whose trajectory changed (one launch)."""
+    st = [o for o in opts if o._def_stale]
+    if not st:
+        return
+    engine.geometry_records(rt.stack([o._geom_def for o in st], engine.GEOM_DT),
+                            [o.xt.data_ptr() + o.Tsize * o._slice_bytes for o in st])
+    for o in st:
+        o.fvDef_dev.vertices = o.xt[-1]
+        o._def_stale = False
+
+
 def batch_objective(opts: Sequence[DeviceMatching]) -> list:
     """objectiveFun (surfaceMatching.py:233-250): obj0 once, then the energy of at plus the
     data term of the deformed template."""
+    return launch_objective(opts).result()
+
+
+def launch_objective(opts: Sequence[DeviceMatching]) -> Pending:
+    """batch_objective with its answers pending."""
     dev = rt.device()
     lens = [o.Tsize + 2 for o in opts]
     scal = torch.zeros(sum(lens), dtype=torch.float64, device=dev)
@@ -202,26 +281,30 @@ def batch_objective(opts: Sequence[DeviceMatching]) -> list:
     engine.shoot(flows)
     _data_energies(opts, [o.xt[-1] for o in opts], [o.fvDef_dev for o in opts],
                    [scal[off + o.Tsize:off + o.Tsize + 1] for o, off in zip(opts, offs)])
+    for o in opts:
+        o._def_stale = False
     # Target self-term (currentNorm0); zero for the landmark data term.
     for o, off in zip(opts, offs):
         if o.data_kind == CURRENT:
             T = o.Tsize
             norms.append(engine.KNormJob(o.c2, o.cr2, o.Kdist.energy(), scal[off + T + 1:off + T + 2]))
     engine.knorm(norms)
-    vals = _readback(scal)
-    out = []
-    for o, off in zip(opts, offs):
-        T = o.Tsize
-        s2 = o.sigmaError ** 2
-        o.obj0 = vals[off + T + 1] / s2
-        obj = 0
-        dt = 1.0 / T
-        for t in range(T):
-            obj = obj + o.regweight * dt * vals[off + t]
-        o.obj = obj + (o.obj0 + vals[off + T] / s2)
-        o.nobj += 1
-        out.append(o.obj)
-    return out
+
+    def finish(vals):
+        out = []
+        for o, off in zip(opts, offs):
+            T = o.Tsize
+            s2 = o.sigmaError ** 2
+            o.obj0 = vals[off + T + 1] / s2
+            obj = 0
+            dt = 1.0 / T
+            for t in range(T):
+                obj = obj + o.regweight * dt * vals[off + t]
+            o.obj = obj + (o.obj0 + vals[off + T] / s2)
+            o.nobj += 1
+            out.append(o.obj)
+        return out
+    return Pending(scal, finish)
 
 
 def _by_key(keys):
@@ -234,6 +317,12 @@ def _by_key(keys):
 
 def batch_update_try(items) -> list:
     """updateTry (surfaceMatching.py:255-289) for [(opt, dir, eps, objRef), ...]."""
+    return launch_update_try(items).result()
+
+
+def launch_update_try(items) -> Pending:
+    """batch_update_try with its answers pending (the kept trial is recorded when they are
+    read)."""
     dev = rt.device()
     n = len(items)
     opts = [it[0] for it in items]
@@ -268,69 +357,74 @@ def batch_update_try(items) -> list:
     if lmk:
         engine.landmark_batch([opts[i]._xt_scratch[-1] for i in lmk], [opts[i].y for i in lmk],
                               [int(data_total[i]) for i in lmk])
-    vals = _readback(scal).tolist()
-    out = []
-    for (o, dir, eps, objRef), atTry, off in zip(items, ats, offs.tolist()):
-        T = o.Tsize
-        c = o.regweight * (1.0 / T)
-        objTry = o.obj0
-        objDef = 0
-        for e in vals[off:off + T]:
-            objDef = objDef + c * e
-        objTry += objDef
-        objTry += vals[off + T] / (o.sigmaError ** 2)
-        o.nobj += 1
-        if math.isnan(objTry):
-            out.append(float('inf'))
-            continue
-        if (objRef is None) or (objTry < objRef):
-            o.atTry = atTry
-            o.objTry = objTry
-            o._xt_try, o._xt_scratch = o._xt_scratch, o._xt_try
-        out.append(objTry)
-    return out
+
+    def finish(host):
+        vals = host.tolist()
+        out = []
+        for (o, dir, eps, objRef), atTry, off in zip(items, ats, offs.tolist()):
+            T = o.Tsize
+            c = o.regweight * (1.0 / T)
+            objTry = o.obj0
+            objDef = 0
+            for e in vals[off:off + T]:
+                objDef = objDef + c * e
+            objTry += objDef
+            objTry += vals[off + T] / (o.sigmaError ** 2)
+            o.nobj += 1
+            if math.isnan(objTry):
+                out.append(float('inf'))
+                continue
+            if (objRef is None) or (objTry < objRef):
+                o.atTry = atTry
+                o.objTry = objTry
+                o._xt_try, o._xt_scratch = o._xt_scratch, o._xt_try
+            out.append(objTry)
+        return out
+    return Pending(scal, finish)
+
+
+def launch_gradient(items) -> Done:
+    """batch_gradient; the gradients are device tensors, available at once."""
+    return Done(batch_gradient(items))
 
 
 def batch_gradient(items) -> list:
     """getGradient (surfaceMatching.py:301-327): px1 = -grad(data)/sigma^2 at the deformed
     template, covector along the kept trajectory, dat = 2 lambda a - p, scaled by
     1/(coeff T)."""
-    dev = rt.device()
-    px1s = [torch.empty((o.npt, o.dim), dtype=torch.float64, device=dev) for o, _ in items]
-    # Currents data-term gradient: one currents launch, then the vertex gathers.
-    cur = [(o, px1) for (o, _), px1 in zip(items, px1s) if o.data_kind == CURRENT]
-    curs, zs = [], []
-    for o, _ in cur:
-        sh = o.fvDef_dev
-        F, d = sh.centers.shape
-        z1 = torch.empty((F, d), dtype=torch.float64, device=dev)
-        dz = torch.empty((F, d), dtype=torch.float64, device=dev)
-        zs.append((z1, dz))
-        curs.append(engine.CurrentsJob(sh.centers, sh.surfel, o.c2, o.cr2, o.Kdist,
-                                       torch.empty(1, dtype=torch.float64, device=dev),
-                                       z1=z1, dz=dz, coef=o.coef))
+    n = len(items)
+    opts = [it[0] for it in items]
+    _refresh_def(opts)
+    # End-point covector px1, written into slice T of each registration's covector buffer.
+    cur = [i for i, o in enumerate(opts) if o.data_kind == CURRENT]
     if cur:
-        engine.currents(curs)
-        engine.gather_batch([o.fvDef_dev for o, _ in cur], [z for z, _ in zs], [z for _, z in zs],
-                            [-2.0 / (o.sigmaError ** 2) for o, _ in cur], [p for _, p in cur])
-    # Landmark data term: px1 = -2 (x_T - y) / sigmaError^2.
-    lmk = [(o, px1) for (o, _), px1 in zip(items, px1s) if o.data_kind == LANDMARK]
+        # currents gradient (z1, dz), then the vertex gathers with scale -2/sigma^2
+        for idx in _by_key([opts[i]._cur_grad_key for i in cur]):
+            g = [cur[i] for i in idx]
+            key = opts[g[0]]._cur_grad_key
+            rt.launch(_lib.OP_CURRENTS, key[1], key[2], rt.stack([opts[i]._cur_grad for i in g]))
+        engine.gather_records(rt.stack([opts[i]._gather for i in cur], engine.GATHER_DT),
+                              [opts[i].xt.data_ptr() + opts[i].Tsize * opts[i]._slice_bytes
+                               for i in cur])
+    lmk = [o for o in opts if o.data_kind == LANDMARK]
     if lmk:
-        engine.landmark_batch([o.xt[-1] for o, _ in lmk], [o.y for o, _ in lmk],
-                              [None] * len(lmk), [p for _, p in lmk],
-                              [-2.0 / (o.sigmaError ** 2) for o, _ in lmk])
-    covs, pxts = [], []
-    for (o, _), px1 in zip(items, px1s):
-        pxt = torch.empty_like(o.xt)
-        pxts.append(pxt)
-        covs.append(engine.CovectorJob(o.xt, o.at, px1, o.Kdiff, o.regweight, pxt))
-    engine.covector(covs)
+        # px1 = -2 (x_T - y) / sigmaError^2
+        engine.landmark_batch([o.xt[-1] for o in lmk], [o.y for o in lmk], [None] * len(lmk),
+                              [o._pxT for o in lmk], [-2.0 / (o.sigmaError ** 2) for o in lmk])
+    Ts = np.fromiter((o.Tsize for o in opts), np.int64, n)
+    xt = np.fromiter((o.xt.data_ptr() for o in opts), np.uint64, n)
+    at = np.fromiter((o.at.data_ptr() for o in opts), np.uint64, n)
+    px = np.fromiter((o._pxt.data_ptr() for o in opts), np.uint64, n)
+    for idx in _by_key([o._flow_key for o in opts]):
+        o0 = opts[idx[0]]
+        engine.covector_records(rt.stack([opts[i]._cov_base for i in idx]), o0.Kdiff.prec,
+                                o0.dim, Ts[idx], xt[idx], at[idx], px[idx])
     # dat = 2 lambda a - p_{t+1};  grd = dat / (coeff T)
-    dats = torch._foreach_mul([o.at for o, _ in items], [2 * o.regweight for o, _ in items])
-    torch._foreach_sub_(dats, [p[1:] for p in pxts])
-    torch._foreach_div_(dats, [coeff * o.Tsize for o, coeff in items])
+    dats = torch._foreach_mul([o.at for o in opts], [2 * o.regweight for o in opts])
+    torch._foreach_sub_(dats, [o._pxt_tail for o in opts])
+    torch._foreach_div_(dats, [it[1] * it[0].Tsize for it in items])
     out = []
-    for (o, coeff), dat in zip(items, dats):
+    for o, dat in zip(opts, dats):
         g = Direction()
         g.diff = dat
         g.aff = np.zeros(o.Afft.shape)
@@ -341,6 +435,11 @@ def batch_gradient(items) -> list:
 
 def batch_dot(items) -> list:
     """dotProduct (surfaceMatching.py:353-377): sum_t <g2_t, K(x_t) g1_t> for 1..2 g2's."""
+    return launch_dot(items).result()
+
+
+def launch_dot(items) -> Pending:
+    """batch_dot with its answers pending."""
     dev = rt.device()
     n = len(items)
     opts = [it[0] for it in items]
@@ -360,10 +459,12 @@ def batch_dot(items) -> list:
         o0 = opts[idx[0]]
         engine.kdot_records(rt.stack([opts[i]._flow_base for i in idx]), o0.Kdiff.prec,
                             o0.dim, Ts[idx], xt[idx], g1[idx], g2[idx], na[idx], out_ptr[idx])
-    allv = _readback(buf)
-    out = []
-    for i in range(n):
-        # sequential sum over t (numpy's accumulate), as the reference's loop
-        v = allv[first[i]:first[i] + Ts[i], :na[i]]
-        out.append(np.cumsum(v, axis=0)[-1].copy())
-    return out
+
+    def finish(allv):
+        out = []
+        for i in range(n):
+            # sequential sum over t (numpy's accumulate), as the reference's loop
+            v = allv[first[i]:first[i] + Ts[i], :na[i]]
+            out.append(np.cumsum(v, axis=0)[-1].copy())
+        return out
+    return Pending(buf, finish)

@@ -120,7 +120,18 @@ class _Active:
         self.opt = opt
         self.gen = opt.cg_steps()
         self.pending = next(self.gen)
+        while self.pending[0] in _HOST_ONLY:
+            self.pending = self.gen.send(getattr(opt, self.pending[0])())
         self.admitted = admitted
+
+
+_LAUNCH = {
+    cg.TRY: lambda regs: matching.launch_update_try(
+        [(a.opt, a.pending[1], a.pending[2], a.pending[3]) for a in regs]),
+    cg.GRADIENT: lambda regs: matching.launch_gradient([(a.opt, a.pending[1]) for a in regs]),
+    cg.DOT: lambda regs: matching.launch_dot([(a.opt, a.pending[1], a.pending[2]) for a in regs]),
+    cg.OBJECTIVE: lambda regs: matching.launch_objective([a.opt for a in regs]),
+}
 
 
 class Server:
@@ -129,6 +140,12 @@ class Server:
     hbm_cap_bytes: admission cap on the summed footprint of active registrations;
     max_active: cap on concurrently optimised registrations; precision: 'fp32c' or 'fp64';
     keep_momenta: results carry the final momenta (device tensors) for checking.
+
+    Active registrations are split into two cohorts that take turns: a turn collects the
+    answers of one cohort's batched requests (launched on its previous turn), advances its
+    optimisers and launches their next requests, while the GPU is still working on the
+    other cohort's launches.  Each registration's own sequence of operations is unchanged,
+    so its result is the same as alone.
     """
 
     def __init__(self, hbm_cap_bytes: float = 200e9, max_active: int = 256,
@@ -140,13 +157,19 @@ class Server:
         self.precision = precision
         self.clock = clock
         self.queue: collections.deque = collections.deque()
-        self.active: list = []
+        self.cohorts = [[], []]
+        self.inflight = [[], []]   # per cohort: [(registrations, pending answers)]
+        self._turn = 0
         self.results: dict = {}
         self.used = 0.0
-        self.rounds = 0
-        self.answered = 0      # optimiser requests answered (summed over rounds)
-        self.max_batch = 0     # most registrations answered in one round
+        self.rounds = 0        # turns that launched work
+        self.answered = 0      # optimiser requests answered (summed over turns)
+        self.max_batch = 0     # most registrations launched in one turn
         self.t0 = None
+
+    @property
+    def active(self) -> list:
+        return self.cohorts[0] + self.cohorts[1]
 
     def submit(self, req: Request) -> None:
         if footprint_bytes(req) > self.cap:
@@ -159,14 +182,16 @@ class Server:
 
     def _admit(self) -> None:
         now = self._now()
-        while self.queue and self.queue[0].arrival <= now and len(self.active) < self.max_active:
+        while self.queue and self.queue[0].arrival <= now and \
+                len(self.cohorts[0]) + len(self.cohorts[1]) < self.max_active:
             req = self.queue[0]
             need = footprint_bytes(req)
-            if self.used + need > self.cap and self.active:
+            if self.used + need > self.cap and (self.cohorts[0] or self.cohorts[1]):
                 break
             self.queue.popleft()
             self.used += need
-            self.active.append(_Active(req, build_registration(req, self.precision), now))
+            c = 0 if len(self.cohorts[0]) <= len(self.cohorts[1]) else 1
+            self.cohorts[c].append(_Active(req, build_registration(req, self.precision), now))
 
     def _finish(self, a: _Active, value) -> None:
         o = a.opt
@@ -176,52 +201,49 @@ class Server:
         self.used -= footprint_bytes(a.req)
 
     def step(self) -> None:
-        """One scheduling round: answer every active registration's pending request."""
-        groups = collections.defaultdict(list)
-        for a in self.active:
-            groups[a.pending[0]].append(a)
+        """One turn of the next cohort: answer its requests launched on its previous turn,
+        advance its optimisers, launch their next requests."""
+        c = self._turn
+        self._turn ^= 1
         answers = {}
-        if groups.get(cg.TRY):
-            items = [(a.opt, a.pending[1], a.pending[2], a.pending[3]) for a in groups[cg.TRY]]
-            for a, v in zip(groups[cg.TRY], matching.batch_update_try(items)):
+        for regs, pend in self.inflight[c]:
+            for a, v in zip(regs, pend.result()):
                 answers[id(a)] = v
-        if groups.get(cg.GRADIENT):
-            items = [(a.opt, a.pending[1]) for a in groups[cg.GRADIENT]]
-            for a, v in zip(groups[cg.GRADIENT], matching.batch_gradient(items)):
-                answers[id(a)] = v
-        if groups.get(cg.DOT):
-            items = [(a.opt, a.pending[1], a.pending[2]) for a in groups[cg.DOT]]
-            for a, v in zip(groups[cg.DOT], matching.batch_dot(items)):
-                answers[id(a)] = v
-        if groups.get(cg.OBJECTIVE):
-            opts = [a.opt for a in groups[cg.OBJECTIVE]]
-            for a, v in zip(groups[cg.OBJECTIVE], matching.batch_objective(opts)):
-                answers[id(a)] = v
-        for kind in _HOST_ONLY:
-            for a in groups.get(kind, []):
-                answers[id(a)] = getattr(a.opt, a.pending[0])()
+        self.inflight[c] = []
+        self.answered += len(answers)
         still = []
-        for a in self.active:
+        for a in self.cohorts[c]:
+            if id(a) not in answers:          # admitted since the last turn: nothing to answer
+                still.append(a)
+                continue
             try:
                 a.pending = a.gen.send(answers[id(a)])
-                # Requests without device work are answered at once, not in a later round.
+                # Requests without device work are answered at once, not in a later turn.
                 while a.pending[0] in _HOST_ONLY:
                     a.pending = a.gen.send(getattr(a.opt, a.pending[0])())
                 still.append(a)
             except StopIteration as stop:
                 self._finish(a, stop.value)
-        self.answered += len(self.active)
-        self.max_batch = max(self.max_batch, len(self.active))
-        self.active = still
+        self.cohorts[c] = still
+        if not still:
+            return
+        groups = collections.defaultdict(list)
+        for a in still:
+            groups[a.pending[0]].append(a)
+        for kind in (cg.TRY, cg.GRADIENT, cg.DOT, cg.OBJECTIVE):
+            regs = groups.get(kind)
+            if regs:
+                self.inflight[c].append((regs, _LAUNCH[kind](regs)))
+        self.max_batch = max(self.max_batch, len(still))
         self.rounds += 1
 
     def run(self, idle_sleep: float = 1e-4) -> dict:
         """Serve until every submitted request is finished; returns {rid: Result}."""
         if self.t0 is None:
             self.t0 = self.clock()
-        while self.queue or self.active:
+        while self.queue or self.cohorts[0] or self.cohorts[1]:
             self._admit()
-            if not self.active:
+            if not (self.cohorts[0] or self.cohorts[1]):
                 wait = self.queue[0].arrival - self._now()
                 if wait > 0:
                     time.sleep(min(wait, 0.05) if wait > idle_sleep else idle_sleep)

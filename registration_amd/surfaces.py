@@ -68,10 +68,7 @@ class DeviceSurface:
     def __init__(self, faces_np, vertices):
         faces_np = np.asarray(faces_np)
         self.nv = int(vertices.shape[0])
-        self.faces = rt.as_dev(faces_np.astype(np.int32), dtype=torch.int32)
-        ptr, inc = engine.incidence(faces_np, self.nv)
-        self.ptr = rt.as_dev(ptr, dtype=torch.int32)
-        self.inc = rt.as_dev(inc, dtype=torch.int32)
+        self.faces, self.ptr, self.inc = engine.topology(faces_np, self.nv)
         F = faces_np.shape[0]
         dev = rt.device()
         self.centers = torch.empty((F, 3), dtype=torch.float64, device=dev)
