@@ -205,3 +205,61 @@ def test_landmark_cg_sequence_fp64():
     hg, ho = np.array(g.history), np.array(o.history)
     assert len(hg) == len(ho) and len(hg) >= 4
     assert np.all(np.abs(hg - ho) <= 1e-9 * np.abs(ho)), (hg, ho)
+
+
+def _config1(prec):
+    """The reference's CPU-runnable surface case: icosphere s3 (642 vertices) to its scaled and
+    bumped copy; Gaussian sigma_V 0.5, currents sigma_W 0.3, sigma_err 0.1, T 10, 50 CG
+    iterations, momenta from zero."""
+    from registration_amd import kernelFunctions as kfun, surfaceMatching as sm, synthetic
+    c = synthetic.surface_pair_config1()
+    (F, V), (_, Y) = c["template"], c["target"]
+    p = sm.SurfaceMatchingParam(timeStep=1.0 / c["T"],
+                                KparDiff=kfun.Kernel('gauss', sigma=c["sigmaKernel"], precision=prec),
+                                KparDist=kfun.Kernel('gauss', sigma=c["sigmaDist"], precision=prec),
+                                sigmaError=c["sigmaError"], errorType='current')
+    g = sm.SurfaceMatching(Template=sm.surfaces.Surface(FV=(F, V)), Target=sm.surfaces.Surface(FV=(F, Y)),
+                           param=p, maxIter=c["maxIter"], testGradient=False, verb=False)
+    return g, c
+
+
+def test_config1_full_run_fp64_matches_oracle():
+    """All 50 CG iterations of the reference's CPU case: the FP64 GPU path and the oracle
+    restatement take the same steps (objective sequences within 1e-9 relative)."""
+    g, c = _config1("fp64")
+    (F, V), (_, Y) = c["template"], c["target"]
+    o = orc.SurfaceMatching(orc.Surface(FV=(F, V)), orc.Surface(FV=(F, Y)), orc.Kernel('gauss', sigma=0.5),
+                            orc.Kernel('gauss', sigma=0.3), sigmaError=0.1, timeStep=0.1, maxIter=50)
+    print("config 1 fp64: GPU run", flush=True)
+    g.optimizeMatching()
+    print("config 1 fp64: oracle run", flush=True)
+    o.optimizeMatching()
+    hg, ho = np.array(g.history), np.array(o.history)
+    print("iterations", len(hg) - 1, len(ho) - 1, g.stop_reason, o.stop_reason, flush=True)
+    assert len(hg) == len(ho) and g.stop_reason == o.stop_reason
+    assert np.all(np.abs(hg - ho) <= 1e-9 * np.abs(ho)), np.abs(hg - ho).max()
+
+
+def test_config1_full_run_fp32c_energies():
+    """The FP32-compensated run of the same case: every accepted iterate's energy within 1e-6
+    of the FP64 energy of the same momenta, and the final energy within 1e-4 of the FP64 run's."""
+    g, _ = _config1("fp32c")
+    stash = []
+    accept = g.acceptVarTry
+
+    def spy():
+        accept()
+        stash.append((g.at.clone(), g.obj))
+    g.acceptVarTry = spy
+    g.optimizeMatching()
+    g64, _ = _config1("fp64")
+    worst = 0.0
+    for at, e in stash[::5] + stash[-1:]:
+        g64.at = at
+        g64.obj = None
+        e64 = g64.objectiveFun()
+        worst = max(worst, abs(e - e64) / abs(e64))
+    assert worst <= 1e-6, worst
+    f64, _ = _config1("fp64")
+    f64.optimizeMatching()
+    assert abs(g.obj - f64.obj) <= 1e-4 * abs(f64.obj), (g.obj, f64.obj)
