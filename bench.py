@@ -66,7 +66,7 @@ def registration(cfg, prec):
     F, V = cfg["template"]
     _, Y = cfg["target"]
     return sm.SurfaceMatching(Template=surfaces.Surface(FV=(F, V)), Target=surfaces.Surface(FV=(F, Y)),
-                              param=p, maxIter=cfg["maxIter"], testGradient=False, verb=False)
+                              param=p, maxIter=cfg["maxIter"], testGradient=False)
 
 
 def flow_step_roofline(cfg, prec, reps):

@@ -74,7 +74,10 @@ class Result:
 
 
 def build_registration(req: Request, precision: str):
-    """The device-resident matching object of a request (same as the plain path builds)."""
+    """The device-resident matching object of a request (same as the plain path builds).
+    The drivers keep the reference's default verb=True: the reference's cg disables CG on a
+    bad direction only when verb is set (conjugateGradient.py:151-154), so verb=False would
+    let a line search accept an objective increase there."""
     Kd = kfun.Kernel(name=req.typeKernel, sigma=req.sigmaKernel, order=req.order, precision=precision)
     Kw = kfun.Kernel(name='gauss', sigma=req.sigmaDist, precision=precision)
     if req.kind == 'surface':
@@ -82,17 +85,16 @@ def build_registration(req: Request, precision: str):
                                     sigmaError=req.sigmaError, errorType='current')
         return sm.SurfaceMatching(Template=surfaces.Surface(FV=req.template),
                                   Target=surfaces.Surface(FV=req.target), param=p,
-                                  maxIter=req.maxIter, testGradient=False, verb=False)
+                                  maxIter=req.maxIter, testGradient=False)
     if req.kind == 'landmark':
         p = lm.LandmarkMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, sigmaError=req.sigmaError)
         return lm.LandmarkMatching(Template=req.template[1], Target=req.target[1], param=p,
-                                   maxIter=req.maxIter, verb=False)
+                                   maxIter=req.maxIter)
     if req.kind == 'curve':
         p = cm.CurveMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, KparDist=Kw,
                                   sigmaError=req.sigmaError, errorType='current')
         return cm.CurveMatching(Template=curves.Curve(FV=req.template),
-                                Target=curves.Curve(FV=req.target), param=p, maxIter=req.maxIter,
-                                verb=False)
+                                Target=curves.Curve(FV=req.target), param=p, maxIter=req.maxIter)
     raise ValueError(f"unknown request kind {req.kind!r}")
 
 

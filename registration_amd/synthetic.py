@@ -164,48 +164,65 @@ def mixed_service_requests(n=2000, seed=5, shoot=None, rate=200.0, burst_factor=
     shortened ``burst_factor`` times with probability ``burst_prob`` (bursts).
 
     Request i is drawn from its own generator seeded (seed, i), so the first k requests of
-    any longer stream are the stream of length k.  shoot(x0, a0, kernel_name, sigma, T) -> x_T
-    builds the landmark targets; the default uses this package's GPU geodesic shooting
-    (pointEvolution.landmarkEPDiff).  Returns a list of dicts with the Request fields plus
-    'kind', 'size' and 'arrival'."""
-    if shoot is None:
-        shoot = _gpu_shoot
+    any longer stream are the stream of length k (and ``mixed_service_request`` rebuilds one).
+    shoot(x0, a0, kernel_name, sigma, T) -> x_T builds the landmark targets; the default uses
+    this package's GPU geodesic shooting (pointEvolution.landmarkEPDiff).  Returns a list of
+    dicts with the Request fields plus 'kind', 'size' and 'arrival'."""
     meshes = {}
     out = []
     t = 0.0
     for i in range(n):
-        rng = np.random.default_rng((seed, i))
+        req = _mixed_request(seed, i, shoot, rate, burst_factor, burst_prob, meshes)
+        t += req.pop("gap")
+        req["arrival"] = t
+        out.append(req)
+    return out
+
+
+def mixed_service_request(i, seed=5, shoot=None, rate=200.0, burst_factor=5.0, burst_prob=0.2):
+    """Request i of the mixed-service stream alone (same fields and values)."""
+    t = 0.0
+    for j in range(i):
+        rng = np.random.default_rng((seed, j))
         gap = rng.exponential(1.0 / rate)
         t += gap / burst_factor if rng.random() < burst_prob else gap
-        u = rng.random()
-        kernel = 'gauss' if rng.random() < 0.5 else 'laplacian'
-        sigma = float(rng.uniform(0.1, 0.5))
-        T = int(rng.choice([10, 20, 40]))
-        common = dict(sigmaKernel=sigma, sigmaDist=0.1, sigmaError=0.1, T=T, maxIter=30,
-                      typeKernel=kernel, order=10 if kernel == 'gauss' else 3, arrival=t)
-        if u < 0.4:
-            npts = _log_uniform_int(rng, 256, 8192)
-            X, _ = circle(npts)
-            Y = star_curve(npts, int(rng.integers(3, 9)), float(rng.uniform(0.05, 0.3)))
-            E = closed_curve_edges(npts)
-            out.append(dict(kind='curve', template=(E, X), target=(E, Y), size=npts, **common))
-        elif u < 0.8:
-            s = int(rng.integers(3, 8))
-            if s not in meshes:
-                meshes[s] = icosphere(s)
-            V, F = meshes[s]
-            R = random_rotation(rng)
-            Vr = V @ R.T
-            out.append(dict(kind='surface', template=(F, Vr), target=(F, rbf_field(rng, Vr)),
-                            size=V.shape[0], subdiv=s, **common))
-        else:
-            npts = _log_uniform_int(rng, 1000, 200000)
-            X = ball_points(rng, npts)
-            a0 = landmark_momenta_scale(npts, sigma) * rng.standard_normal(X.shape)
-            Y = shoot(X, a0, kernel, sigma, T)
-            out.append(dict(kind='landmark', template=(None, X), target=(None, Y), size=npts,
-                            **common))
-    return out
+    req = _mixed_request(seed, i, shoot, rate, burst_factor, burst_prob, {})
+    req["arrival"] = t + req.pop("gap")
+    return req
+
+
+def _mixed_request(seed, i, shoot, rate, burst_factor, burst_prob, meshes):
+    if shoot is None:
+        shoot = _gpu_shoot
+    rng = np.random.default_rng((seed, i))
+    gap = rng.exponential(1.0 / rate)
+    gap = gap / burst_factor if rng.random() < burst_prob else gap
+    u = rng.random()
+    kernel = 'gauss' if rng.random() < 0.5 else 'laplacian'
+    sigma = float(rng.uniform(0.1, 0.5))
+    T = int(rng.choice([10, 20, 40]))
+    common = dict(sigmaKernel=sigma, sigmaDist=0.1, sigmaError=0.1, T=T, maxIter=30,
+                  typeKernel=kernel, order=10 if kernel == 'gauss' else 3, gap=gap)
+    if u < 0.4:
+        npts = _log_uniform_int(rng, 256, 8192)
+        X, _ = circle(npts)
+        Y = star_curve(npts, int(rng.integers(3, 9)), float(rng.uniform(0.05, 0.3)))
+        E = closed_curve_edges(npts)
+        return dict(kind='curve', template=(E, X), target=(E, Y), size=npts, **common)
+    if u < 0.8:
+        s = int(rng.integers(3, 8))
+        if s not in meshes:
+            meshes[s] = icosphere(s)
+        V, F = meshes[s]
+        R = random_rotation(rng)
+        Vr = V @ R.T
+        return dict(kind='surface', template=(F, Vr), target=(F, rbf_field(rng, Vr)),
+                    size=V.shape[0], subdiv=s, **common)
+    npts = _log_uniform_int(rng, 1000, 200000)
+    X = ball_points(rng, npts)
+    a0 = landmark_momenta_scale(npts, sigma) * rng.standard_normal(X.shape)
+    Y = shoot(X, a0, kernel, sigma, T)
+    return dict(kind='landmark', template=(None, X), target=(None, Y), size=npts, **common)
 
 
 def _gpu_shoot(x0, a0, kernel_name, sigma, T):
