@@ -472,12 +472,15 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   for (int64_t base = l0; base < l1; base += TS) {
     if (sym && base / TS < wg_blk) continue;  // workgroup-uniform: below every wave's block
     const int sym_w = !sym ? 1 : (base / TS > wave_blk ? 2 : (base / TS == wave_blk ? 1 : 0));
-    // Tile origin: FP32 mode evaluates the tile in a frame centred at its first
+    // Tile origin: FP32 mode evaluates the tile in a frame centred at its middle
     // source, so rounding to FP32 costs relative precision of the local
-    // distances, not of the absolute coordinates.  FP64 keeps the raw frame.
+    // distances, not of the absolute coordinates (for sources ordered along a
+    // curve or a mesh, the middle one halves the largest offset in the tile).
+    // FP64 keeps the raw frame.
     double org[D];
+    const int64_t mid = base + ((l1 - base < TS ? l1 - base : TS) >> 1);
 #pragma unroll
-    for (int c = 0; c < D; ++c) org[c] = KAHAN ? src.pos[base * D + c] * scale : 0.0;
+    for (int c = 0; c < D; ++c) org[c] = KAHAN ? src.pos[mid * D + c] * scale : 0.0;
     if constexpr (KAHAN) {
 #pragma unroll
       for (int r = 0; r < RPT; ++r)
