@@ -102,7 +102,10 @@ def segs(n: int) -> np.ndarray:
 
 
 def seg(**kw) -> np.ndarray:
-    """One descriptor (a length-1 record array) from keyword fields; tensors become pointers."""
+    """One descriptor (a length-1 record array) from keyword fields; tensors become pointers.
+    The record keeps raw device addresses only: the caller keeps the tensors referenced until
+    the launch has run (a tensor freed earlier can be handed to a later allocation, e.g. the
+    descriptor upload, which then overwrites it first)."""
     r = segs(1)
     for k, v in kw.items():
         if k in ("tw", "sw", "g"):

@@ -181,3 +181,86 @@ def test_tile_culling_matches_full_sums(prec, name, order):
     g = K.applyDiffKT(pts, a1, a2)
     gr = orc.apply_diff_kt(pts, pts, a1, a2, sigma, K.order)
     assert rel_err(g, gr) < 5 * TOL[prec]
+
+
+def test_max_size_landmark_flow_step_rows():
+    """The largest landmark set of the mixed-service stream (200,000 points in the unit ball,
+    Gaussian sigma 0.1): one fused Euler step, checked on 256 sampled rows against the oracle's
+    FP64 sums (FP32-compensated tolerance) and its energy row-sum identity."""
+    import torch
+    from registration_amd import engine, synthetic
+    from registration_amd.kernelFunctions import Kernel
+    rng = np.random.default_rng(200)
+    X = synthetic.ball_points(rng, 200000)
+    a = 1e-3 * rng.standard_normal(X.shape)
+    K = engine.DevKernel.of(Kernel('gauss', sigma=0.1, precision='fp32c'))
+    x0, at = torch.as_tensor(X, device="cuda"), torch.as_tensor(a[None], device="cuda")
+    xt = torch.empty((2,) + X.shape, dtype=torch.float64, device="cuda")
+    v = torch.empty((1,) + X.shape, dtype=torch.float64, device="cuda")
+    en = torch.zeros(1, dtype=torch.float64, device="cuda")
+    engine.shoot([engine.FlowJob(x0, at, K, xt, en, v=v)])
+    rows = rng.choice(X.shape[0], 256, replace=False)
+    ref = orc.apply_k(X[rows], X, a, 0.1, 10)
+    vv = v[0].cpu().numpy()
+    assert rel_err(vv[rows], ref) < TOL["fp32c"]
+    assert np.allclose(xt[1].cpu().numpy(), X + vv, rtol=0, atol=1e-15)  # dt = 1
+    e = float((a * vv).sum())
+    assert abs(en.item() - e) <= 1e-12 * abs(e)
+
+
+def test_max_size_currents_energy_forms_agree():
+    """Icosphere s7 (327,680 faces, the largest surface class): the FP64 currents energy
+    evaluated in its energy-only form (symmetric self-sum, FP32 exponential for blocks whose
+    kernel values are all below 2^-26) equals the plain FP64 form, whose per-row outputs force
+    every pair, within 1e-12 relative to the sum of |terms| magnitude."""
+    import torch
+    from registration_amd import _lib, runtime as rt, synthetic
+    V, F = synthetic.icosphere(7)
+    Y = synthetic.rbf_field(np.random.default_rng(7), V)
+    c1, c2 = V[F].mean(axis=1), Y[F].mean(axis=1)
+    n1 = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    n2 = np.cross(Y[F[:, 1]] - Y[F[:, 0]], Y[F[:, 2]] - Y[F[:, 0]])
+    # descriptors hold raw addresses: the device tensors must stay referenced until the launch ran
+    dc1, dn1, dc2, dn2 = (torch.as_tensor(np.ascontiguousarray(z), device="cuda") for z in (c1, n1, c2, n2))
+    tots = []
+    for rowdot in (False, True):
+        tot = torch.zeros(1, dtype=torch.float64, device="cuda")
+        rd = torch.zeros(F.shape[0], dtype=torch.float64, device="cuda") if rowdot else None
+        rt.launch(_lib.OP_CURRENTS, _lib.PREC_FP64, 3,
+                  rt.seg(y=dc1, w0=dn1, x1=dc2, w1=dn2, total=tot, rowdot=rd,
+                         ny=F.shape[0], nx1=F.shape[0], sigma=0.1, kernel=0, order=10, coef=1 / 3))
+        tots.append(tot.item())
+    print("energy-only %.17e  plain %.17e" % tuple(tots), flush=True)
+    scale = float((np.abs(n1) ** 2).sum())   # magnitude of the self-term's diagonal
+    assert abs(tots[0] - tots[1]) <= 1e-12 * scale
+
+
+def test_symmetric_and_mixed_forms_small_cases():
+    """Energy-only sums equal the per-row form on curves (d = 2, ragged sizes) and on the
+    target norm (KNORM), FP32 and FP64."""
+    import torch
+    from registration_amd import _lib, runtime as rt, synthetic
+    D = lambda z: torch.as_tensor(np.ascontiguousarray(z), device="cuda")
+    for n, prec in ((256, _lib.PREC_FP64), (777, _lib.PREC_FP64), (3001, _lib.PREC_FP64),
+                    (2500, _lib.PREC_FP32C)):
+        X, _ = synthetic.circle(n)
+        Y = synthetic.star_curve(n, 5, 0.2)
+        E = synthetic.closed_curve_edges(n)
+        c1, l1 = (X[E[:, 0]] + X[E[:, 1]]) / 2, X[E[:, 1]] - X[E[:, 0]]
+        c2, l2 = (Y[E[:, 0]] + Y[E[:, 1]]) / 2, Y[E[:, 1]] - Y[E[:, 0]]
+        # referenced until the launches ran (descriptors hold raw addresses)
+        dc1, dl1, dc2, dl2 = D(c1), D(l1), D(c2), D(l2)
+        for op in (_lib.OP_CURRENTS, _lib.OP_KNORM):
+            out = []
+            for rowdot in (False, True):
+                tot = torch.zeros(1, dtype=torch.float64, device="cuda")
+                rd = torch.zeros(n, dtype=torch.float64, device="cuda") if rowdot else None
+                kw = dict(x1=dc2, w1=dl2, nx1=n) if op == _lib.OP_CURRENTS else {}
+                rt.launch(op, prec, 2, rt.seg(y=dc1, w0=dl1, total=tot, rowdot=rd, ny=n,
+                                              sigma=0.05, kernel=0, order=10, coef=0.5, **kw))
+                out.append(tot.item())
+            if op == _lib.OP_KNORM:
+                ref = float((l1 * orc.apply_k(c1, c1, l1, 0.05, 10)).sum())
+                assert abs(out[0] - ref) <= (1e-12 if prec == _lib.PREC_FP64 else 1e-6) * abs(ref)
+            tol = 1e-12 if prec == _lib.PREC_FP64 else 1e-6
+            assert abs(out[0] - out[1]) <= tol * float((l1 ** 2).sum()), (n, prec, op, out)
