@@ -253,8 +253,9 @@ def mixed_service_leg(k, prec):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (one registration "
-              "pair per GPU; see DESIGN.md for how the path shards)", file=sys.stderr)
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (registrations "
+              "are independent, so several GPUs would each serve their own requests with "
+              "unchanged kernels)", file=sys.stderr)
         return 2
     import numpy as np
     import torch

@@ -372,6 +372,7 @@ def topology(faces_np, nv: int):
     if _TOPOLOGY is None:
         _TOPOLOGY = collections.OrderedDict()
     faces_np = np.ascontiguousarray(faces_np)
+    check_faces(faces_np, nv)
     key = (torch.cuda.current_device(), int(nv), faces_np.shape, faces_np.dtype.str,
            hash(faces_np.tobytes()))
     hit = _TOPOLOGY.get(key)
@@ -385,6 +386,20 @@ def topology(faces_np, nv: int):
     while len(_TOPOLOGY) > TOPOLOGY_CACHE:
         _TOPOLOGY.popitem(last=False)
     return val
+
+
+def check_faces(faces_np, nv: int) -> None:
+    """Reject face arrays the kernels cannot index safely: they must be integer (F, 2) or
+    (F, 3) arrays with every index in [0, nv) and nv below 2^31 (the device copies are int32)."""
+    if faces_np.ndim != 2 or faces_np.shape[1] not in (2, 3) or \
+            not np.issubdtype(faces_np.dtype, np.integer):
+        raise ValueError(f"faces must be an integer (F, 2) or (F, 3) array, got "
+                         f"{faces_np.dtype} {faces_np.shape}")
+    if not 0 <= int(nv) < 2 ** 31:
+        raise ValueError(f"vertex count {nv} is outside [0, 2^31)")
+    if faces_np.size and (int(faces_np.min()) < 0 or int(faces_np.max()) >= nv):
+        raise ValueError(f"face indices must lie in [0, {nv}), got [{int(faces_np.min())}, "
+                         f"{int(faces_np.max())}]")
 
 
 def incidence(faces_np, nv: int):

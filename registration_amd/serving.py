@@ -113,6 +113,10 @@ def footprint_bytes(req: Request) -> int:
 
 _HOST_ONLY = (cg.ACCEPT, cg.END_ITER)
 
+# Errors of a request's own inputs (shapes, indices, options outside the GPU path), raised
+# by build_registration before any device work; device errors are not caught.
+_REJECT = (ValueError, NotImplementedError, IndexError, TypeError)
+
 
 class _Active:
     __slots__ = ("req", "opt", "gen", "pending", "admitted")
@@ -148,6 +152,10 @@ class Server:
     optimisers and launches their next requests, while the GPU is still working on the
     other cohort's launches.  Each registration's own sequence of operations is unchanged,
     so its result is the same as alone.
+
+    A request whose inputs cannot be registered (unknown kind, mismatched shapes, face
+    indices outside the vertex range, options outside the GPU path) is not admitted: its
+    Result has energy NaN, no iterations and stop_reason 'rejected: <error>'.
     """
 
     def __init__(self, hbm_cap_bytes: float = 200e9, max_active: int = 256,
@@ -191,9 +199,17 @@ class Server:
             if self.used + need > self.cap and (self.cohorts[0] or self.cohorts[1]):
                 break
             self.queue.popleft()
+            try:
+                active = _Active(req, build_registration(req, self.precision), now)
+            except _REJECT as exc:
+                # Invalid inputs fail this request only; the others keep being served.
+                self.results[req.rid] = Result(req.rid, float('nan'), 0,
+                                               f"rejected: {type(exc).__name__}: {exc}",
+                                               req.arrival, now, now)
+                continue
             self.used += need
             c = 0 if len(self.cohorts[0]) <= len(self.cohorts[1]) else 1
-            self.cohorts[c].append(_Active(req, build_registration(req, self.precision), now))
+            self.cohorts[c].append(active)
 
     def _finish(self, a: _Active, value) -> None:
         o = a.opt

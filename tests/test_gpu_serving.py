@@ -72,3 +72,34 @@ def test_admission_cap_limits_concurrency():
     assert peak == 1 and len(srv.results) == 3
     with pytest.raises(ValueError):
         serving.Server(hbm_cap_bytes=1000).submit(reqs[0])
+
+
+def test_invalid_requests_are_rejected_and_the_rest_served():
+    """Out-of-range face indices and mismatched landmark shapes fail their own requests at
+    admission (before any kernel reads them); the valid requests in the same stream finish
+    with the results they have alone."""
+    from registration_amd import serving
+    reqs = requests()
+    good = [reqs[0], reqs[4], reqs[6]]
+    F, V = reqs[1].template
+    # index len(V): the host Surface's own NumPy indexing fails, as the reference's does;
+    # index -1 passes host NumPy (it wraps) and is caught before the faces reach the device.
+    bad_faces = [serving.Request(rid=rid, kind='surface', template=(np.where(F == 7, bad, F), V),
+                                 target=reqs[1].target, sigmaKernel=0.3, sigmaDist=0.1,
+                                 sigmaError=0.1, T=4, maxIter=6)
+                 for rid, bad in ((90, len(V)), (92, -1))]
+    X = reqs[6].template[1]
+    bad_shape = serving.Request(rid=91, kind='landmark', template=(None, X),
+                                target=(None, X[:-1]), sigmaKernel=0.3, sigmaDist=0.1,
+                                sigmaError=0.1, T=4, maxIter=6)
+    alone = {r.rid: serving.plain_run(r, 'fp32c') for r in good}
+    srv = serving.Server(precision='fp32c')
+    for r in (good[0], bad_faces[0], good[1], bad_shape, bad_faces[1], good[2]):
+        srv.submit(r)
+    res = srv.run()
+    assert res[90].stop_reason.startswith("rejected: IndexError") and np.isnan(res[90].energy)
+    assert res[91].stop_reason.startswith("rejected: ValueError") and np.isnan(res[91].energy)
+    assert res[92].stop_reason.startswith("rejected: ValueError") and "[0, 642)" in res[92].stop_reason
+    for rid, a in alone.items():
+        assert res[rid].history == a.history and res[rid].energy == a.energy
+    assert srv.used == 0.0
