@@ -71,7 +71,18 @@ enum {
    * dp = p_k - p_l                                     (applyDDiffK11and12) */
   LDDMM_OP_DDIFF_K11_12 = 6,
   /* total[0] = <w0, K(y, y) w0>  (currentNorm0 when w0 = surfel) */
-  LDDMM_OP_KNORM = 7
+  LDDMM_OP_KNORM = 7,
+  /* measure data term of a deformed shape with centers c1 = y, face weights
+   * a1 = w0 (ny) and normals n1 = w2 (ny, d), against a target with centers
+   * c2 = x1 and weights a2 = w1 (nx1; nx1 = 0 leaves the target out):
+   *   total[0] = <a1, K(c1,c1) a1> - 2 <a1, K(c1,c2) a2>     (measureNormDef;
+   *              measureNorm0 of a shape when nx1 = 0)
+   * and, when out0/out1 are non-NULL (gradient form, w2 required):
+   *   out0 = z1 = (K(c1,c1) a1 - K(c1,c2) a2)[k] * n1[k] / a1[k]
+   *   out1 = coef * [applyDiffKT(c1,[a1],[a1]) - applyDiffKT(c2,[a1],[a2],c1)]
+   * (surfaces.py:1092-1182, curves.py:621-701; a = sqrt(|normal|^2 + 1e-10) is
+   * computed by the caller, e.g. lddmm_geometry_batch's weights output). */
+  LDDMM_OP_MEASURE = 8
 };
 
 /*
@@ -224,6 +235,7 @@ typedef struct lddmm_geom {
   const int32_t *faces;    /* (nf, k): k = 3 triangles (d = 3) or k = 2 edges */
   double *centers;         /* (nf, d) */
   double *normals;         /* (nf, d): surfel (triangles) or lenel (edges) */
+  double *weights;         /* (nf) measure weights sqrt(|normal|^2 + 1e-10), or NULL */
   int64_t nf;
   int32_t k;
   int32_t d;

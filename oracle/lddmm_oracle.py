@@ -8,7 +8,8 @@ It restates, in Python 3 / NumPy float64, the executed reference path:
   Kernel                  py-lddmm/kernelFunctions.py:238-454 (sums in oracle/kernel_sums.c,
                           restating kernelFunctions.f90:1-114,430-496)
   landmark flow/adjoint   py-lddmm/pointEvolution.py:599-817
-  surfaces / curves       py-lddmm/surfaces.py:61-75,711-717,1009-1089; curves.py:58-73,539-613
+  surfaces / curves       py-lddmm/surfaces.py:61-75,711-717,1009-1182; curves.py:58-73,539-701
+                          (currents and measure data terms)
   SurfaceMatching         py-lddmm/surfaceMatching.py:19-484 (affine='none', symmetric=False)
   CurveMatching           py-lddmm/curveMatching.py:19-352 (affine='none')
   cg                      py-lddmm/conjugateGradient.py:33-217
@@ -337,6 +338,53 @@ def surf_currentNormGradient(fvDef, fv1, K):
     return 2 * px
 
 
+def _measure_weights(cr, eps=1e-10):
+    """Face weights of the measure data term: sqrt(|cr|^2 + eps) per face."""
+    return np.sqrt((cr ** 2).sum(axis=1) + eps)
+
+
+def surf_measureNorm0(fv1, K):
+    """surfaces.py:1092-1099: <a2, K(c2, c2) a2>, a2 = sqrt(|surfel|^2 + 1e-10)."""
+    cr2 = _measure_weights(fv1.surfel)[:, np.newaxis]
+    return np.multiply(cr2, K.applyK(fv1.centers, cr2)).sum()
+
+
+def surf_measureNormDef(fvDef, fv1, K):
+    """surfaces.py:1104-1117."""
+    c1, c2 = fvDef.centers, fv1.centers
+    cr1 = _measure_weights(fvDef.surfel)[:, np.newaxis]
+    cr2 = _measure_weights(fv1.surfel)[:, np.newaxis]
+    return (np.multiply(cr1, K.applyK(c1, cr1)).sum()
+            - 2 * np.multiply(cr1, K.applyK(c2, cr2, firstVar=c1)).sum())
+
+
+def surf_measureNormGradient(fvDef, fv1, K):
+    """surfaces.py:1125-1182: z1 = (K(c1,c1) a1 - K(c1,c2) a2) * surfel/a1 and
+    dz1 = (1/3)[applyDiffKT(c1,[a1],[a1]) - applyDiffKT(c2,[a1],[a2],c1)], then the
+    currents vertex scatter."""
+    xDef = fvDef.vertices
+    c1, c2 = fvDef.centers, fv1.centers
+    dim = c1.shape[1]
+    a1 = _measure_weights(fvDef.surfel)
+    a2 = _measure_weights(fv1.surfel)
+    cr1 = fvDef.surfel / a1[:, np.newaxis]
+    z1 = K.applyK(c1, a1[:, np.newaxis]) - K.applyK(c2, a2[:, np.newaxis], firstVar=c1)
+    z1 = np.multiply(z1, cr1)
+    w1 = a1[np.newaxis, :, np.newaxis]
+    dz1 = (1. / 3.) * (K.applyDiffKT(c1, w1, w1)
+                       - K.applyDiffKT(c2, w1, a2[np.newaxis, :, np.newaxis], firstVar=c1))
+    xDef1 = xDef[fvDef.faces[:, 0]]
+    xDef2 = xDef[fvDef.faces[:, 1]]
+    xDef3 = xDef[fvDef.faces[:, 2]]
+    px = np.zeros([xDef.shape[0], dim])
+    for I, crs in ((fvDef.faces[:, 0], np.cross(xDef3 - xDef2, z1)),
+                   (fvDef.faces[:, 1], np.cross(xDef1 - xDef3, z1)),
+                   (fvDef.faces[:, 2], np.cross(xDef2 - xDef1, z1))):
+        for k in range(I.size):
+            px[I[k]] = px[I[k]] + dz1[k] - crs[k]
+    return 2 * px
+
+
 class Curve:
     """curves.Curve restated for the pointSet / FV / copy constructors (curves.py:15-73)."""
 
@@ -394,6 +442,54 @@ def curve_currentNormGradient(fvDef, fv1, K):
     for k in range(I.size):
         px[I[k]] = px[I[k]] + dz1[k] + z1[k]
     return 2 * px
+
+
+def curve_measureNorm0(fv1, K):
+    """curves.py:621-628: <a2, K(c2, c2) a2> with a2 = |lenel| (no 1e-10 here, unlike
+    measureNormDef and the gradient)."""
+    cr2 = _measure_weights(fv1.lenel, 0.0)[:, np.newaxis]
+    return np.multiply(cr2, K.applyK(fv1.centers, cr2)).sum()
+
+
+def curve_measureNormDef(fvDef, fv1, K):
+    """curves.py:632-645."""
+    c1, c2 = fvDef.centers, fv1.centers
+    cr1 = _measure_weights(fvDef.lenel)[:, np.newaxis]
+    cr2 = _measure_weights(fv1.lenel)[:, np.newaxis]
+    return (np.multiply(cr1, K.applyK(c1, cr1)).sum()
+            - 2 * np.multiply(cr1, K.applyK(c2, cr2, firstVar=c1)).sum())
+
+
+def curve_measureNormGradient(fvDef, fv1, K):
+    """curves.py:653-701: as the surface form with dz1 = (1/2)[...] and the edge scatter
+    px[f0] += dz1 - z1, px[f1] += dz1 + z1."""
+    xDef = fvDef.vertices
+    c1, c2 = fvDef.centers, fv1.centers
+    dim = c1.shape[1]
+    a1 = _measure_weights(fvDef.lenel)
+    a2 = _measure_weights(fv1.lenel)
+    cr1 = fvDef.lenel / a1[:, np.newaxis]
+    z1 = K.applyK(c1, a1[:, np.newaxis]) - K.applyK(c2, a2[:, np.newaxis], firstVar=c1)
+    z1 = np.multiply(z1, cr1)
+    w1 = a1[np.newaxis, :, np.newaxis]
+    dz1 = .5 * (K.applyDiffKT(c1, w1, w1)
+                - K.applyDiffKT(c2, w1, a2[np.newaxis, :, np.newaxis], firstVar=c1))
+    px = np.zeros([xDef.shape[0], dim])
+    I = fvDef.faces[:, 0]
+    for k in range(I.size):
+        px[I[k]] = px[I[k]] + dz1[k] - z1[k]
+    I = fvDef.faces[:, 1]
+    for k in range(I.size):
+        px[I[k]] = px[I[k]] + dz1[k] + z1[k]
+    return 2 * px
+
+
+_DATA_TERMS = {
+    'surface': {'current': (surf_currentNorm0, surf_currentNormDef, surf_currentNormGradient),
+                'measure': (surf_measureNorm0, surf_measureNormDef, surf_measureNormGradient)},
+    'curve': {'current': (curve_currentNorm0, curve_currentNormDef, curve_currentNormGradient),
+              'measure': (curve_measureNorm0, curve_measureNormDef, curve_measureNormGradient)},
+}
 
 
 # --------------------------------------------------------------------------------------
@@ -516,10 +612,11 @@ class _Matching:
 
 
 class SurfaceMatching(_Matching):
-    """surfaceMatching.SurfaceMatching with errorType='current' (surfaceMatching.py:74-484)."""
+    """surfaceMatching.SurfaceMatching with errorType 'current' or 'measure'
+    (surfaceMatching.py:29-36,74-484)."""
 
     def __init__(self, Template, Target, KparDiff, KparDist, sigmaError=1.0, timeStep=.1,
-                 regWeight=1.0, maxIter=1000):
+                 regWeight=1.0, maxIter=1000, errorType='current'):
         self.fv0 = Surface(surf=Template)
         self.fv1 = Surface(surf=Target)
         v0 = self.fv0.surfVolume()
@@ -528,9 +625,7 @@ class SurfaceMatching(_Matching):
             self.fv1.flipFaces()
         self.fvInit = Surface(surf=self.fv0)
         self.fvDef = Surface(surf=self.fvInit)
-        self.fun_obj0 = surf_currentNorm0
-        self.fun_obj = surf_currentNormDef
-        self.fun_objGrad = surf_currentNormGradient
+        self.fun_obj0, self.fun_obj, self.fun_objGrad = _DATA_TERMS['surface'][errorType]
         self._setup(self.fvInit.vertices, KparDiff, KparDist, sigmaError, timeStep, regWeight)
         self.maxIter = maxIter
         self.iter = 0
@@ -549,16 +644,15 @@ class SurfaceMatching(_Matching):
 
 
 class CurveMatching(_Matching):
-    """curveMatching.CurveMatching with errorType='current' (curveMatching.py:58-352)."""
+    """curveMatching.CurveMatching with errorType 'current' or 'measure'
+    (curveMatching.py:24-33,58-352)."""
 
     def __init__(self, Template, Target, KparDiff, KparDist, sigmaError=1.0, timeStep=.1,
-                 regWeight=1.0, maxIter=1000, gradLB=0.001):
+                 regWeight=1.0, maxIter=1000, gradLB=0.001, errorType='current'):
         self.fv0 = Curve(curve=Template)
         self.fv1 = Curve(curve=Target)
         self.fvDef = Curve(curve=self.fv0)
-        self.fun_obj0 = curve_currentNorm0
-        self.fun_obj = curve_currentNormDef
-        self.fun_objGrad = curve_currentNormGradient
+        self.fun_obj0, self.fun_obj, self.fun_objGrad = _DATA_TERMS['curve'][errorType]
         self._setup(self.fv0.vertices, KparDiff, KparDist, sigmaError, timeStep, regWeight)
         self.maxIter = maxIter
         self.iter = 0

@@ -34,6 +34,7 @@ OP_KDOT = 4
 OP_CURRENTS = 5
 OP_DDIFF_K11_12 = 6
 OP_KNORM = 7
+OP_MEASURE = 8
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -65,7 +66,8 @@ class Geom(ctypes.Structure):
     """Mirror of ``lddmm_geom_t``."""
 
     _fields_ = [("vertices", _P), ("faces", _P), ("centers", _P), ("normals", _P),
-                ("nf", _I64), ("k", ctypes.c_int32), ("d", ctypes.c_int32), ("blk0", _I64)]
+                ("weights", _P), ("nf", _I64), ("k", ctypes.c_int32), ("d", ctypes.c_int32),
+                ("blk0", _I64)]
 
 
 class Gather(ctypes.Structure):

@@ -290,6 +290,7 @@ __device__ inline SrcView resolve_src(int op, const lddmm_seg_t *s, int set) {
       v.w[1] = s->w1;
       break;
     case LDDMM_OP_CURRENTS:  // set 0: (c1, cr1); set 1: (c2, -cr2)
+    case LDDMM_OP_MEASURE:   // set 0: (c1, a1);  set 1: (c2, -a2)
       if (set == 0) {
         v.pos = s->y;
         v.w[0] = s->w0;
@@ -325,6 +326,7 @@ __device__ inline TgtView resolve_tgt(int op, const lddmm_seg_t *s) {
       v.w[1] = s->w0;
       break;
     case LDDMM_OP_CURRENTS:  // a1 = [cr1]
+    case LDDMM_OP_MEASURE:   // a1 = [a1]
       v.w[0] = s->w0;
       break;
     default:
@@ -335,6 +337,7 @@ __device__ inline TgtView resolve_tgt(int op, const lddmm_seg_t *s) {
 
 // Row stride of the weight / output arrays of an operation.
 __host__ __device__ inline int weight_ld(int op, const lddmm_seg_t *s, int d) {
+  if (op == LDDMM_OP_MEASURE) return 1;
   return (op == LDDMM_OP_APPLY_K || op == LDDMM_OP_DIFF_KT) ? s->m : d;
 }
 
@@ -454,13 +457,14 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   if (l1 > src.n) l1 = src.n;
 
   // Symmetric self-sums.  When only the total <w, K(y, y) w> is asked for (KNORM, and
-  // the self set of an energy-only CURRENTS segment: no per-row outputs), each
+  // the self set of an energy-only CURRENTS or MEASURE segment: no per-row outputs), each
   // unordered pair of TS-blocks is evaluated once: a wave whose rows lie in block
   // rb takes source tile sb with weight 2 if sb > rb, 1 if sb == rb, 0 if sb < rb.
   // Tiles start at multiples of TS and a wave's rows never straddle a TS block,
   // so the weights are wave-uniform and every unordered pair counts twice.
-  const bool energy_only = !sg->rowdot && !sg->out0 &&
-                           (op == LDDMM_OP_KNORM || (op == LDDMM_OP_CURRENTS && NA == 0));
+  const bool energy_only =
+      !sg->rowdot && !sg->out0 &&
+      (op == LDDMM_OP_KNORM || ((op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE) && NA == 0));
   const bool sym = energy_only && set == 0;
   const int64_t wave_blk = ((int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT)) / TS;
   const int64_t wg_blk = ((int64_t)rt * (BLOCK * RPT)) / TS;
@@ -1070,6 +1074,24 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
         }
       }
     }
+  } else if constexpr (OP == LDDMM_OP_MEASURE) {
+    if (valid) {
+      // One weight column: accK0 = (K(c1,c1) a1)_k, accK1 = -(K(c1,c2) a2)_k.
+      const double a = sg->w0[k];
+      rd0 = a * accK0[0] + 2.0 * (a * accK1[0]);
+      if (sg->rowdot) sg->rowdot[k] = rd0;
+      if (sg->out0) {
+        const double zs = accK0[0] + accK1[0];
+#pragma unroll
+        for (int c = 0; c < D; ++c) sg->out0[k * D + c] = zs * (sg->w2[k * D + c] / a);
+      }
+      if constexpr (NA > 0) {
+        if (sg->out1) {
+#pragma unroll
+          for (int c = 0; c < D; ++c) sg->out1[k * D + c] = sg->coef * (dscale * accD[c]);
+        }
+      }
+    }
   } else if constexpr (OP == LDDMM_OP_DDIFF_K11_12) {
     if (valid) {
       const double inv = 1.0 / (sigma * sigma);
@@ -1079,7 +1101,8 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
   }
 
   constexpr int NRED = (OP == LDDMM_OP_FLOW_STEP || OP == LDDMM_OP_KNORM ||
-                        OP == LDDMM_OP_CURRENTS) ? 1 : (OP == LDDMM_OP_KDOT ? 2 : 0);
+                        OP == LDDMM_OP_CURRENTS || OP == LDDMM_OP_MEASURE)
+                           ? 1 : (OP == LDDMM_OP_KDOT ? 2 : 0);
   if constexpr (NRED > 0) {
     double t0 = block_sum(rd0, sh);
     double t1 = 0.0;
@@ -1116,8 +1139,18 @@ __global__ __launch_bounds__(TOT_BLOCK) void k_totals(const lddmm_seg_t *__restr
 // ---------------------------------------------------------------------------
 // Geometry and gathers evaluate each expression in the reference's NumPy order without
 // FMA contraction, so their FP64 results are the reference's bit for bit.
+// Measure weight of a face: sqrt(|n|^2 + 1e-10), the sum of squares in index order
+// (surfaces.py:1098,1130; curves.py:658).
+__device__ inline double measure_weight(const double *n, int d) {
+#pragma clang fp contract(off)
+  double s = n[0] * n[0];
+  for (int q = 1; q < d; ++q) s = s + n[q] * n[q];
+  return sqrt(s + 1e-10);
+}
+
 __device__ inline void surface_face(const double *__restrict__ v, const int32_t *__restrict__ f,
-                                    int64_t i, double *__restrict__ cen, double *__restrict__ nrm) {
+                                    int64_t i, double *__restrict__ cen, double *__restrict__ nrm,
+                                    double *__restrict__ wts = nullptr) {
 #pragma clang fp contract(off)
   const double *a = v + 3 * (int64_t)f[3 * i + 0];
   const double *b = v + 3 * (int64_t)f[3 * i + 1];
@@ -1126,21 +1159,26 @@ __device__ inline void surface_face(const double *__restrict__ v, const int32_t 
   for (int q = 0; q < 3; ++q) cen[3 * i + q] = (a[q] + b[q] + c[q]) / 3.0;
   double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
   double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-  nrm[3 * i + 0] = e1[1] * e2[2] - e1[2] * e2[1];
-  nrm[3 * i + 1] = e1[2] * e2[0] - e1[0] * e2[2];
-  nrm[3 * i + 2] = e1[0] * e2[1] - e1[1] * e2[0];
+  double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                 e1[0] * e2[1] - e1[1] * e2[0]};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) nrm[3 * i + q] = n[q];
+  if (wts) wts[i] = measure_weight(n, 3);
 }
 
 __device__ inline void curve_edge(const double *__restrict__ v, const int32_t *__restrict__ e,
                                   int64_t i, int d, double *__restrict__ cen,
-                                  double *__restrict__ len) {
+                                  double *__restrict__ len, double *__restrict__ wts = nullptr) {
 #pragma clang fp contract(off)
   const double *a = v + (int64_t)d * e[2 * i + 0];
   const double *b = v + (int64_t)d * e[2 * i + 1];
+  double l[8];  // d <= 8, checked by lddmm_curve_geometry and lddmm_geometry_batch
   for (int q = 0; q < d; ++q) {
     cen[d * i + q] = (a[q] + b[q]) / 2.0;
-    len[d * i + q] = b[q] - a[q];
+    l[q] = b[q] - a[q];
+    len[d * i + q] = l[q];
   }
+  if (wts) wts[i] = measure_weight(l, d);
 }
 
 __device__ inline void surface_gather(const double *__restrict__ v, const int32_t *__restrict__ f,
@@ -1229,9 +1267,9 @@ __global__ __launch_bounds__(256) void k_geometry_batch(const lddmm_geom_t *__re
   const int64_t i = (b - e->blk0) * 256 + threadIdx.x;
   if (i >= e->nf) return;
   if (e->k == 3)
-    surface_face(e->vertices, e->faces, i, e->centers, e->normals);
+    surface_face(e->vertices, e->faces, i, e->centers, e->normals, e->weights);
   else
-    curve_edge(e->vertices, e->faces, i, e->d, e->centers, e->normals);
+    curve_edge(e->vertices, e->faces, i, e->d, e->centers, e->normals, e->weights);
 }
 
 __global__ __launch_bounds__(256) void k_gather_batch(const lddmm_gather_t *__restrict__ g, int n) {
@@ -1311,6 +1349,7 @@ int op_nred(int op) {
     case LDDMM_OP_FLOW_STEP:
     case LDDMM_OP_KNORM:
     case LDDMM_OP_CURRENTS:
+    case LDDMM_OP_MEASURE:
       return 1;
     case LDDMM_OP_KDOT:
       return 2;
@@ -1319,7 +1358,7 @@ int op_nred(int op) {
   }
 }
 
-bool op_valid(int op) { return op >= LDDMM_OP_APPLY_K && op <= LDDMM_OP_KNORM; }
+bool op_valid(int op) { return op >= LDDMM_OP_APPLY_K && op <= LDDMM_OP_MEASURE; }
 
 // Launch-uniform shape of a planned operation.
 struct Shape {
@@ -1349,6 +1388,11 @@ int op_shape(int op, int d, const lddmm_seg_t &s0, Shape *sh) {
     case LDDMM_OP_CURRENTS: {
       bool grad = s0.out1 != nullptr;
       *sh = {d, grad ? 1 : 0, true, grad ? 2 * d : d};
+      return LDDMM_OK;
+    }
+    case LDDMM_OP_MEASURE: {
+      bool grad = s0.out1 != nullptr;
+      *sh = {1, grad ? 1 : 0, true, grad ? 1 + d : 1};
       return LDDMM_OK;
     }
     case LDDMM_OP_DDIFF_K11_12:
@@ -1387,7 +1431,7 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     if (s.kernel == LDDMM_KERNEL_LAPLACIAN) {
       if (s.order < 0 || s.order > 4) return LDDMM_EINVAL;
       bool deriv = op == LDDMM_OP_DIFF_KT || op == LDDMM_OP_COVECTOR_STEP ||
-                   (op == LDDMM_OP_CURRENTS && s.out1);
+                   ((op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE) && s.out1);
       if (deriv && s.order < 1) return LDDMM_EINVAL;
       if (op == LDDMM_OP_DDIFF_K11_12 && s.order < 2) return LDDMM_EINVAL;
     }
@@ -1429,6 +1473,12 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
         nx0 = s.ny;
         nx1 = s.nx1;
         break;
+      case LDDMM_OP_MEASURE:
+        if (!s.w0 || s.nx1 < 0 || (s.nx1 > 0 && (!s.x1 || !s.w1))) return LDDMM_EINVAL;
+        if (s.out0 && !s.w2) return LDDMM_EINVAL;  // z1 needs the normals
+        nx0 = s.ny;
+        nx1 = s.nx1;
+        break;
       case LDDMM_OP_DDIFF_K11_12:
         if (!s.w0 || !s.w1 || !s.w2 || !s.out0) return LDDMM_EINVAL;
         nx0 = s.ny;
@@ -1436,7 +1486,7 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     }
     if (nx0 < 1) return LDDMM_EINVAL;
     s.nx0 = nx0;
-    if (op != LDDMM_OP_CURRENTS) s.nx1 = 0;
+    if (op != LDDMM_OP_CURRENTS && op != LDDMM_OP_MEASURE) s.nx1 = 0;
     const int64_t row_tiles = ceil_div(s.ny, rows);
     const int64_t src_tiles = ceil_div(nx0, TS) + (nx1 ? ceil_div(nx1, TS) : 0);
     int64_t chunk_tiles = ceil_div(src_tiles * row_tiles, ITEMS_PER_SEGMENT);
@@ -1589,6 +1639,15 @@ int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, 
       } else {
         dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
         launch_fin<LDDMM_OP_CURRENTS, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+      }
+      return LDDMM_OK;
+    case LDDMM_OP_MEASURE:
+      if (sh.NA > 0) {
+        dispatch_pairs<D, 1, true, 1>(prec, kern, segs, nseg, op, n_items, part, st);
+        launch_fin<LDDMM_OP_MEASURE, D, 1, true, 1>(segs, nseg, prec, kern, n_fin, part, red, st);
+      } else {
+        dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+        launch_fin<LDDMM_OP_MEASURE, D, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       }
       return LDDMM_OK;
     case LDDMM_OP_DDIFF_K11_12:

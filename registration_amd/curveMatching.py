@@ -3,7 +3,7 @@
 
 ``CurveMatchingParam`` and ``CurveMatching`` keep the reference constructors and optimiser
 protocol on a GPU-resident registration (matching.DeviceMatching).  Supported: the currents
-data term, affine='none'.  The per-iteration Jacobian flow and the VTK snapshots of
+and measure data terms (errorType 'current' or 'measure', the default), affine='none'.  The per-iteration Jacobian flow and the VTK snapshots of
 endOfIteration / endOptim only produce output files and are not run.
 """
 from __future__ import annotations
@@ -27,12 +27,17 @@ class CurveMatchingParam(matchingParam.MatchingParam):
             self.fun_obj0 = curves.currentNorm0
             self.fun_obj = curves.currentNormDef
             self.fun_objGrad = curves.currentNormGradient
+        elif errorType == 'measure':
+            self.fun_obj0 = curves.measureNorm0
+            self.fun_obj = curves.measureNormDef
+            self.fun_objGrad = curves.measureNormGradient
         else:
             self.fun_obj0 = self.fun_obj = self.fun_objGrad = None
 
 
 class CurveMatching(DeviceMatching):
     coef = 0.5
+    measure_norm0_eps = 0.0   # curves.measureNorm0 weights are |lenel| (curves.py:624)
 
     def __init__(self, Template=None, Target=None, fileTempl=None, fileTarg=None, param=None,
                  maxIter=1000, regWeight=1.0, affineWeight=1.0, verb=True, gradLB=0.001,
@@ -45,9 +50,9 @@ class CurveMatching(DeviceMatching):
         self.fv0 = curves.Curve(curve=Template)
         self.fv1 = curves.Curve(curve=Target)
         self.param = CurveMatchingParam() if param is None else param
-        if self.param.errorType != 'current':
-            raise NotImplementedError("the GPU path implements the currents data term "
-                                      "(errorType='current')")
+        if self.param.errorType not in ('current', 'measure'):
+            raise NotImplementedError("the GPU path implements the currents and measure data "
+                                      "terms (errorType 'current' or 'measure')")
         self.fvDef = curves.Curve(curve=self.fv0)
         self.outputDir = outputDir
         self.maxIter = maxIter
@@ -60,10 +65,11 @@ class CurveMatching(DeviceMatching):
         Tsize = int(round(1.0 / self.param.timeStep))
         self._init_device(np.copy(self.fv0.vertices), self.fv0.faces, self.fv1.centers,
                           self.fv1.lenel, self.param.KparDiff, self.param.KparDist,
-                          self.param.sigmaError, Tsize, regWeight)
+                          self.param.sigmaError, Tsize, regWeight,
+                          data_term=self.param.errorType)
 
-    def _new_shape(self, vertices):
-        return curves.DeviceCurve(self.faces_np, vertices)
+    def _new_shape(self, vertices, measure=False):
+        return curves.DeviceCurve(self.faces_np, vertices, measure=measure)
 
 
     def endOptim(self):

@@ -4,7 +4,9 @@
 midpoints, lenel = edge vectors; py-lddmm/curves.py:15-73) and the constructors
 ``Curve(curve=...)``, ``Curve(FV=(edges, vertices))``, ``Curve(pointSet=..., isOpen=...)``.
 ``currentNorm0`` / ``currentNormDef`` / ``currentNorm`` / ``currentNormGradient`` keep the
-reference signatures (curves.py:539-613) and run on the GPU.
+reference signatures (curves.py:539-613) and run on the GPU, and so do the measure-term
+``measureNorm0`` / ``measureNormDef`` / ``measureNorm`` / ``measureNormGradient``
+(curves.py:621-701).
 """
 from __future__ import annotations
 
@@ -57,9 +59,10 @@ class Curve:
 
 
 class DeviceCurve:
-    """A curve resident on the GPU (vertices, edges, centers, edge vectors, incidence CSR)."""
+    """A curve resident on the GPU (vertices, edges, centers, edge vectors, incidence CSR;
+    with ``measure``, the edge weights sqrt(|lenel|^2 + 1e-10), else ``weights`` is None)."""
 
-    def __init__(self, faces_np, vertices):
+    def __init__(self, faces_np, vertices, measure: bool = False):
         faces_np = np.asarray(faces_np)
         self.nv, self.d = int(vertices.shape[0]), int(vertices.shape[1])
         self.faces, self.ptr, self.inc = engine.topology(faces_np, self.nv)
@@ -67,10 +70,14 @@ class DeviceCurve:
         dev = rt.device()
         self.centers = torch.empty((E, self.d), dtype=torch.float64, device=dev)
         self.lenel = torch.empty((E, self.d), dtype=torch.float64, device=dev)
+        self.weights = torch.empty(E, dtype=torch.float64, device=dev) if measure else None
         self.vertices = None
         self.update(vertices)
 
     def update(self, vertices):
+        if self.weights is not None:
+            engine.geometry_batch([self], [rt.as_dev(vertices)])
+            return
         self.vertices = rt.as_dev(vertices)
         engine.curve_geometry(self.vertices, self.faces, self.centers, self.lenel)
 
@@ -126,4 +133,52 @@ def currentNormGradient(fvDef, fv1, KparDist):
     b = DeviceCurve(fv1.faces, fv1.vertices)
     px = torch.empty((a.nv, a.d), dtype=torch.float64, device=rt.device())
     current_gradient_dev(a, b.centers, b.lenel, _kernel(KparDist), 2.0, px)
+    return px.cpu().numpy()
+
+
+# ---- measure data term (curves.py:621-701) ------------------------------------------------
+def measureNorm0(fv1, KparDist):
+    """<a, K(c, c) a> with a = |lenel| (curves.py:621-628: no 1e-10 here, unlike
+    measureNormDef and measureNormGradient)."""
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.measure([engine.MeasureJob(rt.as_dev(fv1.centers),
+                                      rt.as_dev(engine.measure_weights(fv1.lenel, 0.0)),
+                                      None, None, _kernel(KparDist).energy(), tot)])
+    return float(tot.item())
+
+
+def measureNormDef(fvDef, fv1, KparDist):
+    """curves.py:632-645."""
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.measure([engine.MeasureJob(rt.as_dev(fvDef.centers),
+                                      rt.as_dev(engine.measure_weights(fvDef.lenel)),
+                                      rt.as_dev(fv1.centers), rt.as_dev(engine.measure_weights(fv1.lenel)),
+                                      _kernel(KparDist).energy(), tot, coef=0.5)])
+    return float(tot.item())
+
+
+def measureNorm(fvDef, fv1, KparDist):
+    return measureNormDef(fvDef, fv1, KparDist) + measureNorm0(fv1, KparDist)
+
+
+def measure_gradient_dev(s: DeviceCurve, c2, a2, K: engine.DevKernel, scale: float,
+                         px: torch.Tensor, total: torch.Tensor | None = None) -> torch.Tensor:
+    E, d = s.centers.shape
+    dev = s.centers.device
+    z1 = torch.empty((E, d), dtype=torch.float64, device=dev)
+    dz = torch.empty((E, d), dtype=torch.float64, device=dev)
+    if total is None:
+        total = torch.empty(1, dtype=torch.float64, device=dev)
+    engine.measure([engine.MeasureJob(s.centers, s.weights, c2, a2, K, total, n1=s.lenel,
+                                      z1=z1, dz=dz, coef=0.5)])
+    engine.curve_vertex_gradient(z1, dz, s.ptr, s.inc, s.nv, scale, px)
+    return px
+
+
+def measureNormGradient(fvDef, fv1, KparDist):
+    """curves.py:653-701."""
+    a = DeviceCurve(fvDef.faces, fvDef.vertices, measure=True)
+    px = torch.empty((a.nv, a.d), dtype=torch.float64, device=rt.device())
+    measure_gradient_dev(a, rt.as_dev(fv1.centers), rt.as_dev(engine.measure_weights(fv1.lenel)),
+                         _kernel(KparDist), 2.0, px)
     return px.cpu().numpy()

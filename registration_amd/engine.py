@@ -291,12 +291,12 @@ def currents_record(c1, cr1, c2, cr2, K: DevKernel, coef: float) -> np.ndarray:
     return r
 
 
-def currents_records(recs, prec, d, totals) -> None:
-    """Currents energies of n records of one (kernel family, precision, d) into ``totals``
-    (device addresses), one launch."""
+def currents_records(recs, prec, d, totals, op=_lib.OP_CURRENTS) -> None:
+    """Data-term energies (currents, or measure with op=OP_MEASURE) of n records of one
+    (kernel family, precision, d) into ``totals`` (device addresses), one launch."""
     r = np.array(recs, copy=True)
     r["total"] = _u64(totals)
-    rt.launch(_lib.OP_CURRENTS, prec, d, r)
+    rt.launch(op, prec, d, r)
 
 
 def currents(jobs: Sequence[CurrentsJob]) -> None:
@@ -317,6 +317,60 @@ def currents(jobs: Sequence[CurrentsJob]) -> None:
         r["kernel"] = [j.K.kernel for j in group]
         r["order"] = [j.K.order for j in group]
         rt.launch(_lib.OP_CURRENTS, group[0].K.prec, d, r)
+
+
+# --------------------------------------------------------------------------------------
+# Measure data term
+# --------------------------------------------------------------------------------------
+def measure_weights(cr: np.ndarray, eps: float = 1e-10) -> np.ndarray:
+    """Host face weights of the measure data term, sqrt(|cr|^2 + eps) in the reference's
+    NumPy form (surfaces.py:1098, curves.py:637); curves.measureNorm0 uses eps = 0
+    (curves.py:625)."""
+    cr = np.asarray(cr, dtype=np.float64)
+    return np.sqrt((cr ** 2).sum(axis=1) + eps)
+
+
+@dataclass
+class MeasureJob:
+    c1: torch.Tensor            # (F1, d) centers
+    a1: torch.Tensor            # (F1,) weights
+    c2: Optional[torch.Tensor]  # (F2, d) target centers, or None for <a1, K(c1, c1) a1>
+    a2: Optional[torch.Tensor]  # (F2,) target weights
+    K: DevKernel
+    total: torch.Tensor         # (1,) output: measureNormDef (measureNorm0 without target)
+    n1: Optional[torch.Tensor] = None   # (F1, d) normals, for the gradient form
+    z1: Optional[torch.Tensor] = None   # (F1, d) gradient form outputs
+    dz: Optional[torch.Tensor] = None
+    coef: float = 1.0 / 3.0
+
+
+def measure_record(c1, a1, c2, a2, K: DevKernel, coef: float, n1=None) -> np.ndarray:
+    """The measure descriptor of (c1, a1) against (c2, a2) without its outputs (one record);
+    c2 None leaves the target out (the shape's own norm)."""
+    r = rt.segs(1)
+    r["y"] = c1.data_ptr()
+    r["w0"] = a1.data_ptr()
+    r["w2"] = rt.ptr(n1)
+    r["x1"] = rt.ptr(c2)
+    r["w1"] = rt.ptr(a2)
+    r["ny"] = c1.shape[0]
+    r["nx1"] = 0 if c2 is None else c2.shape[0]
+    r["sigma"] = K.sigma
+    r["coef"] = coef
+    r["kernel"] = K.kernel
+    r["order"] = K.order
+    return r
+
+
+def measure(jobs: Sequence[MeasureJob]) -> None:
+    for group in _groups(jobs, lambda j: (j.K.family, j.c1.shape[1], j.dz is not None)):
+        d = group[0].c1.shape[1]
+        r = rt.stack([rt.raw(measure_record(j.c1, j.a1, j.c2, j.a2, j.K, j.coef, j.n1))
+                      for j in group])
+        r["out0"] = [rt.ptr(j.z1) for j in group]
+        r["out1"] = [rt.ptr(j.dz) for j in group]
+        r["total"] = [j.total.data_ptr() for j in group]
+        rt.launch(_lib.OP_MEASURE, group[0].K.prec, d, r)
 
 
 @dataclass
@@ -449,8 +503,8 @@ def _blk0(counts):
 
 
 def geometry_batch(shapes, vertices) -> None:
-    """shape.update(v) for every (shape, v): face centers and normals of many surfaces and
-    curves in one launch."""
+    """shape.update(v) for every (shape, v): face centers and normals (and measure weights,
+    for shapes that hold them) of many surfaces and curves in one launch."""
     if not shapes:
         return
     lib = _lib.load()
@@ -461,6 +515,7 @@ def geometry_batch(shapes, vertices) -> None:
     r["faces"] = [sh.faces.data_ptr() for sh in shapes]
     r["centers"] = [sh.centers.data_ptr() for sh in shapes]
     r["normals"] = [sh.surfel.data_ptr() for sh in shapes]
+    r["weights"] = [rt.ptr(getattr(sh, "weights", None)) for sh in shapes]
     r["nf"] = [sh.faces.shape[0] for sh in shapes]
     r["k"] = [sh.faces.shape[1] for sh in shapes]
     r["d"] = [sh.centers.shape[1] for sh in shapes]
@@ -476,6 +531,7 @@ def geometry_record(sh) -> np.ndarray:
     r["faces"] = sh.faces.data_ptr()
     r["centers"] = sh.centers.data_ptr()
     r["normals"] = sh.surfel.data_ptr()
+    r["weights"] = rt.ptr(getattr(sh, "weights", None))
     r["nf"] = sh.faces.shape[0]
     r["k"] = sh.faces.shape[1]
     r["d"] = sh.centers.shape[1]

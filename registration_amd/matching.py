@@ -9,10 +9,12 @@ them with a list of one, which is the plain single-pair path; the serving layer 
 with every registration waiting on the same request.
 
 Reference: py-lddmm/surfaceMatching.py:187-484 and curveMatching.py:148-352
-(affine='none', symmetric=False, errorType='current'), plus a landmark-set data term
-sum |x_T - y|^2 / sigmaError^2 over corresponding points (landmarkMatching.LandmarkMatching).
-Data-term energies (currents norms) are always evaluated in FP64 pairwise arithmetic; the flow, the covector, the RKHS products and
-the data-term gradient use the kernels' precision.  Differences that do not change any result: the trajectory of an accepted trial is kept instead of being recomputed in
+(affine='none', symmetric=False, errorType 'current' or 'measure'), plus a landmark-set data
+term sum |x_T - y|^2 / sigmaError^2 over corresponding points
+(landmarkMatching.LandmarkMatching).
+Data-term energies (currents and measure norms) are always evaluated in FP64 pairwise
+arithmetic; the flow, the covector, the RKHS products and the data-term gradient use the
+kernels' precision.  Differences that do not change any result: the trajectory of an accepted trial is kept instead of being recomputed in
 endOfIteration / landmarkHamiltonianCovector (the recomputation is deterministic and
 bit-identical), and the energy sum reuses the flow's K(x_t) a_t instead of a second applyK.
 """
@@ -30,7 +32,9 @@ from . import runtime as rt
 
 
 CURRENT = 'current'
+MEASURE = 'measure'
 LANDMARK = 'landmark'
+SHAPE_TERMS = (CURRENT, MEASURE)
 
 
 class Direction:
@@ -45,14 +49,18 @@ class Direction:
 class DeviceMatching:
     """Common device state of SurfaceMatching / CurveMatching / LandmarkMatching."""
 
-    # subclasses set: coef (1/3 surfaces, 1/2 curves) and _new_shape(vertices)
+    # subclasses set: coef (1/3 surfaces, 1/2 curves), _new_shape(vertices, measure) and
+    # measure_norm0_eps (the offset inside the target weights of measureNorm0: 1e-10 for
+    # surfaces, surfaces.py:1095; none for curves, curves.py:624)
     coef = 1.0 / 3.0
+    measure_norm0_eps = 1e-10
 
     def _init_device(self, x0_np, faces_np, target_centers, target_cr, KparDiff, KparDist,
-                     sigmaError, Tsize, regweight, target_points=None):
+                     sigmaError, Tsize, regweight, target_points=None, data_term=CURRENT):
         """Device state.  With ``target_points`` (N, d) the data term is the landmark one,
         sum |x_T - y|^2 / sigmaError^2 over corresponding points, and the shape arguments
-        are unused; otherwise it is the currents norm against (target_centers, target_cr)."""
+        are unused; otherwise it is the currents (data_term 'current') or measure
+        ('measure') norm against the target faces (target_centers, target_cr)."""
         dev = rt.device()
         self.npt, self.dim = x0_np.shape
         self.Tsize = Tsize
@@ -68,22 +76,29 @@ class DeviceMatching:
                 raise ValueError("landmark target must have the template's shape (N, d)")
             self.Kdist = None
         else:
-            self.data_kind = CURRENT
+            if data_term not in SHAPE_TERMS:
+                raise NotImplementedError(f"data term {data_term!r} is not part of the GPU path")
+            self.data_kind = data_term
             self.Kdist = engine.DevKernel.of(KparDist)
             if self.Kdiff.prec != self.Kdist.prec:
                 raise ValueError("deformation and data kernels must share one precision")
             self.faces_np = np.asarray(faces_np)
             self.c2 = rt.as_dev(target_centers)
             self.cr2 = rt.as_dev(target_cr)
+            if data_term == MEASURE:
+                self.a2 = rt.as_dev(engine.measure_weights(target_cr))
+                self.a2_norm0 = rt.as_dev(engine.measure_weights(target_cr,
+                                                                 self.measure_norm0_eps))
         self.at = torch.zeros((T, N, d), dtype=torch.float64, device=dev)
         self.atTry = torch.zeros_like(self.at)
         # Slice 0 of every trajectory buffer is x0 for the whole optimisation.
         self.xt = self.x0.expand(T + 1, N, d).clone()
         self._xt_try = self.xt.clone()
         self._xt_scratch = self.xt.clone()
-        if self.data_kind == CURRENT:
-            self.fvDef_dev = self._new_shape(self.x0)
-            self._try_shape = self._new_shape(self.x0)
+        if self.data_kind in SHAPE_TERMS:
+            measure = self.data_kind == MEASURE
+            self.fvDef_dev = self._new_shape(self.x0, measure)
+            self._try_shape = self._new_shape(self.x0, measure)
         else:
             self.fvDef_dev = self._try_shape = None
         # Pointer-free descriptor parts of this registration's launches, built once; the
@@ -100,25 +115,23 @@ class DeviceMatching:
         self._pxT = self._pxt.data_ptr() + T * self._slice_bytes
         self._cov_base = rt.raw(engine.covector_record(N, T, self.Kdiff, self.regweight))
         self._def_stale = False
-        if self.data_kind == CURRENT:
+        if self.data_kind in SHAPE_TERMS:
             Ke = self.Kdist.energy()
-            self._cur_try = rt.raw(engine.currents_record(self._try_shape.centers,
-                                                          self._try_shape.surfel, self.c2,
-                                                          self.cr2, Ke, self.coef))
-            self._cur_key = (Ke.kernel, Ke.prec, d)
+            op = _lib.OP_CURRENTS if self.data_kind == CURRENT else _lib.OP_MEASURE
+            self._cur_try = rt.raw(self._data_record(self._try_shape, Ke))
+            self._cur_key = (Ke.kernel, Ke.prec, d, op)
             self._geom_try = rt.raw(engine.geometry_record(self._try_shape))
             sh = self.fvDef_dev
             F = sh.centers.shape[0]
             self._z1 = torch.empty((F, d), dtype=torch.float64, device=dev)
             self._dz = torch.empty((F, d), dtype=torch.float64, device=dev)
             self._gtot = torch.empty(1, dtype=torch.float64, device=dev)
-            g = engine.currents_record(sh.centers, sh.surfel, self.c2, self.cr2, self.Kdist,
-                                       self.coef)
+            g = self._data_record(sh, self.Kdist, gradient=True)
             g["out0"] = self._z1.data_ptr()
             g["out1"] = self._dz.data_ptr()
             g["total"] = self._gtot.data_ptr()
             self._cur_grad = rt.raw(g)
-            self._cur_grad_key = (self.Kdist.kernel, self.Kdist.prec, d)
+            self._cur_grad_key = (self.Kdist.kernel, self.Kdist.prec, d, op)
             self._gather = rt.raw(engine.gather_record(sh, self._z1, self._dz,
                                                        -2.0 / (self.sigmaError ** 2), self._pxT))
             self._geom_def = rt.raw(engine.geometry_record(sh))
@@ -132,6 +145,14 @@ class DeviceMatching:
         self.nobj = 0
         self.ngrad = 0
         self.iter = 0
+
+    def _data_record(self, sh, K, gradient=False):
+        """Descriptor of the data term of device shape ``sh`` against the target (outputs
+        unset)."""
+        if self.data_kind == CURRENT:
+            return engine.currents_record(sh.centers, sh.surfel, self.c2, self.cr2, K, self.coef)
+        return engine.measure_record(sh.centers, sh.weights, self.c2, self.a2, K, self.coef,
+                                     n1=sh.surfel if gradient else None)
 
     # ----- reference protocol, single registration -----------------------------------
     def objectiveFun(self):
@@ -236,13 +257,18 @@ def _data_energies(opts, vertices, shapes, totals) -> None:
     views): currents norm against the target for currents registrations (after updating
     their face geometry in ``shapes``), squared distance to the target points for landmark
     registrations.  One launch per stage for the whole list."""
-    cur = [i for i, o in enumerate(opts) if o.data_kind == CURRENT]
+    cur = [i for i, o in enumerate(opts) if o.data_kind in SHAPE_TERMS]
     lmk = [i for i, o in enumerate(opts) if o.data_kind == LANDMARK]
     if cur:
         engine.geometry_batch([shapes[i] for i in cur], [vertices[i] for i in cur])
         engine.currents([engine.CurrentsJob(shapes[i].centers, shapes[i].surfel, opts[i].c2,
                                             opts[i].cr2, opts[i].Kdist.energy(), totals[i],
-                                            coef=opts[i].coef) for i in cur])
+                                            coef=opts[i].coef)
+                         for i in cur if opts[i].data_kind == CURRENT])
+        engine.measure([engine.MeasureJob(shapes[i].centers, shapes[i].weights, opts[i].c2,
+                                          opts[i].a2, opts[i].Kdist.energy(), totals[i],
+                                          coef=opts[i].coef)
+                        for i in cur if opts[i].data_kind == MEASURE])
     if lmk:
         engine.landmark_batch([vertices[i] for i in lmk], [opts[i].y for i in lmk],
                               [totals[i] for i in lmk])
@@ -283,12 +309,17 @@ def launch_objective(opts: Sequence[DeviceMatching]) -> Pending:
                    [scal[off + o.Tsize:off + o.Tsize + 1] for o, off in zip(opts, offs)])
     for o in opts:
         o._def_stale = False
-    # Target self-term (currentNorm0); zero for the landmark data term.
+    # Target self-term (currentNorm0 / measureNorm0); zero for the landmark data term.
+    mnorms = []
     for o, off in zip(opts, offs):
+        T = o.Tsize
         if o.data_kind == CURRENT:
-            T = o.Tsize
             norms.append(engine.KNormJob(o.c2, o.cr2, o.Kdist.energy(), scal[off + T + 1:off + T + 2]))
+        elif o.data_kind == MEASURE:
+            mnorms.append(engine.MeasureJob(o.c2, o.a2_norm0, None, None, o.Kdist.energy(),
+                                            scal[off + T + 1:off + T + 2]))
     engine.knorm(norms)
+    engine.measure(mnorms)
 
     def finish(vals):
         out = []
@@ -344,15 +375,15 @@ def launch_update_try(items) -> Pending:
         o0 = opts[idx[0]]
         engine.shoot_records(rt.stack([opts[i]._flow_base for i in idx]), o0.Kdiff.prec,
                              o0.dim, Ts[idx], xt[idx], at[idx], energy[idx])
-    cur = [i for i, o in enumerate(opts) if o.data_kind == CURRENT]
+    cur = [i for i, o in enumerate(opts) if o.data_kind in SHAPE_TERMS]
     if cur:
         engine.geometry_records(rt.stack([opts[i]._geom_try for i in cur], engine.GEOM_DT),
                                 final[cur])
         for idx in _by_key([opts[i]._cur_key for i in cur]):
             g = [cur[i] for i in idx]
-            o0 = opts[g[0]]
+            key = opts[g[0]]._cur_key
             engine.currents_records(rt.stack([opts[i]._cur_try for i in g]),
-                                    o0._cur_key[1], o0._cur_key[2], data_total[g])
+                                    key[1], key[2], data_total[g], op=key[3])
     lmk = [i for i, o in enumerate(opts) if o.data_kind == LANDMARK]
     if lmk:
         engine.landmark_batch([opts[i]._xt_scratch[-1] for i in lmk], [opts[i].y for i in lmk],
@@ -396,13 +427,13 @@ def batch_gradient(items) -> list:
     opts = [it[0] for it in items]
     _refresh_def(opts)
     # End-point covector px1, written into slice T of each registration's covector buffer.
-    cur = [i for i, o in enumerate(opts) if o.data_kind == CURRENT]
+    cur = [i for i, o in enumerate(opts) if o.data_kind in SHAPE_TERMS]
     if cur:
-        # currents gradient (z1, dz), then the vertex gathers with scale -2/sigma^2
+        # data-term gradient (z1, dz), then the vertex gathers with scale -2/sigma^2
         for idx in _by_key([opts[i]._cur_grad_key for i in cur]):
             g = [cur[i] for i in idx]
             key = opts[g[0]]._cur_grad_key
-            rt.launch(_lib.OP_CURRENTS, key[1], key[2], rt.stack([opts[i]._cur_grad for i in g]))
+            rt.launch(key[3], key[1], key[2], rt.stack([opts[i]._cur_grad for i in g]))
         engine.gather_records(rt.stack([opts[i]._gather for i in cur], engine.GATHER_DT),
                               [opts[i].xt.data_ptr() + opts[i].Tsize * opts[i]._slice_bytes
                                for i in cur])

@@ -8,7 +8,7 @@ from . import kernelFunctions as kfun
 class MatchingParam:
     """timeStep: time discretisation; KparDiff: deformation kernel (default typeKernel of width
     sigmaKernel); KparDist: data-term kernel (default Gaussian of width sigmaDist); sigmaError:
-    data-term normalisation; errorType: 'current' (the GPU path) or 'measure'; typeKernel:
+    data-term normalisation; errorType: 'measure' (the default) or 'current'; typeKernel:
     'gauss' or 'laplacian'."""
 
     def __init__(self, timeStep=.1, KparDiff=None, KparDist=None, sigmaKernel=6.5, sigmaDist=2.5,

@@ -39,8 +39,9 @@ from . import surfaces
 class Request:
     """One registration: ``kind`` 'surface', 'curve' or 'landmark'; template/target are
     (faces, vertices), with faces None for landmark sets (corresponding points); kernel
-    names/widths as in SurfaceMatchingParam (sigmaDist unused by landmark sets); ``arrival``
-    in seconds from service start."""
+    names/widths as in SurfaceMatchingParam (sigmaDist unused by landmark sets); errorType
+    'current' or 'measure' selects the data term of surfaces and curves; ``arrival`` in
+    seconds from service start."""
 
     rid: int
     kind: str
@@ -54,6 +55,7 @@ class Request:
     typeKernel: str = 'gauss'
     order: int = 10
     arrival: float = 0.0
+    errorType: str = 'current'
 
 
 @dataclass
@@ -82,7 +84,7 @@ def build_registration(req: Request, precision: str):
     Kw = kfun.Kernel(name='gauss', sigma=req.sigmaDist, precision=precision)
     if req.kind == 'surface':
         p = sm.SurfaceMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, KparDist=Kw,
-                                    sigmaError=req.sigmaError, errorType='current')
+                                    sigmaError=req.sigmaError, errorType=req.errorType)
         return sm.SurfaceMatching(Template=surfaces.Surface(FV=req.template),
                                   Target=surfaces.Surface(FV=req.target), param=p,
                                   maxIter=req.maxIter, testGradient=False)
@@ -92,7 +94,7 @@ def build_registration(req: Request, precision: str):
                                    maxIter=req.maxIter)
     if req.kind == 'curve':
         p = cm.CurveMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, KparDist=Kw,
-                                  sigmaError=req.sigmaError, errorType='current')
+                                  sigmaError=req.sigmaError, errorType=req.errorType)
         return cm.CurveMatching(Template=curves.Curve(FV=req.template),
                                 Target=curves.Curve(FV=req.target), param=p, maxIter=req.maxIter)
     raise ValueError(f"unknown request kind {req.kind!r}")

@@ -4,8 +4,8 @@
 ``SurfaceMatchingParam`` and ``SurfaceMatching`` keep the reference constructors and the
 optimiser protocol; the registration lives on the GPU (see matching.DeviceMatching) and
 ``optimizeMatching`` runs the reference's conjugate gradient over it.  Supported: the
-currents data term (errorType='current'), affine='none', symmetric=False,
-subsampleTargetSize=-1.  File output (VTK snapshots every saveRate iterations) is not written.
+currents and measure data terms (errorType 'current' or 'measure', the default),
+affine='none', symmetric=False, subsampleTargetSize=-1.  File output (VTK snapshots every saveRate iterations) is not written.
 """
 from __future__ import annotations
 
@@ -32,7 +32,11 @@ class SurfaceMatchingParam:
             self.fun_obj0 = surfaces.currentNorm0
             self.fun_obj = surfaces.currentNormDef
             self.fun_objGrad = surfaces.currentNormGradient
-        elif errorType in ('measure', 'varifold'):
+        elif errorType == 'measure':
+            self.fun_obj0 = surfaces.measureNorm0
+            self.fun_obj = surfaces.measureNormDef
+            self.fun_objGrad = surfaces.measureNormGradient
+        elif errorType == 'varifold':
             self.fun_obj0 = self.fun_obj = self.fun_objGrad = None
         else:
             logging.error('Unknown error Type: ' + self.errorType)
@@ -60,9 +64,9 @@ class SurfaceMatching(DeviceMatching):
         self.fv0 = surfaces.Surface(surf=Template)
         self.fv1 = surfaces.Surface(surf=Target)
         self.param = SurfaceMatchingParam() if param is None else param
-        if self.param.errorType != 'current':
-            raise NotImplementedError("the GPU path implements the currents data term "
-                                      "(errorType='current')")
+        if self.param.errorType not in ('current', 'measure'):
+            raise NotImplementedError("the GPU path implements the currents and measure data "
+                                      "terms (errorType 'current' or 'measure')")
         self.saveRate = 10
         self.outputDir = outputDir
         self.maxIter = maxIter
@@ -81,10 +85,10 @@ class SurfaceMatching(DeviceMatching):
         Tsize = int(round(1.0 / self.param.timeStep))
         self._init_device(x0, self.fvInit.faces, self.fv1.centers, self.fv1.surfel,
                           self.param.KparDiff, self.param.KparDist, self.param.sigmaError, Tsize,
-                          regWeight)
+                          regWeight, data_term=self.param.errorType)
 
-    def _new_shape(self, vertices):
-        return surfaces.DeviceSurface(self.faces_np, vertices)
+    def _new_shape(self, vertices, measure=False):
+        return surfaces.DeviceSurface(self.faces_np, vertices, measure=measure)
 
 
     def endOptim(self):

@@ -9,6 +9,9 @@ writers (BYU/OFF/VTK/OBJ) are not part of the GPU path.
 reference signatures (surfaces.py:1009-1089) and run on the GPU: the two F x F sums of the
 norm and the four of the gradient are fused into one segmented launch, and the per-face
 scatter of the gradient is a deterministic vertex gather in the reference's summation order.
+``measureNorm0`` / ``measureNormDef`` / ``measureNorm`` / ``measureNormGradient``
+(surfaces.py:1092-1182) do the same for the measure data term (scalar face weights
+sqrt(|surfel|^2 + 1e-10) in place of the surfels).
 """
 from __future__ import annotations
 
@@ -62,10 +65,11 @@ class Surface:
 
 
 class DeviceSurface:
-    """A surface resident on the GPU: vertices, faces, centers, surfels and the
-    vertex-incidence CSR used by the gradient gather."""
+    """A surface resident on the GPU: vertices, faces, centers, surfels, the
+    vertex-incidence CSR used by the gradient gather and, with ``measure``, the face weights
+    sqrt(|surfel|^2 + 1e-10) of the measure data term (else ``weights`` is None)."""
 
-    def __init__(self, faces_np, vertices):
+    def __init__(self, faces_np, vertices, measure: bool = False):
         faces_np = np.asarray(faces_np)
         self.nv = int(vertices.shape[0])
         self.faces, self.ptr, self.inc = engine.topology(faces_np, self.nv)
@@ -73,10 +77,14 @@ class DeviceSurface:
         dev = rt.device()
         self.centers = torch.empty((F, 3), dtype=torch.float64, device=dev)
         self.surfel = torch.empty((F, 3), dtype=torch.float64, device=dev)
+        self.weights = torch.empty(F, dtype=torch.float64, device=dev) if measure else None
         self.vertices = None
         self.update(vertices)
 
     def update(self, vertices):
+        if self.weights is not None:
+            engine.geometry_batch([self], [rt.as_dev(vertices)])
+            return
         self.vertices = rt.as_dev(vertices)
         engine.surface_geometry(self.vertices, self.faces, self.centers, self.surfel)
 
@@ -131,4 +139,52 @@ def currentNormGradient(fvDef, fv1, KparDist):
     a, b = _dev_surface(fvDef), _dev_surface(fv1)
     px = torch.empty((a.nv, 3), dtype=torch.float64, device=rt.device())
     current_gradient_dev(a, b.centers, b.surfel, _kernel(KparDist), 2.0, px)
+    return px.cpu().numpy()
+
+
+# ---- measure data term (surfaces.py:1092-1182) -------------------------------------------
+def measureNorm0(fv1, KparDist):
+    """<a, K(c, c) a> of fv1 with a = sqrt(|surfel|^2 + 1e-10) (surfaces.py:1092-1099)."""
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.measure([engine.MeasureJob(rt.as_dev(fv1.centers), rt.as_dev(engine.measure_weights(fv1.surfel)),
+                                      None, None, _kernel(KparDist).energy(), tot)])
+    return float(tot.item())
+
+
+def measureNormDef(fvDef, fv1, KparDist):
+    """|fvDef|^2 - 2 <fvDef, fv1> with the measure dot product (surfaces.py:1104-1117)."""
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.measure([engine.MeasureJob(rt.as_dev(fvDef.centers),
+                                      rt.as_dev(engine.measure_weights(fvDef.surfel)),
+                                      rt.as_dev(fv1.centers), rt.as_dev(engine.measure_weights(fv1.surfel)),
+                                      _kernel(KparDist).energy(), tot)])
+    return float(tot.item())
+
+
+def measureNorm(fvDef, fv1, KparDist):
+    return measureNormDef(fvDef, fv1, KparDist) + measureNorm0(fv1, KparDist)
+
+
+def measure_gradient_dev(s: DeviceSurface, c2, a2, K: engine.DevKernel, scale: float,
+                         px: torch.Tensor, total: torch.Tensor | None = None) -> torch.Tensor:
+    """px = scale * (gradient of the measure norm wrt the vertices of s) / 2 for a shape
+    built with measure=True; also writes measureNormDef to ``total``."""
+    F = s.centers.shape[0]
+    dev = s.centers.device
+    z1 = torch.empty((F, 3), dtype=torch.float64, device=dev)
+    dz = torch.empty((F, 3), dtype=torch.float64, device=dev)
+    if total is None:
+        total = torch.empty(1, dtype=torch.float64, device=dev)
+    engine.measure([engine.MeasureJob(s.centers, s.weights, c2, a2, K, total, n1=s.surfel,
+                                      z1=z1, dz=dz, coef=1.0 / 3.0)])
+    engine.surface_vertex_gradient(s.vertices, s.faces, z1, dz, s.ptr, s.inc, scale, px)
+    return px
+
+
+def measureNormGradient(fvDef, fv1, KparDist):
+    """Gradient of measureNorm wrt the vertices of fvDef (surfaces.py:1125-1182)."""
+    a = DeviceSurface(fvDef.faces, fvDef.vertices, measure=True)
+    px = torch.empty((a.nv, 3), dtype=torch.float64, device=rt.device())
+    measure_gradient_dev(a, rt.as_dev(fv1.centers), rt.as_dev(engine.measure_weights(fv1.surfel)),
+                         _kernel(KparDist), 2.0, px)
     return px.cpu().numpy()

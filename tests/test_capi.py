@@ -100,3 +100,32 @@ def test_product_never_imports_the_oracle():
         if f.endswith(".py"):
             src = open(os.path.join(pkg, f)).read()
             assert not re.search(r"(import\s+oracle|from\s+oracle|lddmm_oracle|liboracle)", src), f
+
+
+def test_struct_mirrors_match_the_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives the header's
+    struct (a layout slip would hand the kernels wrong addresses)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    mirrors = {"lddmm_seg_t": _lib.Seg, "lddmm_geom_t": _lib.Geom,
+               "lddmm_gather_t": _lib.Gather, "lddmm_lmk_t": _lib.Landmark}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lddmm.h"', 'int main(void) {']
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, *_ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append('  return 0;\n}')
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in mirrors.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for fname, *_ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

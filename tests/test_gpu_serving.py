@@ -25,6 +25,16 @@ def requests():
                                     sigmaKernel=0.3, sigmaDist=0.1, sigmaError=0.1, T=5,
                                     maxIter=6, typeKernel='laplacian' if i else 'gauss',
                                     order=1 if i else 10))
+    # the measure data term (the reference's default errorType) in the same mix
+    V, F = synthetic.icosphere(2)
+    reqs.append(serving.Request(rid=30, kind='surface', template=(F, V),
+                                target=(F, synthetic.bump_target(V)), sigmaKernel=0.4,
+                                sigmaDist=0.2, sigmaError=0.1, T=4, maxIter=6, errorType='measure'))
+    X, _ = synthetic.circle(160)
+    E = synthetic.closed_curve_edges(160)
+    reqs.append(serving.Request(rid=31, kind='curve', template=(E, X),
+                                target=(E, synthetic.star_curve(160, 4, 0.2)), sigmaKernel=0.3,
+                                sigmaDist=0.1, sigmaError=0.1, T=5, maxIter=6, errorType='measure'))
     for i, n in enumerate((200, 513)):
         X = synthetic.ball_points(rng, n)
         a0 = 0.2 * rng.standard_normal(X.shape)
@@ -79,8 +89,8 @@ def test_invalid_requests_are_rejected_and_the_rest_served():
     admission (before any kernel reads them); the valid requests in the same stream finish
     with the results they have alone."""
     from registration_amd import serving
-    reqs = requests()
-    good = [reqs[0], reqs[4], reqs[6]]
+    reqs = {r.rid: r for r in requests()}
+    good = [reqs[0], reqs[10], reqs[20]]
     F, V = reqs[1].template
     # index len(V): the host Surface's own NumPy indexing fails, as the reference's does;
     # index -1 passes host NumPy (it wraps) and is caught before the faces reach the device.
@@ -88,7 +98,7 @@ def test_invalid_requests_are_rejected_and_the_rest_served():
                                  target=reqs[1].target, sigmaKernel=0.3, sigmaDist=0.1,
                                  sigmaError=0.1, T=4, maxIter=6)
                  for rid, bad in ((90, len(V)), (92, -1))]
-    X = reqs[6].template[1]
+    X = reqs[20].template[1]
     bad_shape = serving.Request(rid=91, kind='landmark', template=(None, X),
                                 target=(None, X[:-1]), sigmaKernel=0.3, sigmaDist=0.1,
                                 sigmaError=0.1, T=4, maxIter=6)

@@ -223,6 +223,84 @@ def test_currents_gradient_fd(kind):
         assert abs(fd - g[i, c]) < 1e-6 * max(1.0, abs(fd))
 
 
+# ---------------------------------------------------------------- measure data term
+def _measure_brute(c1, a1, c2, a2, sigma):
+    """|mu1 - mu2|^2 for mu = sum_k a_k delta_{c_k} in the Gaussian RKHS, by a double loop."""
+    def dot(ca, aa, cb, ab):
+        return sum(aa[i] * ab[j] * np.exp(-((ca[i] - cb[j]) ** 2).sum() / (2 * sigma ** 2))
+                   for i in range(len(aa)) for j in range(len(ab)))
+    return dot(c1, a1, c1, a1) - 2 * dot(c1, a1, c2, a2) + dot(c2, a2, c2, a2)
+
+
+def test_measure_norm_brute_force_and_invariances():
+    V, F = synthetic.icosphere(1)
+    K = orc.Kernel('gauss', sigma=0.35)
+    S = orc.Surface(FV=(F, V))
+    T = orc.Surface(FV=(F, synthetic.bump_target(V)))
+    a = lambda s: np.sqrt((s.surfel ** 2).sum(axis=1) + 1e-10)
+    full = orc.surf_measureNormDef(S, T, K) + orc.surf_measureNorm0(T, K)
+    brute = _measure_brute(S.centers, a(S), T.centers, a(T), 0.35)
+    assert abs(full - brute) < 1e-12 * orc.surf_measureNorm0(T, K)
+    assert abs(orc.surf_measureNormDef(S, S, K) + orc.surf_measureNorm0(S, K)) < 1e-12 * orc.surf_measureNorm0(S, K)
+    # The measure ignores orientation (currents do not): flipping every face of T keeps it.
+    Tf = orc.Surface(surf=T)
+    Tf.flipFaces()
+    assert abs(orc.surf_measureNormDef(S, Tf, K) - orc.surf_measureNormDef(S, T, K)) < 1e-12 * abs(full)
+    assert abs(orc.surf_currentNormDef(S, Tf, K) - orc.surf_currentNormDef(S, T, K)) > 1e-3
+    # Rigid motions of both shapes leave it unchanged.
+    R = synthetic.random_rotation(np.random.default_rng(2))
+    Sr, Tr = orc.Surface(FV=(F, S.vertices @ R.T + 0.3)), orc.Surface(FV=(F, T.vertices @ R.T + 0.3))
+    assert abs(orc.surf_measureNormDef(Sr, Tr, K) - orc.surf_measureNormDef(S, T, K)) < 1e-11 * abs(full)
+
+
+def test_curve_measure_norm0_has_no_offset():
+    """curves.measureNorm0 weights are |lenel| while measureNormDef's are sqrt(|lenel|^2 +
+    1e-10): the two-edge closed forms tell them apart."""
+    K = orc.Kernel('gauss', sigma=0.5)
+    C = orc.Curve(FV=(np.array([[0, 1]]), np.array([[0., 0], [1e-5, 0]])))
+    l = 1e-5
+    assert abs(orc.curve_measureNorm0(C, K) - l * l) < 1e-24
+    assert abs(orc.curve_measureNormDef(C, C, K) - (-(l * l + 1e-10))) < 1e-24
+
+
+@pytest.mark.parametrize("kind", ["surface", "curve"])
+def test_measure_gradient_fd(kind):
+    rng = np.random.default_rng(19)
+    if kind == "surface":
+        V, F = synthetic.icosphere(1)
+        mk = lambda X: orc.Surface(FV=(F, X))
+        norm = orc.surf_measureNormDef
+        grad = orc.surf_measureNormGradient
+        Y = synthetic.bump_target(V)
+    else:
+        V, _ = synthetic.circle(24)
+        F = synthetic.closed_curve_edges(24)
+        mk = lambda X: orc.Curve(FV=(F, X))
+        norm = orc.curve_measureNormDef
+        grad = orc.curve_measureNormGradient
+        Y = synthetic.star_curve(24, 5, 0.2)
+    K = orc.Kernel('gauss', sigma=0.4)
+    X = V + 0.05 * rng.standard_normal(V.shape)
+    g = grad(mk(X), mk(Y), K)
+    h = 1e-6
+    for (i, c) in [(0, 0), (3, 1), (7, 0), (10, 1), (5, V.shape[1] - 1)]:
+        e = np.zeros_like(X)
+        e[i, c] = h
+        fd = (norm(mk(X + e), mk(Y), K) - norm(mk(X - e), mk(Y), K)) / (2 * h)
+        assert abs(fd - g[i, c]) < 1e-6 * max(1.0, abs(fd)), (i, c, fd, g[i, c])
+
+
+def test_measure_matching_reduces_objective():
+    V, F = synthetic.icosphere(1)
+    Kd, Kw = orc.Kernel('gauss', sigma=0.5), orc.Kernel('gauss', sigma=0.3)
+    S, T = orc.Surface(FV=(F, V)), orc.Surface(FV=(F, synthetic.bump_target(V)))
+    o = orc.SurfaceMatching(S, T, Kd, Kw, sigmaError=0.1, timeStep=0.25, maxIter=4,
+                            errorType='measure')
+    e0 = o.objectiveFun()
+    o.optimizeMatching()
+    assert o.obj < 0.5 * e0
+
+
 # ---------------------------------------------------------------- landmarks
 def test_landmark_data_term_closed_form_and_fd():
     rng = np.random.default_rng(13)
