@@ -82,7 +82,21 @@ enum {
    *   out1 = coef * [applyDiffKT(c1,[a1],[a1]) - applyDiffKT(c2,[a1],[a2],c1)]
    * (surfaces.py:1092-1182, curves.py:621-701; a = sqrt(|normal|^2 + 1e-10) is
    * computed by the caller, e.g. lddmm_geometry_batch's weights output). */
-  LDDMM_OP_MEASURE = 8
+  LDDMM_OP_MEASURE = 8,
+  /* varifold data term of a deformed surface (d = 3 only): centers c1 = y,
+   * weights a1 = w0 (ny), normals n1 = w2 (ny, 3), against a target with
+   * centers c2 = x1, weights a2 = w1 (nx1) and normals n2 = sw[0] (nx1, 3)
+   * (nx1 = 0 leaves the target out).  With u = n / a and g = <u_k, u_l>:
+   *   total[0] = sum_kl K(c1_k,c1_l) a1_k a1_l (1 + g^2)
+   *            - 2 sum_kl K(c1_k,c2_l) a1_k a2_l (1 + g^2)       (varifoldNormDef;
+   *              varifoldNorm0 of a shape when nx1 = 0)
+   * and, when out1 (and optionally out0) are non-NULL (gradient form):
+   *   out0 = z1 = sum_l K a_l (2 g u_l + (1 - g^2) u1_k)
+   *   out1 = coef * sum_l 2K'(r^2)(c1_k - x_l) a1_k a_l (1 + g^2)
+   * where l runs over the shape's own faces and, with sign -1, the target's.
+   * (surfaces.py:1184-1276 with kernelFunctions.f90:499-610; the (F, F) pair
+   * weights are formed per pair, never stored). */
+  LDDMM_OP_VARIFOLD = 9
 };
 
 /*

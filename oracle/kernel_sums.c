@@ -12,6 +12,10 @@
  *   oracle_apply_k            <- applyk             kernelFunctions.f90:1-54
  *   oracle_apply_diff_kt      <- applykdifft        kernelFunctions.f90:56-114
  *   oracle_apply_ddiff_k11and12 <- applykdiff11and12 kernelFunctions.f90:430-496
+ *   oracle_apply_k_mat        <- applykmat          kernelFunctions.f90:499-550
+ *   oracle_apply_diff_k_mat   <- applykdiffmat      kernelFunctions.f90:552-610
+ * (the last two carry one weight per (target, source) pair: the varifold data
+ * term's matrix weights, surfaces.py:1184-1276).
  *
  * Semantics kept from the Fortran:
  *   - each target row k sums over the sources l in increasing l order, in FP64
@@ -185,6 +189,46 @@ int oracle_apply_ddiff_k11and12(const double *x, const double *a1,
       }
       for (int c = 0; c < d; ++c)
         df[c] += 2 * aa * (2 * kd2 * xp * dx[c] + kd * dp[c]);
+    }
+    for (int c = 0; c < d; ++c) f[k * d + c] = df[c];
+  }
+  return 0;
+}
+
+/* f(k,:) = sum_l K(|y_k - x_l|) beta(k,l,:)      (applykmat, f90:499-550)
+ * beta: (ny, nx, m) row-major, i.e. beta[(k*nx + l)*m + c]. */
+int oracle_apply_k_mat(const double *y, const double *x, const double *beta,
+                       int64_t ny, int64_t nx, int d, int m, int ord, double sig,
+                       double *f) {
+  if (d < 1 || d > 8 || m < 1 || m > 16 || ord < 0) return 1;
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < ny; ++k) {
+    double df[16];
+    for (int c = 0; c < m; ++c) df[c] = 0.0;
+    for (int64_t l = 0; l < nx; ++l) {
+      double kh = kval(dist(y + k * d, x + l * d, d), sig, ord);
+      for (int c = 0; c < m; ++c) df[c] += kh * beta[(k * nx + l) * m + c];
+    }
+    for (int c = 0; c < m; ++c) f[k * m + c] = df[c];
+  }
+  return 0;
+}
+
+/* f(k,:) = sum_l 2 K'(r^2) (y_k - x_l) beta(k,l)   (applykdiffmat, f90:552-610)
+ * beta: (ny, nx) row-major.  Order 0 is rejected: the Fortran indexes its
+ * derivative table at row 0 there. */
+int oracle_apply_diff_k_mat(const double *y, const double *x, const double *beta,
+                            int64_t ny, int64_t nx, int d, int ord, double sig,
+                            double *f) {
+  if (d < 1 || d > 8 || ord < 1) return 1;
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < ny; ++k) {
+    double df[8];
+    for (int c = 0; c < d; ++c) df[c] = 0.0;
+    for (int64_t l = 0; l < nx; ++l) {
+      double kd = kdiff(dist(y + k * d, x + l * d, d), sig, ord);
+      for (int c = 0; c < d; ++c)
+        df[c] += kd * 2 * (y[k * d + c] - x[l * d + c]) * beta[k * nx + l];
     }
     for (int c = 0; c < d; ++c) f[k * d + c] = df[c];
   }

@@ -8,8 +8,8 @@ It restates, in Python 3 / NumPy float64, the executed reference path:
   Kernel                  py-lddmm/kernelFunctions.py:238-454 (sums in oracle/kernel_sums.c,
                           restating kernelFunctions.f90:1-114,430-496)
   landmark flow/adjoint   py-lddmm/pointEvolution.py:599-817
-  surfaces / curves       py-lddmm/surfaces.py:61-75,711-717,1009-1182; curves.py:58-73,539-701
-                          (currents and measure data terms)
+  surfaces / curves       py-lddmm/surfaces.py:61-75,711-717,1009-1276; curves.py:58-73,539-701
+                          (currents, measure and varifold data terms)
   SurfaceMatching         py-lddmm/surfaceMatching.py:19-484 (affine='none', symmetric=False)
   CurveMatching           py-lddmm/curveMatching.py:19-352 (affine='none')
   cg                      py-lddmm/conjugateGradient.py:33-217
@@ -52,7 +52,10 @@ def lib():
         _lib.oracle_apply_k.argtypes = [_P, _P, _P, i64, i64, i, i, i, d, _P]
         _lib.oracle_apply_diff_kt.argtypes = [_P, _P, _P, _P, i64, i64, i, i, i, i, d, _P]
         _lib.oracle_apply_ddiff_k11and12.argtypes = [_P, _P, _P, _P, i64, i, i, d, _P]
-        for f in (_lib.oracle_apply_k, _lib.oracle_apply_diff_kt, _lib.oracle_apply_ddiff_k11and12):
+        _lib.oracle_apply_k_mat.argtypes = [_P, _P, _P, i64, i64, i, i, i, d, _P]
+        _lib.oracle_apply_diff_k_mat.argtypes = [_P, _P, _P, i64, i64, i, i, d, _P]
+        for f in (_lib.oracle_apply_k, _lib.oracle_apply_diff_kt, _lib.oracle_apply_ddiff_k11and12,
+                  _lib.oracle_apply_k_mat, _lib.oracle_apply_diff_k_mat):
             f.restype = ctypes.c_int
     return _lib
 
@@ -143,6 +146,45 @@ def apply_ddiff_k11and12(x, a1, a2, p, sigma, order, sums="oracle"):
     return f
 
 
+def apply_k_mat(y, x, beta, sigma, order, sums="oracle"):
+    """f = sum_l K(|y_k - x_l|) beta[k, l, :] (kernelFunctions.f90:499-550)."""
+    y, x, beta = _c(y), _c(x), _c(beta)
+    ny, d = y.shape
+    nx = x.shape[0]
+    m = beta.shape[2]
+    if sums == "reference":
+        f = np.zeros((ny, m), order="F")
+        ci = lambda v: ctypes.byref(ctypes.c_int(v))
+        ref().applykmat_(_p(np.asfortranarray(y)), _p(np.asfortranarray(x)),
+                         _p(np.asfortranarray(beta)), ctypes.byref(ctypes.c_double(sigma)),
+                         ci(order), ci(ny), ci(nx), ci(d), ci(m), _p(f))
+        return np.ascontiguousarray(f)
+    f = np.empty((ny, m))
+    rc = lib().oracle_apply_k_mat(_p(y), _p(x), _p(beta), ny, nx, d, m, order, sigma, _p(f))
+    if rc:
+        raise ValueError("oracle_apply_k_mat: invalid arguments")
+    return f
+
+
+def apply_diff_k_mat(y, x, beta, sigma, order, sums="oracle"):
+    """f = sum_l 2K'(r^2)(y_k - x_l) beta[k, l] (kernelFunctions.f90:552-610)."""
+    y, x, beta = _c(y), _c(x), _c(beta)
+    ny, d = y.shape
+    nx = x.shape[0]
+    if sums == "reference":
+        f = np.zeros((ny, d), order="F")
+        ci = lambda v: ctypes.byref(ctypes.c_int(v))
+        ref().applykdiffmat_(_p(np.asfortranarray(y)), _p(np.asfortranarray(x)),
+                             _p(np.asfortranarray(beta)), ctypes.byref(ctypes.c_double(sigma)),
+                             ci(order), ci(ny), ci(nx), ci(d), _p(f))
+        return np.ascontiguousarray(f)
+    f = np.empty((ny, d))
+    rc = lib().oracle_apply_diff_k_mat(_p(y), _p(x), _p(beta), ny, nx, d, order, sigma, _p(f))
+    if rc:
+        raise ValueError("oracle_apply_diff_k_mat: invalid arguments")
+    return f
+
+
 class Kernel:
     """kernelFunctions.Kernel restated (kernelFunctions.py:238-454), affine='none' only."""
 
@@ -160,11 +202,18 @@ class Kernel:
         if affine != 'none':
             raise NotImplementedError("oracle restates affine='none' only")
 
-    def applyK(self, x, a, firstVar=None):
+    def applyK(self, x, a, firstVar=None, matrixWeights=False):
+        y = x if firstVar is None else firstVar
+        if matrixWeights:   # a: (n_y, n_x, m) per-pair weights (kernelFunctions.py:317-325)
+            return apply_k_mat(y, x, a, self.sigma, self.order, self.sums)
         if self.name == 'none':
             return np.zeros((x.shape[0], a.shape[1]))
-        y = x if firstVar is None else firstVar
         return apply_k(y, x, a, self.sigma, self.order, self.sums)
+
+    def applyDiffKmat(self, x, beta, firstVar=None):
+        """kernelFunctions.py:405-411: sum_l 2K'(r^2)(y_k - x_l) beta[k, l]."""
+        y = x if firstVar is None else firstVar
+        return apply_diff_k_mat(y, x, beta, self.sigma, self.order, self.sums)
 
     def applyDiffKT(self, x, a1, a2, firstVar=None):
         y = x if firstVar is None else firstVar
@@ -385,6 +434,73 @@ def surf_measureNormGradient(fvDef, fv1, K):
     return 2 * px
 
 
+def surf_varifoldNorm0(fv1, K):
+    """surfaces.py:1184-1197 (d = 1): sum_kl K(c_k, c_l) a_k a_l (1 + <u_k, u_l>^2), with
+    a = sqrt(|surfel|^2 + 1e-10) and u = surfel / a, through dense (F, F) pair weights."""
+    d = 1
+    c2 = fv1.centers
+    a2 = _measure_weights(fv1.surfel)
+    cr2 = fv1.surfel / a2[:, np.newaxis]
+    cr2cr2 = (cr2[:, np.newaxis, :] * cr2[np.newaxis, :, :]).sum(axis=2)
+    a2a2 = a2[:, np.newaxis] * a2[np.newaxis, :]
+    beta2 = (1 + d * cr2cr2 ** 2) * a2a2
+    return K.applyK(c2, beta2[..., np.newaxis], matrixWeights=True).sum()
+
+
+def surf_varifoldNormDef(fvDef, fv1, K):
+    """surfaces.py:1200-1221."""
+    d = 1
+    c1, c2 = fvDef.centers, fv1.centers
+    a1 = _measure_weights(fvDef.surfel)
+    a2 = _measure_weights(fv1.surfel)
+    cr1 = fvDef.surfel / a1[:, np.newaxis]
+    cr2 = fv1.surfel / a2[:, np.newaxis]
+    cr1cr1 = (cr1[:, np.newaxis, :] * cr1[np.newaxis, :, :]).sum(axis=2)
+    a1a1 = a1[:, np.newaxis] * a1[np.newaxis, :]
+    cr1cr2 = (cr1[:, np.newaxis, :] * cr2[np.newaxis, :, :]).sum(axis=2)
+    a1a2 = a1[:, np.newaxis] * a2[np.newaxis, :]
+    beta1 = (1 + d * cr1cr1 ** 2) * a1a1
+    beta2 = (1 + d * cr1cr2 ** 2) * a1a2
+    return (K.applyK(c1, beta1[..., np.newaxis], matrixWeights=True).sum()
+            - 2 * K.applyK(c2, beta2[..., np.newaxis], firstVar=c1, matrixWeights=True).sum())
+
+
+def surf_varifoldNormGradient(fvDef, fv1, K):
+    """surfaces.py:1228-1276: z1 from the matrix-weighted sums of
+    u1[k,l] = (2 g_kl u_l - g_kl^2 u_k + u_k) a_l (g = <u_k, u_l>), dz1 from applyDiffKmat of
+    the pair weights, then the currents vertex scatter."""
+    d = 1
+    xDef = fvDef.vertices
+    c1, c2 = fvDef.centers, fv1.centers
+    dim = c1.shape[1]
+    a1 = _measure_weights(fvDef.surfel)
+    a2 = _measure_weights(fv1.surfel)
+    cr1 = fvDef.surfel / a1[:, np.newaxis]
+    cr2 = fv1.surfel / a2[:, np.newaxis]
+    cr1cr1 = (cr1[:, np.newaxis, :] * cr1[np.newaxis, :, :]).sum(axis=2)
+    cr1cr2 = (cr1[:, np.newaxis, :] * cr2[np.newaxis, :, :]).sum(axis=2)
+    beta1 = a1[:, np.newaxis] * a1[np.newaxis, :] * (1 + d * cr1cr1 ** 2)
+    beta2 = a1[:, np.newaxis] * a2[np.newaxis, :] * (1 + d * cr1cr2 ** 2)
+    u1 = (2 * d * cr1cr1[..., np.newaxis] * cr1[np.newaxis, ...]
+          - d * (cr1cr1 ** 2)[..., np.newaxis] * cr1[:, np.newaxis, :]
+          + cr1[:, np.newaxis, :]) * a1[np.newaxis, :, np.newaxis]
+    u2 = (2 * d * cr1cr2[..., np.newaxis] * cr2[np.newaxis, ...]
+          - d * (cr1cr2 ** 2)[..., np.newaxis] * cr1[:, np.newaxis, :]
+          + cr1[:, np.newaxis, :]) * a2[np.newaxis, :, np.newaxis]
+    z1 = K.applyK(c1, u1, matrixWeights=True) - K.applyK(c2, u2, firstVar=c1, matrixWeights=True)
+    dz1 = (1. / 3.) * (K.applyDiffKmat(c1, beta1) - K.applyDiffKmat(c2, beta2, firstVar=c1))
+    xDef1 = xDef[fvDef.faces[:, 0]]
+    xDef2 = xDef[fvDef.faces[:, 1]]
+    xDef3 = xDef[fvDef.faces[:, 2]]
+    px = np.zeros([xDef.shape[0], dim])
+    for I, crs in ((fvDef.faces[:, 0], np.cross(xDef3 - xDef2, z1)),
+                   (fvDef.faces[:, 1], np.cross(xDef1 - xDef3, z1)),
+                   (fvDef.faces[:, 2], np.cross(xDef2 - xDef1, z1))):
+        for k in range(I.size):
+            px[I[k]] = px[I[k]] + dz1[k] - crs[k]
+    return 2 * px
+
+
 class Curve:
     """curves.Curve restated for the pointSet / FV / copy constructors (curves.py:15-73)."""
 
@@ -486,7 +602,8 @@ def curve_measureNormGradient(fvDef, fv1, K):
 
 _DATA_TERMS = {
     'surface': {'current': (surf_currentNorm0, surf_currentNormDef, surf_currentNormGradient),
-                'measure': (surf_measureNorm0, surf_measureNormDef, surf_measureNormGradient)},
+                'measure': (surf_measureNorm0, surf_measureNormDef, surf_measureNormGradient),
+                'varifold': (surf_varifoldNorm0, surf_varifoldNormDef, surf_varifoldNormGradient)},
     'curve': {'current': (curve_currentNorm0, curve_currentNormDef, curve_currentNormGradient),
               'measure': (curve_measureNorm0, curve_measureNormDef, curve_measureNormGradient)},
 }
@@ -612,8 +729,8 @@ class _Matching:
 
 
 class SurfaceMatching(_Matching):
-    """surfaceMatching.SurfaceMatching with errorType 'current' or 'measure'
-    (surfaceMatching.py:29-36,74-484)."""
+    """surfaceMatching.SurfaceMatching with errorType 'current', 'measure' or 'varifold'
+    (surfaceMatching.py:29-40,74-484)."""
 
     def __init__(self, Template, Target, KparDiff, KparDist, sigmaError=1.0, timeStep=.1,
                  regWeight=1.0, maxIter=1000, errorType='current'):

@@ -945,6 +945,263 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ddk(const lddmm_seg_t *__restri
 }
 
 // ---------------------------------------------------------------------------
+// Varifold data term of triangulated surfaces (d = 3): pair weight
+// a_k a_l (1 + g^2) with g = <u_k, u_l>, u = n / a the unit normals and
+// a = sqrt(|n|^2 + 1e-10) the face weights.  Per target row k and source l
+// (weight sign -1 for the target-shape set):
+//   E += K a_l (1 + g^2)                      (energy; total = sum_k a_k E_k)
+//   P += K a_l (1 - g^2),  V += K a_l g u_l   (z1 = 2 V + P u_k)
+//   D += k1 a_l (1 + g^2) dx                  (dz, scaled by a_k in finalize)
+// The reference forms the (F, F) pair weights as dense arrays and sums them with
+// applykmat / applykdiffmat (surfaces.py:1184-1276, kernelFunctions.f90:499-610);
+// here they are formed per pair from O(F) face data.  Same work decomposition,
+// culling, Kahan folds and symmetric energy-only path as k_pairs.  Partials:
+// W = 1 (E) or 8 (E, P, V[3], D[3]) doubles per row and chunk.
+// ---------------------------------------------------------------------------
+template <typename R, int KT, bool GRAD>
+__global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__restrict__ segs,
+                                                         int nseg, int prec,
+                                                         double *__restrict__ part) {
+  constexpr int D = 3;
+  constexpr int RPT = rows_per_thread<R>();
+  constexpr int NACC = GRAD ? 5 : 1;
+  constexpr int W = NACC + (GRAD ? D : 0);
+  constexpr bool KAHAN = std::is_same<R, float>::value;
+
+  __shared__ R s_pos[D][TS];
+  __shared__ R s_u[D][TS];
+  __shared__ R s_a[TS];
+  __shared__ R s_box[TS / 64][2 * D];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t item = blockIdx.x;
+  const int si = find_segment(segs, nseg, item, 0);
+  const lddmm_seg_t *sg = segs + si;
+
+  const int64_t ny = sg->ny;
+  const int row_tiles = sg->row_tiles;
+  const int64_t it = item - sg->item0;
+  const int rt = (int)(it % row_tiles);
+  const int cg = (int)(it / row_tiles);
+  const int set = cg < sg->chunks0 ? 0 : 1;
+  const int ch = set ? cg - sg->chunks0 : cg;
+
+  // Sources: set 0 = the shape itself (y, w0, w2), set 1 = the target (x1, w1, sw[0]).
+  const double *spos = set ? sg->x1 : sg->y;
+  const double *swt = set ? sg->w1 : sg->w0;
+  const double *snr = set ? sg->sw[0] : sg->w2;
+  const int64_t sn = set ? sg->nx1 : sg->ny;
+  const double sgn = set ? -1.0 : 1.0;
+  const double scale = coord_scale(prec, KT, sg->sigma);
+
+  KEval<R, KT> kev;
+  kev.init(sg->order, sg->sigma);
+
+  double yd[RPT][D];
+  R ty[RPT][D], tu[RPT][D];
+  int64_t rows[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = (int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT) + r * 64 + lane;
+    rows[r] = k;
+    int64_t kk = k < ny ? k : ny - 1;
+    const double a = sg->w0[kk];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      yd[r][c] = sg->y[kk * D + c] * scale;
+      ty[r][c] = (R)yd[r][c];
+      tu[r][c] = (R)(sg->w2[kk * D + c] / a);
+    }
+  }
+  double tlo[D], thi[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    double lo = yd[0][c], hi = yd[0][c];
+#pragma unroll
+    for (int r = 1; r < RPT; ++r) { lo = fmin(lo, yd[r][c]); hi = fmax(hi, yd[r][c]); }
+    tlo[c] = wave_min(lo);
+    thi[c] = wave_max(hi);
+  }
+  const R cut2 = (R)cull_r2(prec, KT, sg->sigma);
+
+  R sA[RPT][NACC], cA[RPT][NACC], sD[RPT][D], cD[RPT][D];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+    for (int c = 0; c < NACC; ++c) { sA[r][c] = 0; cA[r][c] = 0; }
+#pragma unroll
+    for (int c = 0; c < D; ++c) { sD[r][c] = 0; cD[r][c] = 0; }
+  }
+
+  const int64_t l0 = (int64_t)ch * sg->chunk_len;
+  int64_t l1 = l0 + sg->chunk_len;
+  if (l1 > sn) l1 = sn;
+
+  // Energy-only self set: each unordered pair of TS blocks once (see k_pairs).
+  const bool sym = !GRAD && !sg->rowdot && !sg->out0 && set == 0;
+  const int64_t wave_blk = ((int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT)) / TS;
+  const int64_t wg_blk = ((int64_t)rt * (BLOCK * RPT)) / TS;
+
+  for (int64_t base = l0; base < l1; base += TS) {
+    if (sym && base / TS < wg_blk) continue;
+    const int sym_w = !sym ? 1 : (base / TS > wave_blk ? 2 : (base / TS == wave_blk ? 1 : 0));
+    double org[D];
+    const int64_t mid = base + ((l1 - base < TS ? l1 - base : TS) >> 1);
+#pragma unroll
+    for (int c = 0; c < D; ++c) org[c] = KAHAN ? spos[mid * D + c] * scale : 0.0;
+    if constexpr (KAHAN) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) ty[r][c] = (R)(yd[r][c] - org[c]);
+    }
+    __syncthreads();
+    {
+      int64_t l = base + tid;
+      R lo[D], hi[D];
+      if (l < l1) {
+        const double a = swt[l];
+        s_a[tid] = (R)(sgn * a);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          R v = (R)(spos[l * D + c] * scale - org[c]);
+          s_pos[c][tid] = v;
+          s_u[c][tid] = (R)(snr[l * D + c] / a);
+          lo[c] = v;
+          hi[c] = v;
+        }
+      } else {
+        s_a[tid] = (R)0;  // padding sources: zero weight, exact zero terms
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          s_pos[c][tid] = (R)0;
+          s_u[c][tid] = (R)0;
+          lo[c] = (R)INFINITY;
+          hi[c] = (R)-INFINITY;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        lo[c] = wave_min(lo[c]);
+        hi[c] = wave_max(hi[c]);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          s_box[wave][c] = lo[c];
+          s_box[wave][D + c] = hi[c];
+        }
+      }
+    }
+    __syncthreads();
+    unsigned live = 0;
+#pragma unroll
+    for (int q = 0; q < TS / 64; ++q) {
+      R d2 = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        R g = fmax(fmax((R)(tlo[c] - org[c]) - s_box[q][D + c], s_box[q][c] - (R)(thi[c] - org[c])),
+                   (R)0);
+        d2 += g * g;
+      }
+      if (!(d2 > cut2)) live |= 1u << q;
+    }
+    if (!live || sym_w == 0) continue;
+    const R wfac = (R)sym_w;
+
+    for (int j0 = 0; j0 < TS; j0 += KF) {
+      if (!((live >> (j0 / 64)) & 1u)) continue;
+      R tA[RPT][NACC], tD[RPT][D];
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int c = 0; c < NACC; ++c) tA[r][c] = 0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) tD[r][c] = 0;
+      }
+#pragma unroll 4
+      for (int j = j0; j < j0 + KF; ++j) {
+        R sx[D], su[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) { sx[c] = s_pos[c][j]; su[c] = s_u[c][j]; }
+        const R sa = s_a[j];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          R dx[D];
+          R r2 = KEval<R, KT>::R2_0, g = 0;
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            dx[c] = ty[r][c] - sx[c];
+            r2 = dx[c] * dx[c] + r2;
+            g = tu[r][c] * su[c] + g;
+          }
+          R k0, k1, k2;
+          kev(r2, k0, k1, k2);
+          const R g2 = g * g;
+          const R w = k0 * sa;
+          tA[r][0] = (w * g2 + w) + tA[r][0];
+          if constexpr (GRAD) {
+            tA[r][1] = (w - w * g2) + tA[r][1];
+            const R wg = w * g;
+#pragma unroll
+            for (int c = 0; c < D; ++c) tA[r][2 + c] = wg * su[c] + tA[r][2 + c];
+            const R f0 = k1 * sa;
+            const R f = f0 * g2 + f0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) tD[r][c] = f * dx[c] + tD[r][c];
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int c = 0; c < NACC; ++c) {
+          tA[r][c] *= wfac;  // exact: 1 or 2
+          if constexpr (KAHAN) {
+            R y = tA[r][c] - cA[r][c];
+            R t = sA[r][c] + y;
+            cA[r][c] = (t - sA[r][c]) - y;
+            sA[r][c] = t;
+          } else {
+            sA[r][c] += tA[r][c];
+          }
+        }
+        if constexpr (GRAD) {
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            if constexpr (KAHAN) {
+              R y = tD[r][c] - cD[r][c];
+              R t = sD[r][c] + y;
+              cD[r][c] = (t - sD[r][c]) - y;
+              sD[r][c] = t;
+            } else {
+              sD[r][c] += tD[r][c];
+            }
+          }
+        }
+      }
+    }
+  }
+
+  double *pp = part + sg->part0 + (int64_t)cg * ny * W;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = rows[r];
+    if (k >= ny) continue;
+    double *o = pp + k * W;
+    int q = 0;
+#pragma unroll
+    for (int c = 0; c < NACC; ++c) o[q++] = (double)sA[r][c] - (double)cA[r][c];
+    if constexpr (GRAD) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) o[q++] = (double)sD[r][c] - (double)cD[r][c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Finalize: combine chunk partials in chunk order and apply the epilogue.
 // ---------------------------------------------------------------------------
 __device__ inline double block_sum(double v, double *sh) {
@@ -1092,6 +1349,25 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
         }
       }
     }
+  } else if constexpr (OP == LDDMM_OP_VARIFOLD) {
+    if (valid) {
+      // Column 0: E (energy); gradient form adds P, V[3] and the derivative sum.
+      const double a = sg->w0[k];
+      rd0 = a * accK0[0] + 2.0 * (a * accK1[0]);
+      if (sg->rowdot) sg->rowdot[k] = rd0;
+      if constexpr (M == 5 && NA > 0) {
+        if (sg->out0) {
+          const double pk = accK0[1] + accK1[1];
+#pragma unroll
+          for (int c = 0; c < D; ++c)
+            sg->out0[k * D + c] = 2.0 * (accK0[2 + c] + accK1[2 + c]) + pk * (sg->w2[k * D + c] / a);
+        }
+        if (sg->out1) {
+#pragma unroll
+          for (int c = 0; c < D; ++c) sg->out1[k * D + c] = sg->coef * (a * (dscale * accD[c]));
+        }
+      }
+    }
   } else if constexpr (OP == LDDMM_OP_DDIFF_K11_12) {
     if (valid) {
       const double inv = 1.0 / (sigma * sigma);
@@ -1101,7 +1377,8 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
   }
 
   constexpr int NRED = (OP == LDDMM_OP_FLOW_STEP || OP == LDDMM_OP_KNORM ||
-                        OP == LDDMM_OP_CURRENTS || OP == LDDMM_OP_MEASURE)
+                        OP == LDDMM_OP_CURRENTS || OP == LDDMM_OP_MEASURE ||
+                        OP == LDDMM_OP_VARIFOLD)
                            ? 1 : (OP == LDDMM_OP_KDOT ? 2 : 0);
   if constexpr (NRED > 0) {
     double t0 = block_sum(rd0, sh);
@@ -1350,6 +1627,7 @@ int op_nred(int op) {
     case LDDMM_OP_KNORM:
     case LDDMM_OP_CURRENTS:
     case LDDMM_OP_MEASURE:
+    case LDDMM_OP_VARIFOLD:
       return 1;
     case LDDMM_OP_KDOT:
       return 2;
@@ -1358,7 +1636,7 @@ int op_nred(int op) {
   }
 }
 
-bool op_valid(int op) { return op >= LDDMM_OP_APPLY_K && op <= LDDMM_OP_MEASURE; }
+bool op_valid(int op) { return op >= LDDMM_OP_APPLY_K && op <= LDDMM_OP_VARIFOLD; }
 
 // Launch-uniform shape of a planned operation.
 struct Shape {
@@ -1393,6 +1671,12 @@ int op_shape(int op, int d, const lddmm_seg_t &s0, Shape *sh) {
     case LDDMM_OP_MEASURE: {
       bool grad = s0.out1 != nullptr;
       *sh = {1, grad ? 1 : 0, true, grad ? 1 + d : 1};
+      return LDDMM_OK;
+    }
+    case LDDMM_OP_VARIFOLD: {
+      if (d != 3) return LDDMM_EINVAL;  // triangulated surfaces only
+      bool grad = s0.out1 != nullptr;
+      *sh = {grad ? 5 : 1, grad ? 1 : 0, true, grad ? 8 : 1};
       return LDDMM_OK;
     }
     case LDDMM_OP_DDIFF_K11_12:
@@ -1431,7 +1715,8 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     if (s.kernel == LDDMM_KERNEL_LAPLACIAN) {
       if (s.order < 0 || s.order > 4) return LDDMM_EINVAL;
       bool deriv = op == LDDMM_OP_DIFF_KT || op == LDDMM_OP_COVECTOR_STEP ||
-                   ((op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE) && s.out1);
+                   ((op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE ||
+                     op == LDDMM_OP_VARIFOLD) && s.out1);
       if (deriv && s.order < 1) return LDDMM_EINVAL;
       if (op == LDDMM_OP_DDIFF_K11_12 && s.order < 2) return LDDMM_EINVAL;
     }
@@ -1479,6 +1764,13 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
         nx0 = s.ny;
         nx1 = s.nx1;
         break;
+      case LDDMM_OP_VARIFOLD:
+        if (!s.w0 || !s.w2 || s.nx1 < 0) return LDDMM_EINVAL;
+        if (s.nx1 > 0 && (!s.x1 || !s.w1 || !s.sw[0])) return LDDMM_EINVAL;
+        if (s.out0 && !s.out1) return LDDMM_EINVAL;  // z1 only in the gradient form
+        nx0 = s.ny;
+        nx1 = s.nx1;
+        break;
       case LDDMM_OP_DDIFF_K11_12:
         if (!s.w0 || !s.w1 || !s.w2 || !s.out0) return LDDMM_EINVAL;
         nx0 = s.ny;
@@ -1486,7 +1778,7 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     }
     if (nx0 < 1) return LDDMM_EINVAL;
     s.nx0 = nx0;
-    if (op != LDDMM_OP_CURRENTS && op != LDDMM_OP_MEASURE) s.nx1 = 0;
+    if (op != LDDMM_OP_CURRENTS && op != LDDMM_OP_MEASURE && op != LDDMM_OP_VARIFOLD) s.nx1 = 0;
     const int64_t row_tiles = ceil_div(s.ny, rows);
     const int64_t src_tiles = ceil_div(nx0, TS) + (nx1 ? ceil_div(nx1, TS) : 0);
     int64_t chunk_tiles = ceil_div(src_tiles * row_tiles, ITEMS_PER_SEGMENT);
@@ -1575,6 +1867,23 @@ int dispatch_pairs_ddk(int prec, int kern, const lddmm_seg_t *segs, int nseg, in
   return LDDMM_OK;
 }
 
+template <bool GRAD>
+void dispatch_pairs_varifold(int prec, int kern, const lddmm_seg_t *segs, int nseg,
+                             int64_t n_items, double *part, hipStream_t st) {
+  dim3 g((unsigned)n_items), b(BLOCK);
+  if (prec == LDDMM_PREC_FP32C) {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      hipLaunchKernelGGL((k_pairs_varifold<float, LDDMM_KERNEL_GAUSS, GRAD>), g, b, 0, st, segs, nseg, prec, part);
+    else
+      hipLaunchKernelGGL((k_pairs_varifold<float, LDDMM_KERNEL_LAPLACIAN, GRAD>), g, b, 0, st, segs, nseg, prec, part);
+  } else {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      hipLaunchKernelGGL((k_pairs_varifold<double, LDDMM_KERNEL_GAUSS, GRAD>), g, b, 0, st, segs, nseg, prec, part);
+    else
+      hipLaunchKernelGGL((k_pairs_varifold<double, LDDMM_KERNEL_LAPLACIAN, GRAD>), g, b, 0, st, segs, nseg, prec, part);
+  }
+}
+
 template <int OP, int D, int M, bool KSUM, int NA>
 void launch_fin(const lddmm_seg_t *segs, int nseg, int prec, int kern, int64_t n_fin,
                 const double *part, double *red, hipStream_t st) {
@@ -1650,6 +1959,18 @@ int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, 
         launch_fin<LDDMM_OP_MEASURE, D, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       }
       return LDDMM_OK;
+    case LDDMM_OP_VARIFOLD:
+      if constexpr (D == 3) {
+        if (sh.NA > 0) {
+          dispatch_pairs_varifold<true>(prec, kern, segs, nseg, n_items, part, st);
+          launch_fin<LDDMM_OP_VARIFOLD, 3, 5, true, 1>(segs, nseg, prec, kern, n_fin, part, red, st);
+        } else {
+          dispatch_pairs_varifold<false>(prec, kern, segs, nseg, n_items, part, st);
+          launch_fin<LDDMM_OP_VARIFOLD, 3, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+        }
+        return LDDMM_OK;
+      }
+      return LDDMM_EINVAL;
     case LDDMM_OP_DDIFF_K11_12:
       dispatch_pairs_ddk<D>(prec, kern, segs, nseg, n_items, part, st);
       launch_fin<LDDMM_OP_DDIFF_K11_12, D, D, false, 1>(segs, nseg, prec, kern, n_fin, part, red, st);

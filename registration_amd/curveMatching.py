@@ -68,8 +68,8 @@ class CurveMatching(DeviceMatching):
                           self.param.sigmaError, Tsize, regWeight,
                           data_term=self.param.errorType)
 
-    def _new_shape(self, vertices, measure=False):
-        return curves.DeviceCurve(self.faces_np, vertices, measure=measure)
+    def _new_shape(self, vertices, with_weights=False):
+        return curves.DeviceCurve(self.faces_np, vertices, with_weights=with_weights)
 
 
     def endOptim(self):

@@ -60,9 +60,10 @@ class Curve:
 
 class DeviceCurve:
     """A curve resident on the GPU (vertices, edges, centers, edge vectors, incidence CSR;
-    with ``measure``, the edge weights sqrt(|lenel|^2 + 1e-10), else ``weights`` is None)."""
+    with ``with_weights``, the edge weights sqrt(|lenel|^2 + 1e-10), else ``weights`` is
+    None)."""
 
-    def __init__(self, faces_np, vertices, measure: bool = False):
+    def __init__(self, faces_np, vertices, with_weights: bool = False):
         faces_np = np.asarray(faces_np)
         self.nv, self.d = int(vertices.shape[0]), int(vertices.shape[1])
         self.faces, self.ptr, self.inc = engine.topology(faces_np, self.nv)
@@ -70,7 +71,7 @@ class DeviceCurve:
         dev = rt.device()
         self.centers = torch.empty((E, self.d), dtype=torch.float64, device=dev)
         self.lenel = torch.empty((E, self.d), dtype=torch.float64, device=dev)
-        self.weights = torch.empty(E, dtype=torch.float64, device=dev) if measure else None
+        self.weights = torch.empty(E, dtype=torch.float64, device=dev) if with_weights else None
         self.vertices = None
         self.update(vertices)
 
@@ -177,7 +178,7 @@ def measure_gradient_dev(s: DeviceCurve, c2, a2, K: engine.DevKernel, scale: flo
 
 def measureNormGradient(fvDef, fv1, KparDist):
     """curves.py:653-701."""
-    a = DeviceCurve(fvDef.faces, fvDef.vertices, measure=True)
+    a = DeviceCurve(fvDef.faces, fvDef.vertices, with_weights=True)
     px = torch.empty((a.nv, a.d), dtype=torch.float64, device=rt.device())
     measure_gradient_dev(a, rt.as_dev(fv1.centers), rt.as_dev(engine.measure_weights(fv1.lenel)),
                          _kernel(KparDist), 2.0, px)

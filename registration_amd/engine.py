@@ -373,6 +373,53 @@ def measure(jobs: Sequence[MeasureJob]) -> None:
         rt.launch(_lib.OP_MEASURE, group[0].K.prec, d, r)
 
 
+# --------------------------------------------------------------------------------------
+# Varifold data term (surfaces)
+# --------------------------------------------------------------------------------------
+@dataclass
+class VarifoldJob:
+    c1: torch.Tensor            # (F1, 3) centers
+    a1: torch.Tensor            # (F1,) weights sqrt(|n|^2 + 1e-10)
+    n1: torch.Tensor            # (F1, 3) surfels
+    c2: Optional[torch.Tensor]  # (F2, 3) target, or None for the shape's own norm
+    a2: Optional[torch.Tensor]
+    n2: Optional[torch.Tensor]
+    K: DevKernel
+    total: torch.Tensor         # (1,) varifoldNormDef (varifoldNorm0 without target)
+    z1: Optional[torch.Tensor] = None   # (F1, 3) gradient form outputs
+    dz: Optional[torch.Tensor] = None
+    coef: float = 1.0 / 3.0
+
+
+def varifold_record(c1, a1, n1, c2, a2, n2, K: DevKernel, coef: float) -> np.ndarray:
+    """The varifold descriptor of (c1, a1, n1) against (c2, a2, n2) without its outputs (one
+    record); c2 None leaves the target out."""
+    r = rt.segs(1)
+    r["y"] = c1.data_ptr()
+    r["w0"] = a1.data_ptr()
+    r["w2"] = n1.data_ptr()
+    r["x1"] = rt.ptr(c2)
+    r["w1"] = rt.ptr(a2)
+    r["sw"][0, 0] = rt.ptr(n2)
+    r["ny"] = c1.shape[0]
+    r["nx1"] = 0 if c2 is None else c2.shape[0]
+    r["sigma"] = K.sigma
+    r["coef"] = coef
+    r["kernel"] = K.kernel
+    r["order"] = K.order
+    return r
+
+
+def varifold(jobs: Sequence[VarifoldJob]) -> None:
+    for group in _groups(jobs, lambda j: (j.K.family, j.dz is not None)):
+        r = rt.stack([rt.raw(varifold_record(j.c1, j.a1, j.n1, j.c2, j.a2, j.n2, j.K, j.coef))
+                      for j in group])
+        r["out0"] = [rt.ptr(j.z1) for j in group]
+        r["out1"] = [rt.ptr(j.dz) for j in group]
+        r["total"] = [j.total.data_ptr() for j in group]
+        rt.launch(_lib.OP_VARIFOLD, group[0].K.prec, 3, r)
+
+
 @dataclass
 class KNormJob:
     c: torch.Tensor

@@ -9,10 +9,11 @@ them with a list of one, which is the plain single-pair path; the serving layer 
 with every registration waiting on the same request.
 
 Reference: py-lddmm/surfaceMatching.py:187-484 and curveMatching.py:148-352
-(affine='none', symmetric=False, errorType 'current' or 'measure'), plus a landmark-set data
+(affine='none', symmetric=False, errorType 'current', 'measure' or, for surfaces,
+'varifold'), plus a landmark-set data
 term sum |x_T - y|^2 / sigmaError^2 over corresponding points
 (landmarkMatching.LandmarkMatching).
-Data-term energies (currents and measure norms) are always evaluated in FP64 pairwise
+Data-term energies (currents, measure and varifold norms) are always evaluated in FP64 pairwise
 arithmetic; the flow, the covector, the RKHS products and the data-term gradient use the
 kernels' precision.  Differences that do not change any result: the trajectory of an accepted trial is kept instead of being recomputed in
 endOfIteration / landmarkHamiltonianCovector (the recomputation is deterministic and
@@ -33,8 +34,10 @@ from . import runtime as rt
 
 CURRENT = 'current'
 MEASURE = 'measure'
+VARIFOLD = 'varifold'
 LANDMARK = 'landmark'
-SHAPE_TERMS = (CURRENT, MEASURE)
+SHAPE_TERMS = (CURRENT, MEASURE, VARIFOLD)
+_DATA_OP = {CURRENT: _lib.OP_CURRENTS, MEASURE: _lib.OP_MEASURE, VARIFOLD: _lib.OP_VARIFOLD}
 
 
 class Direction:
@@ -49,7 +52,7 @@ class Direction:
 class DeviceMatching:
     """Common device state of SurfaceMatching / CurveMatching / LandmarkMatching."""
 
-    # subclasses set: coef (1/3 surfaces, 1/2 curves), _new_shape(vertices, measure) and
+    # subclasses set: coef (1/3 surfaces, 1/2 curves), _new_shape(vertices, with_weights) and
     # measure_norm0_eps (the offset inside the target weights of measureNorm0: 1e-10 for
     # surfaces, surfaces.py:1095; none for curves, curves.py:624)
     coef = 1.0 / 3.0
@@ -59,8 +62,9 @@ class DeviceMatching:
                      sigmaError, Tsize, regweight, target_points=None, data_term=CURRENT):
         """Device state.  With ``target_points`` (N, d) the data term is the landmark one,
         sum |x_T - y|^2 / sigmaError^2 over corresponding points, and the shape arguments
-        are unused; otherwise it is the currents (data_term 'current') or measure
-        ('measure') norm against the target faces (target_centers, target_cr)."""
+        are unused; otherwise it is the currents (data_term 'current'), measure ('measure') or
+        varifold ('varifold', surfaces) norm against the target faces (target_centers,
+        target_cr)."""
         dev = rt.device()
         self.npt, self.dim = x0_np.shape
         self.Tsize = Tsize
@@ -85,7 +89,9 @@ class DeviceMatching:
             self.faces_np = np.asarray(faces_np)
             self.c2 = rt.as_dev(target_centers)
             self.cr2 = rt.as_dev(target_cr)
-            if data_term == MEASURE:
+            if data_term == VARIFOLD and self.dim != 3:
+                raise NotImplementedError("the varifold data term is defined for surfaces")
+            if data_term in (MEASURE, VARIFOLD):
                 self.a2 = rt.as_dev(engine.measure_weights(target_cr))
                 self.a2_norm0 = rt.as_dev(engine.measure_weights(target_cr,
                                                                  self.measure_norm0_eps))
@@ -96,9 +102,9 @@ class DeviceMatching:
         self._xt_try = self.xt.clone()
         self._xt_scratch = self.xt.clone()
         if self.data_kind in SHAPE_TERMS:
-            measure = self.data_kind == MEASURE
-            self.fvDef_dev = self._new_shape(self.x0, measure)
-            self._try_shape = self._new_shape(self.x0, measure)
+            weights = self.data_kind != CURRENT
+            self.fvDef_dev = self._new_shape(self.x0, weights)
+            self._try_shape = self._new_shape(self.x0, weights)
         else:
             self.fvDef_dev = self._try_shape = None
         # Pointer-free descriptor parts of this registration's launches, built once; the
@@ -117,7 +123,7 @@ class DeviceMatching:
         self._def_stale = False
         if self.data_kind in SHAPE_TERMS:
             Ke = self.Kdist.energy()
-            op = _lib.OP_CURRENTS if self.data_kind == CURRENT else _lib.OP_MEASURE
+            op = _DATA_OP[self.data_kind]
             self._cur_try = rt.raw(self._data_record(self._try_shape, Ke))
             self._cur_key = (Ke.kernel, Ke.prec, d, op)
             self._geom_try = rt.raw(engine.geometry_record(self._try_shape))
@@ -151,6 +157,9 @@ class DeviceMatching:
         unset)."""
         if self.data_kind == CURRENT:
             return engine.currents_record(sh.centers, sh.surfel, self.c2, self.cr2, K, self.coef)
+        if self.data_kind == VARIFOLD:
+            return engine.varifold_record(sh.centers, sh.weights, sh.surfel, self.c2, self.a2,
+                                          self.cr2, K, self.coef)
         return engine.measure_record(sh.centers, sh.weights, self.c2, self.a2, K, self.coef,
                                      n1=sh.surfel if gradient else None)
 
@@ -269,6 +278,11 @@ def _data_energies(opts, vertices, shapes, totals) -> None:
                                           opts[i].a2, opts[i].Kdist.energy(), totals[i],
                                           coef=opts[i].coef)
                         for i in cur if opts[i].data_kind == MEASURE])
+        engine.varifold([engine.VarifoldJob(shapes[i].centers, shapes[i].weights,
+                                            shapes[i].surfel, opts[i].c2, opts[i].a2,
+                                            opts[i].cr2, opts[i].Kdist.energy(), totals[i],
+                                            coef=opts[i].coef)
+                         for i in cur if opts[i].data_kind == VARIFOLD])
     if lmk:
         engine.landmark_batch([vertices[i] for i in lmk], [opts[i].y for i in lmk],
                               [totals[i] for i in lmk])
@@ -309,8 +323,9 @@ def launch_objective(opts: Sequence[DeviceMatching]) -> Pending:
                    [scal[off + o.Tsize:off + o.Tsize + 1] for o, off in zip(opts, offs)])
     for o in opts:
         o._def_stale = False
-    # Target self-term (currentNorm0 / measureNorm0); zero for the landmark data term.
-    mnorms = []
+    # Target self-term (currentNorm0 / measureNorm0 / varifoldNorm0); zero for the landmark
+    # data term.
+    mnorms, vnorms = [], []
     for o, off in zip(opts, offs):
         T = o.Tsize
         if o.data_kind == CURRENT:
@@ -318,8 +333,12 @@ def launch_objective(opts: Sequence[DeviceMatching]) -> Pending:
         elif o.data_kind == MEASURE:
             mnorms.append(engine.MeasureJob(o.c2, o.a2_norm0, None, None, o.Kdist.energy(),
                                             scal[off + T + 1:off + T + 2]))
+        elif o.data_kind == VARIFOLD:
+            vnorms.append(engine.VarifoldJob(o.c2, o.a2, o.cr2, None, None, None,
+                                             o.Kdist.energy(), scal[off + T + 1:off + T + 2]))
     engine.knorm(norms)
     engine.measure(mnorms)
+    engine.varifold(vnorms)
 
     def finish(vals):
         out = []

@@ -40,8 +40,8 @@ class Request:
     """One registration: ``kind`` 'surface', 'curve' or 'landmark'; template/target are
     (faces, vertices), with faces None for landmark sets (corresponding points); kernel
     names/widths as in SurfaceMatchingParam (sigmaDist unused by landmark sets); errorType
-    'current' or 'measure' selects the data term of surfaces and curves; ``arrival`` in
-    seconds from service start."""
+    'current', 'measure' or (surfaces) 'varifold' selects the data term of surfaces and
+    curves; ``arrival`` in seconds from service start."""
 
     rid: int
     kind: str

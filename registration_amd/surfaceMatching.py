@@ -4,8 +4,8 @@
 ``SurfaceMatchingParam`` and ``SurfaceMatching`` keep the reference constructors and the
 optimiser protocol; the registration lives on the GPU (see matching.DeviceMatching) and
 ``optimizeMatching`` runs the reference's conjugate gradient over it.  Supported: the
-currents and measure data terms (errorType 'current' or 'measure', the default),
-affine='none', symmetric=False, subsampleTargetSize=-1.  File output (VTK snapshots every saveRate iterations) is not written.
+currents, measure and varifold data terms (errorType 'current', 'measure' (the default) or
+'varifold'), affine='none', symmetric=False, subsampleTargetSize=-1.  File output (VTK snapshots every saveRate iterations) is not written.
 """
 from __future__ import annotations
 
@@ -37,7 +37,9 @@ class SurfaceMatchingParam:
             self.fun_obj = surfaces.measureNormDef
             self.fun_objGrad = surfaces.measureNormGradient
         elif errorType == 'varifold':
-            self.fun_obj0 = self.fun_obj = self.fun_objGrad = None
+            self.fun_obj0 = surfaces.varifoldNorm0
+            self.fun_obj = surfaces.varifoldNormDef
+            self.fun_objGrad = surfaces.varifoldNormGradient
         else:
             logging.error('Unknown error Type: ' + self.errorType)
         if KparDiff is None:
@@ -64,9 +66,9 @@ class SurfaceMatching(DeviceMatching):
         self.fv0 = surfaces.Surface(surf=Template)
         self.fv1 = surfaces.Surface(surf=Target)
         self.param = SurfaceMatchingParam() if param is None else param
-        if self.param.errorType not in ('current', 'measure'):
-            raise NotImplementedError("the GPU path implements the currents and measure data "
-                                      "terms (errorType 'current' or 'measure')")
+        if self.param.errorType not in ('current', 'measure', 'varifold'):
+            raise NotImplementedError("the GPU path implements the currents, measure and "
+                                      "varifold data terms")
         self.saveRate = 10
         self.outputDir = outputDir
         self.maxIter = maxIter
@@ -87,8 +89,8 @@ class SurfaceMatching(DeviceMatching):
                           self.param.KparDiff, self.param.KparDist, self.param.sigmaError, Tsize,
                           regWeight, data_term=self.param.errorType)
 
-    def _new_shape(self, vertices, measure=False):
-        return surfaces.DeviceSurface(self.faces_np, vertices, measure=measure)
+    def _new_shape(self, vertices, with_weights=False):
+        return surfaces.DeviceSurface(self.faces_np, vertices, with_weights=with_weights)
 
 
     def endOptim(self):

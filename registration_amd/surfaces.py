@@ -11,7 +11,9 @@ norm and the four of the gradient are fused into one segmented launch, and the p
 scatter of the gradient is a deterministic vertex gather in the reference's summation order.
 ``measureNorm0`` / ``measureNormDef`` / ``measureNorm`` / ``measureNormGradient``
 (surfaces.py:1092-1182) do the same for the measure data term (scalar face weights
-sqrt(|surfel|^2 + 1e-10) in place of the surfels).
+sqrt(|surfel|^2 + 1e-10) in place of the surfels), and ``varifoldNorm0`` /
+``varifoldNormDef`` / ``varifoldNorm`` / ``varifoldNormGradient`` (surfaces.py:1184-1276) for
+the varifold term, whose (F, F) pair weights the GPU forms per pair instead of storing them.
 """
 from __future__ import annotations
 
@@ -66,10 +68,11 @@ class Surface:
 
 class DeviceSurface:
     """A surface resident on the GPU: vertices, faces, centers, surfels, the
-    vertex-incidence CSR used by the gradient gather and, with ``measure``, the face weights
-    sqrt(|surfel|^2 + 1e-10) of the measure data term (else ``weights`` is None)."""
+    vertex-incidence CSR used by the gradient gather and, with ``with_weights``, the face
+    weights sqrt(|surfel|^2 + 1e-10) of the measure and varifold data terms (else
+    ``weights`` is None)."""
 
-    def __init__(self, faces_np, vertices, measure: bool = False):
+    def __init__(self, faces_np, vertices, with_weights: bool = False):
         faces_np = np.asarray(faces_np)
         self.nv = int(vertices.shape[0])
         self.faces, self.ptr, self.inc = engine.topology(faces_np, self.nv)
@@ -77,7 +80,7 @@ class DeviceSurface:
         dev = rt.device()
         self.centers = torch.empty((F, 3), dtype=torch.float64, device=dev)
         self.surfel = torch.empty((F, 3), dtype=torch.float64, device=dev)
-        self.weights = torch.empty(F, dtype=torch.float64, device=dev) if measure else None
+        self.weights = torch.empty(F, dtype=torch.float64, device=dev) if with_weights else None
         self.vertices = None
         self.update(vertices)
 
@@ -168,7 +171,7 @@ def measureNorm(fvDef, fv1, KparDist):
 def measure_gradient_dev(s: DeviceSurface, c2, a2, K: engine.DevKernel, scale: float,
                          px: torch.Tensor, total: torch.Tensor | None = None) -> torch.Tensor:
     """px = scale * (gradient of the measure norm wrt the vertices of s) / 2 for a shape
-    built with measure=True; also writes measureNormDef to ``total``."""
+    built with with_weights=True; also writes measureNormDef to ``total``."""
     F = s.centers.shape[0]
     dev = s.centers.device
     z1 = torch.empty((F, 3), dtype=torch.float64, device=dev)
@@ -183,8 +186,61 @@ def measure_gradient_dev(s: DeviceSurface, c2, a2, K: engine.DevKernel, scale: f
 
 def measureNormGradient(fvDef, fv1, KparDist):
     """Gradient of measureNorm wrt the vertices of fvDef (surfaces.py:1125-1182)."""
-    a = DeviceSurface(fvDef.faces, fvDef.vertices, measure=True)
+    a = DeviceSurface(fvDef.faces, fvDef.vertices, with_weights=True)
     px = torch.empty((a.nv, 3), dtype=torch.float64, device=rt.device())
     measure_gradient_dev(a, rt.as_dev(fv1.centers), rt.as_dev(engine.measure_weights(fv1.surfel)),
                          _kernel(KparDist), 2.0, px)
+    return px.cpu().numpy()
+
+
+# ---- varifold data term (surfaces.py:1184-1276) -----------------------------------------
+def _varifold_parts(fv):
+    """Device centers, weights sqrt(|surfel|^2 + 1e-10) and surfels of a host surface."""
+    return (rt.as_dev(fv.centers), rt.as_dev(engine.measure_weights(fv.surfel)),
+            rt.as_dev(fv.surfel))
+
+
+def varifoldNorm0(fv1, KparDist):
+    """sum_kl K(c_k, c_l) a_k a_l (1 + <u_k, u_l>^2) of fv1 (surfaces.py:1184-1197)."""
+    c, a, n = _varifold_parts(fv1)
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.varifold([engine.VarifoldJob(c, a, n, None, None, None, _kernel(KparDist).energy(), tot)])
+    return float(tot.item())
+
+
+def varifoldNormDef(fvDef, fv1, KparDist):
+    """|fvDef|^2 - 2 <fvDef, fv1> with the varifold dot product (surfaces.py:1200-1221)."""
+    c1, a1, n1 = _varifold_parts(fvDef)
+    c2, a2, n2 = _varifold_parts(fv1)
+    tot = torch.empty(1, dtype=torch.float64, device=rt.device())
+    engine.varifold([engine.VarifoldJob(c1, a1, n1, c2, a2, n2, _kernel(KparDist).energy(), tot)])
+    return float(tot.item())
+
+
+def varifoldNorm(fvDef, fv1, KparDist):
+    return varifoldNormDef(fvDef, fv1, KparDist) + varifoldNorm0(fv1, KparDist)
+
+
+def varifold_gradient_dev(s: DeviceSurface, c2, a2, n2, K: engine.DevKernel, scale: float,
+                          px: torch.Tensor, total: torch.Tensor | None = None) -> torch.Tensor:
+    """px = scale * (gradient of the varifold norm wrt the vertices of s) / 2 for a shape
+    built with with_weights=True; also writes varifoldNormDef to ``total``."""
+    F = s.centers.shape[0]
+    dev = s.centers.device
+    z1 = torch.empty((F, 3), dtype=torch.float64, device=dev)
+    dz = torch.empty((F, 3), dtype=torch.float64, device=dev)
+    if total is None:
+        total = torch.empty(1, dtype=torch.float64, device=dev)
+    engine.varifold([engine.VarifoldJob(s.centers, s.weights, s.surfel, c2, a2, n2, K, total,
+                                        z1=z1, dz=dz, coef=1.0 / 3.0)])
+    engine.surface_vertex_gradient(s.vertices, s.faces, z1, dz, s.ptr, s.inc, scale, px)
+    return px
+
+
+def varifoldNormGradient(fvDef, fv1, KparDist):
+    """Gradient of varifoldNorm wrt the vertices of fvDef (surfaces.py:1228-1276)."""
+    a = DeviceSurface(fvDef.faces, fvDef.vertices, with_weights=True)
+    px = torch.empty((a.nv, 3), dtype=torch.float64, device=rt.device())
+    c2, a2, n2 = _varifold_parts(fv1)
+    varifold_gradient_dev(a, c2, a2, n2, _kernel(KparDist), 2.0, px)
     return px.cpu().numpy()

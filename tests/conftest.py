@@ -26,12 +26,23 @@ def pytest_collection_modifyitems(config, items):
             item.add_marker(skip)
 
 
-@pytest.fixture(scope="session")
-def golden():
+def _load_cases(name):
     import numpy as np
-    path = os.path.join(ROOT, "tests", "golden", "kernel_sums.npz")
+    path = os.path.join(ROOT, "tests", "golden", name)
     with np.load(path, allow_pickle=False) as z:
         data = {k: z[k] for k in z.files}
     cases = sorted({k.split("/")[0] for k in data})
     return {c: {k.split("/", 1)[1]: v for k, v in data.items() if k.startswith(c + "/")}
             for c in cases}
+
+
+@pytest.fixture(scope="session")
+def golden():
+    """Reference Fortran outputs of applyk, applykdifft, applykdiff11and12."""
+    return _load_cases("kernel_sums.npz")
+
+
+@pytest.fixture(scope="session")
+def golden_mat():
+    """Reference Fortran outputs of the matrix-weight sums applykmat, applykdiffmat."""
+    return _load_cases("kernel_mat_sums.npz")

@@ -1,6 +1,8 @@
-"""Writes tests/golden/kernel_sums.npz: inputs and outputs of the reference's own Fortran
-kernel sums (py-lddmm/kernelFunctions.f90 applyk, applykdifft, applykdiff11and12), compiled by
-oracle/Makefile (`make -C oracle ref`) into oracle/_ref/libkff_ref.so in this container.
+"""Writes tests/golden/kernel_sums.npz and tests/golden/kernel_mat_sums.npz: inputs and
+outputs of the reference's own Fortran kernel sums (py-lddmm/kernelFunctions.f90 applyk,
+applykdifft, applykdiff11and12; and the matrix-weight sums applykmat, applykdiffmat of the
+varifold data term), compiled by oracle/Makefile (`make -C oracle ref`) into
+oracle/_ref/libkff_ref.so in this container.
 
 Run:  python tests/golden/make_golden.py
 The fixture holds data only (seeded inputs, reference outputs), so the GPU tests and the
@@ -58,6 +60,36 @@ def main():
             out[f"{name}/applykdiff11and12"] = orc.apply_ddiff_k11and12(xs, b1, b2, p, sigma,
                                                                          order, sums="reference")
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kernel_sums.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, len(out), "arrays")
+    mat_cases()
+
+
+MAT_CASES = [
+    # (name, d, ny, nx, m, order, sigma)
+    ("gauss_d3", 3, 37, 29, 1, 10, 0.6),
+    ("gauss_m3", 3, 30, 41, 3, 10, 0.9),
+    ("lap1_d3", 3, 33, 33, 1, 1, 0.5),
+    ("lap3_d3", 3, 28, 35, 3, 3, 0.7),
+    ("lap4_d2", 2, 31, 26, 2, 4, 0.8),
+]
+
+
+def mat_cases():
+    """Golden vectors of the matrix-weight sums (own seed, so kernel_sums.npz is unchanged)."""
+    rng = np.random.default_rng(20261204)
+    out = {}
+    for name, d, ny, nx, m, order, sigma in MAT_CASES:
+        y = rng.standard_normal((ny, d)) * 0.8
+        x = rng.standard_normal((nx, d)) * 0.8
+        x[:4] = y[:4]            # coincidences: the diagonal cut-off
+        beta = rng.standard_normal((ny, nx, m))
+        out[f"{name}/y"], out[f"{name}/x"], out[f"{name}/beta"] = y, x, beta
+        out[f"{name}/params"] = np.array([d, ny, nx, m, order, sigma])
+        out[f"{name}/applykmat"] = orc.apply_k_mat(y, x, beta, sigma, order, sums="reference")
+        out[f"{name}/applykdiffmat"] = orc.apply_diff_k_mat(y, x, beta[..., 0], sigma, order,
+                                                            sums="reference")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kernel_mat_sums.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, len(out), "arrays")
 

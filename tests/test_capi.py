@@ -59,7 +59,8 @@ def test_plan_layout_and_chunking():
     assert (a.chunks0 - 1) * a.chunk_len < 40962
     assert b.item0 == a.row_tiles * a.chunks0
     assert items == b.item0 + b.row_tiles * b.chunks0
-    assert a.fin_blocks == -(-40962 // 64) and b.fin0   # 64 rows per finalize workgroup == a.fin_blocks and fins == a.fin_blocks + b.fin_blocks
+    # 64 rows per finalize workgroup
+    assert a.fin_blocks == -(-40962 // 64) and b.fin0 == a.fin_blocks and fins == a.fin_blocks + b.fin_blocks
     # The chunking of a segment depends only on its own sizes (result independent of the batch).
     rc2, alone, *_ = _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP32C, 3, [_seg(642)])
     assert (alone[0].chunk_len, alone[0].chunks0, alone[0].row_tiles) == (b.chunk_len, b.chunks0, b.row_tiles)
@@ -83,6 +84,27 @@ def test_plan_rejects_invalid():
     assert _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP64, 3, [_seg(10, sigma=0.0)])[0] == _lib.EINVAL
     # Currents need a target set.
     assert _plan(_lib.OP_CURRENTS, _lib.PREC_FP64, 3, [_seg(10)])[0] == _lib.EINVAL
+    # Measure: the target set is optional (its own norm without one); z1 needs the normals.
+    meas = _seg(10, w2=0, out0=0)
+    assert _plan(_lib.OP_MEASURE, _lib.PREC_FP64, 3, [meas])[0] == 0
+    assert _plan(_lib.OP_MEASURE, _lib.PREC_FP64, 3, [_seg(10, w2=0)])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_MEASURE, _lib.PREC_FP64, 3, [_seg(10, out0=0, nx1=5)])[0] == _lib.EINVAL
+    # Varifold: surfaces only (d = 3), normals required, z1 only with the gradient form, and a
+    # target needs its normals too (sw[0]).
+    vf = _seg(10, out0=0)
+    assert _plan(_lib.OP_VARIFOLD, _lib.PREC_FP64, 3, [vf])[0] == 0
+    assert _plan(_lib.OP_VARIFOLD, _lib.PREC_FP64, 2, [_seg(10, out0=0)])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_VARIFOLD, _lib.PREC_FP64, 3, [_seg(10, out0=0, w2=0)])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_VARIFOLD, _lib.PREC_FP64, 3, [_seg(10)])[0] == _lib.EINVAL
+    tgt = _seg(10, out0=0, nx1=7, x1=4096)
+    assert _plan(_lib.OP_VARIFOLD, _lib.PREC_FP64, 3, [tgt])[0] == _lib.EINVAL
+    tgt.sw[0] = 4096
+    rc, arr, *_ = _plan(_lib.OP_VARIFOLD, _lib.PREC_FP64, 3, [tgt])
+    assert rc == 0 and arr[0].nx1 == 7 and arr[0].chunks1 == 1
+    grad = _seg(10, out1=4096)
+    assert _plan(_lib.OP_VARIFOLD, _lib.PREC_FP64, 3, [grad])[0] == 0
+    assert _plan(_lib.OP_VARIFOLD, _lib.PREC_FP64, 3,
+                 [_seg(10, out1=4096, kernel=_lib.KERNEL_LAPLACIAN, order=0)])[0] == _lib.EINVAL
 
 
 def test_workspace_queries():

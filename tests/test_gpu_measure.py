@@ -34,7 +34,7 @@ def test_surface_measure(prec, tol, subdiv, sigma):
     gr = orc.surf_measureNormGradient(So, To, Ko)
     assert rel_err(g, gr) < 10 * tol
     # the geometry kernel's face weights are NumPy's sqrt(|surfel|^2 + 1e-10) bit for bit
-    ds = surfaces.DeviceSurface(F, X, measure=True)
+    ds = surfaces.DeviceSurface(F, X, with_weights=True)
     assert np.array_equal(ds.weights.cpu().numpy(), np.sqrt((So.surfel ** 2).sum(axis=1) + 1e-10))
     assert np.array_equal(ds.surfel.cpu().numpy(), So.surfel)
 
@@ -60,7 +60,7 @@ def test_curve_measure(prec, tol, d):
     assert abs(curves.measureNormDef(C, D, K) - ndr) < 1e-12 * abs(n0r)
     g = curves.measureNormGradient(C, D, K)
     assert rel_err(g, orc.curve_measureNormGradient(Co, Do, Ko)) < 10 * tol
-    dc = curves.DeviceCurve(C.faces, X, measure=True)
+    dc = curves.DeviceCurve(C.faces, X, with_weights=True)
     assert np.array_equal(dc.weights.cpu().numpy(), np.sqrt((Co.lenel ** 2).sum(axis=1) + 1e-10))
 
 

@@ -30,6 +30,10 @@ def requests():
     reqs.append(serving.Request(rid=30, kind='surface', template=(F, V),
                                 target=(F, synthetic.bump_target(V)), sigmaKernel=0.4,
                                 sigmaDist=0.2, sigmaError=0.1, T=4, maxIter=6, errorType='measure'))
+    reqs.append(serving.Request(rid=32, kind='surface', template=(F, 0.9 * V),
+                                target=(F, synthetic.bump_target(V)), sigmaKernel=0.4,
+                                sigmaDist=0.25, sigmaError=0.1, T=5, maxIter=6,
+                                errorType='varifold'))
     X, _ = synthetic.circle(160)
     E = synthetic.closed_curve_edges(160)
     reqs.append(serving.Request(rid=31, kind='curve', template=(E, X),

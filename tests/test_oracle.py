@@ -33,6 +33,32 @@ def test_oracle_matches_reference_golden(golden):
                                        err_msg=name)
 
 
+def test_oracle_matrix_sums_match_reference_golden(golden_mat):
+    for name, c in golden_mat.items():
+        d, ny, nx, m, order, sigma = c["params"]
+        f = orc.apply_k_mat(c["y"], c["x"], c["beta"], sigma, int(order))
+        np.testing.assert_allclose(f, c["applykmat"], rtol=1e-12,
+                                   atol=1e-13 * np.abs(c["applykmat"]).max(), err_msg=name)
+        g = orc.apply_diff_k_mat(c["y"], c["x"], c["beta"][..., 0], sigma, int(order))
+        np.testing.assert_allclose(g, c["applykdiffmat"], rtol=1e-11,
+                                   atol=1e-12 * np.abs(c["applykdiffmat"]).max(), err_msg=name)
+
+
+def test_matrix_sums_reduce_to_vector_sums():
+    """beta[k, l] = w[l] turns applyKmat into applyK; beta[k, l] = <a1_k, a2_l> turns
+    applyDiffKmat into applyDiffKT (one pair)."""
+    rng = np.random.default_rng(8)
+    y, x = rng.standard_normal((20, 3)), rng.standard_normal((25, 3))
+    w = rng.standard_normal((25, 2))
+    for order in (10, 2):
+        np.testing.assert_allclose(orc.apply_k_mat(y, x, np.broadcast_to(w, (20, 25, 2)), 0.7, order),
+                                   orc.apply_k(y, x, w, 0.7, order), rtol=1e-13, atol=1e-14)
+        a1, a2 = rng.standard_normal((20, 3)), rng.standard_normal((25, 3))
+        np.testing.assert_allclose(orc.apply_diff_k_mat(y, x, a1 @ a2.T, 0.7, order),
+                                   orc.apply_diff_kt(y, x, a1[None], a2[None], 0.7, order),
+                                   rtol=1e-11, atol=1e-12)
+
+
 @pytest.mark.skipif(not orc.ref_available(), reason="oracle/_ref not built here")
 @pytest.mark.parametrize("order,sigma", [(10, 0.5), (1, 0.3), (2, 0.6), (3, 0.4), (4, 0.8)])
 def test_oracle_matches_reference_live(order, sigma):
@@ -287,6 +313,62 @@ def test_measure_gradient_fd(kind):
         e = np.zeros_like(X)
         e[i, c] = h
         fd = (norm(mk(X + e), mk(Y), K) - norm(mk(X - e), mk(Y), K)) / (2 * h)
+        assert abs(fd - g[i, c]) < 1e-6 * max(1.0, abs(fd)), (i, c, fd, g[i, c])
+
+
+def _varifold_brute(c1, n1, c2, n2, sigma):
+    """|V1 - V2|^2 for varifolds sum_k a_k delta_{(c_k, u_k)} with the kernel
+    K(c, c') (1 + <u, u'>^2), a = sqrt(|n|^2 + 1e-10), u = n / a, by a double loop."""
+    def parts(n):
+        a = np.sqrt((n ** 2).sum(axis=1) + 1e-10)
+        return a, n / a[:, None]
+
+    def dot(ca, na, cb, nb):
+        aa, ua = parts(na)
+        ab, ub = parts(nb)
+        return sum(aa[i] * ab[j] * (1 + (ua[i] @ ub[j]) ** 2)
+                   * np.exp(-((ca[i] - cb[j]) ** 2).sum() / (2 * sigma ** 2))
+                   for i in range(len(aa)) for j in range(len(ab)))
+    return dot(c1, n1, c1, n1) - 2 * dot(c1, n1, c2, n2) + dot(c2, n2, c2, n2)
+
+
+def test_varifold_norm_brute_force_and_invariances():
+    V, F = synthetic.icosphere(1)
+    K = orc.Kernel('gauss', sigma=0.35)
+    S = orc.Surface(FV=(F, V))
+    T = orc.Surface(FV=(F, synthetic.bump_target(V)))
+    full = orc.surf_varifoldNormDef(S, T, K) + orc.surf_varifoldNorm0(T, K)
+    brute = _varifold_brute(S.centers, S.surfel, T.centers, T.surfel, 0.35)
+    n0 = orc.surf_varifoldNorm0(T, K)
+    assert abs(full - brute) < 1e-12 * n0
+    assert abs(orc.surf_varifoldNormDef(S, S, K) + orc.surf_varifoldNorm0(S, K)) < 1e-12 * n0
+    # orientation-blind (g^2), and differs from the measure term (which ignores normals)
+    Tf = orc.Surface(surf=T)
+    Tf.flipFaces()
+    assert abs(orc.surf_varifoldNormDef(S, Tf, K) - orc.surf_varifoldNormDef(S, T, K)) < 1e-12 * n0
+    assert abs(orc.surf_varifoldNorm0(T, K) - 2 * orc.surf_measureNorm0(T, K)) > 1e-3 * n0
+    # coplanar faces (every g = 1 up to the 1e-10 offset in a): twice the measure norm
+    Vp = np.array([[0., 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], [2, 0.5, 0]])
+    Fp = np.array([[0, 1, 2], [1, 3, 2], [1, 4, 3]])
+    P = orc.Surface(FV=(Fp, Vp))
+    m0 = orc.surf_measureNorm0(P, K)
+    assert abs(orc.surf_varifoldNorm0(P, K) - 2 * m0) < 1e-9 * m0
+
+
+def test_varifold_gradient_fd():
+    rng = np.random.default_rng(23)
+    V, F = synthetic.icosphere(1)
+    mk = lambda X: orc.Surface(FV=(F, X))
+    Y = synthetic.bump_target(V)
+    K = orc.Kernel('gauss', sigma=0.4)
+    X = V + 0.05 * rng.standard_normal(V.shape)
+    g = orc.surf_varifoldNormGradient(mk(X), mk(Y), K)
+    h = 1e-6
+    for (i, c) in [(0, 0), (3, 1), (7, 2), (10, 1), (25, 0)]:
+        e = np.zeros_like(X)
+        e[i, c] = h
+        fd = (orc.surf_varifoldNormDef(mk(X + e), mk(Y), K)
+              - orc.surf_varifoldNormDef(mk(X - e), mk(Y), K)) / (2 * h)
         assert abs(fd - g[i, c]) < 1e-6 * max(1.0, abs(fd)), (i, c, fd, g[i, c])
 
 
