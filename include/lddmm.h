@@ -96,7 +96,12 @@ enum {
    * where l runs over the shape's own faces and, with sign -1, the target's.
    * (surfaces.py:1184-1276 with kernelFunctions.f90:499-610; the (F, F) pair
    * weights are formed per pair, never stored). */
-  LDDMM_OP_VARIFOLD = 9
+  LDDMM_OP_VARIFOLD = 9,
+  /* divergence of v = K(., x0) w0 at the targets (x0 NULL means y):
+   *   out0[k] = sum_l 2K'(r^2) <w0_l, y_k - x0_l>                (applyDivergence,
+   * kernelFunctions.py:510-530; Laplacian orders 1..4 with the dense path's
+   * exact-double coefficients, kernelFunctions.py:75-88).  w0: (nx0, d), out0: (ny). */
+  LDDMM_OP_DIVERGENCE = 10
 };
 
 /*

@@ -185,12 +185,57 @@ def apply_diff_k_mat(y, x, beta, sigma, order, sums="oracle"):
     return f
 
 
+def _lap_pol_diff(u, order):
+    """lapPolDiff (kernelFunctions.py:75-88): exact double expressions."""
+    if order == 1:
+        return np.ones_like(u)
+    if order == 2:
+        return (1 + u) / 3.
+    if order == 3:
+        return (3 + 3 * u + u * u) / 15.
+    if order == 4:
+        return (15 + 15 * u + 6 * u * u + u * u * u) / 105.
+    raise ValueError("the Laplacian derivative needs order 1..4")
+
+
+def kernel_matrix_diff(y, x, sigma, order, self_set):
+    """The dense K'(r^2) matrix of precompute(diff=True) (kernelFunctions.py:10-55,152-183):
+    Gaussian -exp(-r^2/2s^2)/(2s^2); Laplacian -lapPolDiff(u) e^-u/(2s^2) with u = r/s.  With
+    ``self_set`` (firstVar None: pdist + squareform) the diagonal is filled as the reference
+    does (Gaussian K = 1 before scaling; Laplacian -1/((2 order - 1) 2 s^2))."""
+    from scipy.spatial import distance as dfun
+    if order > 4:
+        if self_set:
+            K = dfun.squareform(np.exp(-dfun.pdist(x, 'sqeuclidean') / (2 * sigma * sigma)),
+                                checks=False)
+            np.fill_diagonal(K, 1)
+        else:
+            K = np.exp(-dfun.cdist(y, x, 'sqeuclidean') / (2 * sigma * sigma))
+        return -K / (2 * sigma * sigma)
+    if self_set:
+        u = dfun.pdist(x) / sigma
+        K = dfun.squareform(-_lap_pol_diff(u, order) * np.exp(-u) / (2 * sigma * sigma))
+        np.fill_diagonal(K, -1. / ((2 * order - 1) * 2 * sigma * sigma))
+        return K
+    u = dfun.cdist(y, x) / sigma
+    return -_lap_pol_diff(u, order) * np.exp(-u) / (2 * sigma * sigma)
+
+
+def apply_divergence(x, a, sigma, order, firstVar=None):
+    """Kernel.applyDivergence (kernelFunctions.py:510-530), dense as in the reference:
+    2 (<(r a)_k, y_k> - (r <a, x>)_k) with r the K' matrix."""
+    r = kernel_matrix_diff(firstVar, x, sigma, order, firstVar is None)
+    y = x if firstVar is None else firstVar
+    return 2 * (np.multiply(np.dot(r, a), y).sum(axis=1) - np.dot(r, np.multiply(a, x).sum(axis=1)))
+
+
 class Kernel:
     """kernelFunctions.Kernel restated (kernelFunctions.py:238-454), affine='none' only."""
 
     def __init__(self, name='gauss', affine='none', sigma=6.5, order=10, w1=1.0, w2=1.0, dim=3,
                  center=(0, 0, 0), weight=1.0, sums="oracle"):
         self.name, self.sigma, self.order, self.affine = name, sigma, order, affine
+        self.weight = weight    # scales the dense matrices only (kernelFunctions.py:298)
         self.sums = sums
         if name == 'laplacian':
             if self.order > 4:
@@ -224,6 +269,10 @@ class Kernel:
     def applyDDiffK11and12(self, x, n, a, p):
         return apply_ddiff_k11and12(x, n, a, p, self.sigma, self.order, self.sums)
 
+    def applyDivergence(self, x, a, firstVar=None):
+        """kernelFunctions.py:510-530 with the dense K' matrix times ``weight`` (:298)."""
+        return self.weight * apply_divergence(x, a, self.sigma, self.order, firstVar)
+
 
 # --------------------------------------------------------------------------------------
 # Landmark flow and adjoint (pointEvolution.py:599-817), affine=None or A=0 arrays
@@ -246,8 +295,8 @@ def landmarkEPDiff(T, x0, a0, KparDiff):
 
 
 def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, withPointSet=None,
-                                 withNormals=None):
-    """pointEvolution.py:666-730 (Jacobian option omitted)."""
+                                 withNormals=None, withJacobian=False):
+    """pointEvolution.py:666-730."""
     N, dim = x0.shape
     M = at.shape[0] + 1
     timeStep = 1.0 / (M - 1)
@@ -260,6 +309,7 @@ def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, withPointSet=Non
     if withNormals is not None:
         nt = np.zeros([M, N, dim])
         nt[0] = withNormals
+    Jt = np.zeros([M, N if yt is None else yt.shape[1]]) if withJacobian else None
     for k in range(M - 1):
         z = xt[k]
         a = at[k]
@@ -272,18 +322,28 @@ def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, withPointSet=Non
             yt[k + 1] = zy + timeStep * KparDiff.applyK(z, a, firstVar=zy)
             if affine is not None:
                 yt[k + 1] += timeStep * (np.dot(zy, affine[0][k].T) + affine[1][k])
+            if withJacobian:
+                Jt[k + 1] = Jt[k] + timeStep * KparDiff.applyDivergence(z, a, firstVar=zy)
+                if affine is not None:
+                    Jt[k + 1] += timeStep * np.trace(affine[0][k])
+        elif withJacobian:
+            Jt[k + 1] = Jt[k] + timeStep * KparDiff.applyDivergence(z, a)
+            if affine is not None:
+                Jt[k + 1] += timeStep * np.trace(affine[0][k])
         if nt is not None:
             zn = nt[k]
             nt[k + 1] = zn - timeStep * KparDiff.applyDiffKT(z, zn[np.newaxis], a[np.newaxis])
             if affine is not None:
                 nt[k + 1] += timeStep * np.dot(zn, affine[0][k])
-    if yt is None and nt is None:
+    if yt is None and nt is None and Jt is None:
         return xt
     out = [xt]
     if yt is not None:
         out.append(yt)
     if nt is not None:
         out.append(nt)
+    if Jt is not None:
+        out.append(Jt)
     return out
 
 

@@ -36,6 +36,7 @@ OP_DDIFF_K11_12 = 6
 OP_KNORM = 7
 OP_MEASURE = 8
 OP_VARIFOLD = 9
+OP_DIVERGENCE = 10
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64

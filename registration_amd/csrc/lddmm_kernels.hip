@@ -1202,6 +1202,140 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
 }
 
 // ---------------------------------------------------------------------------
+// Divergence of the velocity field v = K(., x) a at the targets y:
+//   div_k = sum_l 2K'(r^2) <a_l, y_k - x_l>
+// (Kernel.applyDivergence, kernelFunctions.py:510-530).  The reference forms the
+// dense K' matrix (precompute(diff=True): kernelMatrixGauss :10-55,
+// kernelMatrixLaplacian :152-183), whose Laplacian polynomials lapPolDiff
+// (:75-88) are exact double expressions, not the Fortran's float32 tables; the
+// first-derivative coefficients below are those doubles.  Its diagonal fill does
+// not matter here: the self term carries y_k - x_k = 0.  One FP64 partial per
+// row and chunk; no culling (an output-only path).
+// ---------------------------------------------------------------------------
+__constant__ double LAP_Q_EXACT[5][4] = {
+    {0.0, 0.0, 0.0, 0.0},  // order 0 has no first derivative (rejected on the host)
+    {1.0, 0.0, 0.0, 0.0},
+    {1.0 / 3.0, 1.0 / 3.0, 0.0, 0.0},
+    {3.0 / 15.0, 3.0 / 15.0, 1.0 / 15.0, 0.0},
+    {15.0 / 105.0, 15.0 / 105.0, 6.0 / 105.0, 1.0 / 105.0}};
+
+template <typename R, int KT, int D>
+__global__ __launch_bounds__(BLOCK) void k_pairs_div(const lddmm_seg_t *__restrict__ segs,
+                                                    int nseg, int prec,
+                                                    double *__restrict__ part) {
+  constexpr int RPT = rows_per_thread<R>();
+  constexpr bool KAHAN = std::is_same<R, float>::value;
+  __shared__ R s_pos[D][TS];
+  __shared__ R s_a[D][TS];
+
+  const int tid = threadIdx.x;
+  const int64_t item = blockIdx.x;
+  const int si = find_segment(segs, nseg, item, 0);
+  const lddmm_seg_t *sg = segs + si;
+  const int64_t ny = sg->ny;
+  const int row_tiles = sg->row_tiles;
+  const int64_t it = item - sg->item0;
+  const int rt = (int)(it % row_tiles);
+  const int cg = (int)(it / row_tiles);
+  const double *xs = sg->x0 ? sg->x0 : sg->y;
+  const double scale = coord_scale(prec, KT, sg->sigma);
+  KEval<R, KT> kev;
+  kev.init(sg->order, sg->sigma);
+  if constexpr (KT == LDDMM_KERNEL_LAPLACIAN) {
+    kev.q0 = (R)LAP_Q_EXACT[sg->order][0];
+    kev.q1 = (R)LAP_Q_EXACT[sg->order][1];
+    kev.q2 = (R)LAP_Q_EXACT[sg->order][2];
+    kev.q3 = (R)LAP_Q_EXACT[sg->order][3];
+  }
+
+  double yd[RPT][D];
+  R ty[RPT][D];
+  int64_t rows[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = (int64_t)rt * (BLOCK * RPT) + r * BLOCK + tid;
+    rows[r] = k;
+    int64_t kk = k < ny ? k : ny - 1;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      yd[r][c] = sg->y[kk * D + c] * scale;
+      ty[r][c] = (R)yd[r][c];
+    }
+  }
+  R sA[RPT], cA[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) { sA[r] = 0; cA[r] = 0; }
+
+  const int64_t l0 = (int64_t)cg * sg->chunk_len;
+  int64_t l1 = l0 + sg->chunk_len;
+  if (l1 > sg->nx0) l1 = sg->nx0;
+  for (int64_t base = l0; base < l1; base += TS) {
+    double org[D];
+    const int64_t mid = base + ((l1 - base < TS ? l1 - base : TS) >> 1);
+#pragma unroll
+    for (int c = 0; c < D; ++c) org[c] = KAHAN ? xs[mid * D + c] * scale : 0.0;
+    if constexpr (KAHAN) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) ty[r][c] = (R)(yd[r][c] - org[c]);
+    }
+    __syncthreads();
+    {
+      int64_t l = base + tid;
+      bool ok = l < l1;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        s_pos[c][tid] = ok ? (R)(xs[l * D + c] * scale - org[c]) : (R)0;
+        s_a[c][tid] = ok ? (R)sg->w0[l * D + c] : (R)0;
+      }
+    }
+    __syncthreads();
+    for (int j0 = 0; j0 < TS; j0 += KF) {
+      R tA[RPT];
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) tA[r] = 0;
+#pragma unroll 4
+      for (int j = j0; j < j0 + KF; ++j) {
+        R sx[D], sa[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) { sx[c] = s_pos[c][j]; sa[c] = s_a[c][j]; }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          R r2 = KEval<R, KT>::R2_0, dot = 0;
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            const R dx = ty[r][c] - sx[c];
+            r2 = dx * dx + r2;
+            dot = dx * sa[c] + dot;
+          }
+          R k0, k1, k2;
+          kev(r2, k0, k1, k2);
+          tA[r] = k1 * dot + tA[r];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        if constexpr (KAHAN) {
+          R y = tA[r] - cA[r];
+          R t = sA[r] + y;
+          cA[r] = (t - sA[r]) - y;
+          sA[r] = t;
+        } else {
+          sA[r] += tA[r];
+        }
+      }
+    }
+  }
+  double *pp = part + sg->part0 + (int64_t)cg * ny;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = rows[r];
+    if (k < ny) pp[k] = (double)sA[r] - (double)cA[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Finalize: combine chunk partials in chunk order and apply the epilogue.
 // ---------------------------------------------------------------------------
 __device__ inline double block_sum(double v, double *sh) {
@@ -1368,6 +1502,8 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
         }
       }
     }
+  } else if constexpr (OP == LDDMM_OP_DIVERGENCE) {
+    if (valid) sg->out0[k] = dscale * accK0[0];
   } else if constexpr (OP == LDDMM_OP_DDIFF_K11_12) {
     if (valid) {
       const double inv = 1.0 / (sigma * sigma);
@@ -1636,7 +1772,7 @@ int op_nred(int op) {
   }
 }
 
-bool op_valid(int op) { return op >= LDDMM_OP_APPLY_K && op <= LDDMM_OP_VARIFOLD; }
+bool op_valid(int op) { return op >= LDDMM_OP_APPLY_K && op <= LDDMM_OP_DIVERGENCE; }
 
 // Launch-uniform shape of a planned operation.
 struct Shape {
@@ -1682,6 +1818,9 @@ int op_shape(int op, int d, const lddmm_seg_t &s0, Shape *sh) {
     case LDDMM_OP_DDIFF_K11_12:
       *sh = {d, 0, false, d};
       return LDDMM_OK;
+    case LDDMM_OP_DIVERGENCE:
+      *sh = {1, 0, true, 1};
+      return LDDMM_OK;
   }
   return LDDMM_EINVAL;
 }
@@ -1715,6 +1854,7 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     if (s.kernel == LDDMM_KERNEL_LAPLACIAN) {
       if (s.order < 0 || s.order > 4) return LDDMM_EINVAL;
       bool deriv = op == LDDMM_OP_DIFF_KT || op == LDDMM_OP_COVECTOR_STEP ||
+                   op == LDDMM_OP_DIVERGENCE ||
                    ((op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE ||
                      op == LDDMM_OP_VARIFOLD) && s.out1);
       if (deriv && s.order < 1) return LDDMM_EINVAL;
@@ -1774,6 +1914,10 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
       case LDDMM_OP_DDIFF_K11_12:
         if (!s.w0 || !s.w1 || !s.w2 || !s.out0) return LDDMM_EINVAL;
         nx0 = s.ny;
+        break;
+      case LDDMM_OP_DIVERGENCE:
+        if (!s.w0 || !s.out0) return LDDMM_EINVAL;
+        if (!s.x0) nx0 = s.ny;
         break;
     }
     if (nx0 < 1) return LDDMM_EINVAL;
@@ -1865,6 +2009,23 @@ int dispatch_pairs_ddk(int prec, int kern, const lddmm_seg_t *segs, int nseg, in
       hipLaunchKernelGGL((k_pairs_ddk<double, LDDMM_KERNEL_LAPLACIAN, D>), g, b, 0, st, segs, nseg, prec, part);
   }
   return LDDMM_OK;
+}
+
+template <int D>
+void dispatch_pairs_div(int prec, int kern, const lddmm_seg_t *segs, int nseg, int64_t n_items,
+                        double *part, hipStream_t st) {
+  dim3 g((unsigned)n_items), b(BLOCK);
+  if (prec == LDDMM_PREC_FP32C) {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      hipLaunchKernelGGL((k_pairs_div<float, LDDMM_KERNEL_GAUSS, D>), g, b, 0, st, segs, nseg, prec, part);
+    else
+      hipLaunchKernelGGL((k_pairs_div<float, LDDMM_KERNEL_LAPLACIAN, D>), g, b, 0, st, segs, nseg, prec, part);
+  } else {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      hipLaunchKernelGGL((k_pairs_div<double, LDDMM_KERNEL_GAUSS, D>), g, b, 0, st, segs, nseg, prec, part);
+    else
+      hipLaunchKernelGGL((k_pairs_div<double, LDDMM_KERNEL_LAPLACIAN, D>), g, b, 0, st, segs, nseg, prec, part);
+  }
 }
 
 template <bool GRAD>
@@ -1971,6 +2132,10 @@ int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, 
         return LDDMM_OK;
       }
       return LDDMM_EINVAL;
+    case LDDMM_OP_DIVERGENCE:
+      dispatch_pairs_div<D>(prec, kern, segs, nseg, n_items, part, st);
+      launch_fin<LDDMM_OP_DIVERGENCE, D, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
+      return LDDMM_OK;
     case LDDMM_OP_DDIFF_K11_12:
       dispatch_pairs_ddk<D>(prec, kern, segs, nseg, n_items, part, st);
       launch_fin<LDDMM_OP_DDIFF_K11_12, D, D, false, 1>(segs, nseg, prec, kern, n_fin, part, red, st);

@@ -200,6 +200,29 @@ class Kernel(KernelSpec):
                     zpx = zpx + self.w1 * (yy * a2d[k]).sum() * bb
         return _unwrap(zpx, np_out)
 
+    def applyDivergence(self, x, a, firstVar=None):
+        """sum_l div_1(K(y_k, x_l) a_l) = sum_l 2K'(r^2) <a_l, y_k - x_l> at y = firstVar or x
+        (kernelFunctions.py:510-530), shape (n_y,).  The reference builds the dense (n_y, n_x)
+        K' matrix (precompute(diff=True)); here it is a tiled pair sum with the same exact-double
+        Laplacian polynomials (kernelFunctions.py:75-88), scaled by ``weight`` as the dense
+        matrices are (:298).  A kernel named 'none' returns zeros of shape (1, n_y), as the
+        reference's transposed (n_y, 1) array."""
+        xd, np_out = _wrap(x)
+        ad, _ = _wrap(a)
+        yd = xd if firstVar is None else _wrap(firstVar)[0]
+        if xd.shape != ad.shape or xd.shape[1] != yd.shape[1]:
+            raise ValueError("applyDivergence: shape mismatch between x, a and firstVar")
+        if self.name == 'none':
+            return _unwrap(torch.zeros((1, yd.shape[0]), dtype=torch.float64, device=xd.device),
+                           np_out)
+        z = apply_divergence_dev(yd, xd, ad, self)
+        if self.weight != 1.0:
+            z = z * self.weight
+        if self.affine == 'affine':
+            c = torch.as_tensor(self.center, dtype=torch.float64, device=xd.device)
+            z = z + self.w1 * ((yd - c) * ad).sum(dim=1)
+        return _unwrap(z, np_out)
+
     def applyDDiffK11and12(self, x, n, a, p):
         """kernelFunctions.py:452-454 -> kernelFunctions.f90:430-496."""
         xd, np_out = _wrap(x)
@@ -281,3 +304,16 @@ def apply_ddiff_k11and12_dev(x, a1, a2, p, K: KernelSpec) -> torch.Tensor:
                                               ws.data_ptr(), ws.numel(), rt.stream_handle()),
                "lddmm_apply_ddiff_k11and12")
     return out[:, :d0].contiguous()
+
+
+def apply_divergence_dev(y, x, a, K: KernelSpec) -> torch.Tensor:
+    """(n_y,) divergence sum of Kernel.applyDivergence on device tensors."""
+    d0 = x.shape[1]
+    _check_dim(d0)
+    y, x, a = _pad_dim(y, x, a)
+    a = a.contiguous()
+    out = torch.empty(y.shape[0], dtype=torch.float64, device=x.device)
+    r = rt.seg(y=y, x0=x, w0=a, out0=out, ny=y.shape[0], nx0=x.shape[0], sigma=float(K.sigma),
+               kernel=K.kernel_code, order=int(K.order))
+    rt.launch(_lib.OP_DIVERGENCE, K.prec_code, y.shape[1], r)
+    return out

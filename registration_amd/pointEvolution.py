@@ -13,7 +13,7 @@ import torch
 
 from . import engine
 from . import runtime as rt
-from .kernelFunctions import apply_diff_kt_dev, apply_k_dev
+from .kernelFunctions import apply_diff_kt_dev, apply_divergence_dev, apply_k_dev
 
 
 def _in(a):
@@ -42,10 +42,10 @@ def _affine_dev(affine, T, dim, dev):
 def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, withJacobian=False,
                                  withNormals=None, withPointSet=None):
     """Solves dx/dt = K(x,x) a(t) + A(t) x + b(t), x(0) = x0 by explicit Euler with T = at.shape[0]
-    steps (pointEvolution.py:666-730).  Returns xt (T+1, N, d), or [xt, yt?, nt?] with the
-    optional point set / normals trajectories."""
-    if withJacobian:
-        raise NotImplementedError("the Jacobian flow (dense applyDivergence) is not part of the GPU path")
+    steps (pointEvolution.py:666-730).  Returns xt (T+1, N, d), or [xt, yt?, nt?, Jt?] with the
+    optional point set / normals trajectories and, with ``withJacobian``, the log-Jacobian
+    J_{t+1} = J_t + dt (div v_t + tr A_t), J_0 = 0, at the points (or at the point set when
+    one is given; shape (T+1, n))."""
     x0d, np_out = _in(x0)
     atd, _ = _in(at)
     T, N, dim = atd.shape
@@ -63,8 +63,12 @@ def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, withJacobian=Fal
         n0, _ = _in(withNormals)
         nt = torch.empty((T + 1, N, dim), dtype=torch.float64, device=dev)
         nt[0] = n0
+    Jt = None
+    if withJacobian:
+        nj = yt.shape[1] if yt is not None else N
+        Jt = torch.zeros((T + 1, nj), dtype=torch.float64, device=dev)
     dt = 1.0 / T
-    if aff is None and yt is None and nt is None:
+    if aff is None and yt is None and nt is None and Jt is None:
         engine.shoot([engine.FlowJob(x0d, atd, K, xt, energy)])
     else:
         xt[0] = x0d
@@ -78,19 +82,38 @@ def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, withJacobian=Fal
                 yt[k + 1] = zy + dt * apply_k_dev(zy, z, a, KparDiff)
                 if aff is not None:
                     yt[k + 1] += dt * (zy @ aff[0][k].T + aff[1][k])
+            if Jt is not None:
+                # the divergence at the point set when there is one, else at the points
+                at_pts = yt[k] if yt is not None else z
+                Jt[k + 1] = Jt[k] + dt * _divergence(at_pts, z, a, KparDiff)
+                if aff is not None:
+                    Jt[k + 1] += dt * torch.trace(aff[0][k])
             if nt is not None:
                 zn = nt[k]
                 nt[k + 1] = zn - dt * apply_diff_kt_dev(z, z, zn[None], a[None], KparDiff)
                 if aff is not None:
                     nt[k + 1] += dt * (zn @ aff[0][k])
-    if yt is None and nt is None:
+    if yt is None and nt is None and Jt is None:
         return _out(xt, np_out)
     out = [_out(xt, np_out)]
     if yt is not None:
         out.append(_out(yt, np_out))
     if nt is not None:
         out.append(_out(nt, np_out))
+    if Jt is not None:
+        out.append(_out(Jt, np_out))
     return out
+
+
+def _divergence(y, x, a, K):
+    """Kernel.applyDivergence on device tensors, including a kernel-level affine term."""
+    z = apply_divergence_dev(y, x, a, K)
+    if getattr(K, 'weight', 1.0) != 1.0:
+        z = z * K.weight
+    if getattr(K, 'affine', 'none') == 'affine':
+        c = torch.as_tensor(K.center, dtype=torch.float64, device=x.device)
+        z = z + K.w1 * ((y - c) * a).sum(dim=1)
+    return z
 
 
 def landmarkHamiltonianCovector(x0, at, px1, KparDiff, regweight, affine=None):

@@ -467,3 +467,55 @@ def test_cg_restatement_minimises_quadratic():
     q.gradEps = 1e-9
     orc.cg(q, maxIter=500, epsInit=0.01)
     np.testing.assert_allclose(q.v, np.linalg.solve(q.A, q.b), rtol=1e-3, atol=1e-4)
+
+
+# ---------------------------------------------------------------- divergence and Jacobian flow
+@pytest.mark.parametrize("order,sigma,tol", [(10, 0.6, 1e-8), (1, 0.5, 1e-6), (3, 0.4, 1e-6)])
+def test_divergence_is_trace_of_velocity_gradient(order, sigma, tol):
+    """applyDivergence = sum_c d v_c / d y_c of v(y) = K(y, x) a, by central FD of applyK
+    (Laplacian: the dense exact-double polynomials differ from applyK's float32 tables at the
+    1e-8 level, hence the looser bound)."""
+    rng = np.random.default_rng(order)
+    x, a = rng.standard_normal((30, 3)) * 0.5, rng.standard_normal((30, 3))
+    y = rng.standard_normal((12, 3)) * 0.5
+    K = orc.Kernel('gauss' if order > 4 else 'laplacian', sigma=sigma, order=order)
+    div = K.applyDivergence(x, a, firstVar=y)
+    h = 1e-5
+    fd = np.zeros(len(y))
+    for c in range(3):
+        e = np.zeros(3)
+        e[c] = h
+        fd += (K.applyK(x, a, firstVar=y + e)[:, c] - K.applyK(x, a, firstVar=y - e)[:, c]) / (2 * h)
+    np.testing.assert_allclose(div, fd, rtol=tol, atol=tol * np.abs(fd).max())
+    # at the sources themselves the self term drops out (y_k - x_k = 0) whatever the diagonal
+    d0 = K.applyDivergence(x, a)
+    d1 = np.array([K.applyDivergence(np.delete(x, k, 0), np.delete(a, k, 0), firstVar=x[k:k + 1])[0]
+                   for k in range(len(x))])
+    np.testing.assert_allclose(d0, d1, rtol=1e-12, atol=1e-13 * np.abs(d0).max())
+
+
+def test_jacobian_flow_tracks_log_det_of_the_flow_map():
+    """J_T = sum_t dt div v_t approximates log det of the discrete flow map's Jacobian
+    (measured by FD on a tracked point set) to O(dt)."""
+    rng = np.random.default_rng(4)
+    x0 = rng.standard_normal((20, 3)) * 0.4
+    T = 40
+    at = np.tile(0.3 * rng.standard_normal((1, 20, 3)), (T, 1, 1))
+    K = orc.Kernel('gauss', sigma=0.5)
+    y0 = rng.standard_normal((5, 3)) * 0.3
+    xt, yt, Jt = orc.landmarkDirectEvolutionEuler(x0, at, K, withPointSet=y0, withJacobian=True)
+    assert Jt.shape == (T + 1, 5) and np.all(Jt[0] == 0)
+    h = 1e-6
+    for i in range(5):
+        Jm = np.zeros((3, 3))
+        for c in range(3):
+            e = np.zeros_like(y0)
+            e[i, c] = h
+            yp = orc.landmarkDirectEvolutionEuler(x0, at, K, withPointSet=y0 + e)[1][-1, i]
+            ym = orc.landmarkDirectEvolutionEuler(x0, at, K, withPointSet=y0 - e)[1][-1, i]
+            Jm[:, c] = (yp - ym) / (2 * h)
+        logdet = np.log(np.linalg.det(Jm))
+        assert abs(Jt[-1, i] - logdet) < 0.05 * max(abs(logdet), 0.05), (i, Jt[-1, i], logdet)
+    # without a point set the Jacobian is taken at the moving points
+    xt2, Jx = orc.landmarkDirectEvolutionEuler(x0, at, K, withJacobian=True)
+    assert Jx.shape == (T + 1, 20) and np.allclose(xt2, xt)
