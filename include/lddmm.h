@@ -68,7 +68,10 @@ enum {
    *   out1 = coef * [applyDiffKT(c1,[cr1],[cr1]) - applyDiffKT(c2,[cr1],[cr2],c1)] */
   LDDMM_OP_CURRENTS = 5,
   /* out0[k,:] = sum_l 2<w0_k, w1_l>(2K''<dx,dp>dx + K' dp), p = w2, dx = y_k - y_l,
-   * dp = p_k - p_l                                     (applyDDiffK11and12) */
+   * dp = p_k - p_l (na = 0: applyDDiffK11and12, kernelFunctions.f90:430-496);
+   * dp = p_k (na = 1: applyDDiffK11) or -p_l (na = 2: applyDDiffK12), the dense
+   * siblings kernelFunctions.py:459-506, with their exact-double Laplacian
+   * polynomials (:75-104) */
   LDDMM_OP_DDIFF_K11_12 = 6,
   /* total[0] = <w0, K(y, y) w0>  (currentNorm0 when w0 = surfel) */
   LDDMM_OP_KNORM = 7,
@@ -101,7 +104,12 @@ enum {
    *   out0[k] = sum_l 2K'(r^2) <w0_l, y_k - x0_l>                (applyDivergence,
    * kernelFunctions.py:510-530; Laplacian orders 1..4 with the dense path's
    * exact-double coefficients, kernelFunctions.py:75-88).  w0: (nx0, d), out0: (ny). */
-  LDDMM_OP_DIVERGENCE = 10
+  LDDMM_OP_DIVERGENCE = 10,
+  /* applyDiffK family (kernelFunctions.f90:116-290), targets = sources = y,
+   * a1 = w0, a2 = w1 (n, d):  out0[k,:] = sum_l 2K'(r^2) <y_k - y_l, b_kl> a2_l with
+   * b_kl = a1_k (na = 1: applyDiffK), -a1_l (na = 2: applyDiffK2) or a1_k - a1_l
+   * (na = 3: applyDiffK1and2) */
+  LDDMM_OP_DIFF_K = 11
 };
 
 /*
@@ -209,6 +217,21 @@ int lddmm_apply_ddiff_k11and12(const double *x, const double *a1,
                                int d, int kernel, int order, double sigma,
                                int prec, double *out, void *work,
                                size_t work_bytes, void *stream);
+
+/* kernelFunctions.f90:499-550 applykmat (via Kernel.applyK(..., matrixWeights=True),
+ * kernelFunctions.py:317-325): out(ny, m) = sum_l K(|y_k - x_l|) beta(k, l, :), beta
+ * row-major (ny, nx, m), m = 1..3.  FP64 (the sum is bound by reading beta); Fortran
+ * tables and cut-off; needs no workspace. */
+int lddmm_apply_k_mat(const double *y, const double *x, const double *beta, int64_t ny,
+                      int64_t nx, int d, int m, int kernel, int order, double sigma,
+                      double *out, void *stream);
+
+/* kernelFunctions.f90:552-610 applykdiffmat (via Kernel.applyDiffKmat,
+ * kernelFunctions.py:405-411): out(ny, d) = sum_l 2K'(r^2) (y_k - x_l) beta(k, l),
+ * beta row-major (ny, nx).  FP64; Laplacian orders 1..4. */
+int lddmm_apply_diff_k_mat(const double *y, const double *x, const double *beta,
+                           int64_t ny, int64_t nx, int d, int kernel, int order,
+                           double sigma, double *out, void *stream);
 
 /* ---- geometry of the currents data term -------------------------------- */
 

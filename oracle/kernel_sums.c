@@ -12,6 +12,8 @@
  *   oracle_apply_k            <- applyk             kernelFunctions.f90:1-54
  *   oracle_apply_diff_kt      <- applykdifft        kernelFunctions.f90:56-114
  *   oracle_apply_ddiff_k11and12 <- applykdiff11and12 kernelFunctions.f90:430-496
+ *   oracle_apply_diff_k       <- applykdiff1 / applykdiff2 / applykdiff1and2
+ *                                                   kernelFunctions.f90:116-290
  *   oracle_apply_k_mat        <- applykmat          kernelFunctions.f90:499-550
  *   oracle_apply_diff_k_mat   <- applykdiffmat      kernelFunctions.f90:552-610
  * (the last two carry one weight per (target, source) pair: the varifold data
@@ -229,6 +231,38 @@ int oracle_apply_diff_k_mat(const double *y, const double *x, const double *beta
       double kd = kdiff(dist(y + k * d, x + l * d, d), sig, ord);
       for (int c = 0; c < d; ++c)
         df[c] += kd * 2 * (y[k * d + c] - x[l * d + c]) * beta[k * nx + l];
+    }
+    for (int c = 0; c < d; ++c) f[k * d + c] = df[c];
+  }
+  return 0;
+}
+
+/* f(k,:) = sum_l 2 K'(r^2) <x_k - x_l, b_kl> a2(l,:) with b_kl = a1_k (form 1,
+ * applykdiff1, f90:116-173), -a1_l (form 2, applykdiff2, f90:175-231) or
+ * a1_k - a1_l (form 3, applykdiff1and2, f90:234-290).  The first two add into
+ * their f2py output array, which f2py zero-fills, so f is the sum.
+ * x, a1, a2: (n, d). */
+int oracle_apply_diff_k(const double *x, const double *a1, const double *a2,
+                        int64_t n, int d, int form, int ord, double sig, double *f) {
+  if (d < 1 || d > 8 || ord < 1 || form < 1 || form > 3) return 1;
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < n; ++k) {
+    double df[8];
+    for (int c = 0; c < d; ++c) df[c] = 0.0;
+    for (int64_t l = 0; l < n; ++l) {
+      double kd = kdiff(dist(x + k * d, x + l * d, d), sig, ord);
+      double s = 0.0;
+      for (int c = 0; c < d; ++c) {
+        double dx = x[k * d + c] - x[l * d + c];
+        double b = form == 1 ? a1[k * d + c] : (form == 2 ? a1[l * d + c] : a1[k * d + c] - a1[l * d + c]);
+        s += dx * b;
+      }
+      for (int c = 0; c < d; ++c) {
+        if (form == 2)
+          df[c] -= kd * 2 * s * a2[l * d + c];
+        else
+          df[c] += kd * 2 * s * a2[l * d + c];
+      }
     }
     for (int c = 0; c < d; ++c) f[k * d + c] = df[c];
   }

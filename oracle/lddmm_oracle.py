@@ -53,9 +53,10 @@ def lib():
         _lib.oracle_apply_diff_kt.argtypes = [_P, _P, _P, _P, i64, i64, i, i, i, i, d, _P]
         _lib.oracle_apply_ddiff_k11and12.argtypes = [_P, _P, _P, _P, i64, i, i, d, _P]
         _lib.oracle_apply_k_mat.argtypes = [_P, _P, _P, i64, i64, i, i, i, d, _P]
+        _lib.oracle_apply_diff_k.argtypes = [_P, _P, _P, i64, i, i, i, d, _P]
         _lib.oracle_apply_diff_k_mat.argtypes = [_P, _P, _P, i64, i64, i, i, d, _P]
         for f in (_lib.oracle_apply_k, _lib.oracle_apply_diff_kt, _lib.oracle_apply_ddiff_k11and12,
-                  _lib.oracle_apply_k_mat, _lib.oracle_apply_diff_k_mat):
+                  _lib.oracle_apply_k_mat, _lib.oracle_apply_diff_k_mat, _lib.oracle_apply_diff_k):
             f.restype = ctypes.c_int
     return _lib
 
@@ -146,6 +147,29 @@ def apply_ddiff_k11and12(x, a1, a2, p, sigma, order, sums="oracle"):
     return f
 
 
+_DIFF_K_REF = {1: "applykdiff1_", 2: "applykdiff2_", 3: "applykdiff1and2_"}
+
+
+def apply_diff_k(x, a1, a2, sigma, order, form, sums="oracle"):
+    """applyDiffK (form 1), applyDiffK2 (2), applyDiffK1and2 (3): sum_l 2K'(r^2)
+    <x_k - x_l, b_kl> a2_l (kernelFunctions.f90:116-290)."""
+    x, a1, a2 = _c(x), _c(a1), _c(a2)
+    n, d = x.shape
+    if sums == "reference":
+        f = np.zeros((n, d), order="F")   # f2py zero-fills intent(out)
+        ci = lambda v: ctypes.byref(ctypes.c_int(v))
+        getattr(ref(), _DIFF_K_REF[form])(_p(np.asfortranarray(x)), _p(np.asfortranarray(a1)),
+                                          _p(np.asfortranarray(a2)),
+                                          ctypes.byref(ctypes.c_double(sigma)), ci(order), ci(n),
+                                          ci(d), _p(f))
+        return np.ascontiguousarray(f)
+    f = np.empty((n, d))
+    rc = lib().oracle_apply_diff_k(_p(x), _p(a1), _p(a2), n, d, form, order, sigma, _p(f))
+    if rc:
+        raise ValueError("oracle_apply_diff_k: invalid arguments")
+    return f
+
+
 def apply_k_mat(y, x, beta, sigma, order, sums="oracle"):
     """f = sum_l K(|y_k - x_l|) beta[k, l, :] (kernelFunctions.f90:499-550)."""
     y, x, beta = _c(y), _c(x), _c(beta)
@@ -221,6 +245,58 @@ def kernel_matrix_diff(y, x, sigma, order, self_set):
     return -_lap_pol_diff(u, order) * np.exp(-u) / (2 * sigma * sigma)
 
 
+def _lap_pol_diff2(u, order):
+    """lapPolDiff2 (kernelFunctions.py:92-104)."""
+    if order == 2:
+        return np.full_like(u, 1.0 / 3.)
+    if order == 3:
+        return (1 + u) / 15.
+    if order == 4:
+        return (3 + 3 * u + u * u) / 105.
+    raise ValueError("the Laplacian second derivative needs order 2..4")
+
+
+def kernel_matrix_diff2(x, sigma, order):
+    """The dense K''(r^2) matrix of precompute(diff2=True) on a self set: Gaussian
+    K/(4 s^4) (diagonal 1/(4 s^4)); Laplacian lapPolDiff2(u) e^-u/(4 s^4) with the
+    reference's diagonal fill 1/(35 * 4 s^4) (kernelFunctions.py:10-55,152-183)."""
+    from scipy.spatial import distance as dfun
+    s4 = 4 * sigma ** 4
+    if order > 4:
+        K = dfun.squareform(np.exp(-dfun.pdist(x, 'sqeuclidean') / (2 * sigma * sigma)),
+                            checks=False)
+        np.fill_diagonal(K, 1)
+        return K / s4
+    u = dfun.pdist(x) / sigma
+    K = dfun.squareform(_lap_pol_diff2(u, order) * np.exp(-u) / s4)
+    np.fill_diagonal(K, 1. / (35 * s4))
+    return K
+
+
+def apply_ddiff_k11(x, n, a, p, sigma, order):
+    """Kernel.applyDDiffK11 (kernelFunctions.py:459-478), dense."""
+    r1 = kernel_matrix_diff(None, x, sigma, order, True)
+    r2 = kernel_matrix_diff2(x, sigma, order)
+    xxp = -np.dot(p, x.T) + (x * p).sum(axis=1)[:, np.newaxis]
+    na = np.dot(n, a.T)
+    u = r2 * xxp * na
+    zpx = 4 * ((u.sum(axis=1))[:, np.newaxis] * x - np.dot(u, x))
+    zpx += 2 * (r1 * na).sum(axis=1)[:, np.newaxis] * p
+    return zpx
+
+
+def apply_ddiff_k12(x, n, a, p, sigma, order):
+    """Kernel.applyDDiffK12 (kernelFunctions.py:481-506), dense, affine='none'."""
+    r1 = kernel_matrix_diff(None, x, sigma, order, True)
+    r2 = kernel_matrix_diff2(x, sigma, order)
+    xxp = np.dot(x, p.T) - (x * p).sum(axis=1)[np.newaxis, :]
+    na = np.dot(n, a.T)
+    u = r2 * xxp * na
+    zpx = -4 * ((u.sum(axis=1))[:, np.newaxis] * x - np.dot(u, x))
+    zpx -= 2 * np.dot(r1 * na, p)
+    return zpx
+
+
 def apply_divergence(x, a, sigma, order, firstVar=None):
     """Kernel.applyDivergence (kernelFunctions.py:510-530), dense as in the reference:
     2 (<(r a)_k, y_k> - (r <a, x>)_k) with r the K' matrix."""
@@ -268,6 +344,21 @@ class Kernel:
 
     def applyDDiffK11and12(self, x, n, a, p):
         return apply_ddiff_k11and12(x, n, a, p, self.sigma, self.order, self.sums)
+
+    def applyDiffK(self, x, a1, a2):
+        return apply_diff_k(x, a1, a2, self.sigma, self.order, 1, self.sums)
+
+    def applyDiffK2(self, x, a1, a2):
+        return apply_diff_k(x, a1, a2, self.sigma, self.order, 2, self.sums)
+
+    def applyDiffK1and2(self, x, a1, a2):
+        return apply_diff_k(x, a1, a2, self.sigma, self.order, 3, self.sums)
+
+    def applyDDiffK11(self, x, n, a, p):
+        return self.weight * apply_ddiff_k11(x, n, a, p, self.sigma, self.order)
+
+    def applyDDiffK12(self, x, n, a, p):
+        return self.weight * apply_ddiff_k12(x, n, a, p, self.sigma, self.order)
 
     def applyDivergence(self, x, a, firstVar=None):
         """kernelFunctions.py:510-530 with the dense K' matrix times ``weight`` (:298)."""

@@ -37,6 +37,7 @@ OP_KNORM = 7
 OP_MEASURE = 8
 OP_VARIFOLD = 9
 OP_DIVERGENCE = 10
+OP_DIFF_K = 11
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -101,6 +102,8 @@ SIGNATURES = {
     "lddmm_apply_ddiff_k11and12_workspace_bytes": (_SZ, [_I64, _I]),
     "lddmm_apply_ddiff_k11and12": (_I, [_P, _P, _P, _P, _I64, _I, _I, _I, _D, _I, _P, _P,
                                         _SZ, _P]),
+    "lddmm_apply_k_mat": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _D, _P, _P]),
+    "lddmm_apply_diff_k_mat": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _D, _P, _P]),
     "lddmm_surface_geometry": (_I, [_P, _P, _I64, _P, _P, _P]),
     "lddmm_curve_geometry": (_I, [_P, _P, _I64, _I, _P, _P, _P]),
     "lddmm_surface_vertex_gradient": (_I, [_P, _P, _P, _P, _P, _P, _I64, _D, _P, _P]),

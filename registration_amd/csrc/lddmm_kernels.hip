@@ -825,7 +825,38 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ksum_mfma(const lddmm_seg_t *__
 // ---------------------------------------------------------------------------
 // applyDDiffK11and12 pair kernel: targets = sources = y; a1 = w0, a2 = w1, p = w2.
 //   acc += <a1_k, a2_l> (k2 g <dx', dp> dx' - k1 dp),  out = acc / sigma^2.
+// Form (seg.na): 0 = applykdiff11and12, dp = p_k - p_l (Fortran tables,
+// kernelFunctions.f90:430-496); 1 = applyDDiffK11, dp = p_k, and 2 = applyDDiffK12,
+// dp = -p_l (the dense siblings, kernelFunctions.py:459-506, whose K' and K''
+// matrices use the exact-double Laplacian polynomials lapPolDiff / lapPolDiff2,
+// :75-104).  Forms 1 and 2 add up to form 0.
 // ---------------------------------------------------------------------------
+__constant__ double LAP_R_EXACT[5][3] = {
+    {0.0, 0.0, 0.0},  // orders 0, 1 have no second derivative (rejected on the host)
+    {0.0, 0.0, 0.0},
+    {1.0 / 3.0, 0.0, 0.0},
+    {1.0 / 15.0, 1.0 / 15.0, 0.0},
+    {3.0 / 105.0, 3.0 / 105.0, 1.0 / 105.0}};
+__constant__ double LAP_Q_EXACT[5][4] = {
+    {0.0, 0.0, 0.0, 0.0},  // order 0 has no first derivative (rejected on the host)
+    {1.0, 0.0, 0.0, 0.0},
+    {1.0 / 3.0, 1.0 / 3.0, 0.0, 0.0},
+    {3.0 / 15.0, 3.0 / 15.0, 1.0 / 15.0, 0.0},
+    {15.0 / 105.0, 15.0 / 105.0, 6.0 / 105.0, 1.0 / 105.0}};
+
+// The dense paths' exact-double derivative polynomials in place of the Fortran tables.
+template <typename R, int KT>
+__device__ inline void use_exact_tables(KEval<R, KT> &kev, int order) {
+  if constexpr (KT == LDDMM_KERNEL_LAPLACIAN) {
+    kev.q0 = (R)LAP_Q_EXACT[order][0];
+    kev.q1 = (R)LAP_Q_EXACT[order][1];
+    kev.q2 = (R)LAP_Q_EXACT[order][2];
+    kev.q3 = (R)LAP_Q_EXACT[order][3];
+    kev.c0 = (R)LAP_R_EXACT[order][0];
+    kev.c1 = (R)LAP_R_EXACT[order][1];
+    kev.c2 = (R)LAP_R_EXACT[order][2];
+  }
+}
 template <typename R, int KT, int D>
 __global__ __launch_bounds__(BLOCK) void k_pairs_ddk(const lddmm_seg_t *__restrict__ segs,
                                                     int nseg, int prec,
@@ -851,6 +882,10 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ddk(const lddmm_seg_t *__restri
                                            : 1.0 / (sg->sigma * sg->sigma));
   KEval<R, KT> kev;
   kev.init(sg->order, sg->sigma);
+  const int form = sg->na;
+  if (form != 0) use_exact_tables(kev, sg->order);
+  // dp = tp - sp with tp = alpha p_k, sp = beta p_l (alpha, beta in {0, 1} by form)
+  const double alpha = form == 2 ? 0.0 : 1.0, beta = form == 1 ? 0.0 : 1.0;
 
   R ty[RPT][D], ta[RPT][D], tp[RPT][D];
   int64_t rows[RPT];
@@ -863,7 +898,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ddk(const lddmm_seg_t *__restri
     for (int c = 0; c < D; ++c) {
       ty[r][c] = (R)(sg->y[kk * D + c] * scale);
       ta[r][c] = (R)sg->w0[kk * D + c];
-      tp[r][c] = (R)sg->w2[kk * D + c];
+      tp[r][c] = (R)(alpha * sg->w2[kk * D + c]);
     }
   }
   R sD[RPT][D], cD[RPT][D];
@@ -884,7 +919,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ddk(const lddmm_seg_t *__restri
       for (int c = 0; c < D; ++c) {
         s_pos[c][tid] = ok ? (R)(sg->y[l * D + c] * scale) : (R)0;
         s_a[c][tid] = ok ? (R)sg->w1[l * D + c] : (R)0;
-        s_p[c][tid] = ok ? (R)sg->w2[l * D + c] : (R)0;
+        s_p[c][tid] = ok ? (R)(beta * sg->w2[l * D + c]) : (R)0;
       }
     }
     __syncthreads();
@@ -1202,6 +1237,137 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
 }
 
 // ---------------------------------------------------------------------------
+// applyDiffK family (kernelFunctions.f90:116-290), targets = sources = y,
+// a1 = w0, a2 = w1 (n, d):
+//   out_k = sum_l 2K'(r^2) <y_k - y_l, b_kl> a2_l,  b_kl = a1_k (form 1, applykdiff1),
+//   -a1_l (form 2, applykdiff2), a1_k - a1_l (form 3, applykdiff1and2); form = seg.na.
+// Fortran tables and cut-off (standard KEval).  The reference's applykdiff1/2 add
+// into their f2py output array (f = f + df), which f2py zero-fills; out is the sum.
+// ---------------------------------------------------------------------------
+template <typename R, int KT, int D>
+__global__ __launch_bounds__(BLOCK) void k_pairs_dk(const lddmm_seg_t *__restrict__ segs,
+                                                   int nseg, int prec,
+                                                   double *__restrict__ part) {
+  constexpr int RPT = rows_per_thread<R>();
+  constexpr bool KAHAN = std::is_same<R, float>::value;
+  __shared__ R s_pos[D][TS];
+  __shared__ R s_b[D][TS];   // source part of b: a1_l (forms 2, 3), else 0
+  __shared__ R s_w[D][TS];   // a2_l
+
+  const int tid = threadIdx.x;
+  const int64_t item = blockIdx.x;
+  const int si = find_segment(segs, nseg, item, 0);
+  const lddmm_seg_t *sg = segs + si;
+  const int64_t ny = sg->ny;
+  const int row_tiles = sg->row_tiles;
+  const int64_t it = item - sg->item0;
+  const int rt = (int)(it % row_tiles);
+  const int cg = (int)(it / row_tiles);
+  const double scale = coord_scale(prec, KT, sg->sigma);
+  KEval<R, KT> kev;
+  kev.init(sg->order, sg->sigma);
+  const double tb = (sg->na & 1) ? 1.0 : 0.0, sb = (sg->na & 2) ? 1.0 : 0.0;
+
+  double yd[RPT][D];
+  R ty[RPT][D], tbv[RPT][D];
+  int64_t rows[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = (int64_t)rt * (BLOCK * RPT) + r * BLOCK + tid;
+    rows[r] = k;
+    int64_t kk = k < ny ? k : ny - 1;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      yd[r][c] = sg->y[kk * D + c] * scale;
+      ty[r][c] = (R)yd[r][c];
+      tbv[r][c] = (R)(tb * sg->w0[kk * D + c]);
+    }
+  }
+  R sD[RPT][D], cD[RPT][D];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int c = 0; c < D; ++c) { sD[r][c] = 0; cD[r][c] = 0; }
+
+  const int64_t l0 = (int64_t)cg * sg->chunk_len;
+  int64_t l1 = l0 + sg->chunk_len;
+  if (l1 > ny) l1 = ny;
+  for (int64_t base = l0; base < l1; base += TS) {
+    double org[D];
+    const int64_t mid = base + ((l1 - base < TS ? l1 - base : TS) >> 1);
+#pragma unroll
+    for (int c = 0; c < D; ++c) org[c] = KAHAN ? sg->y[mid * D + c] * scale : 0.0;
+    if constexpr (KAHAN) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) ty[r][c] = (R)(yd[r][c] - org[c]);
+    }
+    __syncthreads();
+    {
+      int64_t l = base + tid;
+      bool ok = l < l1;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        s_pos[c][tid] = ok ? (R)(sg->y[l * D + c] * scale - org[c]) : (R)0;
+        s_b[c][tid] = ok ? (R)(sb * sg->w0[l * D + c]) : (R)0;
+        s_w[c][tid] = ok ? (R)sg->w1[l * D + c] : (R)0;
+      }
+    }
+    __syncthreads();
+    for (int j0 = 0; j0 < TS; j0 += KF) {
+      R tD[RPT][D];
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) tD[r][c] = 0;
+#pragma unroll 4
+      for (int j = j0; j < j0 + KF; ++j) {
+        R sx[D], sbv[D], sw[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) { sx[c] = s_pos[c][j]; sbv[c] = s_b[c][j]; sw[c] = s_w[c][j]; }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          R r2 = KEval<R, KT>::R2_0, dot = 0;
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            const R dx = ty[r][c] - sx[c];
+            r2 = dx * dx + r2;
+            dot = dx * (tbv[r][c] - sbv[c]) + dot;
+          }
+          R k0, k1, k2;
+          kev(r2, k0, k1, k2);
+          const R f = k1 * dot;
+#pragma unroll
+          for (int c = 0; c < D; ++c) tD[r][c] = f * sw[c] + tD[r][c];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          if constexpr (KAHAN) {
+            R y = tD[r][c] - cD[r][c];
+            R t = sD[r][c] + y;
+            cD[r][c] = (t - sD[r][c]) - y;
+            sD[r][c] = t;
+          } else {
+            sD[r][c] += tD[r][c];
+          }
+        }
+    }
+  }
+  double *pp = part + sg->part0 + (int64_t)cg * ny * D;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int64_t k = rows[r];
+    if (k >= ny) continue;
+#pragma unroll
+    for (int c = 0; c < D; ++c) pp[k * D + c] = (double)sD[r][c] - (double)cD[r][c];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Divergence of the velocity field v = K(., x) a at the targets y:
 //   div_k = sum_l 2K'(r^2) <a_l, y_k - x_l>
 // (Kernel.applyDivergence, kernelFunctions.py:510-530).  The reference forms the
@@ -1212,13 +1378,6 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
 // not matter here: the self term carries y_k - x_k = 0.  One FP64 partial per
 // row and chunk; no culling (an output-only path).
 // ---------------------------------------------------------------------------
-__constant__ double LAP_Q_EXACT[5][4] = {
-    {0.0, 0.0, 0.0, 0.0},  // order 0 has no first derivative (rejected on the host)
-    {1.0, 0.0, 0.0, 0.0},
-    {1.0 / 3.0, 1.0 / 3.0, 0.0, 0.0},
-    {3.0 / 15.0, 3.0 / 15.0, 1.0 / 15.0, 0.0},
-    {15.0 / 105.0, 15.0 / 105.0, 6.0 / 105.0, 1.0 / 105.0}};
-
 template <typename R, int KT, int D>
 __global__ __launch_bounds__(BLOCK) void k_pairs_div(const lddmm_seg_t *__restrict__ segs,
                                                     int nseg, int prec,
@@ -1241,12 +1400,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_div(const lddmm_seg_t *__restri
   const double scale = coord_scale(prec, KT, sg->sigma);
   KEval<R, KT> kev;
   kev.init(sg->order, sg->sigma);
-  if constexpr (KT == LDDMM_KERNEL_LAPLACIAN) {
-    kev.q0 = (R)LAP_Q_EXACT[sg->order][0];
-    kev.q1 = (R)LAP_Q_EXACT[sg->order][1];
-    kev.q2 = (R)LAP_Q_EXACT[sg->order][2];
-    kev.q3 = (R)LAP_Q_EXACT[sg->order][3];
-  }
+  use_exact_tables(kev, sg->order);
 
   double yd[RPT][D];
   R ty[RPT][D];
@@ -1504,6 +1658,11 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
     }
   } else if constexpr (OP == LDDMM_OP_DIVERGENCE) {
     if (valid) sg->out0[k] = dscale * accK0[0];
+  } else if constexpr (OP == LDDMM_OP_DIFF_K) {
+    if (valid) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) sg->out0[k * D + c] = dscale * accD[c];
+    }
   } else if constexpr (OP == LDDMM_OP_DDIFF_K11_12) {
     if (valid) {
       const double inv = 1.0 / (sigma * sigma);
@@ -1772,7 +1931,7 @@ int op_nred(int op) {
   }
 }
 
-bool op_valid(int op) { return op >= LDDMM_OP_APPLY_K && op <= LDDMM_OP_DIVERGENCE; }
+bool op_valid(int op) { return op >= LDDMM_OP_APPLY_K && op <= LDDMM_OP_DIFF_K; }
 
 // Launch-uniform shape of a planned operation.
 struct Shape {
@@ -1821,6 +1980,9 @@ int op_shape(int op, int d, const lddmm_seg_t &s0, Shape *sh) {
     case LDDMM_OP_DIVERGENCE:
       *sh = {1, 0, true, 1};
       return LDDMM_OK;
+    case LDDMM_OP_DIFF_K:
+      *sh = {d, 1, false, d};
+      return LDDMM_OK;
   }
   return LDDMM_EINVAL;
 }
@@ -1854,7 +2016,7 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     if (s.kernel == LDDMM_KERNEL_LAPLACIAN) {
       if (s.order < 0 || s.order > 4) return LDDMM_EINVAL;
       bool deriv = op == LDDMM_OP_DIFF_KT || op == LDDMM_OP_COVECTOR_STEP ||
-                   op == LDDMM_OP_DIVERGENCE ||
+                   op == LDDMM_OP_DIVERGENCE || op == LDDMM_OP_DIFF_K ||
                    ((op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE ||
                      op == LDDMM_OP_VARIFOLD) && s.out1);
       if (deriv && s.order < 1) return LDDMM_EINVAL;
@@ -1913,6 +2075,11 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
         break;
       case LDDMM_OP_DDIFF_K11_12:
         if (!s.w0 || !s.w1 || !s.w2 || !s.out0) return LDDMM_EINVAL;
+        if (s.na < 0 || s.na > 2) return LDDMM_EINVAL;  // form
+        nx0 = s.ny;
+        break;
+      case LDDMM_OP_DIFF_K:
+        if (!s.w0 || !s.w1 || !s.out0 || s.na < 1 || s.na > 3) return LDDMM_EINVAL;
         nx0 = s.ny;
         break;
       case LDDMM_OP_DIVERGENCE:
@@ -2009,6 +2176,23 @@ int dispatch_pairs_ddk(int prec, int kern, const lddmm_seg_t *segs, int nseg, in
       hipLaunchKernelGGL((k_pairs_ddk<double, LDDMM_KERNEL_LAPLACIAN, D>), g, b, 0, st, segs, nseg, prec, part);
   }
   return LDDMM_OK;
+}
+
+template <int D>
+void dispatch_pairs_dk(int prec, int kern, const lddmm_seg_t *segs, int nseg, int64_t n_items,
+                       double *part, hipStream_t st) {
+  dim3 g((unsigned)n_items), b(BLOCK);
+  if (prec == LDDMM_PREC_FP32C) {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      hipLaunchKernelGGL((k_pairs_dk<float, LDDMM_KERNEL_GAUSS, D>), g, b, 0, st, segs, nseg, prec, part);
+    else
+      hipLaunchKernelGGL((k_pairs_dk<float, LDDMM_KERNEL_LAPLACIAN, D>), g, b, 0, st, segs, nseg, prec, part);
+  } else {
+    if (kern == LDDMM_KERNEL_GAUSS)
+      hipLaunchKernelGGL((k_pairs_dk<double, LDDMM_KERNEL_GAUSS, D>), g, b, 0, st, segs, nseg, prec, part);
+    else
+      hipLaunchKernelGGL((k_pairs_dk<double, LDDMM_KERNEL_LAPLACIAN, D>), g, b, 0, st, segs, nseg, prec, part);
+  }
 }
 
 template <int D>
@@ -2132,6 +2316,10 @@ int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, 
         return LDDMM_OK;
       }
       return LDDMM_EINVAL;
+    case LDDMM_OP_DIFF_K:
+      dispatch_pairs_dk<D>(prec, kern, segs, nseg, n_items, part, st);
+      launch_fin<LDDMM_OP_DIFF_K, D, D, false, 1>(segs, nseg, prec, kern, n_fin, part, red, st);
+      return LDDMM_OK;
     case LDDMM_OP_DIVERGENCE:
       dispatch_pairs_div<D>(prec, kern, segs, nseg, n_items, part, st);
       launch_fin<LDDMM_OP_DIVERGENCE, D, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
@@ -2353,6 +2541,96 @@ extern "C" int lddmm_apply_ddiff_k11and12(const double *x, const double *a1, con
   if (need > work_bytes) return LDDMM_EWORKSPACE;
   return lddmm_segments_run(LDDMM_OP_DDIFF_K11_12, prec, d, &s, nullptr, 1, items, fins, work,
                             work_bytes, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-weight sums (applykmat / applykdiffmat, kernelFunctions.f90:499-610):
+// one weight per (target, source) pair, beta row-major (ny, nx[, m]).  Reading
+// beta (8 ny nx m bytes) bounds them, so each target row is one wave whose lanes
+// stride over the sources (coalesced beta reads), everything in FP64 with the
+// Fortran tables and cut-off, and the 64 lane sums are combined by a fixed
+// butterfly (deterministic).
+// ---------------------------------------------------------------------------
+template <int KT, int D, bool DIFF>
+__global__ __launch_bounds__(256) void k_kmat(const double *__restrict__ y, const double *__restrict__ x,
+                                             const double *__restrict__ beta, int64_t ny, int64_t nx,
+                                             int m, int order, double sigma, double *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= ny) return;  // wave-uniform; no barriers below
+  KEval<double, KT> kev;
+  kev.init(order, sigma);
+  double yk[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) yk[c] = y[k * D + c];
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int mw = DIFF ? 1 : m;
+  const double *brow = beta + k * nx * mw;
+  for (int64_t l = lane; l < nx; l += 64) {
+    double dx[D], r2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      dx[c] = yk[c] - x[l * D + c];
+      r2 += dx[c] * dx[c];
+    }
+    double k0, k1, k2;
+    kev(r2, k0, k1, k2);
+    if constexpr (DIFF) {
+      const double f = k1 * brow[l];
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc[c] += f * dx[c];
+    } else {
+      for (int c = 0; c < mw; ++c) acc[c] += k0 * brow[l * mw + c];
+    }
+  }
+  const int nout = DIFF ? D : m;
+  for (int c = 0; c < nout; ++c) {
+    double v = acc[c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    // 2K'(r^2) = -k1 / sigma^2 in the FP64 kernel evaluation
+    if (lane == 0) out[k * nout + c] = DIFF ? -v / (sigma * sigma) : v;
+  }
+}
+
+namespace {
+template <bool DIFF>
+int launch_kmat(const double *y, const double *x, const double *beta, int64_t ny, int64_t nx,
+                int d, int m, int kernel, int order, double sigma, double *out, void *stream) {
+  if (!y || !x || !beta || !out || ny < 1 || nx < 1 || (d != 2 && d != 3)) return LDDMM_EINVAL;
+  if (!DIFF && (m < 1 || m > 3)) return LDDMM_EINVAL;
+  if (!(sigma > 0.0) || !isfinite(sigma)) return LDDMM_EINVAL;
+  if (kernel == LDDMM_KERNEL_LAPLACIAN) {
+    if (order < (DIFF ? 1 : 0) || order > 4) return LDDMM_EINVAL;
+  } else if (kernel != LDDMM_KERNEL_GAUSS) {
+    return LDDMM_EINVAL;
+  }
+  const int64_t blocks = ceil_div(ny, 4);
+  if (blocks > INT32_MAX) return LDDMM_EINVAL;
+  dim3 g((unsigned)blocks), b(256);
+  hipStream_t st = (hipStream_t)stream;
+#define LDDMM_KMAT(KT, DD) \
+  hipLaunchKernelGGL((k_kmat<KT, DD, DIFF>), g, b, 0, st, y, x, beta, ny, nx, m, order, sigma, out)
+  if (kernel == LDDMM_KERNEL_GAUSS) {
+    if (d == 2) LDDMM_KMAT(LDDMM_KERNEL_GAUSS, 2); else LDDMM_KMAT(LDDMM_KERNEL_GAUSS, 3);
+  } else {
+    if (d == 2) LDDMM_KMAT(LDDMM_KERNEL_LAPLACIAN, 2); else LDDMM_KMAT(LDDMM_KERNEL_LAPLACIAN, 3);
+  }
+#undef LDDMM_KMAT
+  return hip_check(hipGetLastError());
+}
+}  // namespace
+
+extern "C" int lddmm_apply_k_mat(const double *y, const double *x, const double *beta, int64_t ny,
+                                 int64_t nx, int d, int m, int kernel, int order, double sigma,
+                                 double *out, void *stream) {
+  return launch_kmat<false>(y, x, beta, ny, nx, d, m, kernel, order, sigma, out, stream);
+}
+
+extern "C" int lddmm_apply_diff_k_mat(const double *y, const double *x, const double *beta,
+                                      int64_t ny, int64_t nx, int d, int kernel, int order,
+                                      double sigma, double *out, void *stream) {
+  return launch_kmat<true>(y, x, beta, ny, nx, d, 1, kernel, order, sigma, out, stream);
 }
 
 // ---------------------------------------------------------------------------

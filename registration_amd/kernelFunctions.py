@@ -132,9 +132,11 @@ class Kernel(KernelSpec):
 
     # ---------------------------------------------------------------------------------
     def applyK(self, x, a, firstVar=None, grid=None, matrixWeights=False):
-        """K(x,x)a or K(firstVar,x)a (kernelFunctions.py:310-369 -> kernelFunctions.f90:1-54)."""
+        """K(x,x)a or K(firstVar,x)a (kernelFunctions.py:310-369 -> kernelFunctions.f90:1-54);
+        with ``matrixWeights`` a is (n_y, n_x, m), one weight per pair (applykmat,
+        kernelFunctions.f90:499-550)."""
         if matrixWeights:
-            raise NotImplementedError("matrix weights (applykmat) are not part of the GPU path")
+            return self._apply_k_mat(x, a, firstVar)
         xd, np_out = _wrap(x)
         ad, _ = _wrap(a)
         yd = xd if firstVar is None else _wrap(firstVar)[0]
@@ -169,6 +171,31 @@ class Kernel(KernelSpec):
                 else:
                     z = z + self.w1 * s * (yy @ Et.T)
         return _unwrap(z, np_out)
+
+    def _apply_k_mat(self, x, beta, firstVar):
+        xd, np_out = _wrap(x)
+        bd, _ = _wrap(beta)
+        yd = xd if firstVar is None else _wrap(firstVar)[0]
+        if bd.dim() != 3 or bd.shape[0] != yd.shape[0] or bd.shape[1] != xd.shape[0]:
+            raise ValueError("applyK(matrixWeights=True): weights must be (n_y, n_x, m)")
+        if self.affine != 'none':
+            raise NotImplementedError("kernel-level affine terms with matrix weights are not "
+                                      "defined by the reference (its affine branch expects (n, m) "
+                                      "weights)")
+        if self.name == 'none':
+            return _unwrap(torch.zeros((yd.shape[0], bd.shape[2]), dtype=torch.float64,
+                                       device=xd.device), np_out)
+        return _unwrap(apply_k_mat_dev(yd, xd, bd, self), np_out)
+
+    def applyDiffKmat(self, x, beta, firstVar=None):
+        """sum_l 2K'(r^2) (y_k - x_l) beta[k, l] at y = firstVar or x (kernelFunctions.py:405-411
+        -> kernelFunctions.f90:552-610)."""
+        xd, np_out = _wrap(x)
+        bd, _ = _wrap(beta)
+        yd = xd if firstVar is None else _wrap(firstVar)[0]
+        if bd.dim() != 2 or bd.shape[0] != yd.shape[0] or bd.shape[1] != xd.shape[0]:
+            raise ValueError("applyDiffKmat: beta must be (n_y, n_x)")
+        return _unwrap(apply_diff_k_mat_dev(yd, xd, bd, self), np_out)
 
     def applyDiffKT(self, x, a1, a2, firstVar=None):
         """sum_l 2 K'(|y_k-x_l|^2) (y_k - x_l) sum_j <a1[j,k], a2[j,l]>
@@ -221,6 +248,62 @@ class Kernel(KernelSpec):
         if self.affine == 'affine':
             c = torch.as_tensor(self.center, dtype=torch.float64, device=xd.device)
             z = z + self.w1 * ((yd - c) * ad).sum(dim=1)
+        return _unwrap(z, np_out)
+
+    def applyDiffK(self, x, a1, a2):
+        """sum_l 2K'(r^2) <x_k - x_l, a1_k> a2_l (kernelFunctions.py:387-389 ->
+        kernelFunctions.f90:116-173)."""
+        return self._diff_k(x, a1, a2, 1)
+
+    def applyDiffK2(self, x, a1, a2):
+        """-sum_l 2K'(r^2) <x_k - x_l, a1_l> a2_l (kernelFunctions.py:392-394 ->
+        kernelFunctions.f90:175-231)."""
+        return self._diff_k(x, a1, a2, 2)
+
+    def applyDiffK1and2(self, x, a1, a2):
+        """sum_l 2K'(r^2) <x_k - x_l, a1_k - a1_l> a2_l (kernelFunctions.py:396-398 ->
+        kernelFunctions.f90:234-290)."""
+        return self._diff_k(x, a1, a2, 3)
+
+    def _diff_k(self, x, a1, a2, form):
+        xd, np_out = _wrap(x)
+        a1d, a2d = _wrap(a1)[0], _wrap(a2)[0]
+        if a1d.shape != xd.shape or a2d.shape != xd.shape:
+            raise ValueError("applyDiffK: x, a1 and a2 must have one shape (n, d)")
+        if self.name == 'none':
+            return _unwrap(torch.zeros_like(xd), np_out)
+        return _unwrap(apply_diff_k_dev(xd, a1d, a2d, self, form), np_out)
+
+    def applyDDiffK11(self, x, n, a, p):
+        """sum_l 2<n_k, a_l> (2K''<dx, p_k> dx + K' p_k), dx = x_k - x_l: the derivative of
+        applyDiffKT(x, [n], [a]) along p at the targets (kernelFunctions.py:459-478).  The
+        reference's dense K', K'' matrices (exact-double Laplacian polynomials, scaled by
+        ``weight``) become a tiled pair sum."""
+        return self._ddiff_dense(x, n, a, p, 1)
+
+    def applyDDiffK12(self, x, n, a, p):
+        """-sum_l 2<n_k, a_l> (2K''<dx, p_l> dx + K' p_l): the same derivative at the sources
+        (kernelFunctions.py:481-506), plus w1 (n a^T) p for a kernel-level affine part."""
+        z = self._ddiff_dense(x, n, a, p, 2)
+        if self.affine == 'affine':
+            xd, np_out = _wrap(x)
+            nd, ad, pd = (_wrap(v)[0] for v in (n, a, p))
+            zd = _wrap(z)[0] + self.w1 * (nd @ (ad.T @ pd))
+            return _unwrap(zd, np_out)
+        if self.affine == 'euclidean':
+            raise NotImplementedError(
+                "the reference's euclidean branch of applyDDiffK12 multiplies an (n, d) array by "
+                "a (d, d) basis element (kernelFunctions.py:503-505), defined only when n = d")
+        return z
+
+    def _ddiff_dense(self, x, n, a, p, form):
+        xd, np_out = _wrap(x)
+        nd, ad, pd = (_wrap(v)[0] for v in (n, a, p))
+        if self.name == 'none':
+            return _unwrap(torch.zeros_like(xd), np_out)
+        z = apply_ddiff_k11and12_dev(xd, nd, ad, pd, self, form=form)
+        if self.weight != 1.0:
+            z = z * self.weight
         return _unwrap(z, np_out)
 
     def applyDDiffK11and12(self, x, n, a, p):
@@ -289,13 +372,21 @@ def apply_diff_kt_dev(y, x, a1, a2, K: KernelSpec) -> torch.Tensor:
     return out[:, :d0].contiguous()
 
 
-def apply_ddiff_k11and12_dev(x, a1, a2, p, K: KernelSpec) -> torch.Tensor:
+def apply_ddiff_k11and12_dev(x, a1, a2, p, K: KernelSpec, form: int = 0) -> torch.Tensor:
+    """form 0: applyDDiffK11and12 (one-call entry point); 1 / 2: the dense siblings
+    applyDDiffK11 / applyDDiffK12 (segmented launch with seg.na = form)."""
     lib = _lib.load()
     d0 = x.shape[1]
     _check_dim(d0)
     x, a1, a2, p = _pad_dim(x, a1, a2, p)
     n, d = x.shape
     out = torch.empty((n, d), dtype=torch.float64, device=x.device)
+    if form:
+        a1, a2, p = a1.contiguous(), a2.contiguous(), p.contiguous()
+        r = rt.seg(y=x, w0=a1, w1=a2, w2=p, out0=out, ny=n, sigma=float(K.sigma),
+                   kernel=K.kernel_code, order=int(K.order), na=form)
+        rt.launch(_lib.OP_DDIFF_K11_12, K.prec_code, d, r)
+        return out[:, :d0].contiguous()
     nbytes = lib.lddmm_apply_ddiff_k11and12_workspace_bytes(n, d)
     ws = rt.workspace(nbytes)
     _lib.check(lib.lddmm_apply_ddiff_k11and12(x.data_ptr(), a1.data_ptr(), a2.data_ptr(),
@@ -317,3 +408,51 @@ def apply_divergence_dev(y, x, a, K: KernelSpec) -> torch.Tensor:
                kernel=K.kernel_code, order=int(K.order))
     rt.launch(_lib.OP_DIVERGENCE, K.prec_code, y.shape[1], r)
     return out
+
+
+def apply_diff_k_dev(x, a1, a2, K: KernelSpec, form: int) -> torch.Tensor:
+    """applyDiffK (form 1), applyDiffK2 (2), applyDiffK1and2 (3) on device tensors (n, d)."""
+    d0 = x.shape[1]
+    _check_dim(d0)
+    x, a1, a2 = _pad_dim(x, a1, a2)
+    a1, a2 = a1.contiguous(), a2.contiguous()
+    n, d = x.shape
+    out = torch.empty((n, d), dtype=torch.float64, device=x.device)
+    r = rt.seg(y=x, w0=a1, w1=a2, out0=out, ny=n, sigma=float(K.sigma), kernel=K.kernel_code,
+               order=int(K.order), na=form)
+    rt.launch(_lib.OP_DIFF_K, K.prec_code, d, r)
+    return out[:, :d0].contiguous()
+
+
+def apply_k_mat_dev(y, x, beta, K: KernelSpec) -> torch.Tensor:
+    """applyK with matrix weights beta (n_y, n_x, m) on device tensors; FP64 in both precisions
+    (the sum is bound by reading beta)."""
+    lib = _lib.load()
+    _check_dim(x.shape[1])
+    y, x = _pad_dim(y, x)
+    beta = beta.contiguous()
+    ny, nx, m = beta.shape
+    if m > 3:
+        return torch.cat([apply_k_mat_dev(y, x, beta[:, :, j:j + 3], K) for j in range(0, m, 3)],
+                         dim=1)
+    out = torch.empty((ny, m), dtype=torch.float64, device=x.device)
+    _lib.check(lib.lddmm_apply_k_mat(y.data_ptr(), x.data_ptr(), beta.data_ptr(), ny, nx,
+                                     x.shape[1], m, K.kernel_code, int(K.order), float(K.sigma),
+                                     out.data_ptr(), rt.stream_handle()), "lddmm_apply_k_mat")
+    return out
+
+
+def apply_diff_k_mat_dev(y, x, beta, K: KernelSpec) -> torch.Tensor:
+    """applyDiffKmat on device tensors (beta (n_y, n_x)); FP64."""
+    lib = _lib.load()
+    d0 = x.shape[1]
+    _check_dim(d0)
+    y, x = _pad_dim(y, x)
+    beta = beta.contiguous()
+    ny, nx = beta.shape
+    out = torch.empty((ny, x.shape[1]), dtype=torch.float64, device=x.device)
+    _lib.check(lib.lddmm_apply_diff_k_mat(y.data_ptr(), x.data_ptr(), beta.data_ptr(), ny, nx,
+                                          x.shape[1], K.kernel_code, int(K.order),
+                                          float(K.sigma), out.data_ptr(), rt.stream_handle()),
+               "lddmm_apply_diff_k_mat")
+    return out[:, :d0].contiguous()

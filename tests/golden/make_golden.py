@@ -1,7 +1,7 @@
 """Writes tests/golden/kernel_sums.npz and tests/golden/kernel_mat_sums.npz: inputs and
 outputs of the reference's own Fortran kernel sums (py-lddmm/kernelFunctions.f90 applyk,
-applykdifft, applykdiff11and12; and the matrix-weight sums applykmat, applykdiffmat of the
-varifold data term), compiled by oracle/Makefile (`make -C oracle ref`) into
+applykdifft, applykdiff11and12; the matrix-weight sums applykmat, applykdiffmat of the
+varifold data term; applykdiff1, applykdiff2, applykdiff1and2), compiled by oracle/Makefile (`make -C oracle ref`) into
 oracle/_ref/libkff_ref.so in this container.
 
 Run:  python tests/golden/make_golden.py
@@ -76,7 +76,8 @@ MAT_CASES = [
 
 
 def mat_cases():
-    """Golden vectors of the matrix-weight sums (own seed, so kernel_sums.npz is unchanged)."""
+    """Golden vectors of the matrix-weight sums and of the applyDiffK family (own seed, so
+    kernel_sums.npz is unchanged)."""
     rng = np.random.default_rng(20261204)
     out = {}
     for name, d, ny, nx, m, order, sigma in MAT_CASES:
@@ -89,6 +90,13 @@ def mat_cases():
         out[f"{name}/applykmat"] = orc.apply_k_mat(y, x, beta, sigma, order, sums="reference")
         out[f"{name}/applykdiffmat"] = orc.apply_diff_k_mat(y, x, beta[..., 0], sigma, order,
                                                             sums="reference")
+        # the applyDiffK family on the target set (n = ny, d, with a1, a2 of that shape)
+        a1 = rng.standard_normal((ny, d))
+        a2 = rng.standard_normal((ny, d))
+        out[f"{name}/dk_a1"], out[f"{name}/dk_a2"] = a1, a2
+        for form in (1, 2, 3):
+            out[f"{name}/applykdiff_{form}"] = orc.apply_diff_k(y, a1, a2, sigma, order, form,
+                                                                sums="reference")
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kernel_mat_sums.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, len(out), "arrays")

@@ -519,3 +519,51 @@ def test_jacobian_flow_tracks_log_det_of_the_flow_map():
     # without a point set the Jacobian is taken at the moving points
     xt2, Jx = orc.landmarkDirectEvolutionEuler(x0, at, K, withJacobian=True)
     assert Jx.shape == (T + 1, 20) and np.allclose(xt2, xt)
+
+
+# ---------------------------------------------------------------- applyDiffK family, dense DDiffK11 / 12
+def test_diff_k_family_matches_reference_golden(golden_mat):
+    for name, c in golden_mat.items():
+        d, ny, nx, m, order, sigma = c["params"]
+        for form in (1, 2, 3):
+            f = orc.apply_diff_k(c["y"], c["dk_a1"], c["dk_a2"], sigma, int(order), form)
+            ref = c[f"applykdiff_{form}"]
+            np.testing.assert_allclose(f, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max(),
+                                       err_msg=f"{name} form {form}")
+
+
+def test_diff_k_forms_add_up_and_match_diff_kt():
+    """applyDiffK1and2 = applyDiffK + applyDiffK2, and applyDiffK with a1 = e_c equals
+    applyDiffKT's c-th column form by symmetry of the pair sum."""
+    rng = np.random.default_rng(3)
+    x, a1, a2 = rng.standard_normal((40, 3)), rng.standard_normal((40, 3)), rng.standard_normal((40, 3))
+    for order in (10, 2):
+        f1, f2, f3 = (orc.apply_diff_k(x, a1, a2, 0.6, order, f) for f in (1, 2, 3))
+        np.testing.assert_allclose(f3, f1 + f2, rtol=1e-12, atol=1e-13 * np.abs(f3).max())
+        # applyDiffK row k, component c: sum_l 2K' <dx, a1_k> a2_l[c]; applyDiffKT with
+        # a1' = a2[:, c] (sources) and a1_k as the target weight gives the same contraction
+        for c in range(3):
+            w = np.zeros((40, 3))
+            w[:, 0] = a2[:, c]
+            g = orc.apply_diff_kt(x, x, a1[None, :, :1] * 0 + 1, w[None, :, :1], 0.6, order)
+            np.testing.assert_allclose((g * a1).sum(axis=1), f1[:, c], rtol=1e-11,
+                                       atol=1e-12 * np.abs(f1).max())
+
+
+@pytest.mark.parametrize("order,sigma,tol", [(10, 0.7, 1e-12), (3, 0.5, 1e-6), (4, 0.6, 1e-6)])
+def test_dense_ddiff_k11_k12(order, sigma, tol):
+    """The dense siblings add up to applyDDiffK11and12 (Fortran) and are the target / source
+    derivatives of applyDiffKT along p (central FD)."""
+    rng = np.random.default_rng(order)
+    x, n, a, p = (rng.standard_normal((30, 3)) * s for s in (0.5, 1, 1, 1))
+    K = orc.Kernel('gauss' if order > 4 else 'laplacian', sigma=sigma, order=order)
+    k11, k12 = K.applyDDiffK11(x, n, a, p), K.applyDDiffK12(x, n, a, p)
+    both = K.applyDDiffK11and12(x, n, a, p)
+    np.testing.assert_allclose(k11 + k12, both, rtol=tol, atol=tol * np.abs(both).max())
+    h = 1e-6
+    fd11 = (K.applyDiffKT(x, n[None], a[None], firstVar=x + h * p)
+            - K.applyDiffKT(x, n[None], a[None], firstVar=x - h * p)) / (2 * h)
+    fd12 = (K.applyDiffKT(x + h * p, n[None], a[None], firstVar=x)
+            - K.applyDiffKT(x - h * p, n[None], a[None], firstVar=x)) / (2 * h)
+    np.testing.assert_allclose(k11, fd11, rtol=1e-6, atol=1e-6 * np.abs(fd11).max())
+    np.testing.assert_allclose(k12, fd12, rtol=1e-6, atol=1e-6 * np.abs(fd12).max())
