@@ -687,29 +687,33 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// FP32 K-weighted sums with the accumulation on the matrix cores.
+// FP32 K-weighted sums with the accumulation on the matrix cores (applyK, the
+// Euler flow step, RKHS products).
 //
 // v_mfma_f32_4x4x1f32 runs 16 independent 4x4x1 products per wave: for block
 // b = lane/4, lane 4b+i supplies row i of A, lane 4b+j supplies column j of B,
 // and lane 4b+j's result register v holds D[b][v][j] (layout checked on
-// gfx950).  Each lane owns one target row; per source l it computes
-// A = K(y_lane, x_l) on the VALU and supplies B = w_l[lane % 4] (weight column
-// j, zero for j >= M), so one MFMA per source and row set accumulates
-// D[b][v][j] += K(y_{4b+v}, x_l) w_l[j]: the three accumulation FMAs per pair
-// leave the VALU.  Runs of KF sources accumulate in D and are folded into
-// Kahan (sum, compensation) pairs exactly as in k_pairs.
+// gfx950).  Each lane owns one target row per row set; per source l it computes
+// A = K(y_lane, x_l) on the VALU (3 sub, 3 FMA, 1 exp) and supplies
+// B = w_l[lane % 4] (weight column j, zero for j >= M), so one MFMA per source
+// and row set accumulates D[b][v][j] += K(y_{4b+v}, x_l) w_l[j]: the three
+// accumulation FMAs per pair leave the VALU.  Same tiles, per-tile origin and
+// quarter culling as k_pairs; two independent accumulator chains per row set
+// (sources alternate between them), summed and folded into Kahan pairs once
+// per 64-source quarter.
 // ---------------------------------------------------------------------------
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <int KT, int D, int M>
-__global__ __launch_bounds__(BLOCK) void k_pairs_ksum_mfma(const lddmm_seg_t *__restrict__ segs,
-                                                          int nseg, int op, int prec,
-                                                          double *__restrict__ part) {
-  constexpr int RPT = 2;  // rows per lane in this kernel (independent MFMA chains)
+__global__ __launch_bounds__(BLOCK) void k_pairs_mfma(const lddmm_seg_t *__restrict__ segs,
+                                                     int nseg, int op, int prec,
+                                                     double *__restrict__ part) {
+  constexpr int RPT = 2;  // row sets per wave (64 rows each)
   constexpr int W = M;
   static_assert(M >= 1 && M <= 4, "MFMA K-sum handles up to 4 weight columns");
   __shared__ float s_pos[D][TS];
   __shared__ float s_wt[4][TS];
+  __shared__ float s_box[TS / 64][2 * D];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -732,16 +736,28 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ksum_mfma(const lddmm_seg_t *__
   KEval<float, KT> kev;
   kev.init(sg->order, sg->sigma);
 
-  // Each wave owns RPT x 64 consecutive target rows.
+  // Rows as in k_pairs: each wave owns RPT x 64 consecutive target rows.
   const int64_t wbase = (int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT);
+  double yd[RPT][D];
   float ty[RPT][D];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
     int64_t k = wbase + r * 64 + lane;
     int64_t kk = k < ny ? k : ny - 1;
 #pragma unroll
-    for (int c = 0; c < D; ++c) ty[r][c] = (float)(sg->y[kk * D + c] * scale);
+    for (int c = 0; c < D; ++c) yd[r][c] = sg->y[kk * D + c] * scale;
   }
+  double tlo[D], thi[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    double lo = yd[0][c], hi = yd[0][c];
+#pragma unroll
+    for (int r = 1; r < RPT; ++r) { lo = fmin(lo, yd[r][c]); hi = fmax(hi, yd[r][c]); }
+    tlo[c] = wave_min(lo);
+    thi[c] = wave_max(hi);
+  }
+  const float cut2 = (float)cull_r2(prec, KT, sg->sigma);
+
   float sK[RPT][4], cK[RPT][4];
 #pragma unroll
   for (int r = 0; r < RPT; ++r)
@@ -753,24 +769,76 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ksum_mfma(const lddmm_seg_t *__
   if (l1 > src.n) l1 = src.n;
 
   for (int64_t base = l0; base < l1; base += TS) {
+    double org[D];
+    const int64_t mid = base + ((l1 - base < TS ? l1 - base : TS) >> 1);
+#pragma unroll
+    for (int c = 0; c < D; ++c) org[c] = src.pos[mid * D + c] * scale;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+      for (int c = 0; c < D; ++c) ty[r][c] = (float)(yd[r][c] - org[c]);
     __syncthreads();
     {
       int64_t l = base + tid;
-      bool ok = l < l1;
+      float lo[D], hi[D];
+      if (l < l1) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) s_pos[c][tid] = ok ? (float)(src.pos[l * D + c] * scale) : 0.f;
+        for (int c = 0; c < D; ++c) {
+          float v = (float)(src.pos[l * D + c] * scale - org[c]);
+          s_pos[c][tid] = v;
+          lo[c] = v;
+          hi[c] = v;
+        }
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        s_wt[q][tid] = (ok && q < M) ? (float)(src.sign * src.w[0][l * ld + q]) : 0.f;
+        for (int q = 0; q < 4; ++q)
+          s_wt[q][tid] = q < M ? (float)(src.sign * src.w[0][l * ld + q]) : 0.f;
+      } else {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          s_pos[c][tid] = 0.f;
+          lo[c] = INFINITY;
+          hi[c] = -INFINITY;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_wt[q][tid] = 0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        lo[c] = wave_min(lo[c]);
+        hi[c] = wave_max(hi[c]);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          s_box[wave][c] = lo[c];
+          s_box[wave][D + c] = hi[c];
+        }
+      }
     }
     __syncthreads();
-    for (int j0 = 0; j0 < TS; j0 += KF) {
-      v4f acc[RPT];
+    unsigned live = 0;
 #pragma unroll
-      for (int r = 0; r < RPT; ++r) acc[r] = (v4f){0.f, 0.f, 0.f, 0.f};
-      // Kernel values of 4 sources x RPT rows first, then their MFMAs: the
-      // matrix core never waits on a value the VALU just produced.
-      for (int j = j0; j < j0 + KF; j += 4) {
+    for (int q = 0; q < TS / 64; ++q) {
+      float d2 = 0.f;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        float g = fmaxf(fmaxf((float)(tlo[c] - org[c]) - s_box[q][D + c],
+                              s_box[q][c] - (float)(thi[c] - org[c])), 0.f);
+        d2 += g * g;
+      }
+      if (!(d2 > cut2)) live |= 1u << q;
+    }
+    if (!live) continue;
+
+    for (int q = 0; q < TS / 64; ++q) {
+      if (!((live >> q) & 1u)) continue;
+      v4f acc[2][RPT];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) acc[h][r] = (v4f){0.f, 0.f, 0.f, 0.f};
+      for (int j = q * 64; j < q * 64 + 64; j += 4) {
+        // Kernel values of 4 sources x RPT rows first, then their MFMAs.
         float kv[4][RPT], wb[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -794,13 +862,13 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ksum_mfma(const lddmm_seg_t *__
         for (int u = 0; u < 4; ++u)
 #pragma unroll
           for (int r = 0; r < RPT; ++r)
-            acc[r] = __builtin_amdgcn_mfma_f32_4x4x1f32(kv[u][r], wb[u], acc[r], 0, 0, 0);
+            acc[u & 1][r] = __builtin_amdgcn_mfma_f32_4x4x1f32(kv[u][r], wb[u], acc[u & 1][r], 0, 0, 0);
       }
 #pragma unroll
       for (int r = 0; r < RPT; ++r)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          float y = acc[r][v] - cK[r][v];
+          float y = (acc[0][r][v] + acc[1][r][v]) - cK[r][v];
           float t = sK[r][v] + y;
           cK[r][v] = (t - sK[r][v]) - y;
           sK[r][v] = t;
@@ -2120,10 +2188,11 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
 
 namespace {
 
-// LDDMM_MFMA=1 selects the K-sum kernel that accumulates on the matrix cores
-// (same sums up to FP32 accumulation rounding).  It issues ~25% fewer VALU
-// instructions per pair but measured slower on MI355X (MFMA dependency
-// stalls), so the all-VALU kernel is the default.
+// LDDMM_MFMA=1 selects the K-sum kernel that accumulates on the matrix cores for
+// FP32 applyK, flow steps and RKHS products (same sums up to FP32 accumulation
+// rounding; energy-only sums keep k_pairs' symmetric form).  It issues 7 instead
+// of ~11 VALU instructions per pair but measured 12% slower on MI355X on the
+// 40,962-point flow step (0.61 vs 0.54 ms), so the all-VALU kernel is the default.
 bool use_mfma() {
   const char *e = getenv("LDDMM_MFMA");
   return e && e[0] == '1';
@@ -2133,8 +2202,8 @@ template <typename R, int KT, int D, int M, bool KSUM, int NA>
 void launch_pairs(const lddmm_seg_t *segs, int nseg, int op, int prec, int64_t n_items,
                   double *part, hipStream_t st) {
   if constexpr (std::is_same<R, float>::value && KSUM && NA == 0) {
-    if (use_mfma()) {
-      hipLaunchKernelGGL((k_pairs_ksum_mfma<KT, D, M>), dim3((unsigned)n_items), dim3(BLOCK), 0,
+    if ((op == LDDMM_OP_APPLY_K || op == LDDMM_OP_FLOW_STEP || op == LDDMM_OP_KDOT) && use_mfma()) {
+      hipLaunchKernelGGL((k_pairs_mfma<KT, D, M>), dim3((unsigned)n_items), dim3(BLOCK), 0,
                          st, segs, nseg, op, prec, part);
       return;
     }
