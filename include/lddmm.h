@@ -233,6 +233,15 @@ int lddmm_apply_diff_k_mat(const double *y, const double *x, const double *beta,
                            int64_t ny, int64_t nx, int d, int kernel, int order,
                            double sigma, double *out, void *stream);
 
+/* Dense kernel matrix of Kernel.precompute (kernelFunctions.py:10-55,152-198,
+ * 285-298): out(ny, nx) row-major = K(y_k, x_l) (deriv 0), K'(r^2) (deriv 1) or
+ * K''(r^2) (deriv 2), FP64 with the dense path's exact-double Laplacian
+ * polynomials and no cut-off; self_set (y is x, no firstVar / grid in the
+ * reference) applies the reference's diagonal fills.  Writes 8 ny nx bytes. */
+int lddmm_kernel_matrix(const double *y, const double *x, int64_t ny, int64_t nx, int d,
+                        int kernel, int order, double sigma, int deriv, int self_set,
+                        double *out, void *stream);
+
 /* ---- geometry of the currents data term -------------------------------- */
 
 /* surfaces.Surface.updateVertices (py-lddmm/surfaces.py:69-75):

@@ -222,6 +222,38 @@ def _lap_pol_diff(u, order):
     raise ValueError("the Laplacian derivative needs order 1..4")
 
 
+def _lap_pol(u, order):
+    """lapPol (kernelFunctions.py:58-72)."""
+    if order == 0:
+        return np.ones_like(u)
+    if order == 1:
+        return 1. + u
+    if order == 2:
+        return (3. + 3 * u + u * u) / 3.
+    if order == 3:
+        return (15. + 15 * u + 6 * u * u + u * u * u) / 15.
+    return (105. + 105 * u + 45 * u * u + 10 * u * u * u + u * u * u * u) / 105.
+
+
+def kernel_matrix(y, x, sigma, order, self_set):
+    """The dense K matrix of precompute() (kernelFunctions.py:10-55,152-183)."""
+    from scipy.spatial import distance as dfun
+    if order > 4:
+        if self_set:
+            K = dfun.squareform(np.exp(-dfun.pdist(x, 'sqeuclidean') / (2 * sigma * sigma)),
+                                checks=False)
+            np.fill_diagonal(K, 1)
+            return K
+        return np.exp(-dfun.cdist(y, x, 'sqeuclidean') / (2 * sigma * sigma))
+    if self_set:
+        u = dfun.pdist(x) / sigma
+        K = dfun.squareform(_lap_pol(u, order) * np.exp(-u))
+        np.fill_diagonal(K, 1)
+        return K
+    u = dfun.cdist(y, x) / sigma
+    return _lap_pol(u, order) * np.exp(-u)
+
+
 def kernel_matrix_diff(y, x, sigma, order, self_set):
     """The dense K'(r^2) matrix of precompute(diff=True) (kernelFunctions.py:10-55,152-183):
     Gaussian -exp(-r^2/2s^2)/(2s^2); Laplacian -lapPolDiff(u) e^-u/(2s^2) with u = r/s.  With

@@ -104,6 +104,7 @@ SIGNATURES = {
                                         _SZ, _P]),
     "lddmm_apply_k_mat": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _D, _P, _P]),
     "lddmm_apply_diff_k_mat": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _D, _P, _P]),
+    "lddmm_kernel_matrix": (_I, [_P, _P, _I64, _I64, _I, _I, _I, _D, _I, _I, _P, _P]),
     "lddmm_surface_geometry": (_I, [_P, _P, _I64, _P, _P, _P]),
     "lddmm_curve_geometry": (_I, [_P, _P, _I64, _I, _P, _P, _P]),
     "lddmm_surface_vertex_gradient": (_I, [_P, _P, _P, _P, _P, _P, _I64, _D, _P, _P]),

@@ -2703,6 +2703,90 @@ extern "C" int lddmm_apply_diff_k_mat(const double *y, const double *x, const do
 }
 
 // ---------------------------------------------------------------------------
+// Dense kernel matrices (Kernel.precompute, kernelFunctions.py:10-55,152-198,285-298):
+// out[k, l] = K(y_k, x_l) (deriv 0), K'(r^2) (1) or K''(r^2) (2), row-major (ny, nx),
+// FP64 with the dense path's exact-double Laplacian polynomials lapPol / lapPolDiff /
+// lapPolDiff2 (:58-104) and no cut-off.  self_set (firstVar and grid absent, pdist
+// + squareform there) fills the diagonal as the reference does: Gaussian K = 1
+// before the derivative scaling; Laplacian 1, -1/((2 order - 1) 2 sigma^2) and
+// 1/(35 * 4 sigma^4).  One thread per entry, consecutive threads along a row.
+// ---------------------------------------------------------------------------
+__device__ inline double lap_exact(int order, int deriv, double u) {
+  if (deriv == 0) {
+    switch (order) {
+      case 0: return 1.0;
+      case 1: return 1.0 + u;
+      case 2: return (3.0 + 3.0 * u + u * u) / 3.0;
+      case 3: return (15.0 + 15.0 * u + 6.0 * u * u + u * u * u) / 15.0;
+      default: return (105.0 + 105.0 * u + 45.0 * u * u + 10.0 * u * u * u + u * u * u * u) / 105.0;
+    }
+  }
+  if (deriv == 1) {
+    switch (order) {
+      case 1: return 1.0;
+      case 2: return (1.0 + u) / 3.0;
+      case 3: return (3.0 + 3.0 * u + u * u) / 15.0;
+      default: return (15.0 + 15.0 * u + 6.0 * u * u + u * u * u) / 105.0;
+    }
+  }
+  switch (order) {
+    case 2: return 1.0 / 3.0;
+    case 3: return (1.0 + u) / 15.0;
+    default: return (3.0 + 3.0 * u + u * u) / 105.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_kernel_matrix(const double *__restrict__ y,
+                                                      const double *__restrict__ x, int64_t ny,
+                                                      int64_t nx, int d, int kernel, int order,
+                                                      double sigma, int deriv, int self_set,
+                                                      double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= ny * nx) return;
+  const int64_t k = i / nx, l = i - k * nx;
+  const double s2 = 2.0 * sigma * sigma;
+  double r2 = 0.0;
+  for (int c = 0; c < d; ++c) {
+    const double t = y[k * d + c] - x[l * d + c];
+    r2 += t * t;
+  }
+  double v;
+  if (kernel == LDDMM_KERNEL_GAUSS) {
+    const double K = (self_set && k == l) ? 1.0 : exp(-r2 / s2);
+    v = deriv == 0 ? K : (deriv == 1 ? -K / s2 : K / (s2 * s2));
+  } else if (self_set && k == l) {
+    v = deriv == 0 ? 1.0
+                   : (deriv == 1 ? -1.0 / ((2 * order - 1) * s2) : 1.0 / (35.0 * 4.0 * sigma * sigma * sigma * sigma));
+  } else {
+    const double u = sqrt(r2) / sigma;
+    const double e = exp(-u);
+    const double p = lap_exact(order, deriv, u);
+    v = deriv == 0 ? p * e : (deriv == 1 ? -p * e / s2 : p * e / (4.0 * sigma * sigma * sigma * sigma));
+  }
+  out[i] = v;
+}
+
+extern "C" int lddmm_kernel_matrix(const double *y, const double *x, int64_t ny, int64_t nx, int d,
+                                   int kernel, int order, double sigma, int deriv, int self_set,
+                                   double *out, void *stream) {
+  if (!y || !x || !out || ny < 1 || nx < 1 || d < 1 || d > 8) return LDDMM_EINVAL;
+  if (deriv < 0 || deriv > 2 || !(sigma > 0.0) || !isfinite(sigma)) return LDDMM_EINVAL;
+  if (self_set && ny != nx) return LDDMM_EINVAL;
+  if (kernel == LDDMM_KERNEL_LAPLACIAN) {
+    if (order < deriv || order > 4 || (deriv == 2 && order < 2)) return LDDMM_EINVAL;
+  } else if (kernel != LDDMM_KERNEL_GAUSS) {
+    return LDDMM_EINVAL;
+  }
+  const int64_t n = ny * nx;
+  if (nx > 0 && n / nx != ny) return LDDMM_EINVAL;
+  const int64_t blocks = ceil_div(n, 256);
+  if (blocks > INT32_MAX) return LDDMM_EINVAL;
+  hipLaunchKernelGGL(k_kernel_matrix, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     y, x, ny, nx, d, kernel, order, sigma, deriv, self_set, out);
+  return hip_check(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Geometry entry points.
 // ---------------------------------------------------------------------------
 extern "C" int lddmm_surface_geometry(const double *vertices, const int32_t *faces, int64_t nf,

@@ -115,9 +115,35 @@ class Kernel(KernelSpec):
     """Kernel sums (py-lddmm/kernelFunctions.py:284-530) on the GPU."""
 
     def precompute(self, x, firstVar=None, grid=None, diff=False, diff2=False):
-        raise NotImplementedError(
-            "dense kernel matrices (Kernel.precompute, kernelFunctions.py:285-298) are not part "
-            "of the GPU path; use applyK / applyDiffKT")
+        """The dense kernel matrix K(y_k, x_l) (or K'(r^2) with diff, K''(r^2) with diff2) times
+        ``weight``, y = firstVar, grid points (..., d) or x (kernelFunctions.py:285-298 with
+        kernelMatrixGauss / kernelMatrixLaplacian, :10-55,152-198): exact-double polynomials,
+        the reference's diagonal fills when y is x.  None for a kernel named 'none'.  The
+        reference's hold() reuses the previous call's distances whatever x is; here every call
+        recomputes, which is the same for the same x."""
+        if self.name == 'none':
+            return None
+        xd, np_out = _wrap(x)
+        lead = None
+        if grid is not None:
+            gd = _wrap(grid)[0]
+            lead = tuple(gd.shape[:-1])
+            yd = gd.reshape(-1, gd.shape[-1]).contiguous()
+        else:
+            yd = xd if firstVar is None else _wrap(firstVar)[0]
+        self_set = firstVar is None and grid is None
+        deriv = 1 if diff else (2 if diff2 else 0)
+        out = torch.empty((yd.shape[0], xd.shape[0]), dtype=torch.float64, device=xd.device)
+        lib = _lib.load()
+        _lib.check(lib.lddmm_kernel_matrix(yd.data_ptr(), xd.data_ptr(), yd.shape[0], xd.shape[0],
+                                           xd.shape[1], self.kernel_code, int(self.order),
+                                           float(self.sigma), deriv, int(self_set), out.data_ptr(),
+                                           rt.stream_handle()), "lddmm_kernel_matrix")
+        if self.weight != 1.0:
+            out = out * self.weight
+        if lead is not None:
+            out = out.reshape(lead + (xd.shape[0],))
+        return _unwrap(out, np_out)
 
     def hold(self):
         self._state = self._hold

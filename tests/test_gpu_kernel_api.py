@@ -74,3 +74,26 @@ def test_matrix_weight_sums(name, order, sigma):
     assert rel_err(K.applyK(x, bx, matrixWeights=True), Ko.applyK(x, bx, matrixWeights=True)) < 1e-13
     with pytest.raises(ValueError):
         K.applyK(x, beta, matrixWeights=True)      # beta rows must match the targets
+
+
+@pytest.mark.parametrize("name,order,sigma", [("gauss", 10, 0.5), ("laplacian", 2, 0.4),
+                                              ("laplacian", 4, 0.6)])
+def test_precompute_dense_matrices(name, order, sigma):
+    from registration_amd import kernelFunctions as kfun
+    rng = np.random.default_rng(order)
+    x, y = rng.standard_normal((120, 3)) * 0.5, rng.standard_normal((70, 3)) * 0.5
+    K = kfun.Kernel(name, sigma=sigma, order=order, weight=1.5)
+    o = 10 if name == "gauss" else order
+    # K, K' and K'' on the self set (with the reference's diagonal fills) and against firstVar
+    assert rel_err(K.precompute(x), 1.5 * orc.kernel_matrix(None, x, sigma, o, True)) < 1e-14
+    assert rel_err(K.precompute(x, diff=True), 1.5 * orc.kernel_matrix_diff(None, x, sigma, o, True)) < 1e-14
+    assert rel_err(K.precompute(x, diff2=True), 1.5 * orc.kernel_matrix_diff2(x, sigma, o)) < 1e-14
+    assert rel_err(K.precompute(x, firstVar=y), 1.5 * orc.kernel_matrix(y, x, sigma, o, False)) < 1e-14
+    assert rel_err(K.precompute(x, firstVar=y, diff=True),
+                   1.5 * orc.kernel_matrix_diff(y, x, sigma, o, False)) < 1e-14
+    # grid points keep their leading shape
+    g = rng.standard_normal((4, 5, 3))
+    Kg = K.precompute(x, grid=g)
+    assert Kg.shape == (4, 5, 120)
+    assert rel_err(Kg.reshape(20, 120), 1.5 * orc.kernel_matrix(g.reshape(20, 3), x, sigma, o, False)) < 1e-14
+    assert kfun.Kernel('foo', sigma=1.0).precompute(x) is None

@@ -567,3 +567,15 @@ def test_dense_ddiff_k11_k12(order, sigma, tol):
             - K.applyDiffKT(x - h * p, n[None], a[None], firstVar=x)) / (2 * h)
     np.testing.assert_allclose(k11, fd11, rtol=1e-6, atol=1e-6 * np.abs(fd11).max())
     np.testing.assert_allclose(k12, fd12, rtol=1e-6, atol=1e-6 * np.abs(fd12).max())
+
+
+def test_dense_kernel_matrices_reproduce_the_sums():
+    """The dense K matrix times the weights is applyK (Gaussian: the Fortran's cut-off only
+    matters within 1e-4 sigma); the dense K' matrix gives applyDivergence's reference form."""
+    rng = np.random.default_rng(6)
+    x, a, y = rng.standard_normal((50, 3)), rng.standard_normal((50, 3)), rng.standard_normal((20, 3))
+    K = orc.kernel_matrix(y, x, 0.6, 10, False)
+    np.testing.assert_allclose(K @ a, orc.apply_k(y, x, a, 0.6, 10), rtol=1e-13, atol=1e-14)
+    Ks = orc.kernel_matrix(None, x, 0.6, 3, True)
+    np.testing.assert_allclose(Ks @ a, orc.apply_k(x, x, a, 0.6, 3), rtol=1e-6, atol=1e-7)
+    assert np.allclose(np.diag(Ks), 1.0) and np.allclose(Ks, Ks.T)
