@@ -2,7 +2,10 @@
 
 ``Curve`` keeps the reference layout (vertices (N,d), faces = edges (E,2), centers = edge
 midpoints, lenel = edge vectors; py-lddmm/curves.py:15-73) and the constructors
-``Curve(curve=...)``, ``Curve(FV=(edges, vertices))``, ``Curve(pointSet=..., isOpen=...)``.
+``Curve(curve=...)``, ``Curve(FV=(edges, vertices))``, ``Curve(pointSet=..., isOpen=...)``,
+``Curve(filename=...)`` (.dat, .txt, .vtk, read by registration_amd.fileio), with the
+reference's readers and writers (readCurve, readTxt, readVTK, saveCurve, saveVTK,
+readMultipleCurves, saveEvolution; curves.py:269-436,518-535).
 ``currentNorm0`` / ``currentNormDef`` / ``currentNorm`` / ``currentNormGradient`` keep the
 reference signatures (curves.py:539-613) and run on the GPU, and so do the measure-term
 ``measureNorm0`` / ``measureNormDef`` / ``measureNorm`` / ``measureNormGradient``
@@ -14,6 +17,7 @@ import numpy as np
 import torch
 
 from . import engine
+from . import fileio
 from . import runtime as rt
 
 
@@ -22,8 +26,9 @@ class Curve:
         if curve is None:
             if FV is None:
                 if filename is not None:
-                    raise NotImplementedError("curve file formats are not part of the GPU path")
-                if pointSet is None:
+                    self.vertices, self.faces = fileio.read_curve(filename)
+                    self.computeCentersLengths()
+                elif pointSet is None:
                     self.vertices = np.empty(0)
                     self.centers = np.empty(0)
                     self.faces = np.empty(0)
@@ -56,6 +61,44 @@ class Curve:
     def updateVertices(self, x0):
         self.vertices = np.copy(x0)
         self.computeCentersLengths()
+
+    # ---- files (curves.py:269-436) ------------------------------------------------------
+    def _read(self, reader, path):
+        self.vertices, self.faces = reader(path)
+        self.computeCentersLengths()
+
+    def readTxt(self, infile):
+        self._read(fileio.read_curve_txt, infile)
+
+    def readCurve(self, infile):
+        self._read(fileio.read_curve_dat, infile)
+
+    def readVTK(self, fileName):
+        self._read(fileio.read_vtk_curve, fileName)
+
+    def saveCurve(self, outfile):
+        fileio.save_curve_dat(outfile, self.vertices, self.faces)
+
+    def saveVTK(self, fileName, scalars=None, normals=None, scal_name='scalars'):
+        fileio.save_vtk_curve(fileName, self.vertices, self.faces, scalars, normals, scal_name)
+
+
+def readMultipleCurves(regexp, Nmax=0):
+    """Curves of the files matching a glob pattern, at most Nmax (curves.py:518-527)."""
+    import glob
+    files = glob.glob(regexp)
+    nm = min(Nmax, len(files)) if Nmax > 0 else len(files)
+    return [Curve(filename=files[k]) for k in range(nm)]
+
+
+def saveEvolution(fileName, fv0, xt):
+    """One curve .dat file per time point, named as the reference names them
+    (curves.py:530-534)."""
+    fv = Curve(curve=fv0)
+    xt = xt.cpu().numpy() if isinstance(xt, torch.Tensor) else np.asarray(xt)
+    for k in range(xt.shape[0]):
+        fv.vertices = np.squeeze(xt[k, :, :])
+        fv.saveCurve(fileName + '{0: 02d}'.format(k) + '.byu')
 
 
 class DeviceCurve:

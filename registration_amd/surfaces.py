@@ -2,8 +2,10 @@
 
 ``Surface`` keeps the reference's in-memory layout (vertices (N,3) float64, faces (F,3) int,
 centers, surfel = area-weighted normals at twice the area; py-lddmm/surfaces.py:24-75) and
-its constructors ``Surface(surf=...)`` / ``Surface(FV=(faces, vertices))``.  File readers and
-writers (BYU/OFF/VTK/OBJ) are not part of the GPU path.
+its constructors ``Surface(surf=...)`` / ``Surface(FV=(faces, vertices))`` /
+``Surface(filename=...)`` (.byu, .off, .vtk, .obj, read by registration_amd.fileio), with the
+reference's readers and writers (readbyu, readOFF, readVTK, readOBJ, savebyu, saveVTK,
+saveVTK2, readMultipleByu, saveEvolution; surfaces.py:720-1005).
 
 ``currentNorm0``, ``currentNormDef``, ``currentNorm`` and ``currentNormGradient`` keep the
 reference signatures (surfaces.py:1009-1089) and run on the GPU: the two F x F sums of the
@@ -21,7 +23,9 @@ import numpy as np
 import torch
 
 from . import engine
+from . import fileio
 from . import runtime as rt
+from .fileio import vtkFields  # noqa: F401  (part of the module's API)
 
 
 class Surface:
@@ -29,7 +33,9 @@ class Surface:
         if surf is None:
             if FV is None:
                 if filename is not None:
-                    raise NotImplementedError("surface file formats are not part of the GPU path")
+                    self.vertices, self.faces = fileio.read_surface(filename)
+                    self.computeCentersAreas()
+                    return
                 self.vertices = np.empty(0)
                 self.centers = np.empty(0)
                 self.faces = np.empty(0)
@@ -64,6 +70,61 @@ class Surface:
         """Signed volume sum_f det[x1; x2; x3] / 6 (surfaces.py:711-717)."""
         v = self.vertices[self.faces]
         return float(np.linalg.det(v).sum() / 6)
+
+    # ---- files (surfaces.py:720-985) --------------------------------------------------
+    def _read(self, reader, path):
+        self.vertices, self.faces = reader(path)
+        self.computeCentersAreas()
+
+    def readbyu(self, byufile):
+        self._read(fileio.read_byu, byufile)
+
+    def readOFF(self, offfile):
+        self._read(fileio.read_off, offfile)
+
+    def readVTK(self, fileName):
+        self._read(fileio.read_vtk_surface, fileName)
+
+    def readOBJ(self, fileName):
+        self._read(fileio.read_obj, fileName)
+
+    def savebyu(self, byufile):
+        fileio.save_byu(byufile, self.vertices, self.faces)
+
+    def saveVTK(self, fileName, scalars=None, normals=None, tensors=None, scal_name='scalars',
+                vectors=None, vect_name='vectors'):
+        vf = vtkFields()
+        if scalars is not None:
+            vf.scalars += [scal_name, scalars]
+        if vectors is not None:
+            vf.vectors += [vect_name, vectors]
+        if normals is not None:
+            vf.normals += ['normals', normals]
+        if tensors is not None:
+            vf.tensors += ['tensors', tensors]
+        self.saveVTK2(fileName, vf)
+
+    def saveVTK2(self, fileName, vtkFields=None):
+        fileio.save_vtk_surface(fileName, self.vertices, self.faces, vtkFields)
+
+
+def readMultipleByu(regexp, Nmax=0):
+    """Surfaces of the files matching a glob pattern, at most Nmax (surfaces.py:988-997)."""
+    import glob
+    files = glob.glob(regexp)
+    nm = min(Nmax, len(files)) if Nmax > 0 else len(files)
+    return [Surface(filename=files[k]) for k in range(nm)]
+
+
+def saveEvolution(fileName, fv0, xt):
+    """One .byu file per time point of a trajectory xt (T, N, 3) on fv0's faces, named
+    fileName + '{0: 02d}'.format(k) + '.byu' as the reference names them
+    (surfaces.py:1000-1004)."""
+    fv = Surface(surf=fv0)
+    xt = xt.cpu().numpy() if isinstance(xt, torch.Tensor) else np.asarray(xt)
+    for k in range(xt.shape[0]):
+        fv.vertices = np.squeeze(xt[k, :, :])
+        fv.savebyu(fileName + '{0: 02d}'.format(k) + '.byu')
 
 
 class DeviceSurface:
