@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (AMD published)
+FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64 (AMD published)
 FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
 # HBM bytes per fused flow step at 40,962 points (pair kernel + finalize + totals), measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on `bench.py --roofline-only --roofline-reps 3` and
@@ -47,6 +48,8 @@ def parse():
                     help="requests of the served-surfaces stream to serve (0 = skip)")
     ap.add_argument("--mixed-service", type=int, default=5,
                     help="requests of the mixed-service stream to serve (0 = skip)")
+    ap.add_argument("--data-term-reps", type=int, default=10,
+                    help="launches per data-term kernel in the data_terms leg (0 = skip)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the flow-step launches and a calibration copy (for rocprof passes)")
     return ap.parse_args()
@@ -107,6 +110,80 @@ def flow_step_roofline(cfg, prec, reps):
             "kernel": "k_pairs<float,gauss,3,3> + finalize (Euler flow step)",
             "launch_ms": round(ms, 4), "pairs_per_launch": pairs, "flops_per_pair": 16,
             "pairs_per_s": pairs / (ms * 1e-3)}
+
+
+# Algorithmic flops per pair (add/mul 1, FMA 2, exp/sqrt 1; Gaussian d = 3).  Energies:
+# kernel value 3d+1 = 10, plus 2m for the weighted sum (currents m = 3, measure m = 1) or, for
+# the varifold, g = <u_k, u_l> (5), g^2 (1), K a_l (1) and a_l K (1 + g^2) accumulated (3).
+DATA_TERM_FLOPS = {"currents": 16, "measure": 12, "varifold": 20}
+
+
+def data_terms_leg(cfg, reps):
+    """Per-launch time of the data-term pair kernels on the workload's surface pair (deformed
+    template against target, F x F pairs for each of the two source sets), timed with HIP
+    events on the launch stream: FP64 energies (what every line-search trial runs) and FP32
+    gradient forms, plus the divergence of the Jacobian flow at the template's points."""
+    import numpy as np
+    import torch
+    from registration_amd import _lib, runtime as rt, surfaces
+    F, V = cfg["template"]
+    _, Y = cfg["target"]
+    sig = cfg["sigmaDist"]
+    dev = "cuda"
+    S = surfaces.DeviceSurface(F, torch.as_tensor(V, device=dev), with_weights=True)
+    T = surfaces.DeviceSurface(F, torch.as_tensor(Y, device=dev), with_weights=True)
+    nf = F.shape[0]
+    tot = torch.empty(1, dtype=torch.float64, device=dev)
+    z1 = torch.empty((nf, 3), dtype=torch.float64, device=dev)
+    dz = torch.empty((nf, 3), dtype=torch.float64, device=dev)
+    base = dict(y=S.centers, ny=nf, x1=T.centers, nx1=nf, sigma=sig, coef=1.0 / 3.0,
+                kernel=_lib.KERNEL_GAUSS, order=10, total=tot)
+    weights = {
+        "currents": dict(w0=S.surfel, w1=T.surfel),
+        "measure": dict(w0=S.weights, w1=T.weights, w2=S.surfel),
+        "varifold": dict(w0=S.weights, w1=T.weights, w2=S.surfel, sw=[T.surfel]),
+    }
+    ops = {"currents": _lib.OP_CURRENTS, "measure": _lib.OP_MEASURE, "varifold": _lib.OP_VARIFOLD}
+
+    def timed(plan):
+        plan.run()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            plan.run()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    pairs = 2 * nf * nf
+    for name, op in ops.items():
+        ms = timed(rt.Plan(op, _lib.PREC_FP64, 3, [rt.seg(**base, **weights[name])]))
+        tf = DATA_TERM_FLOPS[name] * pairs / (ms * 1e-3) / 1e12
+        # Nominal: every one of the 2 F^2 pairs counted, although tile culling skips far
+        # blocks, the self sum evaluates each unordered block pair once and FP64 far blocks
+        # use the FP32 exponential, so these rates exceed the evaluated FP64 work.
+        out[f"{name}_energy_fp64"] = {"ms": round(ms, 4), "nominal_pairs": pairs,
+                                      "nominal_pairs_per_s": pairs / (ms * 1e-3),
+                                      "flops_per_pair": DATA_TERM_FLOPS[name],
+                                      "nominal_tflops": round(tf, 3),
+                                      "fp64_peak_tflops": FP64_PEAK_TFLOPS}
+        ms = timed(rt.Plan(op, _lib.PREC_FP32C, 3,
+                           [rt.seg(**base, **weights[name], out0=z1, out1=dz)]))
+        out[f"{name}_gradient_fp32c"] = {"ms": round(ms, 4), "nominal_pairs": pairs,
+                                         "nominal_pairs_per_s": pairs / (ms * 1e-3)}
+    # divergence of K(., x) a at the template's points (one Jacobian-flow step)
+    x = torch.as_tensor(V, device=dev)
+    a = torch.as_tensor(1e-2 * np.random.default_rng(1).standard_normal(V.shape), device=dev)
+    div = torch.empty(V.shape[0], dtype=torch.float64, device=dev)
+    ms = timed(rt.Plan(_lib.OP_DIVERGENCE, _lib.PREC_FP32C, 3,
+                       [rt.seg(y=x, w0=a, out0=div, ny=V.shape[0], sigma=cfg["sigmaKernel"],
+                               kernel=_lib.KERNEL_GAUSS, order=10)]))
+    n2 = V.shape[0] ** 2
+    out["divergence_fp32c"] = {"ms": round(ms, 4), "pairs": n2, "pairs_per_s": n2 / (ms * 1e-3)}
+    return out
 
 
 def cpu_baseline(cfg, pair_counts, seconds_target=10.0):
@@ -308,6 +385,7 @@ def main():
     energy_err = abs(final_obj - obj64) / abs(obj64)
 
     roof = flow_step_roofline(cfg, args.precision, args.roofline_reps)
+    data_terms = data_terms_leg(cfg, args.data_term_reps) if args.data_term_reps > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, {k: v / args.steps for k, v in counts.items()})
@@ -343,6 +421,7 @@ def main():
         "pairs_per_registration": pairs_per_reg,
         "pair_rate_overall": pairs_per_reg / (wall / args.steps),
         "roofline": roof,
+        "data_terms": data_terms,
         "cpu_baseline": cpu,
         "served": served,
     }

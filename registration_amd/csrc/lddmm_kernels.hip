@@ -1143,7 +1143,13 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
   if (l1 > sn) l1 = sn;
 
   // Energy-only self set: each unordered pair of TS blocks once (see k_pairs).
-  const bool sym = !GRAD && !sg->rowdot && !sg->out0 && set == 0;
+  const bool energy_only = !GRAD && !sg->rowdot && !sg->out0;
+  const bool sym = energy_only && set == 0;
+  // FP64 Gaussian energies: 64-source runs whose every kernel value is below 2^-26 take the
+  // FP32 exponential of the FP64 exponent, as in k_pairs.
+  constexpr bool MIXED = std::is_same<R, double>::value && KT == LDDMM_KERNEL_GAUSS && !GRAD;
+  const bool mixed = MIXED && energy_only;
+  const R far2 = MIXED ? (R)(26.0 * 2.0 * sg->sigma * sg->sigma / LOG2E) : (R)0;
   const int64_t wave_blk = ((int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT)) / TS;
   const int64_t wg_blk = ((int64_t)rt * (BLOCK * RPT)) / TS;
 
@@ -1199,7 +1205,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
       }
     }
     __syncthreads();
-    unsigned live = 0;
+    unsigned live = 0, far = 0;
 #pragma unroll
     for (int q = 0; q < TS / 64; ++q) {
       R d2 = 0;
@@ -1210,6 +1216,9 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
         d2 += g * g;
       }
       if (!(d2 > cut2)) live |= 1u << q;
+      if constexpr (MIXED) {
+        if (d2 > far2) far |= 1u << q;
+      }
     }
     if (!live || sym_w == 0) continue;
     const R wfac = (R)sym_w;
@@ -1223,6 +1232,30 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
         for (int c = 0; c < NACC; ++c) tA[r][c] = 0;
 #pragma unroll
         for (int c = 0; c < D; ++c) tD[r][c] = 0;
+      }
+      if constexpr (MIXED) {
+        if (mixed && ((far >> (j0 / 64)) & 1u)) {
+#pragma unroll 4
+          for (int j = j0; j < j0 + KF; ++j) {
+            R sx[D], su[D];
+#pragma unroll
+            for (int c = 0; c < D; ++c) { sx[c] = s_pos[c][j]; su[c] = s_u[c][j]; }
+            const R sa = s_a[j];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+              R r2 = 0, g = 0;
+#pragma unroll
+              for (int c = 0; c < D; ++c) {
+                const R dx = ty[r][c] - sx[c];
+                r2 = dx * dx + r2;
+                g = tu[r][c] * su[c] + g;
+              }
+              const R w = (R)__builtin_amdgcn_exp2f((float)(-kev.h * r2)) * sa;
+              tA[r][0] = (w * (g * g) + w) + tA[r][0];
+            }
+          }
+          goto vfold;
+        }
       }
 #pragma unroll 4
       for (int j = j0; j < j0 + KF; ++j) {
@@ -1257,6 +1290,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
           }
         }
       }
+    vfold:
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
 #pragma unroll
