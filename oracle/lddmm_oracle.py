@@ -7,7 +7,7 @@ cpu_baseline leg of bench.py may import it; the product package never does.
 It restates, in Python 3 / NumPy float64, the executed reference path:
   Kernel                  py-lddmm/kernelFunctions.py:238-454 (sums in oracle/kernel_sums.c,
                           restating kernelFunctions.f90:1-114,430-496)
-  landmark flow/adjoint   py-lddmm/pointEvolution.py:599-817
+  landmark flow/adjoint   py-lddmm/pointEvolution.py:599-817 (second order: :882-1084)
   surfaces / curves       py-lddmm/surfaces.py:61-75,711-717,1009-1276; curves.py:58-73,539-701
                           (currents, measure and varifold data terms)
   SurfaceMatching         py-lddmm/surfaceMatching.py:19-484 (affine='none', symmetric=False)
@@ -500,6 +500,131 @@ def landmarkHamiltonianGradient(x0, at, px1, KparDiff, regweight, getCovector=Fa
     if getCovector:
         return dat, xt, pxt
     return dat, xt
+
+
+# --------------------------------------------------------------------------------------
+# Second-order (spline) flow and its adjoint (pointEvolution.py:882-1066)
+# --------------------------------------------------------------------------------------
+def secondOrderEvolution(x0, a0, rhot, KparDiff, withJacobian=False, withPointSet=None,
+                         affine=None):
+    """pointEvolution.py:882-948: x' = K(x)a, a' = -applyDiffKT(x,[a],[a]) + rho, with the
+    affine step U = I + dt A_k (x <- (x + dt zx) U^T + dt b_k, a <- (a + dt za) U^-1).  The
+    Jacobian is taken at the point set when one is given, else at the points."""
+    T = rhot.shape[0]
+    N, dim = x0.shape
+    dt = 1.0 / T
+    at = np.zeros([T + 1, N, dim])
+    xt = np.zeros([T + 1, N, dim])
+    xt[0], at[0] = x0, a0
+    zt = Jt = None
+    if withPointSet is not None:
+        zt = np.zeros([T + 1, withPointSet.shape[0], dim])
+        zt[0] = withPointSet
+    if withJacobian:
+        Jt = np.zeros([T + 1, N if zt is None else zt.shape[1]])
+    for k in range(T):
+        x, a, rho = xt[k], at[k], rhot[k]
+        zx = KparDiff.applyK(x, a)
+        za = -KparDiff.applyDiffKT(x, a[np.newaxis], a[np.newaxis]) + rho
+        if affine is not None:
+            U = np.eye(dim) + dt * affine[0][k]
+            xt[k + 1] = np.dot(x + dt * zx, U.T) + dt * affine[1][k]
+            at[k + 1] = np.dot(a + dt * za, np.linalg.inv(U))
+        else:
+            xt[k + 1] = x + dt * zx
+            at[k + 1] = a + dt * za
+        if zt is not None:
+            z = zt[k]
+            zz = KparDiff.applyK(x, a, firstVar=z)
+            zt[k + 1] = (np.dot(z + dt * zz, U.T) + dt * affine[1][k]) if affine is not None \
+                else z + dt * zz
+        if Jt is not None:
+            at_pts = zt[k] if zt is not None else x
+            Jt[k + 1] = Jt[k] + dt * KparDiff.applyDivergence(x, a, firstVar=None if zt is None else at_pts)
+            if affine is not None:
+                Jt[k + 1] += dt * np.trace(affine[0][k])
+    if zt is None and Jt is None:
+        return xt, at
+    out = [xt, at]
+    if zt is not None:
+        out.append(zt)
+    if Jt is not None:
+        out.append(Jt)
+    return out
+
+
+def secondOrderHamiltonian(x, a, rho, px, pa, KparDiff, affine=None):
+    """pointEvolution.py:952-962."""
+    Ht = (px * KparDiff.applyK(x, a)).sum()
+    Ht += (pa * (-KparDiff.applyDiffKT(x, a[np.newaxis], a[np.newaxis]) + rho)).sum()
+    Ht -= (rho ** 2).sum() / 2
+    if affine is not None:
+        A, b = affine
+        Ht += (px * (np.dot(x, A.T) + b)).sum() - (pa * np.dot(a, A)).sum()
+    return Ht
+
+
+def secondOrderCovector(x0, a0, rhot, px1, pa1, KparDiff, affine=None, times=None):
+    """pointEvolution.py:965-1037: backward adjoint of secondOrderEvolution with the
+    time-series end points px1[j], pa1[j] added at steps (j+1) T / len(px1)."""
+    T = rhot.shape[0]
+    nTarg = len(px1)
+    Tsize1 = T // nTarg
+    N, dim = x0.shape
+    dt = 1.0 / T
+    xt, at = secondOrderEvolution(x0, a0, rhot, KparDiff)
+    pxt = np.zeros([T + 1, N, dim])
+    pat = np.zeros([T + 1, N, dim])
+    pxt[T], pat[T] = px1[nTarg - 1], pa1[nTarg - 1]
+    for t in range(T):
+        px, pa = pxt[T - t], pat[T - t]
+        x, a = xt[T - t - 1], at[T - t - 1]
+        if affine is not None:
+            U = np.eye(dim) + dt * affine[0][T - t - 1]
+            px_ = np.dot(px, U)
+            pa_ = np.dot(pa, np.linalg.inv(U).T)
+        else:
+            px_, pa_ = px, pa
+        a1 = np.concatenate((px_[np.newaxis], a[np.newaxis]))
+        a2 = np.concatenate((a[np.newaxis], px_[np.newaxis]))
+        zpx = KparDiff.applyDiffKT(x, a1, a2) - KparDiff.applyDDiffK11and12(x, a, a, pa_)
+        zpa = KparDiff.applyK(x, px_) - KparDiff.applyDiffK1and2(x, pa_, a)
+        pxt[T - t - 1] = px_ + dt * zpx
+        pat[T - t - 1] = pa_ + dt * zpa
+        if t < T - 1 and (T - 1 - t) % Tsize1 == 0:
+            pxt[T - t - 1] += px1[(T - t - 1) // Tsize1 - 1]
+            pat[T - t - 1] += pa1[(T - t - 1) // Tsize1 - 1]
+    return pxt, pat, xt, at
+
+
+def secondOrderGradient(x0, a0, rhot, px1, pa1, KparDiff, times=None, getCovector=False,
+                        affine=None, controlWeight=1.0):
+    """pointEvolution.py:1040-1084.  drhot and dA, db are computed in the affine branch only
+    (drhot stays zero without affine, as in the reference)."""
+    pxt, pat, xt, at = secondOrderCovector(x0, a0, rhot, px1, pa1, KparDiff, affine=affine,
+                                           times=times)
+    T = rhot.shape[0]
+    dt = 1.0 / T
+    drhot = np.zeros(rhot.shape)
+    if affine is not None:
+        dim = x0.shape[1]
+        dA = np.zeros(affine[0].shape)
+        for k in range(T):
+            x, a, rho = xt[k], at[k], rhot[k]
+            px, pa = pxt[k + 1], pat[k + 1]
+            zx = x + dt * KparDiff.applyK(x, a)
+            za = a + dt * (-KparDiff.applyDiffKT(x, a[np.newaxis], a[np.newaxis]) + rho)
+            Ui = np.linalg.inv(np.eye(dim) + dt * affine[0][k])
+            pa = np.dot(pa, Ui.T)
+            za = np.dot(za, Ui)
+            dA[k] = ((px[:, :, np.newaxis] * zx[:, np.newaxis, :]).sum(axis=0)
+                     - (za[:, :, np.newaxis] * pa[:, np.newaxis, :]).sum(axis=0))
+            drhot[k] = rho * controlWeight - pa
+        db = pxt[1:T + 1].sum(axis=1)
+    da0 = KparDiff.applyK(x0, a0) - pat[0]
+    if affine is None:
+        return (da0, drhot, xt, at, pxt, pat) if getCovector else (da0, drhot, xt, at)
+    return (da0, drhot, dA, db, xt, at, pxt, pat) if getCovector else (da0, drhot, dA, db, xt, at)
 
 
 # --------------------------------------------------------------------------------------

@@ -1,5 +1,6 @@
 """Landmark flow, adjoint and geodesic shooting, drop-in for py-lddmm's pointEvolution
-landmark functions (py-lddmm/pointEvolution.py:599-817).
+landmark functions (py-lddmm/pointEvolution.py:599-817) and its second-order (spline) flow,
+Hamiltonian, covector and gradient (:882-1084).
 
 Same signatures and return conventions as the reference.  Inputs may be NumPy arrays (results
 come back as NumPy float64) or torch tensors (results stay on the GPU).  Each Euler step is one
@@ -13,7 +14,8 @@ import torch
 
 from . import engine
 from . import runtime as rt
-from .kernelFunctions import apply_diff_kt_dev, apply_divergence_dev, apply_k_dev
+from .kernelFunctions import (apply_ddiff_k11and12_dev, apply_diff_k_dev, apply_diff_kt_dev,
+                              apply_divergence_dev, apply_k_dev)
 
 
 def _in(a):
@@ -183,3 +185,175 @@ def landmarkEPDiff(T, x0, a0, KparDiff, affine=None, withJacobian=False, withNor
         if k < T - 1:
             at[k + 1] = a - dt * apply_diff_kt_dev(z, z, a[None], a[None], KparDiff)
     return _out(xt, np_out), _out(at, np_out)
+
+
+# ---------------------------------------------------------------------------------------
+# Second-order (spline) flow: x' = K(x) a, a' = -applyDiffKT(x,[a],[a]) + rho
+# (pointEvolution.py:882-1084).  Host loops over time steps; every kernel sum is one launch.
+# ---------------------------------------------------------------------------------------
+def _affine_given(affine, T, dim, dev):
+    """(A (T,d,d), b (T,d)) device tensors when affine is given (zeros included: the
+    reference branches on presence, not value), else None."""
+    if affine is None:
+        return None
+    A = rt.as_dev(affine[0] if isinstance(affine[0], torch.Tensor)
+                  else np.asarray(affine[0], dtype=np.float64)).reshape(T, dim, dim)
+    b = rt.as_dev(affine[1] if isinstance(affine[1], torch.Tensor)
+                  else np.asarray(affine[1], dtype=np.float64)).reshape(T, dim)
+    return A, b
+
+
+def _second_order_velocity(x, a, rho, K):
+    """(K(x) a, -applyDiffKT(x,[a],[a]) + rho) on device tensors."""
+    zx = apply_k_dev(x, x, a, K)
+    za = rho - apply_diff_kt_dev(x, x, a[None], a[None], K)
+    return zx, za
+
+
+def secondOrderEvolution(x0, a0, rhot, KparDiff, withJacobian=False, withPointSet=None,
+                         affine=None):
+    """Explicit Euler for x' = K(x) a, a' = -applyDiffKT(x,[a],[a]) + rho with T = rhot.shape[0]
+    steps and the affine step U = I + dt A_k (pointEvolution.py:882-948).  Returns (xt, at) or
+    [xt, at, zt?, Jt?]; the Jacobian is taken at the point set when one is given, else at the
+    points (the reference's withJacobian without a point set reads undefined names)."""
+    x0d, np_out = _in(x0)
+    a0d, _ = _in(a0)
+    rhod, _ = _in(rhot)
+    T = rhod.shape[0]
+    N, dim = x0d.shape
+    dev = x0d.device
+    dt = 1.0 / T
+    aff = _affine_given(affine, T, dim, dev)
+    xt = torch.empty((T + 1, N, dim), dtype=torch.float64, device=dev)
+    at = torch.empty_like(xt)
+    xt[0], at[0] = x0d, a0d
+    zt = Jt = None
+    if withPointSet is not None:
+        z0, _ = _in(withPointSet)
+        zt = torch.empty((T + 1,) + tuple(z0.shape), dtype=torch.float64, device=dev)
+        zt[0] = z0
+    if withJacobian:
+        Jt = torch.zeros((T + 1, N if zt is None else zt.shape[1]), dtype=torch.float64, device=dev)
+    eye = torch.eye(dim, dtype=torch.float64, device=dev)
+    for k in range(T):
+        x, a = xt[k], at[k]
+        zx, za = _second_order_velocity(x, a, rhod[k], KparDiff)
+        if aff is not None:
+            U = eye + dt * aff[0][k]
+            xt[k + 1] = (x + dt * zx) @ U.T + dt * aff[1][k]
+            at[k + 1] = (a + dt * za) @ torch.linalg.inv(U)
+        else:
+            xt[k + 1] = x + dt * zx
+            at[k + 1] = a + dt * za
+        if zt is not None:
+            z = zt[k]
+            zz = apply_k_dev(z, x, a, KparDiff)
+            zt[k + 1] = (z + dt * zz) @ U.T + dt * aff[1][k] if aff is not None else z + dt * zz
+        if Jt is not None:
+            Jt[k + 1] = Jt[k] + dt * _divergence(zt[k] if zt is not None else x, x, a, KparDiff)
+            if aff is not None:
+                Jt[k + 1] += dt * torch.trace(aff[0][k])
+    if zt is None and Jt is None:
+        return _out(xt, np_out), _out(at, np_out)
+    out = [_out(xt, np_out), _out(at, np_out)]
+    if zt is not None:
+        out.append(_out(zt, np_out))
+    if Jt is not None:
+        out.append(_out(Jt, np_out))
+    return out
+
+
+def secondOrderHamiltonian(x, a, rho, px, pa, KparDiff, affine=None):
+    """<px, K(x) a> + <pa, -applyDiffKT(x,[a],[a]) + rho> - |rho|^2/2 (+ affine terms)
+    (pointEvolution.py:952-962); a float."""
+    xd, ad, rd, pxd, pad = (_in(v)[0] for v in (x, a, rho, px, pa))
+    zx, za = _second_order_velocity(xd, ad, rd, KparDiff)
+    H = (pxd * zx).sum() + (pad * za).sum() - (rd ** 2).sum() / 2
+    if affine is not None:
+        A, b = (_in(v)[0] for v in affine)
+        H = H + (pxd * (xd @ A.T + b)).sum() - (pad * (ad @ A)).sum()
+    return float(H)
+
+
+def secondOrderCovector(x0, a0, rhot, px1, pa1, KparDiff, affine=None, times=None):
+    """Backward adjoint of secondOrderEvolution (pointEvolution.py:965-1037) with the
+    time-series end points px1[j], pa1[j] added at steps (j+1) T / len(px1).  ``times`` is
+    accepted and unused (the reference computes an unused value from it and, when it is
+    None, raises while doing so).  Returns (pxt, pat, xt, at)."""
+    x0d, np_out = _in(x0)
+    a0d, _ = _in(a0)
+    rhod, _ = _in(rhot)
+    px1d = [_in(p)[0] for p in px1]
+    pa1d = [_in(p)[0] for p in pa1]
+    T = rhod.shape[0]
+    nTarg = len(px1d)
+    Tsize1 = T // nTarg
+    N, dim = x0d.shape
+    dev = x0d.device
+    dt = 1.0 / T
+    aff = _affine_given(affine, T, dim, dev)
+    xt, at = secondOrderEvolution(x0d, a0d, rhod, KparDiff)
+    pxt = torch.zeros((T + 1, N, dim), dtype=torch.float64, device=dev)
+    pat = torch.zeros_like(pxt)
+    pxt[T], pat[T] = px1d[nTarg - 1], pa1d[nTarg - 1]
+    eye = torch.eye(dim, dtype=torch.float64, device=dev)
+    for t in range(T):
+        px, pa = pxt[T - t], pat[T - t]
+        x, a = xt[T - t - 1], at[T - t - 1]
+        if aff is not None:
+            U = eye + dt * aff[0][T - t - 1]
+            px_ = px @ U
+            pa_ = pa @ torch.linalg.inv(U).T
+        else:
+            px_, pa_ = px, pa
+        a1 = torch.stack([px_, a])
+        a2 = torch.stack([a, px_])
+        zpx = (apply_diff_kt_dev(x, x, a1, a2, KparDiff)
+               - apply_ddiff_k11and12_dev(x, a, a, pa_.contiguous(), KparDiff))
+        zpa = apply_k_dev(x, x, px_.contiguous(), KparDiff) - apply_diff_k_dev(x, pa_, a, KparDiff, 3)
+        pxt[T - t - 1] = px_ + dt * zpx
+        pat[T - t - 1] = pa_ + dt * zpa
+        if t < T - 1 and (T - 1 - t) % Tsize1 == 0:
+            pxt[T - t - 1] += px1d[(T - t - 1) // Tsize1 - 1]
+            pat[T - t - 1] += pa1d[(T - t - 1) // Tsize1 - 1]
+    return _out(pxt, np_out), _out(pat, np_out), _out(xt, np_out), _out(at, np_out)
+
+
+def secondOrderGradient(x0, a0, rhot, px1, pa1, KparDiff, times=None, getCovector=False,
+                        affine=None, controlWeight=1.0):
+    """Gradient of 1/2 <a0, K(x0) a0> + sum_t dt cw |rho_t|^2/2 + data terms
+    (pointEvolution.py:1040-1084): da0 = K(x0) a0 - pa_0, and, when ``affine`` is given (zeros
+    included), drhot = cw rho_t - pa_{t+1} (in the affine frame) with dA, db; without affine
+    drhot stays zero, as in the reference."""
+    x0d, np_out = _in(x0)
+    a0d, _ = _in(a0)
+    rhod, _ = _in(rhot)
+    pxt, pat, xt, at = secondOrderCovector(x0d, a0d, rhod, px1, pa1, KparDiff, affine=affine,
+                                           times=times)
+    T = rhod.shape[0]
+    dim = x0d.shape[1]
+    dt = 1.0 / T
+    drhot = torch.zeros_like(rhod)
+    aff = _affine_given(affine, T, dim, x0d.device)
+    if aff is not None:
+        eye = torch.eye(dim, dtype=torch.float64, device=x0d.device)
+        dA = torch.zeros_like(aff[0])
+        for k in range(T):
+            x, a, rho = xt[k], at[k], rhod[k]
+            px, pa = pxt[k + 1], pat[k + 1]
+            zx, za = _second_order_velocity(x, a, rho, KparDiff)
+            zx = x + dt * zx
+            za = a + dt * za
+            Ui = torch.linalg.inv(eye + dt * aff[0][k])
+            pa = pa @ Ui.T
+            za = za @ Ui
+            dA[k] = px.T @ zx - za.T @ pa
+            drhot[k] = rho * controlWeight - pa
+        db = pxt[1:T + 1].sum(dim=1)
+    da0 = apply_k_dev(x0d, x0d, a0d, KparDiff) - pat[0]
+    o = lambda v: _out(v, np_out)
+    if aff is None:
+        base = (o(da0), o(drhot), o(xt), o(at))
+    else:
+        base = (o(da0), o(drhot), o(dA), o(db), o(xt), o(at))
+    return base + (o(pxt), o(pat)) if getCovector else base

@@ -579,3 +579,63 @@ def test_dense_kernel_matrices_reproduce_the_sums():
     Ks = orc.kernel_matrix(None, x, 0.6, 3, True)
     np.testing.assert_allclose(Ks @ a, orc.apply_k(x, x, a, 0.6, 3), rtol=1e-6, atol=1e-7)
     assert np.allclose(np.diag(Ks), 1.0) and np.allclose(Ks, Ks.T)
+
+
+# ---------------------------------------------------------------- second-order flow
+def _second_order_case(seed=5, N=18, T=8):
+    rng = np.random.default_rng(seed)
+    x0 = rng.standard_normal((N, 3)) * 0.5
+    a0 = 0.3 * rng.standard_normal((N, 3))
+    rhot = 0.2 * rng.standard_normal((T, N, 3))
+    return x0, a0, rhot
+
+
+@pytest.mark.parametrize("name,order", [("gauss", 10), ("laplacian", 3)])
+def test_second_order_gradient_is_the_adjoint(name, order):
+    """secondOrderGradient differentiates 1/2 <a0, K(x0) a0> + sum_t dt cw |rho_t|^2 / 2
+    - <px1[0], x_{T/2}> - <px1[1], x_T> (the objective of secondOrderMatching.py:204-212 with
+    linear end-point terms): da0 is the Euclidean gradient in a0, drhot the one in rho_t
+    divided by dt.  Zero affine parts take the reference's affine branch, as its driver
+    does."""
+    K = orc.Kernel(name, sigma=0.6, order=order)
+    x0, a0, rhot = _second_order_case()
+    T, N = rhot.shape[0], x0.shape[0]
+    rng = np.random.default_rng(9)
+    px1 = [rng.standard_normal((N, 3)), rng.standard_normal((N, 3))]
+    pa1 = [np.zeros((N, 3)), np.zeros((N, 3))]
+    cw = 0.7
+    aff = (np.zeros((T, 3, 3)), np.zeros((T, 3)))
+
+    def J(a0_, rho_):
+        xt, at = orc.secondOrderEvolution(x0, a0_, rho_, K, affine=aff)
+        return (0.5 * (a0_ * K.applyK(x0, a0_)).sum() + sum(cw * (r ** 2).sum() / 2 for r in rho_) / T
+                - (px1[0] * xt[T // 2]).sum() - (px1[1] * xt[T]).sum())
+
+    da0, drhot, dA, db, xt, at = orc.secondOrderGradient(x0, a0, rhot, px1, pa1, K, affine=aff,
+                                                         controlWeight=cw)
+    h = 1e-6
+    d = rng.standard_normal(a0.shape)
+    fd = (J(a0 + h * d, rhot) - J(a0 - h * d, rhot)) / (2 * h)
+    assert abs(fd - (da0 * d).sum()) < 1e-6 * max(1.0, abs(fd))
+    dr = rng.standard_normal(rhot.shape)
+    fd = (J(a0, rhot + h * dr) - J(a0, rhot - h * dr)) / (2 * h)
+    assert abs(fd - (drhot * dr).sum() / T) < 1e-6 * max(1.0, abs(fd))
+    # without affine parts the reference leaves drhot at zero
+    assert not orc.secondOrderGradient(x0, a0, rhot, px1, pa1, K)[1].any()
+
+
+def test_second_order_hamiltonian_drives_the_evolution():
+    """dH/dpx = K(x) a and dH/dpa = -applyDiffKT(x,[a],[a]) + rho: the evolution's velocities,
+    by FD of the Hamiltonian."""
+    K = orc.Kernel('gauss', sigma=0.5)
+    x0, a0, rhot = _second_order_case(N=10)
+    rng = np.random.default_rng(2)
+    px, pa = rng.standard_normal(x0.shape), rng.standard_normal(x0.shape)
+    h = 1e-6
+    for c, v in ((0, K.applyK(x0, a0)),
+                 (1, -K.applyDiffKT(x0, a0[None], a0[None]) + rhot[0])):
+        e = rng.standard_normal(x0.shape)
+        args = lambda s: (px + s * e, pa) if c == 0 else (px, pa + s * e)
+        fd = (orc.secondOrderHamiltonian(x0, a0, rhot[0], *args(h), K)
+              - orc.secondOrderHamiltonian(x0, a0, rhot[0], *args(-h), K)) / (2 * h)
+        assert abs(fd - (v * e).sum()) < 1e-6 * max(1.0, abs(fd))
