@@ -148,7 +148,9 @@ struct KEval<float, LDDMM_KERNEL_LAPLACIAN> {
 // t = n + f with n = rint(t) (exact: the 1.5 * 2^52 shift rounds to an integer), |f| <= 1/2;
 // 2^f = sum_k (f ln 2)^k / k! truncated after degree 13 (remainder < 4e-18 on |f| <= 1/2,
 // Horner rounding ~2 ulp); 2^n is added to the exponent field.  t is clamped at -1000
-// (2^-1000 ~ 9e-302), which keeps every result a normal double.
+// (2^-1000 ~ 9e-302), which keeps every result a normal double.  (Estrin's form, depth 4
+// instead of 13 for three more multiplies, measured 6-9% slower in the FP64 energy sums:
+// they are bound by instruction issue, not by this chain's latency.)
 struct Exp2Coef {
   double c[14];
   constexpr Exp2Coef() : c() {

@@ -290,3 +290,22 @@ def test_symmetric_and_mixed_forms_small_cases():
                 assert abs(out[0] - ref) <= (1e-12 if prec == _lib.PREC_FP64 else 1e-6) * abs(ref)
             tol = 1e-12 if prec == _lib.PREC_FP64 else 1e-6
             assert abs(out[0] - out[1]) <= tol * float((l1 ** 2).sum()), (n, prec, op, out)
+
+
+def test_fp64_exponential_is_within_a_few_ulp():
+    """FP64 applyK with one source and unit weight returns the kernel's exponential itself:
+    exp(-r^2 / (2 sigma^2)) = 2^t with t = -log2(e) r^2 / (2 sigma^2), compared with NumPy's exp2
+    of the same t.  Beyond the polynomial's ~2 ulp, rounding t costs |t| ln 2 ulp."""
+    from registration_amd import kernelFunctions as kfun
+    sigma = 0.7
+    r = np.concatenate([np.geomspace(1e-3, 9.0, 4000), np.linspace(0.01, 9.0, 4000)]) * sigma
+    rng = np.random.default_rng(0)
+    u = rng.standard_normal((r.size, 3))
+    y = u / np.linalg.norm(u, axis=1, keepdims=True) * r[:, None]
+    K = kfun.Kernel('gauss', sigma=sigma, precision='fp64')
+    got = K.applyK(np.zeros((1, 3)), np.ones((1, 1)), firstVar=y)[:, 0]
+    r2 = (y * y).sum(axis=1)
+    t = -(np.log2(np.e) / (2 * sigma * sigma)) * r2
+    ref = np.exp2(t)
+    rel = np.abs(got - ref) / ref
+    assert np.all(rel <= 2.3e-16 * (4 + 2 * np.abs(t))), float((rel / (4 + 2 * np.abs(t))).max())
