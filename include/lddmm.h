@@ -50,13 +50,17 @@ enum {
    * over j < na                                             (applyDiffKT) */
   LDDMM_OP_DIFF_KT = 1,
   /* one explicit Euler step of the landmark flow, targets = sources = y:
-   *   v = K(y, y) w0;  out0 = y + dt v;  out1 = v (if non-NULL);
-   *   rowdot[k] = <w0[k], v[k]>;  total[0] = sum_k rowdot[k]   */
+   *   v = K(y, y) w0;  out0 = y + dt v [+ dt (A y + b)];  out1 = v (if non-NULL);
+   *   rowdot[k] = <w0[k], v[k]>;  total[0] = sum_k rowdot[k]
+   * with the optional affine part A = w1 (d x d row-major), b = w2 (d), both or
+   * neither (pointEvolution.py:697-699)                       */
   LDDMM_OP_FLOW_STEP = 2,
   /* one backward step of the Hamiltonian covector, targets = sources = y,
    * a = w0, p = w1:
    *   out0 = p + dt sum_l 2K'(r^2)(y_k - y_l)[<p_k - 2 lambda a_k, a_l> + <a_k, p_l>]
-   * with lambda = coef                                   */
+   *          [+ dt p A]
+   * with lambda = coef and the optional affine matrix A = w2 (d x d row-major,
+   * pointEvolution.py:762-763)                                  */
   LDDMM_OP_COVECTOR_STEP = 3,
   /* RKHS products: u = K(y, y) w0; total[j] = sum_k <g_j[k], u[k]>, j < ndot */
   LDDMM_OP_KDOT = 4,

@@ -492,14 +492,89 @@ def landmarkHamiltonianCovector(x0, at, px1, KparDiff, regweight, affine=None):
 
 
 def landmarkHamiltonianGradient(x0, at, px1, KparDiff, regweight, getCovector=False, affine=None):
-    """pointEvolution.py:793-817 (affine=None)."""
+    """pointEvolution.py:793-817."""
     pxt, xt = landmarkHamiltonianCovector(x0, at, px1, KparDiff, regweight, affine=affine)
     dat = np.zeros(at.shape)
+    if affine is not None:
+        dA = np.zeros(affine[0].shape)
+        db = np.zeros(affine[1].shape)
     for k in range(at.shape[0]):
         dat[k] = 2 * regweight * at[k] - pxt[k + 1]
-    if getCovector:
-        return dat, xt, pxt
-    return dat, xt
+        if affine is not None:
+            dA[k] = np.dot(pxt[k + 1].T, xt[k])
+            db[k] = pxt[k + 1].sum(axis=0)
+    if affine is None:
+        return (dat, xt, pxt) if getCovector else (dat, xt)
+    return (dat, dA, db, xt, pxt) if getCovector else (dat, dA, db, xt)
+
+
+class AffineBasis:
+    """affineBasis.AffineBasis (affineBasis.py:3-79); the entry the reference indexes with the
+    float (i+1.) (:41) is indexed with the integer i+1."""
+
+    def __init__(self, dim=3, affine='affine'):
+        u = 1.0 / np.sqrt(2.0)
+        dimSym = (dim * (dim - 1)) // 2
+        affCode = {'affine': 1, 'similitude': 2, 'euclidean': 3, 'translation': 4}.get(affine, 5)
+        self.dim = dim
+        self.rotComp, self.simComp, self.affComp, self.transComp = [], [], [], []
+        if affCode == 1:
+            self.affineDim = 2 * (dimSym + dim)
+        elif affCode == 2:
+            self.affineDim = dimSym + 1 + dim
+        elif affCode == 3:
+            self.affineDim = dimSym + dim
+        elif affCode == 4:
+            self.affineDim = dim
+        else:
+            self.basis = []
+            self.affineDim = 0
+            return
+        self.basis = np.zeros([2 * (dimSym + dim), self.affineDim])
+        k = 0
+        if affCode <= 1:
+            for i in range(dimSym):
+                for j in range(i + 1, dim):
+                    self.basis[i * dim + j, k] = u
+                    self.basis[j * dim + i, k] = u
+                    k += 1
+            for i in range(dim - 1):
+                uu = np.sqrt((1 - 1.0 / (i + 2))) / (i + 1.)
+                self.basis[(i + 1) * dim + (i + 1), k] = np.sqrt(1 - 1.0 / (i + 2))
+                for j in range(i + 1):
+                    self.basis[j * dim + j, k] = -uu
+                k += 1
+            self.affComp = list(range(k))
+        if affCode <= 2:
+            k0 = k
+            for i in range(dim):
+                self.basis[i * dim + i, k] = 1. / np.sqrt(dim)
+            k += 1
+            self.simComp = list(range(k0, k))
+        if affCode <= 3:
+            k0 = k
+            for i in range(dim):
+                for j in range(i + 1, dim):
+                    self.basis[i * dim + j, k] = u
+                    self.basis[j * dim + i, k] = -u
+                    k += 1
+            self.rotComp = list(range(k0, k))
+        k0 = k
+        for i in range(dim):
+            self.basis[i + dim ** 2, k] = 1
+            k += 1
+        self.transComp = list(range(k0, k))
+
+    def getTransforms(self, Afft):
+        Tsize = Afft.shape[0]
+        dim2 = self.dim ** 2
+        A = [np.zeros([Tsize, self.dim, self.dim]), np.zeros([Tsize, self.dim])]
+        if self.affineDim > 0:
+            for t in range(Tsize):
+                AB = np.dot(self.basis, Afft[t])
+                A[0][t] = AB[0:dim2].reshape([self.dim, self.dim])
+                A[1][t] = AB[dim2:dim2 + self.dim]
+        return A
 
 
 # --------------------------------------------------------------------------------------
@@ -929,9 +1004,15 @@ class Direction:
 
 class _Matching:
     """Shared body of SurfaceMatching (surfaceMatching.py:74-484) and CurveMatching
-    (curveMatching.py:58-352) for affine='none', symmetric=False."""
+    (curveMatching.py:58-352) for symmetric=False, with the global affine part
+    (affine 'affine' / 'similitude' / 'euclidean' / 'translation' / 'none').  ``aff_style``
+    selects the two drivers' affine scalings: 'surface' (coeffAff, surfaceMatching.py:315-322,
+    367-369, 388-389, 475-481) or 'curve' (affineWeight, curveMatching.py:246-252,288-296)."""
 
-    def _setup(self, x0, KparDiff, KparDist, sigmaError, timeStep, regWeight):
+    aff_style = 'surface'
+
+    def _setup(self, x0, KparDiff, KparDist, sigmaError, timeStep, regWeight, affine='none',
+               affineWeight=1.0, rotWeight=None, scaleWeight=None, transWeight=None):
         self.x0 = np.copy(x0)
         self.npt, self.dim = x0.shape
         self.KparDiff, self.KparDist = KparDiff, KparDist
@@ -944,23 +1025,41 @@ class _Matching:
         self.obj = None
         self.objTry = None
         self.gradCoeff = self.npt
-        self.affineDim = 0
-        self.Afft = np.zeros([self.Tsize, 0])
-        self.AfftTry = np.zeros([self.Tsize, 0])
+        self.affB = AffineBasis(self.dim, affine)
+        self.affineDim = self.affB.affineDim
+        self.affineBasis = self.affB.basis
+        self.affineWeight = affineWeight * np.ones([self.affineDim, 1])
+        if self.affB.rotComp and rotWeight is not None:
+            self.affineWeight[self.affB.rotComp] = rotWeight
+        if self.affB.simComp and scaleWeight is not None:
+            self.affineWeight[self.affB.simComp] = scaleWeight
+        if self.affB.transComp and transWeight is not None:
+            self.affineWeight[self.affB.transComp] = transWeight
+        self.Afft = np.zeros([self.Tsize, self.affineDim])
+        self.AfftTry = np.zeros([self.Tsize, self.affineDim])
+        self.coeffAff1, self.coeffAff2 = 1., 100.
+        self.coeffAff = self.coeffAff1
+        self.affBurnIn = 20
         self.nobj = 0  # objective evaluations (updateTry + objectiveFun)
+
+    def _transforms(self, Afft):
+        return self.affB.getTransforms(Afft) if self.affineDim > 0 else None
 
     def dataTerm(self, fvDef):
         return self.fun_obj(fvDef, self.fv1, self.KparDist) / (self.sigmaError ** 2)
 
-    def objectiveFunDef(self, at, withTrajectory=False):
+    def objectiveFunDef(self, at, Afft=None, withTrajectory=False):
         timeStep = 1.0 / self.Tsize
-        xt = landmarkDirectEvolutionEuler(self.x0, at, self.KparDiff)
+        Afft = self.Afft if Afft is None else Afft
+        xt = landmarkDirectEvolutionEuler(self.x0, at, self.KparDiff, affine=self._transforms(Afft))
         obj = 0
         for t in range(self.Tsize):
             z = xt[t]
             a = at[t]
             ra = self.KparDiff.applyK(z, a)
             obj = obj + self.regweight * timeStep * np.multiply(a, ra).sum()
+            if self.affineDim > 0:
+                obj += timeStep * np.multiply(self.affineWeight.reshape(Afft[t].shape), Afft[t] ** 2).sum()
         if withTrajectory:
             return obj, xt
         return obj
@@ -968,7 +1067,7 @@ class _Matching:
     def objectiveFun(self):
         if self.obj is None:
             self.obj0 = self.fun_obj0(self.fv1, self.KparDist) / (self.sigmaError ** 2)
-            self.obj, self.xt = self.objectiveFunDef(self.at, withTrajectory=True)
+            self.obj, self.xt = self.objectiveFunDef(self.at, self.Afft, withTrajectory=True)
             self.fvDef.updateVertices(self.xt[-1])
             self.obj += self.obj0 + self.dataTerm(self.fvDef)
             self.nobj += 1
@@ -980,7 +1079,8 @@ class _Matching:
     def updateTry(self, dir, eps, objRef=None):
         objTry = self.obj0
         atTry = self.at - eps * dir.diff
-        foo = self.objectiveFunDef(atTry, withTrajectory=True)
+        AfftTry = self.Afft - eps * dir.aff if self.affineDim > 0 else self.Afft
+        foo = self.objectiveFunDef(atTry, AfftTry, withTrajectory=True)
         objTry += foo[0]
         ff = self._copy_shape(self.fvDef)
         ff.updateVertices(foo[1][-1])
@@ -991,6 +1091,7 @@ class _Matching:
         if (objRef is None) or (objTry < objRef):
             self.atTry = atTry
             self.objTry = objTry
+            self.AfftTry = AfftTry
         return objTry
 
     def endPointGradient(self):
@@ -998,10 +1099,29 @@ class _Matching:
 
     def getGradient(self, coeff=1.0):
         px1 = -self.endPointGradient()
-        foo = landmarkHamiltonianGradient(self.x0, self.at, px1, self.KparDiff, self.regweight)
+        A = self._transforms(self.Afft)
+        foo = landmarkHamiltonianGradient(self.x0, self.at, px1, self.KparDiff, self.regweight,
+                                          affine=A)
         grd = Direction()
         grd.diff = foo[0] / (coeff * self.Tsize)
         grd.aff = np.zeros(self.Afft.shape)
+        if self.affineDim > 0:
+            dA, db = foo[1], foo[2]
+            dim2 = self.dim ** 2
+            w = self.affineWeight.reshape([1, self.affineDim])
+            grd.aff = 2 * np.multiply(w, self.Afft) if self.aff_style == 'surface' else 2 * self.Afft
+            for t in range(self.Tsize):
+                dAff = np.dot(self.affineBasis.T, np.vstack([dA[t].reshape([dim2, 1]),
+                                                             db[t].reshape([self.dim, 1])]))
+                if self.aff_style == 'surface':
+                    grd.aff[t] -= dAff.reshape(grd.aff[t].shape)
+                else:
+                    grd.aff[t] -= np.divide(dAff.reshape(grd.aff[t].shape),
+                                            self.affineWeight.reshape(grd.aff[t].shape))
+            if self.aff_style == 'surface':
+                grd.aff /= (self.coeffAff * coeff * self.Tsize)
+            else:
+                grd.aff /= (coeff * self.Tsize)
         return grd
 
     def addProd(self, dir1, dir2, beta):
@@ -1021,8 +1141,17 @@ class _Matching:
         for t in range(self.Tsize):
             z = self.xt[t]
             u = self.KparDiff.applyK(z, g1.diff[t])
+            if self.aff_style == 'curve':
+                uu = np.multiply(g1.aff[t], self.affineWeight.reshape(g1.aff[t].shape))
+            else:
+                uu = g1.aff[t]
             for ll, gr in enumerate(g2):
                 res[ll] = res[ll] + np.multiply(gr.diff[t], u).sum()
+                if self.affineDim > 0:
+                    if self.aff_style == 'curve':
+                        res[ll] += np.multiply(uu, gr.aff[t]).sum()
+                    else:
+                        res[ll] += np.multiply(uu, gr.aff[t]).sum() * self.coeffAff
         return res
 
     def acceptVarTry(self):
@@ -1032,7 +1161,9 @@ class _Matching:
 
     def endOfIteration(self):
         self.iter += 1
-        obj1, self.xt = self.objectiveFunDef(self.at, withTrajectory=True)
+        if self.aff_style == 'surface' and self.iter >= self.affBurnIn:
+            self.coeffAff = self.coeffAff2
+        obj1, self.xt = self.objectiveFunDef(self.at, self.Afft, withTrajectory=True)
         self.fvDef.updateVertices(self.xt[-1])
 
 
@@ -1041,7 +1172,8 @@ class SurfaceMatching(_Matching):
     (surfaceMatching.py:29-40,74-484)."""
 
     def __init__(self, Template, Target, KparDiff, KparDist, sigmaError=1.0, timeStep=.1,
-                 regWeight=1.0, maxIter=1000, errorType='current'):
+                 regWeight=1.0, maxIter=1000, errorType='current', affine='none', affineWeight=1.0,
+                 rotWeight=None, scaleWeight=None, transWeight=None):
         self.fv0 = Surface(surf=Template)
         self.fv1 = Surface(surf=Target)
         v0 = self.fv0.surfVolume()
@@ -1051,7 +1183,8 @@ class SurfaceMatching(_Matching):
         self.fvInit = Surface(surf=self.fv0)
         self.fvDef = Surface(surf=self.fvInit)
         self.fun_obj0, self.fun_obj, self.fun_objGrad = _DATA_TERMS['surface'][errorType]
-        self._setup(self.fvInit.vertices, KparDiff, KparDist, sigmaError, timeStep, regWeight)
+        self._setup(self.fvInit.vertices, KparDiff, KparDist, sigmaError, timeStep, regWeight,
+                    affine, affineWeight, rotWeight, scaleWeight, transWeight)
         self.maxIter = maxIter
         self.iter = 0
         self.epsInit = 0.1
@@ -1062,9 +1195,13 @@ class SurfaceMatching(_Matching):
         return Surface(surf=s)
 
     def optimizeMatching(self, cgfun=None):
+        """surfaceMatching.py:474-484: the gradient bound with coeffAff2, the optimisation
+        starting with coeffAff1."""
+        self.coeffAff = self.coeffAff2
         grd = self.getGradient(self.gradCoeff)
         [grd2] = self.dotProduct(grd, [grd])
         self.gradEps = max(self.gradLB, np.sqrt(grd2) / 10000)
+        self.coeffAff = self.coeffAff1
         return cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
 
 
@@ -1072,13 +1209,17 @@ class CurveMatching(_Matching):
     """curveMatching.CurveMatching with errorType 'current' or 'measure'
     (curveMatching.py:24-33,58-352)."""
 
+    aff_style = 'curve'
+
     def __init__(self, Template, Target, KparDiff, KparDist, sigmaError=1.0, timeStep=.1,
-                 regWeight=1.0, maxIter=1000, gradLB=0.001, errorType='current'):
+                 regWeight=1.0, maxIter=1000, gradLB=0.001, errorType='current', affine='none',
+                 affineWeight=1.0, rotWeight=None, scaleWeight=None, transWeight=None):
         self.fv0 = Curve(curve=Template)
         self.fv1 = Curve(curve=Target)
         self.fvDef = Curve(curve=self.fv0)
         self.fun_obj0, self.fun_obj, self.fun_objGrad = _DATA_TERMS['curve'][errorType]
-        self._setup(self.fv0.vertices, KparDiff, KparDist, sigmaError, timeStep, regWeight)
+        self._setup(self.fv0.vertices, KparDiff, KparDist, sigmaError, timeStep, regWeight,
+                    affine, affineWeight, rotWeight, scaleWeight, transWeight)
         self.maxIter = maxIter
         self.iter = 0
         self.epsInit = 10.0

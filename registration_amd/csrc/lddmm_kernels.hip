@@ -1670,10 +1670,22 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
   } else if constexpr (OP == LDDMM_OP_FLOW_STEP) {
     if (valid) {
       const double dt = sg->dt;
+      const double *A = sg->w1, *bv = sg->w2;  // optional affine part (d x d, d)
+      double yk[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) yk[c] = sg->y[k * D + c];
 #pragma unroll
       for (int c = 0; c < D; ++c) {
         double v = accK0[c];
-        sg->out0[k * D + c] = sg->y[k * D + c] + dt * v;
+        double x1 = yk[c] + dt * v;
+        if (A) {
+          // x_{t+1} += dt (A x_t + b), added after the kernel step as the reference does
+          double ax = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) ax += yk[j] * A[c * D + j];
+          x1 += dt * (ax + bv[c]);
+        }
+        sg->out0[k * D + c] = x1;
         if (sg->out1) sg->out1[k * D + c] = v;
         rd0 += sg->w0[k * D + c] * v;
       }
@@ -1682,9 +1694,22 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
   } else if constexpr (OP == LDDMM_OP_COVECTOR_STEP) {
     if (valid) {
       const double dt = sg->dt;
+      const double *A = sg->w2;  // optional affine part (d x d)
+      double pk[D];
 #pragma unroll
-      for (int c = 0; c < D; ++c)
-        sg->out0[k * D + c] = sg->w1[k * D + c] + dt * (dscale * accD[c]);
+      for (int c = 0; c < D; ++c) pk[c] = sg->w1[k * D + c];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double p0 = pk[c] + dt * (dscale * accD[c]);
+        if (A) {
+          // p_t += dt p_{t+1} A, after the kernel term as the reference does
+          double pa = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) pa += pk[j] * A[j * D + c];
+          p0 += dt * pa;
+        }
+        sg->out0[k * D + c] = p0;
+      }
     }
   } else if constexpr (OP == LDDMM_OP_KDOT) {
     if (valid) {
@@ -2145,6 +2170,7 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
         break;
       case LDDMM_OP_FLOW_STEP:
         if (!s.w0 || !s.out0) return LDDMM_EINVAL;
+        if (!s.w1 != !s.w2) return LDDMM_EINVAL;  // affine part: both A and b, or neither
         nx0 = s.ny;
         break;
       case LDDMM_OP_COVECTOR_STEP:

@@ -3,7 +3,9 @@
 
 ``CurveMatchingParam`` and ``CurveMatching`` keep the reference constructors and optimiser
 protocol on a GPU-resident registration (matching.DeviceMatching).  Supported: the currents
-and measure data terms (errorType 'current' or 'measure', the default), affine='none'.  The per-iteration Jacobian flow and the VTK snapshots of
+and measure data terms (errorType 'current' or 'measure', the default) and the global affine
+part (affine 'affine', 'similitude', 'euclidean', 'translation' or 'none'; the curve driver
+weighs its gradient and products by affineWeight, not coeffAff).  The per-iteration Jacobian flow and the VTK snapshots of
 endOfIteration / endOptim only produce output files and are not run.
 """
 from __future__ import annotations
@@ -38,6 +40,7 @@ class CurveMatchingParam(matchingParam.MatchingParam):
 class CurveMatching(DeviceMatching):
     coef = 0.5
     measure_norm0_eps = 0.0   # curves.measureNorm0 weights are |lenel| (curves.py:624)
+    aff_style = 'curve'
 
     def __init__(self, Template=None, Target=None, fileTempl=None, fileTarg=None, param=None,
                  maxIter=1000, regWeight=1.0, affineWeight=1.0, verb=True, gradLB=0.001,
@@ -45,8 +48,6 @@ class CurveMatching(DeviceMatching):
                  testGradient=False, saveFile='evolution', affine='none', outputDir='.'):
         if Template is None or Target is None:
             raise NotImplementedError("Template and Target curves are required (no file readers)")
-        if affine != 'none':
-            raise NotImplementedError("the GPU path supports affine='none'")
         self.fv0 = curves.Curve(curve=Template)
         self.fv1 = curves.Curve(curve=Target)
         self.param = CurveMatchingParam() if param is None else param
@@ -66,7 +67,9 @@ class CurveMatching(DeviceMatching):
         self._init_device(np.copy(self.fv0.vertices), self.fv0.faces, self.fv1.centers,
                           self.fv1.lenel, self.param.KparDiff, self.param.KparDist,
                           self.param.sigmaError, Tsize, regWeight,
-                          data_term=self.param.errorType)
+                          data_term=self.param.errorType, affine=affine,
+                          affineWeight=affineWeight, rotWeight=rotWeight, scaleWeight=scaleWeight,
+                          transWeight=transWeight)
 
     def _new_shape(self, vertices, with_weights=False):
         return curves.DeviceCurve(self.faces_np, vertices, with_weights=with_weights)

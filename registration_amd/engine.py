@@ -90,6 +90,8 @@ class FlowJob:
     v: Optional[torch.Tensor] = None  # (T, N, d) optional velocities K(x_t) a_t
     dt: Optional[float] = None          # time step; 1/T when None
     init: bool = True                   # copy x0 into xt[0] (False: xt[0] already holds x0)
+    A: Optional[torch.Tensor] = None    # (T, d, d) affine part, with b (T, d): x += dt (A x + b)
+    b: Optional[torch.Tensor] = None
 
     @property
     def T(self):
@@ -111,16 +113,18 @@ def flow_record(N: int, T: int, K: DevKernel, dt: Optional[float] = None) -> np.
     return r
 
 
-def shoot_records(base, prec, d, T, xt, at, energy, v=None) -> None:
+def shoot_records(base, prec, d, T, xt, at, energy, v=None, A=None, b=None) -> None:
     """Flow launches for n trajectories of one (kernel family, precision, d) from arrays:
     ``base`` (n,) flow_record rows; T (n,) step counts; xt, at, energy (n,) device base
     addresses of the (T+1, N, d) trajectories, (T, N, d) momenta and (T,) energy rows; v
-    optional velocity outputs (0 = none).  Time step t of every trajectory with t < T goes
-    into launch t."""
+    optional velocity outputs, A, b optional affine parts (T, d, d) and (T, d) (0 = none).
+    Time step t of every trajectory with t < T goes into launch t."""
     T = np.asarray(T, dtype=np.int64)
     stride = base["ny"].astype(np.uint64) * np.uint64(d * 8)
     xt, at, en = _u64(xt), _u64(at), _u64(energy)
     vv = _u64(v) if v is not None else None
+    AA = _u64(A) if A is not None else None
+    bb = _u64(b) if b is not None else None
     for t0, t1, idx in _runs(T, int(T.max())):
         i = np.asarray(idx)
         tt = np.arange(t0, t1, dtype=np.uint64)[:, None]
@@ -131,6 +135,9 @@ def shoot_records(base, prec, d, T, xt, at, energy, v=None) -> None:
         steps["out0"] = xt[i] + (tt + 1) * st
         if vv is not None:
             steps["out1"] = np.where(vv[i] != 0, vv[i] + tt * st, 0)
+        if AA is not None:
+            steps["w1"] = np.where(AA[i] != 0, AA[i] + tt * np.uint64(d * d * 8), 0)
+            steps["w2"] = np.where(bb[i] != 0, bb[i] + tt * np.uint64(d * 8), 0)
         steps["total"] = en[i] + tt * np.uint64(8)
         rt.run_steps(_lib.OP_FLOW_STEP, prec, d, steps)
 
@@ -143,9 +150,12 @@ def shoot(jobs: Sequence[FlowJob]) -> None:
     for group in _groups(jobs, lambda j: (j.K.family, j.x0.shape[1])):
         d = group[0].x0.shape[1]
         base = rt.stack([rt.raw(flow_record(j.xt.shape[1], j.T, j.K, j.step)) for j in group])
+        aff = any(j.A is not None for j in group)
         shoot_records(base, group[0].K.prec, d, [j.T for j in group],
                       [j.xt.data_ptr() for j in group], [j.at.data_ptr() for j in group],
-                      [j.energy.data_ptr() for j in group], [rt.ptr(j.v) for j in group])
+                      [j.energy.data_ptr() for j in group], [rt.ptr(j.v) for j in group],
+                      [rt.ptr(j.A) for j in group] if aff else None,
+                      [rt.ptr(j.b) for j in group] if aff else None)
 
 
 # --------------------------------------------------------------------------------------
@@ -160,6 +170,7 @@ class CovectorJob:
     regweight: float
     pxt: torch.Tensor           # (T+1, N, d) output
     dt: Optional[float] = None  # time step; 1/T when None
+    A: Optional[torch.Tensor] = None  # (T, d, d) affine part: p_t += dt p_{t+1} A_t
 
     @property
     def T(self):
@@ -178,15 +189,16 @@ def covector_record(N: int, T: int, K: DevKernel, regweight: float,
     return r
 
 
-def covector_records(base, prec, d, T, xt, at, pxt) -> None:
+def covector_records(base, prec, d, T, xt, at, pxt, A=None) -> None:
     """Covector launches for n trajectories of one (kernel family, precision, d) from arrays:
     ``base`` (n,) covector_record rows; T (n,) step counts; xt, at, pxt (n,) device base
     addresses of the (T+1, N, d) trajectories, (T, N, d) momenta and (T+1, N, d) covectors,
-    whose slice T already holds the end-point covector.  Launch s handles t = T - 1 - s of
-    every trajectory with s < T."""
+    whose slice T already holds the end-point covector; A optional (T, d, d) affine parts
+    (0 = none).  Launch s handles t = T - 1 - s of every trajectory with s < T."""
     T = np.asarray(T, dtype=np.int64)
     stride = base["ny"].astype(np.uint64) * np.uint64(d * 8)
     xt, at, px = _u64(xt), _u64(at), _u64(pxt)
+    AA = _u64(A) if A is not None else None
     for s0, s1, idx in _runs(T, int(T.max())):
         i = np.asarray(idx)
         ss = np.arange(s0, s1, dtype=np.uint64)[:, None]
@@ -197,6 +209,8 @@ def covector_records(base, prec, d, T, xt, at, pxt) -> None:
         steps["w0"] = at[i] + tt * st
         steps["w1"] = px[i] + (tt + 1) * st
         steps["out0"] = px[i] + tt * st
+        if AA is not None:
+            steps["w2"] = np.where(AA[i] != 0, AA[i] + tt * np.uint64(d * d * 8), 0)
         rt.run_steps(_lib.OP_COVECTOR_STEP, prec, d, steps)
 
 
@@ -207,9 +221,11 @@ def covector(jobs: Sequence[CovectorJob]) -> None:
         d = group[0].xt.shape[2]
         base = rt.stack([rt.raw(covector_record(j.xt.shape[1], j.T, j.K, j.regweight, j.step))
                          for j in group])
+        aff = any(j.A is not None for j in group)
         covector_records(base, group[0].K.prec, d, [j.T for j in group],
                          [j.xt.data_ptr() for j in group], [j.at.data_ptr() for j in group],
-                         [j.pxt.data_ptr() for j in group])
+                         [j.pxt.data_ptr() for j in group],
+                         [rt.ptr(j.A) for j in group] if aff else None)
 
 
 # --------------------------------------------------------------------------------------

@@ -30,6 +30,7 @@ import torch
 from . import _lib
 from . import engine
 from . import runtime as rt
+from .affineBasis import AffineBasis
 
 
 CURRENT = 'current'
@@ -57,14 +58,23 @@ class DeviceMatching:
     # surfaces, surfaces.py:1095; none for curves, curves.py:624)
     coef = 1.0 / 3.0
     measure_norm0_eps = 1e-10
+    # affine scalings of the gradient and dot products: 'surface' (coeffAff, switched from 1
+    # to 100 after affBurnIn iterations; surfaceMatching.py:315-322,367-369,388-389) or 'curve'
+    # (affineWeight; curveMatching.py:246-252,288-296)
+    aff_style = 'surface'
 
     def _init_device(self, x0_np, faces_np, target_centers, target_cr, KparDiff, KparDist,
-                     sigmaError, Tsize, regweight, target_points=None, data_term=CURRENT):
+                     sigmaError, Tsize, regweight, target_points=None, data_term=CURRENT,
+                     affine='none', affineWeight=1.0, rotWeight=None, scaleWeight=None,
+                     transWeight=None):
         """Device state.  With ``target_points`` (N, d) the data term is the landmark one,
         sum |x_T - y|^2 / sigmaError^2 over corresponding points, and the shape arguments
         are unused; otherwise it is the currents (data_term 'current'), measure ('measure') or
         varifold ('varifold', surfaces) norm against the target faces (target_centers,
-        target_cr)."""
+        target_cr).  ``affine`` adds the reference's global affine part A_t x + b_t to the
+        velocity, with coefficients Afft (T, affineDim) on an AffineBasis, penalised by
+        sum_t dt <affineWeight, Afft_t^2>; Afft and the directions' affine parts are device
+        tensors."""
         dev = rt.device()
         self.npt, self.dim = x0_np.shape
         self.Tsize = Tsize
@@ -145,9 +155,27 @@ class DeviceMatching:
         self.objTry = None
         self.obj0 = None
         self.gradCoeff = self.npt
-        self.Afft = np.zeros([self.Tsize, 0])
-        self.AfftTry = np.zeros([self.Tsize, 0])
-        self.affineDim = 0
+        self.affine = affine
+        self.affB = AffineBasis(self.dim, affine)
+        self.affineDim = self.affB.affineDim
+        self.affineBasis = self.affB.basis
+        self.affineWeight = affineWeight * np.ones([self.affineDim, 1])
+        if self.affB.rotComp and rotWeight is not None:
+            self.affineWeight[self.affB.rotComp] = rotWeight
+        if self.affB.simComp and scaleWeight is not None:
+            self.affineWeight[self.affB.simComp] = scaleWeight
+        if self.affB.transComp and transWeight is not None:
+            self.affineWeight[self.affB.transComp] = transWeight
+        self.coeffAff1, self.coeffAff2 = 1., 100.
+        self.coeffAff = self.coeffAff1
+        self.affBurnIn = 20
+        if self.affineDim > 0:
+            self.Afft = torch.zeros((T, self.affineDim), dtype=torch.float64, device=dev)
+            self._basis = rt.as_dev(self.affineBasis)                   # (d^2 + d, affineDim)
+            self._aff_w = rt.as_dev(self.affineWeight.reshape(-1))      # (affineDim,)
+        else:
+            self.Afft = np.zeros([self.Tsize, 0])
+        self.AfftTry = self.Afft
         self.nobj = 0
         self.ngrad = 0
         self.iter = 0
@@ -162,6 +190,25 @@ class DeviceMatching:
                                           self.cr2, K, self.coef)
         return engine.measure_record(sh.centers, sh.weights, self.c2, self.a2, K, self.coef,
                                      n1=sh.surfel if gradient else None)
+
+    # ----- global affine part -----------------------------------------------------------
+    def _aff_transforms(self, Afft):
+        """(A (T, d, d), b (T, d)) device tensors of the coefficients Afft (T, affineDim):
+        [vec(A_t); b_t] = basis Afft_t (affineBasis.py:67-79)."""
+        d = self.dim
+        AB = Afft @ self._basis.T
+        return AB[:, :d * d].reshape(-1, d, d).contiguous(), AB[:, d * d:].contiguous()
+
+    def _aff_penalty(self, Afft):
+        """(T,) device tensor of dt <affineWeight, Afft_t^2> (surfaceMatching.py:214-215)."""
+        return (Afft * Afft * self._aff_w).sum(dim=1) / self.Tsize
+
+    def _aff_dot_weight(self):
+        """(affineDim,) weights of the affine part of dot products: coeffAff (surfaces) or
+        affineWeight (curves)."""
+        if self.aff_style == 'curve':
+            return self._aff_w
+        return torch.full_like(self._aff_w, self.coeffAff)
 
     # ----- reference protocol, single registration -----------------------------------
     def objectiveFun(self):
@@ -184,6 +231,7 @@ class DeviceMatching:
     def acceptVarTry(self):
         self.obj = self.objTry
         self.at = self.atTry
+        self.Afft = self.AfftTry
         self.xt, self._xt_try = self._xt_try, self.xt
         # The deformed shape's geometry follows xt[-1]; it is refreshed (batched over
         # registrations) before the next request that reads it.
@@ -192,6 +240,8 @@ class DeviceMatching:
 
     def endOfIteration(self):
         self.iter += 1
+        if self.affineDim > 0 and self.aff_style == 'surface' and self.iter >= self.affBurnIn:
+            self.coeffAff = self.coeffAff2
 
     def addProd(self, dir1, dir2, beta):
         d = Direction()
@@ -202,14 +252,15 @@ class DeviceMatching:
     def copyDir(self, dir0):
         d = Direction()
         d.diff = dir0.diff.clone()
-        d.aff = np.copy(dir0.aff)
+        d.aff = dir0.aff.clone() if isinstance(dir0.aff, torch.Tensor) else np.copy(dir0.aff)
         return d
 
     def randomDir(self):
         d = Direction()
         d.diff = torch.randn((self.Tsize, self.npt, self.dim), dtype=torch.float64,
                              device=self.x0.device)
-        d.aff = np.random.randn(self.Tsize, self.affineDim)
+        d.aff = (torch.randn((self.Tsize, self.affineDim), dtype=torch.float64, device=self.x0.device)
+                 if self.affineDim > 0 else np.zeros([self.Tsize, 0]))
         return d
 
     # ----- helpers ---------------------------------------------------------------------
@@ -309,14 +360,20 @@ def batch_objective(opts: Sequence[DeviceMatching]) -> list:
 def launch_objective(opts: Sequence[DeviceMatching]) -> Pending:
     """batch_objective with its answers pending."""
     dev = rt.device()
-    lens = [o.Tsize + 2 for o in opts]
+    # per registration: T energies, the data term, the target norm, and T affine penalties
+    lens = [o.Tsize + 2 + (o.Tsize if o.affineDim > 0 else 0) for o in opts]
     scal = torch.zeros(sum(lens), dtype=torch.float64, device=dev)
     flows, norms, offs = [], [], []
     off = 0
     for o, L in zip(opts, lens):
         T = o.Tsize
         offs.append(off)
-        flows.append(engine.FlowJob(o.x0, o.at, o.Kdiff, o.xt, scal[off:off + T], init=False))
+        A = b = None
+        if o.affineDim > 0:
+            A, b = o._aff_transforms(o.Afft)
+            scal[off + T + 2:off + 2 * T + 2] = o._aff_penalty(o.Afft)
+        flows.append(engine.FlowJob(o.x0, o.at, o.Kdiff, o.xt, scal[off:off + T], init=False,
+                                    A=A, b=b))
         off += L
     engine.shoot(flows)
     _data_energies(opts, [o.xt[-1] for o in opts], [o.fvDef_dev for o in opts],
@@ -350,6 +407,8 @@ def launch_objective(opts: Sequence[DeviceMatching]) -> Pending:
             dt = 1.0 / T
             for t in range(T):
                 obj = obj + o.regweight * dt * vals[off + t]
+                if o.affineDim > 0:
+                    obj += vals[off + T + 2 + t]
             o.obj = obj + (o.obj0 + vals[off + T] / s2)
             o.nobj += 1
             out.append(o.obj)
@@ -379,7 +438,14 @@ def launch_update_try(items) -> Pending:
     Ts = np.fromiter((o.Tsize for o in opts), np.int64, n)
     offs = np.zeros(n, np.int64)
     np.cumsum(Ts[:-1] + 1, out=offs[1:])
-    scal = torch.zeros(int(offs[-1] + Ts[-1] + 1), dtype=torch.float64, device=dev)
+    aff = [i for i, o in enumerate(opts) if o.affineDim > 0]
+    # affine penalties of the trials after every registration's (T energies, data term)
+    pen_off = {}
+    end = int(offs[-1] + Ts[-1] + 1)
+    for i in aff:
+        pen_off[i] = end
+        end += int(Ts[i])
+    scal = torch.zeros(end, dtype=torch.float64, device=dev)
     sbase = np.uint64(scal.data_ptr())
     # atTry = at - eps dir for every registration (multi-tensor launches)
     ats = torch._foreach_mul([it[1].diff for it in items], [-float(it[2]) for it in items])
@@ -390,10 +456,24 @@ def launch_update_try(items) -> Pending:
     energy = sbase + np.uint64(8) * offs.astype(np.uint64)
     data_total = energy + np.uint64(8) * Ts.astype(np.uint64)
     final = xt + Ts.astype(np.uint64) * slice_bytes          # address of x_T in each trajectory
+    afft_try = {}
+    Aptr = np.zeros(n, np.uint64)
+    bptr = np.zeros(n, np.uint64)
+    keep = []                                                # affine parts live until launched
+    for i in aff:
+        o, dir_, eps = items[i][0], items[i][1], items[i][2]
+        afft_try[i] = o.Afft - float(eps) * dir_.aff
+        A, b = o._aff_transforms(afft_try[i])
+        keep.append((A, b))
+        Aptr[i], bptr[i] = A.data_ptr(), b.data_ptr()
+        scal[pen_off[i]:pen_off[i] + int(Ts[i])] = o._aff_penalty(afft_try[i])
     for idx in _by_key([o._flow_key for o in opts]):
         o0 = opts[idx[0]]
+        has_aff = bool(Aptr[idx].any())
         engine.shoot_records(rt.stack([opts[i]._flow_base for i in idx]), o0.Kdiff.prec,
-                             o0.dim, Ts[idx], xt[idx], at[idx], energy[idx])
+                             o0.dim, Ts[idx], xt[idx], at[idx], energy[idx],
+                             A=Aptr[idx] if has_aff else None, b=bptr[idx] if has_aff else None)
+    del keep
     cur = [i for i, o in enumerate(opts) if o.data_kind in SHAPE_TERMS]
     if cur:
         engine.geometry_records(rt.stack([opts[i]._geom_try for i in cur], engine.GEOM_DT),
@@ -411,13 +491,16 @@ def launch_update_try(items) -> Pending:
     def finish(host):
         vals = host.tolist()
         out = []
-        for (o, dir, eps, objRef), atTry, off in zip(items, ats, offs.tolist()):
+        for i, ((o, dir, eps, objRef), atTry, off) in enumerate(zip(items, ats, offs.tolist())):
             T = o.Tsize
             c = o.regweight * (1.0 / T)
             objTry = o.obj0
             objDef = 0
-            for e in vals[off:off + T]:
+            pen = vals[pen_off[i]:pen_off[i] + T] if i in pen_off else None
+            for t, e in enumerate(vals[off:off + T]):
                 objDef = objDef + c * e
+                if pen is not None:
+                    objDef += pen[t]
             objTry += objDef
             objTry += vals[off + T] / (o.sigmaError ** 2)
             o.nobj += 1
@@ -427,6 +510,8 @@ def launch_update_try(items) -> Pending:
             if (objRef is None) or (objTry < objRef):
                 o.atTry = atTry
                 o.objTry = objTry
+                if i in afft_try:
+                    o.AfftTry = afft_try[i]
                 o._xt_try, o._xt_scratch = o._xt_scratch, o._xt_try
             out.append(objTry)
         return out
@@ -465,22 +550,45 @@ def batch_gradient(items) -> list:
     xt = np.fromiter((o.xt.data_ptr() for o in opts), np.uint64, n)
     at = np.fromiter((o.at.data_ptr() for o in opts), np.uint64, n)
     px = np.fromiter((o._pxt.data_ptr() for o in opts), np.uint64, n)
+    Aptr = np.zeros(n, np.uint64)
+    keep = []                                                # affine parts live until launched
+    for i, o in enumerate(opts):
+        if o.affineDim > 0:
+            A, _ = o._aff_transforms(o.Afft)
+            keep.append(A)
+            Aptr[i] = A.data_ptr()
     for idx in _by_key([o._flow_key for o in opts]):
         o0 = opts[idx[0]]
+        has_aff = bool(Aptr[idx].any())
         engine.covector_records(rt.stack([opts[i]._cov_base for i in idx]), o0.Kdiff.prec,
-                                o0.dim, Ts[idx], xt[idx], at[idx], px[idx])
+                                o0.dim, Ts[idx], xt[idx], at[idx], px[idx],
+                                A=Aptr[idx] if has_aff else None)
+    del keep
     # dat = 2 lambda a - p_{t+1};  grd = dat / (coeff T)
     dats = torch._foreach_mul([o.at for o in opts], [2 * o.regweight for o in opts])
     torch._foreach_sub_(dats, [o._pxt_tail for o in opts])
     torch._foreach_div_(dats, [it[1] * it[0].Tsize for it in items])
     out = []
-    for o, dat in zip(opts, dats):
+    for (o, coeff), dat in zip(items, dats):
         g = Direction()
         g.diff = dat
-        g.aff = np.zeros(o.Afft.shape)
+        g.aff = _affine_gradient(o, coeff) if o.affineDim > 0 else np.zeros(o.Afft.shape)
         o.ngrad += 1
         out.append(g)
     return out
+
+
+def _affine_gradient(o, coeff):
+    """Affine part of getGradient from the covector (surfaceMatching.py:313-322,
+    curveMatching.py:246-252): dA_t = p_{t+1}^T x_t, db_t = sum p_{t+1}, projected on the
+    basis, with the drivers' own scalings."""
+    T, d = o.Tsize, o.dim
+    dA = torch.einsum('tni,tnj->tij', o._pxt[1:], o.xt[:-1])
+    db = o._pxt[1:].sum(dim=1)
+    dAff = torch.cat([dA.reshape(T, d * d), db], dim=1) @ o._basis     # (T, affineDim)
+    if o.aff_style == 'curve':
+        return (2 * o.Afft - dAff / o._aff_w) / (coeff * T)
+    return (2 * (o._aff_w * o.Afft) - dAff) / (o.coeffAff * coeff * T)
 
 
 def batch_dot(items) -> list:
@@ -509,12 +617,30 @@ def launch_dot(items) -> Pending:
         o0 = opts[idx[0]]
         engine.kdot_records(rt.stack([opts[i]._flow_base for i in idx]), o0.Kdiff.prec,
                             o0.dim, Ts[idx], xt[idx], g1[idx], g2[idx], na[idx], out_ptr[idx])
+    aff = [i for i, o in enumerate(opts) if o.affineDim > 0]
+    if aff:
+        # affine products per time step in columns 2..3 (surfaceMatching.py:366-369,
+        # curveMatching.py:288-296)
+        abuf = torch.zeros_like(buf)
+        for i in aff:
+            o, a1 = opts[i], items[i][1].aff * opts[i]._aff_dot_weight()
+            for q, g in enumerate(items[i][2]):
+                abuf[first[i]:first[i] + Ts[i], q] = (a1 * g.aff).sum(dim=1)
+        buf = torch.cat([buf, abuf], dim=1)
 
     def finish(allv):
         out = []
         for i in range(n):
             # sequential sum over t (numpy's accumulate), as the reference's loop
             v = allv[first[i]:first[i] + Ts[i], :na[i]]
-            out.append(np.cumsum(v, axis=0)[-1].copy())
+            if opts[i].affineDim > 0:
+                a = allv[first[i]:first[i] + Ts[i], 2:2 + na[i]]
+                r = np.zeros(na[i])
+                for t in range(Ts[i]):
+                    r = r + v[t]
+                    r += a[t]
+                out.append(r)
+            else:
+                out.append(np.cumsum(v, axis=0)[-1].copy())
         return out
     return Pending(buf, finish)

@@ -70,15 +70,18 @@ def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, withJacobian=Fal
         nj = yt.shape[1] if yt is not None else N
         Jt = torch.zeros((T + 1, nj), dtype=torch.float64, device=dev)
     dt = 1.0 / T
-    if aff is None and yt is None and nt is None and Jt is None:
-        engine.shoot([engine.FlowJob(x0d, atd, K, xt, energy)])
+    A = None if aff is None else aff[0].contiguous()
+    bv = None if aff is None else aff[1].contiguous()
+    if yt is None and nt is None and Jt is None:
+        # all T steps in one sequence of fused launches (affine part in the step's epilogue)
+        engine.shoot([engine.FlowJob(x0d, atd, K, xt, energy, A=A, b=bv)])
     else:
         xt[0] = x0d
         for k in range(T):
             z, a = xt[k], atd[k]
-            engine.shoot([engine.FlowJob(z, a[None], K, xt[k:k + 2], energy[k:k + 1], dt=dt)])
-            if aff is not None:
-                xt[k + 1] += dt * (z @ aff[0][k].T + aff[1][k])
+            engine.shoot([engine.FlowJob(z, a[None], K, xt[k:k + 2], energy[k:k + 1], dt=dt,
+                                         A=None if A is None else A[k:k + 1],
+                                         b=None if bv is None else bv[k:k + 1])])
             if yt is not None:
                 zy = yt[k]
                 yt[k + 1] = zy + dt * apply_k_dev(zy, z, a, KparDiff)
@@ -130,17 +133,8 @@ def landmarkHamiltonianCovector(x0, at, px1, KparDiff, regweight, affine=None):
     xt = landmarkDirectEvolutionEuler(x0d, atd, KparDiff, affine=affine)
     pxt = torch.empty((T + 1, N, dim), dtype=torch.float64, device=dev)
     aff = _affine_dev(affine, T, dim, dev)
-    if aff is None:
-        engine.covector([engine.CovectorJob(xt, atd, px1d, K, float(regweight), pxt)])
-    else:
-        pxt[T] = px1d
-        dt = 1.0 / T
-        for t in range(T):
-            px = pxt[T - t]
-            engine.covector([engine.CovectorJob(xt[T - t - 1:T - t + 1], atd[T - t - 1:T - t], px,
-                                                K, float(regweight), pxt[T - t - 1:T - t + 1],
-                                                dt=dt)])
-            pxt[T - t - 1] += dt * (px @ aff[0][T - t - 1])
+    engine.covector([engine.CovectorJob(xt, atd, px1d, K, float(regweight), pxt,
+                                        A=None if aff is None else aff[0].contiguous())])
     return _out(pxt, np_out), _out(xt, np_out)
 
 

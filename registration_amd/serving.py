@@ -41,7 +41,9 @@ class Request:
     (faces, vertices), with faces None for landmark sets (corresponding points); kernel
     names/widths as in SurfaceMatchingParam (sigmaDist unused by landmark sets); errorType
     'current', 'measure' or (surfaces) 'varifold' selects the data term of surfaces and
-    curves; ``arrival`` in seconds from service start."""
+    curves; affine ('none', 'affine', 'similitude', 'euclidean', 'translation') and
+    affineWeight the global affine part of surfaces and curves; ``arrival`` in seconds from
+    service start."""
 
     rid: int
     kind: str
@@ -56,6 +58,8 @@ class Request:
     order: int = 10
     arrival: float = 0.0
     errorType: str = 'current'
+    affine: str = 'none'
+    affineWeight: float = 1.0
 
 
 @dataclass
@@ -87,7 +91,8 @@ def build_registration(req: Request, precision: str):
                                     sigmaError=req.sigmaError, errorType=req.errorType)
         return sm.SurfaceMatching(Template=surfaces.Surface(FV=req.template),
                                   Target=surfaces.Surface(FV=req.target), param=p,
-                                  maxIter=req.maxIter, testGradient=False)
+                                  maxIter=req.maxIter, testGradient=False, affine=req.affine,
+                                  affineWeight=req.affineWeight)
     if req.kind == 'landmark':
         p = lm.LandmarkMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, sigmaError=req.sigmaError)
         return lm.LandmarkMatching(Template=req.template[1], Target=req.target[1], param=p,
@@ -96,7 +101,8 @@ def build_registration(req: Request, precision: str):
         p = cm.CurveMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, KparDist=Kw,
                                   sigmaError=req.sigmaError, errorType=req.errorType)
         return cm.CurveMatching(Template=curves.Curve(FV=req.template),
-                                Target=curves.Curve(FV=req.target), param=p, maxIter=req.maxIter)
+                                Target=curves.Curve(FV=req.target), param=p, maxIter=req.maxIter,
+                                affine=req.affine, affineWeight=req.affineWeight)
     raise ValueError(f"unknown request kind {req.kind!r}")
 
 

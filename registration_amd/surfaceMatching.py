@@ -5,7 +5,9 @@
 optimiser protocol; the registration lives on the GPU (see matching.DeviceMatching) and
 ``optimizeMatching`` runs the reference's conjugate gradient over it.  Supported: the
 currents, measure and varifold data terms (errorType 'current', 'measure' (the default) or
-'varifold'), affine='none', symmetric=False, subsampleTargetSize=-1.  File output (VTK snapshots every saveRate iterations) is not written.
+'varifold'), the global affine part (affine 'affine', 'similitude', 'euclidean',
+'translation' or 'none', with affineWeight / rotWeight / scaleWeight / transWeight),
+symmetric=False, subsampleTargetSize=-1.  File output (VTK snapshots every saveRate iterations) is not written.
 """
 from __future__ import annotations
 
@@ -61,8 +63,9 @@ class SurfaceMatching(DeviceMatching):
                  testGradient=True, saveFile='evolution', affine='none', outputDir='.'):
         if Template is None or Target is None:
             raise NotImplementedError("Template and Target surfaces are required (no file readers)")
-        if affine != 'none' or symmetric or subsampleTargetSize > 0:
-            raise NotImplementedError("the GPU path supports affine='none', symmetric=False")
+        if symmetric or subsampleTargetSize > 0:
+            raise NotImplementedError("the GPU path supports symmetric=False and no target "
+                                      "subsampling")
         self.fv0 = surfaces.Surface(surf=Template)
         self.fv1 = surfaces.Surface(surf=Target)
         self.param = SurfaceMatchingParam() if param is None else param
@@ -87,7 +90,9 @@ class SurfaceMatching(DeviceMatching):
         Tsize = int(round(1.0 / self.param.timeStep))
         self._init_device(x0, self.fvInit.faces, self.fv1.centers, self.fv1.surfel,
                           self.param.KparDiff, self.param.KparDist, self.param.sigmaError, Tsize,
-                          regWeight, data_term=self.param.errorType)
+                          regWeight, data_term=self.param.errorType, affine=affine,
+                          affineWeight=affineWeight, rotWeight=rotWeight, scaleWeight=scaleWeight,
+                          transWeight=transWeight)
 
     def _new_shape(self, vertices, with_weights=False):
         return surfaces.DeviceSurface(self.faces_np, vertices, with_weights=with_weights)
@@ -97,18 +102,24 @@ class SurfaceMatching(DeviceMatching):
         self.fvDef.updateVertices(self.xt[-1].cpu().numpy())
 
     def optimizeMatching(self):
+        """surfaceMatching.py:474-484: the gradient bound is taken with coeffAff2 and the
+        optimisation starts with coeffAff1."""
+        self.coeffAff = self.coeffAff2
         grd = self.getGradient(self.gradCoeff)
         [grd2] = self.dotProduct(grd, [grd])
         self.gradEps = max(0.001, np.sqrt(grd2) / 10000)
         logging.info('Gradient lower bound: %f' % (self.gradEps))
+        self.coeffAff = self.coeffAff1
         cg.cg(self, verb=self.verb, maxIter=self.maxIter, TestGradient=self.testGradient,
               epsInit=0.1)
 
     def cg_steps(self):
         """The optimiser of optimizeMatching as a generator (see conjugateGradient.cg_steps);
         the initial gradient bound is one more request."""
+        self.coeffAff = self.coeffAff2
         grd = yield (cg.GRADIENT, self.gradCoeff)
         [grd2] = yield (cg.DOT, grd, [grd])
         self.gradEps = max(0.001, np.sqrt(grd2) / 10000)
+        self.coeffAff = self.coeffAff1
         return (yield from cg.cg_steps(self, verb=self.verb, maxIter=self.maxIter,
                                        TestGradient=self.testGradient, epsInit=0.1))

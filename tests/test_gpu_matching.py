@@ -294,3 +294,74 @@ def test_bad_direction_branch_follows_reference():
     assert np.all(np.diff(hg) <= 0)
     assert np.all(np.abs(hg[:7] - ho[:7]) <= 1e-9 * np.abs(ho[:7])), (hg[:7], ho[:7])
     assert hg[-1] < 0.1 * hg[0] and ho[-1] < 0.1 * ho[0], (hg[-1], ho[-1])
+
+
+# ---------------------------------------------------------------- global affine part
+def _affine_pair(kind, prec, affine, T=4):
+    from registration_amd import curveMatching as cm, kernelFunctions as kfun
+    from registration_amd import surfaceMatching as sm, synthetic
+    Kd = kfun.Kernel('gauss', sigma=0.5, precision=prec)
+    Kw = kfun.Kernel('gauss', sigma=0.3, precision=prec)
+    if kind == "surface":
+        V, F = synthetic.icosphere(2)
+        R = synthetic.random_rotation(np.random.default_rng(3))
+        Y = 1.15 * synthetic.bump_target(V) @ R.T + 0.1
+        p = sm.SurfaceMatchingParam(timeStep=1.0 / T, KparDiff=Kd, KparDist=Kw, sigmaError=0.1,
+                                    errorType='current')
+        g = sm.SurfaceMatching(Template=sm.surfaces.Surface(FV=(F, V)), Target=sm.surfaces.Surface(FV=(F, Y)),
+                               param=p, testGradient=False, affine=affine, affineWeight=0.5,
+                               rotWeight=2.0)
+        o = orc.SurfaceMatching(orc.Surface(FV=(F, V)), orc.Surface(FV=(F, Y)), orc.Kernel('gauss', sigma=0.5),
+                                orc.Kernel('gauss', sigma=0.3), sigmaError=0.1, timeStep=1.0 / T,
+                                affine=affine, affineWeight=0.5, rotWeight=2.0)
+    else:
+        X, _ = synthetic.circle(150)
+        E = synthetic.closed_curve_edges(150)
+        Y = 1.2 * synthetic.star_curve(150, 4, 0.2) + 0.1
+        p = cm.CurveMatchingParam(timeStep=1.0 / T, KparDiff=Kd, KparDist=Kw, sigmaError=0.1,
+                                  errorType='current')
+        g = cm.CurveMatching(Template=cm.curves.Curve(FV=(E, X)), Target=cm.curves.Curve(FV=(E, Y)),
+                             param=p, affine=affine, affineWeight=0.5)
+        o = orc.CurveMatching(orc.Curve(FV=(E, X)), orc.Curve(FV=(E, Y)), orc.Kernel('gauss', sigma=0.5),
+                              orc.Kernel('gauss', sigma=0.3), sigmaError=0.1, timeStep=1.0 / T,
+                              affine=affine, affineWeight=0.5)
+    return g, o
+
+
+@pytest.mark.parametrize("kind,affine", [("surface", "similitude"), ("surface", "affine"),
+                                         ("curve", "euclidean"), ("curve", "affine")])
+def test_affine_requests_match_oracle(kind, affine):
+    import torch
+    g, o = _affine_pair(kind, "fp64", affine)
+    rng = np.random.default_rng(8)
+    a = 0.02 * rng.standard_normal(o.at.shape)
+    aff = 0.05 * rng.standard_normal(o.Afft.shape)
+    g.at, o.at = torch.as_tensor(a, device="cuda"), a.copy()
+    g.Afft, o.Afft = torch.as_tensor(aff, device="cuda"), aff.copy()
+    print(f"affine requests {kind} {affine}", flush=True)
+    assert abs(g.objectiveFun() - o.objectiveFun()) < 1e-11 * abs(o.obj)
+    o.endOfIteration()
+    gg, go = g.getGradient(g.gradCoeff), o.getGradient(o.gradCoeff)
+    assert np.abs(gg.aff.cpu().numpy() - go.aff).max() < 1e-10 * np.abs(go.aff).max()
+    assert np.abs(gg.diff.cpu().numpy() - go.diff).max() < 1e-10 * np.abs(go.diff).max()
+    [d1] = g.dotProduct(gg, [gg])
+    [r1] = o.dotProduct(go, [go])
+    assert abs(d1 - r1) < 1e-10 * abs(r1)
+    t1, t2 = g.updateTry(gg, 0.01, None), o.updateTry(go, 0.01, None)
+    assert abs(t1 - t2) < 1e-11 * abs(t2)
+    g.acceptVarTry()
+    o.acceptVarTry()
+    assert np.abs(g.Afft.cpu().numpy() - o.Afft).max() < 1e-12 * max(1.0, np.abs(o.Afft).max())
+
+
+@pytest.mark.parametrize("kind,affine", [("surface", "similitude"), ("curve", "affine")])
+def test_affine_cg_sequence_fp64(kind, affine):
+    g, o = _affine_pair(kind, "fp64", affine)
+    g.maxIter = o.maxIter = 8
+    print(f"affine cg {kind} {affine}", flush=True)
+    g.optimizeMatching()
+    o.optimizeMatching()
+    hg, ho = np.array(g.history), np.array(o.history)
+    assert len(hg) == len(ho) and len(hg) >= 5
+    assert np.all(np.abs(hg - ho) <= 1e-9 * np.abs(ho)), (hg, ho)
+    assert np.abs(g.Afft.cpu().numpy() - o.Afft).max() < 1e-7 * max(1.0, np.abs(o.Afft).max())

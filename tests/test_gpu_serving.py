@@ -34,6 +34,17 @@ def requests():
                                 target=(F, synthetic.bump_target(V)), sigmaKernel=0.4,
                                 sigmaDist=0.25, sigmaError=0.1, T=5, maxIter=6,
                                 errorType='varifold'))
+    # global affine parts (surface similitude, curve affine) in the same mix
+    reqs.append(serving.Request(rid=33, kind='surface', template=(F, V),
+                                target=(F, 1.2 * synthetic.bump_target(V) + 0.05), sigmaKernel=0.4,
+                                sigmaDist=0.2, sigmaError=0.1, T=4, maxIter=6,
+                                affine='similitude', affineWeight=0.5))
+    Xa, _ = synthetic.circle(140)
+    Ea = synthetic.closed_curve_edges(140)
+    reqs.append(serving.Request(rid=34, kind='curve', template=(Ea, Xa),
+                                target=(Ea, 1.1 * synthetic.star_curve(140, 3, 0.2) + 0.1),
+                                sigmaKernel=0.3, sigmaDist=0.1, sigmaError=0.1, T=5, maxIter=6,
+                                affine='affine'))
     X, _ = synthetic.circle(160)
     E = synthetic.closed_curve_edges(160)
     reqs.append(serving.Request(rid=31, kind='curve', template=(E, X),

@@ -639,3 +639,67 @@ def test_second_order_hamiltonian_drives_the_evolution():
         fd = (orc.secondOrderHamiltonian(x0, a0, rhot[0], *args(h), K)
               - orc.secondOrderHamiltonian(x0, a0, rhot[0], *args(-h), K)) / (2 * h)
         assert abs(fd - (v * e).sum()) < 1e-6 * max(1.0, abs(fd))
+
+
+# ---------------------------------------------------------------- global affine part
+def test_affine_bases_are_orthonormal_and_span_their_groups():
+    for d in (2, 3):
+        for aff, dim in (('affine', d * d + d), ('similitude', d * (d - 1) // 2 + 1 + d),
+                         ('euclidean', d * (d - 1) // 2 + d), ('translation', d)):
+            B = orc.AffineBasis(d, aff)
+            assert B.affineDim == dim
+            np.testing.assert_allclose(B.basis.T @ B.basis, np.eye(dim), atol=1e-15)
+            for k in B.rotComp:   # rotations: antisymmetric A, no translation
+                A = B.basis[:d * d, k].reshape(d, d)
+                assert np.allclose(A, -A.T) and not B.basis[d * d:, k].any()
+            for k in B.affComp:   # the symmetric trace-free part
+                A = B.basis[:d * d, k].reshape(d, d)
+                assert np.allclose(A, A.T) and abs(np.trace(A)) < 1e-15
+        assert orc.AffineBasis(d, 'none').affineDim == 0
+
+
+@pytest.mark.parametrize("kind", ["surface", "curve"])
+def test_affine_matching_gradient_is_the_derivative(kind):
+    """The affine part of getGradient is the derivative of the objective in Afft under the
+    drivers' own scalings (surface: grd.aff coeffAff coeff; curve: grd.aff affineWeight coeff;
+    the penalty's dt already carries the 1/T), by central FD of updateTry."""
+    rng = np.random.default_rng(4)
+    if kind == "surface":
+        V, F = synthetic.icosphere(1)
+        o = orc.SurfaceMatching(orc.Surface(FV=(F, V)), orc.Surface(FV=(F, synthetic.bump_target(V))),
+                                orc.Kernel('gauss', sigma=0.5), orc.Kernel('gauss', sigma=0.3),
+                                sigmaError=0.1, timeStep=0.25, affine='similitude', affineWeight=2.0)
+    else:
+        X, _ = synthetic.circle(30)
+        E = synthetic.closed_curve_edges(30)
+        o = orc.CurveMatching(orc.Curve(FV=(E, X)), orc.Curve(FV=(E, 1.2 * synthetic.star_curve(30, 4, 0.2))),
+                              orc.Kernel('gauss', sigma=0.4), orc.Kernel('gauss', sigma=0.2),
+                              sigmaError=0.1, timeStep=0.25, affine='affine', affineWeight=2.0)
+    o.at = 0.05 * rng.standard_normal(o.at.shape)
+    o.Afft = 0.1 * rng.standard_normal(o.Afft.shape)
+    o.objectiveFun()
+    o.endOfIteration()
+    g = o.getGradient(1.0)
+    d = orc.Direction()
+    d.diff = np.zeros_like(o.at)
+    d.aff = rng.standard_normal(o.Afft.shape)
+    h = 1e-6
+    fd = (o.updateTry(d, -h) - o.updateTry(d, h)) / (2 * h)
+    if kind == "surface":
+        gd = (g.aff * d.aff).sum() * o.coeffAff
+    else:
+        gd = (g.aff * d.aff * o.affineWeight.reshape(1, -1)).sum()
+    assert abs(fd - gd) < 1e-6 * max(1.0, abs(fd)), (fd, gd)
+
+
+def test_affine_surface_matching_recovers_a_scaling():
+    """A pure scaling of the template is matched mostly by the similitude part."""
+    V, F = synthetic.icosphere(1)
+    o = orc.SurfaceMatching(orc.Surface(FV=(F, V)), orc.Surface(FV=(F, 1.3 * V)),
+                            orc.Kernel('gauss', sigma=0.5), orc.Kernel('gauss', sigma=0.3),
+                            sigmaError=0.1, timeStep=0.25, maxIter=10, affine='similitude')
+    e0 = o.objectiveFun()
+    o.optimizeMatching()
+    assert o.obj < 0.2 * e0
+    B = o.affB
+    assert np.abs(o.Afft[:, B.simComp]).sum() > 0
