@@ -35,6 +35,11 @@ FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
 FLOW_STEP_TRAFFIC_S6 = 65.8e6
 
 
+def _progress(msg):
+    """One progress line per finished leg on stderr (the JSON result stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -356,6 +361,7 @@ def main():
         m = registration(cfg, args.precision)
         m.optimizeMatching()
     torch.cuda.synchronize()
+    _progress(f"{args.warmup} warmup registration(s) done")
 
     rt.COUNT_PAIRS = True
     rt.pair_counts.clear()
@@ -372,6 +378,7 @@ def main():
         iters.append(len(m.history) - 1)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    _progress(f"{args.steps} timed registration(s): {wall:.3f} s")
     rt.COUNT_PAIRS = False
     counts = dict(rt.pair_counts)
     pairs_per_reg = sum(counts.values()) / args.steps
@@ -386,14 +393,19 @@ def main():
 
     roof = flow_step_roofline(cfg, args.precision, args.roofline_reps)
     data_terms = data_terms_leg(cfg, args.data_term_reps) if args.data_term_reps > 0 else None
+    _progress("energy check, roofline and data-term legs done")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, {k: v / args.steps for k, v in counts.items()})
+        _progress("cpu baseline done")
 
     served = served_leg(args.served, args.precision) if args.served > 0 else None
+    if served is not None:
+        _progress("served legs done")
     if args.mixed_service > 0:
         served = dict(served or {})
         served["mixed_service"] = mixed_service_leg(args.mixed_service, args.precision)
+        _progress("mixed-service leg done")
 
     lat_ms = np.array(lat) * 1e3
     out = {
