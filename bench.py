@@ -30,9 +30,10 @@ FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64 (AMD published)
 FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
 # HBM bytes per fused flow step at 40,962 points (pair kernel + finalize + totals), measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on `bench.py --roofline-only --roofline-reps 3` and
-# scaled by the counters' reading for 256 MiB device copies (FETCH x2.0, WRITE x1.0):
-# 42.2 MB read + 23.6 MB written (profiles/flow_step_s6_fp32c.md).
-FLOW_STEP_TRAFFIC_S6 = 65.8e6
+# scaled by the counters' reading for 256 MiB device copies (FETCH x2.0, WRITE x1.0), in the
+# template's spatial processing order: 43.9 MB read + 27.3 MB written
+# (profiles/flow_step_s6_fp32c.md).
+FLOW_STEP_TRAFFIC_S6 = 71.2e6
 
 
 def _progress(msg):
@@ -77,12 +78,14 @@ def registration(cfg, prec):
                               param=p, maxIter=cfg["maxIter"], testGradient=False)
 
 
-def flow_step_roofline(cfg, prec, reps):
-    """Average duration of the fused flow-step launch at the workload's size, timed with HIP
-    events on the stream the launches go to."""
+def flow_step_roofline(cfg, prec, reps, natural=True):
+    """Average duration of the fused flow-step launch at the workload's size, in the
+    processing order the registration uses (the template's spatial order), timed with HIP
+    events on the stream the launches go to; the natural vertex order is timed beside it
+    unless ``natural`` is False (profiled runs, which then hold one launch form only)."""
     import numpy as np
     import torch
-    from registration_amd import _lib, runtime as rt
+    from registration_amd import _lib, engine, runtime as rt
     from registration_amd.kernelFunctions import Kernel
     F, V = cfg["template"]
     N = V.shape[0]
@@ -92,19 +95,28 @@ def flow_step_roofline(cfg, prec, reps):
     out = torch.empty_like(x)
     tot = torch.empty(1, dtype=torch.float64, device="cuda")
     K = Kernel('gauss', sigma=cfg["sigmaKernel"], precision=prec)
-    plan = rt.Plan(_lib.OP_FLOW_STEP, K.prec_code, 3,
-                   [rt.seg(y=x, w0=a, out0=out, total=tot, ny=N, sigma=K.sigma, dt=0.05,
-                           kernel=K.kernel_code, order=K.order)])
-    plan.run()
-    torch.cuda.synchronize()
-    st = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for _ in range(reps):
-        plan.run()
-    e1.record(st)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+    order = engine.order_for(V, engine.DevKernel.of(K))
+
+    def timed(perm):
+        fields = dict(y=x, w0=a, out0=out, total=tot, ny=N, sigma=K.sigma, dt=0.05,
+                      kernel=K.kernel_code, order=K.order)
+        if perm is not None:
+            fields["perm"] = perm
+        plan = rt.Plan(_lib.OP_FLOW_STEP, K.prec_code, 3, [rt.seg(**fields)])
+        for _ in range(reps):  # untimed: the clocks settle over the first launches
+            plan.run()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            plan.run()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms = timed(order) if order is not None else timed(None)
+    ms_natural = timed(None) if (natural and order is not None) else (ms if order is None else None)
     pairs = N * N
     flops = FLOPS_PER_PAIR["apply_k_gauss_d3_m3"] * pairs
     tflops = flops / (ms * 1e-3) / 1e12
@@ -114,7 +126,8 @@ def flow_step_roofline(cfg, prec, reps):
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
             "kernel": "k_pairs<float,gauss,3,3> + finalize (Euler flow step)",
             "launch_ms": round(ms, 4), "pairs_per_launch": pairs, "flops_per_pair": 16,
-            "pairs_per_s": pairs / (ms * 1e-3)}
+            "pairs_per_s": pairs / (ms * 1e-3), "spatial_order": order is not None,
+            "launch_ms_natural_order": round(ms_natural, 4) if ms_natural is not None else None}
 
 
 # Algorithmic flops per pair (add/mul 1, FMA 2, exp/sqrt 1; Gaussian d = 3).  Energies:
@@ -347,7 +360,7 @@ def main():
     cfg = make_workload(args.workload)
 
     if args.roofline_only:
-        roof = flow_step_roofline(cfg, args.precision, args.roofline_reps)
+        roof = flow_step_roofline(cfg, args.precision, args.roofline_reps, natural=False)
         # Calibration copy of a known size: 256 MiB read + 256 MiB written.
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda").fill_(1)
         dst = torch.empty_like(src)

@@ -155,6 +155,16 @@ typedef struct lddmm_seg {
   int32_t chunk_len;   /* sources per chunk (multiple of the LDS tile)          */
   int32_t fin_blocks;  /* finalize blocks                                       */
   int32_t pad_;
+  /* [in] optional processing order of FLOW_STEP and KDOT segments (rows = sources),
+   * NULL for the natural one: a permutation of 0..ny-1 in device memory, perm[i] =
+   * the point taken as the i-th row and the i-th source.  Outputs stay indexed by
+   * point, and the sums are the same up to rounding.  An order that keeps
+   * neighbouring indices close in space makes the LDS tiles compact; the FP32
+   * Gaussian sums then evaluate the exponent of compact 64-source blocks as
+   * 2<x, y> - |y|^2 - |x|^2 in tile coordinates (3 instead of 6 instructions per
+   * pair).  The permutation itself is not checked (it is not read on the host).
+   * No counterpart in the reference, whose sums run in point order. */
+  const int32_t *perm;
 } lddmm_seg_t;
 
 /* Plan a segmented launch: fills the [plan] fields of segs[0..nseg) and

@@ -62,6 +62,7 @@ class Seg(ctypes.Structure):
         ("row_tiles", ctypes.c_int32), ("chunks0", ctypes.c_int32),
         ("chunks1", ctypes.c_int32), ("chunk_len", ctypes.c_int32),
         ("fin_blocks", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("perm", _P),
     ]
 
 

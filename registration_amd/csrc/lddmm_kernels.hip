@@ -19,11 +19,14 @@
 // length of a segment depends only on that segment's sizes, so a segment's
 // result is bit-identical alone and inside any batch.
 //
-// Precision.  FP64 mode evaluates every pair in double (exp from the device
-// math library) with the reference's diagonal cut-off.  FP32 mode prescales
-// positions so that the Gaussian is a bare v_exp_f32 of -r'^2, accumulates
-// each KF-source run in FP32 and folds it into a Kahan-compensated (sum, comp)
-// pair; partials leave the kernel as double(sum) - double(comp).
+// Precision.  FP64 mode evaluates every pair in double (the pair kernels' own
+// exp2_nonpos; the dense kernel-matrix kernel uses the device math library's exp)
+// with the reference's diagonal cut-off.  FP32 mode prescales positions so that
+// the Gaussian is a bare v_exp_f32 of -r'^2, accumulates each run of 32 or 64
+// sources in FP32 and folds it into a Kahan-compensated (sum, comp) pair;
+// partials leave the kernel as double(sum) - double(comp).  Segments with a
+// processing order (spatially compact tiles) evaluate the FP32 Gaussian exponent
+// of compact blocks as 2<x, y> - |y|^2 - |x|^2 in tile coordinates.
 
 #include <hip/hip_runtime.h>
 
@@ -238,6 +241,20 @@ __device__ inline double cull_r2(int prec, int kernel, double sigma) {
   return u_cut * u_cut;
 }
 
+// Dot-product exponent (FP32 Gaussian K-sums of segments with a processing order).
+// In tile-origin coordinates, -r'^2 = (2 x.y - |y|^2) - |x|^2: the pair loop seeds the
+// exponent with -|y|^2 (staged per source, next to 2y in place of y) and adds 2 x.y with
+// three FMAs, and the row factor 2^-|x|^2 multiplies each 32-source run at its fold, which
+// saves the three differences and the square of the direct form.  Its rounding error grows
+// with the offsets from the origin, so a 64-source quarter takes it only when every source
+// lies within R'^2 <= EXP_R2MAX of the origin.  Rows with |x|'^2 > EXP_X2MAX drop their runs
+// over such a quarter at the fold: by the triangle inequality their r'^2 exceeds
+// (sqrt(60) - sqrt(2))^2 > 40, so every kernel value is below the 2^-40 culling bound.  For
+// the other rows 2^(2x.y - |y|^2) <= 2^|x|^2 <= 2^60, which cannot overflow for weights below
+// 2^60; a dropped row's run may overflow, and the fold discards it.
+constexpr float EXP_R2MAX = 2.0f;
+constexpr float EXP_X2MAX = 60.0f;
+
 template <typename R>
 __device__ inline R wave_min(R v) {
 #pragma unroll
@@ -372,10 +389,21 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   // Sources per unrolled step of the pair loop (measured on gfx950: 4 is fastest for the
   // FP32 loop, 8 for the FP64 one).
   constexpr int PAIR_UNROLL = KAHAN ? 4 : 8;
+  // Sources per FP32 run before its Kahan fold: 64 for the plain K-sums (one culling
+  // quarter), which halves the fold cost of the 32 used elsewhere; measured on the
+  // 40,962-point flow step: 2-3% faster, the step's error against FP64 from 1.6e-7 to 2.3e-7
+  // of its largest move.  Sums with derivative terms (the currents gradient: a difference of
+  // large sums) keep 32.
+  constexpr int KFP = (KAHAN && NA == 0) ? 64 : KF;
+
+  // Dot-product exponent: FP32 Gaussian K-sums whose segments carry a processing order.
+  constexpr bool EXPAND = KAHAN && KT == LDDMM_KERNEL_GAUSS && KSUM && NA == 0;
 
   __shared__ R s_pos[D][TS];
   __shared__ R s_w[NSW][M][TS];
   __shared__ R s_box[TS / 64][2 * D];  // bounding box of each 64-source quarter
+  __shared__ R s_n2[EXPAND ? TS : 1];  // -|y|^2 of each source (dot-product exponent)
+  __shared__ R s_r2max[TS / 64];       // max |y|^2 of each quarter
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -396,6 +424,11 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   const SrcView src = resolve_src(op, sg, set);
   const TgtView tgt = resolve_tgt(op, sg);
   const double scale = coord_scale(prec, KT, sg->sigma);
+  // Processing order of the rows and of the set-0 sources (the same point set): row i
+  // of the sums is point perm[i]; partials are written at the point's own index.
+  const int32_t *__restrict__ perm = sg->perm;
+  const int32_t *__restrict__ sperm = set == 0 ? perm : nullptr;
+  const bool expand_seg = EXPAND && perm != nullptr;
 
   KEval<R, KT> kev;
   kev.init(sg->order, sg->sigma);
@@ -407,11 +440,14 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   R ty[RPT][D];
   R tw[RPT][NA > 0 ? NA : 1][M];
   int64_t rows[RPT];
+  int64_t orow[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
     int64_t k = (int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT) + r * 64 + lane;
     rows[r] = k;
     int64_t kk = k < ny ? k : ny - 1;
+    if (perm) kk = perm[kk];
+    orow[r] = kk;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       yd[r][c] = tgt.pos[kk * D + c] * scale;
@@ -486,11 +522,23 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
     // distances, not of the absolute coordinates (for sources ordered along a
     // curve or a mesh, the middle one halves the largest offset in the tile).
     // FP64 keeps the raw frame.
+    // The origin is the mean of the tile's four quarter-middle sources (sources 32, 96,
+    // 160, 224, clamped to the tile), close to the centre of the tile's extent for
+    // spatially ordered sources.
     double org[D];
-    const int64_t mid = base + ((l1 - base < TS ? l1 - base : TS) >> 1);
+    const int64_t tlen = l1 - base < TS ? l1 - base : TS;
 #pragma unroll
-    for (int c = 0; c < D; ++c) org[c] = KAHAN ? src.pos[mid * D + c] * scale : 0.0;
+    for (int c = 0; c < D; ++c) org[c] = 0.0;
     if constexpr (KAHAN) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int64_t ls = base + (32 + 64 * q < tlen ? 32 + 64 * q : tlen - 1);
+        if (sperm) ls = sperm[ls];
+#pragma unroll
+        for (int c = 0; c < D; ++c) org[c] += src.pos[ls * D + c];
+      }
+#pragma unroll
+      for (int c = 0; c < D; ++c) org[c] *= 0.25 * scale;
 #pragma unroll
       for (int r = 0; r < RPT; ++r)
 #pragma unroll
@@ -499,24 +547,26 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
     __syncthreads();
     {
       int64_t l = base + tid;
-      R lo[D], hi[D];
+      R lo[D], hi[D], v[D];
+      R n2 = 0;
       if (l < l1) {
+        const int64_t lg = sperm ? (int64_t)sperm[l] : l;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-          R v = (R)(src.pos[l * D + c] * scale - org[c]);
-          s_pos[c][tid] = v;
-          lo[c] = v;
-          hi[c] = v;
+          v[c] = (R)(src.pos[lg * D + c] * scale - org[c]);
+          lo[c] = v[c];
+          hi[c] = v[c];
+          n2 = v[c] * v[c] + n2;
         }
 #pragma unroll
         for (int j = 0; j < NSW; ++j)
 #pragma unroll
-          for (int c = 0; c < M; ++c) s_w[j][c][tid] = (R)(src.sign * src.w[j][l * ld + c]);
+          for (int c = 0; c < M; ++c) s_w[j][c][tid] = (R)(src.sign * src.w[j][lg * ld + c]);
       } else {
         // Padding sources carry zero weight: their terms are exact zeros.
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-          s_pos[c][tid] = (R)0;
+          v[c] = (R)0;
           lo[c] = (R)INFINITY;
           hi[c] = (R)-INFINITY;
         }
@@ -530,6 +580,19 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
         lo[c] = wave_min(lo[c]);
         hi[c] = wave_max(hi[c]);
       }
+      // Each wave stages one 64-source quarter: a compact quarter (dot-product form) is
+      // staged as 2y with -|y|^2 beside it.
+      bool dbl = false;
+      if constexpr (EXPAND) {
+        if (expand_seg) {
+          const R q2 = wave_max(n2);
+          dbl = q2 <= (R)EXP_R2MAX;
+          s_n2[tid] = -n2;
+          if (lane == 0) s_r2max[wave] = q2;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < D; ++c) s_pos[c][tid] = dbl ? (R)2 * v[c] : v[c];
       if (lane == 0) {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
@@ -541,9 +604,12 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
     __syncthreads();
     // Which 64-source quarters of the tile hold a pair above the cut for this
     // wave (squared distance between the target box and each quarter's box).
-    unsigned live = 0, far = 0;
+    unsigned live = 0, far = 0, expq = 0;
 #pragma unroll
     for (int q = 0; q < TS / 64; ++q) {
+      if constexpr (EXPAND) {
+        if (expand_seg && s_r2max[q] <= (R)EXP_R2MAX) expq |= 1u << q;
+      }
       R d2 = 0;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
@@ -558,8 +624,22 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
     }
     if (!live || sym_w == 0) continue;  // wave-uniform: nothing of this tile to add
     const R wfac = (R)sym_w;
+    // Run factor of the dot-product form for this tile: wfac * 2^-|x|^2, zero for rows
+    // beyond EXP_X2MAX (their runs are dropped at the fold).
+    R xfac[EXPAND ? RPT : 1];
+    if constexpr (EXPAND) {
+      if (expq & live) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          R x2 = 0;
+#pragma unroll
+          for (int c = 0; c < D; ++c) x2 = ty[r][c] * ty[r][c] + x2;
+          xfac[r] = x2 <= (R)EXP_X2MAX ? wfac * __builtin_amdgcn_exp2f(-x2) : (R)0;
+        }
+      }
+    }
 
-    for (int j0 = 0; j0 < TS; j0 += KF) {
+    for (int j0 = 0; j0 < TS; j0 += KFP) {
       if (!((live >> (j0 / 64)) & 1u)) continue;
       R tK[RPT][KSUM ? M : 1], tD[RPT][D];
 #pragma unroll
@@ -569,13 +649,48 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
 #pragma unroll
         for (int c = 0; c < D; ++c) tD[r][c] = 0;
       }
+      if constexpr (EXPAND) {
+        if ((expq >> (j0 / 64)) & 1u) {
+          // exponent = -|y|^2 + x.(2y) (three FMAs); the row factor is applied at the fold.
+#pragma unroll PAIR_UNROLL
+          for (int j = j0; j < j0 + KFP; ++j) {
+            R sx[D], sw[M];
+#pragma unroll
+            for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j];
+#pragma unroll
+            for (int c = 0; c < M; ++c) sw[c] = s_w[0][c][j];
+            const R n2j = s_n2[j];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+              R e = n2j;
+#pragma unroll
+              for (int c = 0; c < D; ++c) e = ty[r][c] * sx[c] + e;
+              const R k = __builtin_amdgcn_exp2f(e);
+#pragma unroll
+              for (int c = 0; c < M; ++c) tK[r][c] = k * sw[c] + tK[r][c];
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < RPT; ++r) {
+            const bool keep = xfac[r] > (R)0;
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+              R y = keep ? tK[r][c] * xfac[r] - cK[r][c] : -cK[r][c];
+              R t = sK[r][c] + y;
+              cK[r][c] = (t - sK[r][c]) - y;
+              sK[r][c] = t;
+            }
+          }
+          continue;
+        }
+      }
       if constexpr (MIXED) {
         if (mixed && ((far >> (j0 / 64)) & 1u)) {
           // Every pair of this run has K < 2^-26: v_exp_f32 of the FP64 exponent (relative
           // error ~4e-6 from rounding an exponent of magnitude < 80 to FP32) changes each
           // term by less than 1e-13 of a unit kernel value; the products accumulate in FP64.
 #pragma unroll 8
-          for (int j = j0; j < j0 + KF; ++j) {
+          for (int j = j0; j < j0 + KFP; ++j) {
             R sx[D], sw[M];
 #pragma unroll
             for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j];
@@ -598,7 +713,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
         }
       }
 #pragma unroll PAIR_UNROLL
-      for (int j = j0; j < j0 + KF; ++j) {
+      for (int j = j0; j < j0 + KFP; ++j) {
         R sx[D], sw[NSW][M];
 #pragma unroll
         for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j];
@@ -673,9 +788,8 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   double *pp = part + sg->part0 + (int64_t)cg * ny * W;
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    int64_t k = rows[r];
-    if (k >= ny) continue;
-    double *o = pp + k * W;
+    if (rows[r] >= ny) continue;
+    double *o = pp + orow[r] * W;
     int q = 0;
     if constexpr (KSUM) {
 #pragma unroll
@@ -2217,6 +2331,8 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
         if (!s.x0) nx0 = s.ny;
         break;
     }
+    // A processing order: the flow step and the RKHS products (rows = set-0 sources).
+    if (s.perm && op != LDDMM_OP_FLOW_STEP && op != LDDMM_OP_KDOT) return LDDMM_EINVAL;
     if (nx0 < 1) return LDDMM_EINVAL;
     s.nx0 = nx0;
     if (op != LDDMM_OP_CURRENTS && op != LDDMM_OP_MEASURE && op != LDDMM_OP_VARIFOLD) s.nx1 = 0;
@@ -2262,9 +2378,9 @@ bool use_mfma() {
 
 template <typename R, int KT, int D, int M, bool KSUM, int NA>
 void launch_pairs(const lddmm_seg_t *segs, int nseg, int op, int prec, int64_t n_items,
-                  double *part, hipStream_t st) {
+                  double *part, hipStream_t st, bool mfma) {
   if constexpr (std::is_same<R, float>::value && KSUM && NA == 0) {
-    if ((op == LDDMM_OP_APPLY_K || op == LDDMM_OP_FLOW_STEP || op == LDDMM_OP_KDOT) && use_mfma()) {
+    if ((op == LDDMM_OP_APPLY_K || op == LDDMM_OP_FLOW_STEP || op == LDDMM_OP_KDOT) && mfma) {
       hipLaunchKernelGGL((k_pairs_mfma<KT, D, M>), dim3((unsigned)n_items), dim3(BLOCK), 0,
                          st, segs, nseg, op, prec, part);
       return;
@@ -2276,17 +2392,17 @@ void launch_pairs(const lddmm_seg_t *segs, int nseg, int op, int prec, int64_t n
 
 template <int D, int M, bool KSUM, int NA>
 int dispatch_pairs(int prec, int kern, const lddmm_seg_t *segs, int nseg, int op,
-                   int64_t n_items, double *part, hipStream_t st) {
+                   int64_t n_items, double *part, hipStream_t st, bool mfma) {
   if (prec == LDDMM_PREC_FP32C) {
     if (kern == LDDMM_KERNEL_GAUSS)
-      launch_pairs<float, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st);
+      launch_pairs<float, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, mfma);
     else
-      launch_pairs<float, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st);
+      launch_pairs<float, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, mfma);
   } else {
     if (kern == LDDMM_KERNEL_GAUSS)
-      launch_pairs<double, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st);
+      launch_pairs<double, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, mfma);
     else
-      launch_pairs<double, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st);
+      launch_pairs<double, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, mfma);
   }
   return LDDMM_OK;
 }
@@ -2369,20 +2485,21 @@ void launch_fin(const lddmm_seg_t *segs, int nseg, int prec, int kern, int64_t n
 
 template <int D>
 int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, int nseg,
-          int64_t n_items, int64_t n_fin, double *part, double *red, hipStream_t st) {
+          int64_t n_items, int64_t n_fin, double *part, double *red, hipStream_t st,
+          bool mfma) {
   switch (op) {
     case LDDMM_OP_APPLY_K:
       switch (sh.M) {
         case 1:
-          dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+          dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
           launch_fin<LDDMM_OP_APPLY_K, D, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
           break;
         case 2:
-          dispatch_pairs<D, 2, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+          dispatch_pairs<D, 2, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
           launch_fin<LDDMM_OP_APPLY_K, D, 2, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
           break;
         case 3:
-          dispatch_pairs<D, 3, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+          dispatch_pairs<D, 3, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
           launch_fin<LDDMM_OP_APPLY_K, D, 3, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
           break;
         default:
@@ -2392,7 +2509,7 @@ int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, 
     case LDDMM_OP_DIFF_KT: {
 #define LDDMM_DKT(MM, NN)                                                                  \
   if (sh.M == MM && sh.NA == NN) {                                                         \
-    dispatch_pairs<D, MM, false, NN>(prec, kern, segs, nseg, op, n_items, part, st);       \
+    dispatch_pairs<D, MM, false, NN>(prec, kern, segs, nseg, op, n_items, part, st, mfma);       \
     launch_fin<LDDMM_OP_DIFF_KT, D, MM, false, NN>(segs, nseg, prec, kern, n_fin, part, red, st); \
     return LDDMM_OK;                                                                       \
   }
@@ -2402,36 +2519,36 @@ int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, 
       return LDDMM_EINVAL;
     }
     case LDDMM_OP_FLOW_STEP:
-      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
       launch_fin<LDDMM_OP_FLOW_STEP, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       return LDDMM_OK;
     case LDDMM_OP_KDOT:
-      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
       launch_fin<LDDMM_OP_KDOT, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       return LDDMM_OK;
     case LDDMM_OP_KNORM:
-      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
       launch_fin<LDDMM_OP_KNORM, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       return LDDMM_OK;
     case LDDMM_OP_COVECTOR_STEP:
-      dispatch_pairs<D, D, false, 2>(prec, kern, segs, nseg, op, n_items, part, st);
+      dispatch_pairs<D, D, false, 2>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
       launch_fin<LDDMM_OP_COVECTOR_STEP, D, D, false, 2>(segs, nseg, prec, kern, n_fin, part, red, st);
       return LDDMM_OK;
     case LDDMM_OP_CURRENTS:
       if (sh.NA > 0) {
-        dispatch_pairs<D, D, true, 1>(prec, kern, segs, nseg, op, n_items, part, st);
+        dispatch_pairs<D, D, true, 1>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
         launch_fin<LDDMM_OP_CURRENTS, D, D, true, 1>(segs, nseg, prec, kern, n_fin, part, red, st);
       } else {
-        dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+        dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
         launch_fin<LDDMM_OP_CURRENTS, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       }
       return LDDMM_OK;
     case LDDMM_OP_MEASURE:
       if (sh.NA > 0) {
-        dispatch_pairs<D, 1, true, 1>(prec, kern, segs, nseg, op, n_items, part, st);
+        dispatch_pairs<D, 1, true, 1>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
         launch_fin<LDDMM_OP_MEASURE, D, 1, true, 1>(segs, nseg, prec, kern, n_fin, part, red, st);
       } else {
-        dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st);
+        dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
         launch_fin<LDDMM_OP_MEASURE, D, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       }
       return LDDMM_OK;
@@ -2499,8 +2616,12 @@ extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *se
   double *part = (double *)(wb + desc_bytes(nseg));
   double *red = part + parts;
   const int kern = segs_host[0].kernel;
-  rc = d == 2 ? run_d<2>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st)
-              : run_d<3>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st);
+  // The matrix-core K-sum (opt-in) has no processing-order support.
+  bool mfma = use_mfma();
+  for (int i = 0; i < nseg && mfma; ++i)
+    if (segs_host[i].perm) mfma = false;
+  rc = d == 2 ? run_d<2>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, mfma)
+              : run_d<3>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, mfma);
   if (rc) return rc;
   if (nred > 0)
     hipLaunchKernelGGL(k_totals, dim3((unsigned)nseg), dim3(TOT_BLOCK), 0, st, segs, nred, red);

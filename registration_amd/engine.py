@@ -78,6 +78,94 @@ def _u64(vals):
 
 
 # --------------------------------------------------------------------------------------
+# Processing order of a point set
+# --------------------------------------------------------------------------------------
+TILE = 256      # sources per LDS tile of the pair kernels
+QUARTER = 64    # sources per culling / exponent-form block
+
+
+def spatial_order(points) -> np.ndarray:
+    """A processing order of a point set for the pair kernels: recursive bisection along
+    the longest side of the bounding box, with split points at multiples of 256 (then 64)
+    so that every LDS tile and every 64-point quarter of it is one spatially compact
+    subtree.  Returns int32 indices, order[i] = the point processed i-th.  Deterministic
+    (the same points give the same order).  Sums over the points do not depend on it
+    beyond rounding; compact tiles let the FP32 Gaussian sums use the dot-product exponent
+    form."""
+    P = np.ascontiguousarray(np.asarray(points, dtype=np.float64))
+    n = P.shape[0]
+    if n > np.iinfo(np.int32).max:
+        raise ValueError("point set too large for an int32 order")
+    out = []
+    stack = [np.arange(n, dtype=np.int64)]
+    while stack:
+        ix = stack.pop()
+        m = ix.size
+        if m <= QUARTER:
+            out.append(ix)
+            continue
+        unit = TILE if m > TILE else QUARTER
+        h = min(max(int(round(m / (2 * unit))) * unit, unit), m - 1)
+        p = P[ix]
+        ax = int(np.argmax(p.max(axis=0) - p.min(axis=0)))
+        part = np.argpartition(p[:, ax], h - 1)
+        stack.append(ix[part[h:]])   # popped second: the high half follows the low half
+        stack.append(ix[part[:h]])
+    return np.concatenate(out).astype(np.int32) if out else np.zeros(0, np.int32)
+
+
+def spatial_order_dev(points, device=None) -> torch.Tensor:
+    """spatial_order as a device int32 tensor (the ``perm`` of a pair-sum descriptor)."""
+    o = spatial_order(points)
+    return torch.from_numpy(o).to(device if device is not None else rt.device())
+
+
+def wants_order(K: "DevKernel") -> bool:
+    """Whether a processing order can pay for K: the dot-product exponent form exists for
+    the FP32 Gaussian K-sums only (the gathers of other forms would only add loads)."""
+    return K.kernel == _lib.KERNEL_GAUSS and K.prec == _lib.PREC_FP32C
+
+
+# The pair kernel's bound on a quarter's squared prescaled offset from its tile origin for
+# the dot-product exponent (EXP_R2MAX in the kernel source), and the FP32 Gaussian prescale.
+EXP_R2MAX = 2.0
+_PRESCALE = float(np.sqrt(np.log2(np.e) / 2.0))
+
+
+def compact_fraction(points, order, sigma: float) -> float:
+    """Fraction of 64-point quarters of ``order`` that the FP32 Gaussian pair kernel would
+    evaluate in dot-product form at kernel width sigma: every point within R'^2 <= EXP_R2MAX
+    of its tile origin (the mean of the tile's points 32, 96, 160, 224, clamped), in the
+    prescaled coordinates x * sqrt(log2(e) / 2) / sigma."""
+    P = np.asarray(points, dtype=np.float64)[np.asarray(order)] * (_PRESCALE / sigma)
+    n = P.shape[0]
+    if n == 0:
+        return 0.0
+    ok = total = 0
+    for base in range(0, n, TILE):
+        t = P[base:base + TILE]
+        org = t[[min(i, len(t) - 1) for i in (32, 96, 160, 224)]].mean(axis=0)
+        r2 = ((t - org) ** 2).sum(axis=1)
+        for q in range(0, len(t), QUARTER):
+            ok += bool(r2[q:q + QUARTER].max() <= EXP_R2MAX)
+            total += 1
+    return ok / total
+
+
+def order_for(points, K: "DevKernel", min_fraction: float = 0.5):
+    """The processing order a registration of ``points`` under K should use, as a device
+    int32 tensor, or None when it would not pay: K has no dot-product form, or fewer than
+    ``min_fraction`` of the quarters would be compact enough for it (small point sets at
+    wide kernels, where the order only adds gathers)."""
+    if not wants_order(K):
+        return None
+    o = spatial_order(points)
+    if compact_fraction(points, o, K.sigma) < min_fraction:
+        return None
+    return torch.from_numpy(o).to(rt.device())
+
+
+# --------------------------------------------------------------------------------------
 # Shooting (flow + energy)
 # --------------------------------------------------------------------------------------
 @dataclass
@@ -92,6 +180,7 @@ class FlowJob:
     init: bool = True                   # copy x0 into xt[0] (False: xt[0] already holds x0)
     A: Optional[torch.Tensor] = None    # (T, d, d) affine part, with b (T, d): x += dt (A x + b)
     b: Optional[torch.Tensor] = None
+    perm: Optional[torch.Tensor] = None  # (N,) int32 processing order (spatial_order_dev)
 
     @property
     def T(self):
@@ -102,14 +191,20 @@ class FlowJob:
         return 1.0 / self.T if self.dt is None else self.dt
 
 
-def flow_record(N: int, T: int, K: DevKernel, dt: Optional[float] = None) -> np.ndarray:
-    """The pointer-free part of a registration's flow-step descriptor (one record)."""
+def flow_record(N: int, T: int, K: DevKernel, dt: Optional[float] = None,
+                perm: Optional[torch.Tensor] = None) -> np.ndarray:
+    """The pointer-free part of a registration's flow-step descriptor (one record), with the
+    optional processing order of its points (a device int32 permutation of 0..N-1)."""
     r = rt.segs(1)
     r["ny"] = N
     r["sigma"] = K.sigma
     r["dt"] = 1.0 / T if dt is None else dt
     r["kernel"] = K.kernel
     r["order"] = K.order
+    if perm is not None:
+        if perm.dtype != torch.int32 or perm.numel() != N or not perm.is_contiguous():
+            raise ValueError("perm must be a contiguous int32 tensor of N entries")
+        r["perm"] = perm.data_ptr()
     return r
 
 
@@ -149,7 +244,8 @@ def shoot(jobs: Sequence[FlowJob]) -> None:
         torch._foreach_copy_([j.xt[0] for j in init], [j.x0 for j in init])
     for group in _groups(jobs, lambda j: (j.K.family, j.x0.shape[1])):
         d = group[0].x0.shape[1]
-        base = rt.stack([rt.raw(flow_record(j.xt.shape[1], j.T, j.K, j.step)) for j in group])
+        base = rt.stack([rt.raw(flow_record(j.xt.shape[1], j.T, j.K, j.step, j.perm))
+                         for j in group])
         aff = any(j.A is not None for j in group)
         shoot_records(base, group[0].K.prec, d, [j.T for j in group],
                       [j.xt.data_ptr() for j in group], [j.at.data_ptr() for j in group],
@@ -238,6 +334,7 @@ class KDotJob:
     g2: Sequence[torch.Tensor]  # 1..2 tensors (T, N, d)
     K: DevKernel
     out: torch.Tensor           # (T, 2) output totals: out[t, j] = <g2[j][t], K(xt[t]) g1[t]>
+    perm: Optional[torch.Tensor] = None  # (N,) int32 processing order (spatial_order_dev)
 
 
 def kdot_records(base, prec, d, T, xt, g1, g2, na, out) -> None:
@@ -268,7 +365,8 @@ def kdot(jobs: Sequence[KDotJob]) -> None:
     """One launch: every time step of every job is a segment."""
     for group in _groups(jobs, lambda j: (j.K.family, j.xt.shape[2])):
         d = group[0].xt.shape[2]
-        base = rt.stack([rt.raw(flow_record(j.xt.shape[1], j.g1.shape[0], j.K)) for j in group])
+        base = rt.stack([rt.raw(flow_record(j.xt.shape[1], j.g1.shape[0], j.K, perm=j.perm))
+                         for j in group])
         g2 = [[g.data_ptr() for g in j.g2] + [0] * (2 - len(j.g2)) for j in group]
         kdot_records(base, group[0].K.prec, d, [j.g1.shape[0] for j in group],
                      [j.xt.data_ptr() for j in group], [j.g1.data_ptr() for j in group], g2,

@@ -82,6 +82,9 @@ class DeviceMatching:
         self.sigmaError = float(sigmaError)
         self.Kdiff = engine.DevKernel.of(KparDiff)
         self.x0 = rt.as_dev(x0_np)
+        # Processing order of the template's points for the flow and RKHS-product sums
+        # (spatially compact tiles; the trajectory keeps the points' own order).
+        self._order = engine.order_for(x0_np, self.Kdiff)
         T, N, d = Tsize, self.npt, self.dim
         if target_points is not None:
             self.data_kind = LANDMARK
@@ -120,7 +123,7 @@ class DeviceMatching:
         # Pointer-free descriptor parts of this registration's launches, built once; the
         # batched requests fill in the buffer addresses of each call with NumPy.
         # Kept as raw bytes (runtime.raw / runtime.stack).
-        self._flow_base = rt.raw(engine.flow_record(N, T, self.Kdiff))
+        self._flow_base = rt.raw(engine.flow_record(N, T, self.Kdiff, perm=self._order))
         self._flow_key = (self.Kdiff.kernel, self.Kdiff.prec, d)
         self._slice_bytes = N * d * 8
         # Gradient buffers, reused by every gradient request: the covector (T+1, N, d) with
@@ -373,7 +376,7 @@ def launch_objective(opts: Sequence[DeviceMatching]) -> Pending:
             A, b = o._aff_transforms(o.Afft)
             scal[off + T + 2:off + 2 * T + 2] = o._aff_penalty(o.Afft)
         flows.append(engine.FlowJob(o.x0, o.at, o.Kdiff, o.xt, scal[off:off + T], init=False,
-                                    A=A, b=b))
+                                    A=A, b=b, perm=o._order))
         off += L
     engine.shoot(flows)
     _data_energies(opts, [o.xt[-1] for o in opts], [o.fvDef_dev for o in opts],

@@ -107,6 +107,17 @@ def test_plan_rejects_invalid():
                  [_seg(10, out1=4096, kernel=_lib.KERNEL_LAPLACIAN, order=0)])[0] == _lib.EINVAL
 
 
+def test_plan_processing_order():
+    """A processing order is accepted for the flow step and the RKHS products only."""
+    assert _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP32C, 3, [_seg(100, perm=4096)])[0] == 0
+    kd = _seg(100, perm=4096, na=1)
+    kd.g[0] = 4096
+    assert _plan(_lib.OP_KDOT, _lib.PREC_FP32C, 3, [kd])[0] == 0
+    assert _plan(_lib.OP_APPLY_K, _lib.PREC_FP32C, 3, [_seg(100, perm=4096, m=3)])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_COVECTOR_STEP, _lib.PREC_FP32C, 3, [_seg(100, perm=4096)])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_KNORM, _lib.PREC_FP64, 3, [_seg(100, perm=4096, out0=0)])[0] == _lib.EINVAL
+
+
 def test_workspace_queries():
     lib = _lib.load()
     assert lib.lddmm_apply_k_workspace_bytes(1000, 2000, 3, 5) > 1000 * 5 * 8
