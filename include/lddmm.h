@@ -350,6 +350,26 @@ typedef struct lddmm_lmk {
 
 int lddmm_landmark_batch(const lddmm_lmk_t *host, const lddmm_lmk_t *dev, int n, void *stream);
 
+/* ---- processing order (host only) ----------------------------------------
+ * No counterpart in the reference (its sums run in point order).  They build
+ * and check the optional lddmm_seg_t.perm of flow steps and RKHS products. */
+
+/* Spatial processing order of n points (row-major (n, d) host array, d = 1..3):
+ * recursive bisection along the longest side of the bounding box, with split
+ * points at multiples of 256 (then 64), so that every 256-point tile and every
+ * 64-point quarter of the order is one compact subtree.  Writes order[i] = the
+ * point taken i-th (a permutation of 0..n-1); deterministic. */
+int lddmm_spatial_order(const double *points, int64_t n, int d, int32_t *order);
+
+/* Fraction of the 64-point quarters of `order` whose points all lie within
+ * squared distance r2max of their tile origin (the mean of the tile's points
+ * 32, 96, 160, 224, clamped to the tile), distances taken after scaling the
+ * coordinates by `scale`: the quarters an FP32 Gaussian pair kernel evaluates
+ * in dot-product form (scale = sqrt(log2(e) / 2) / sigma, r2max = 2).
+ * Returns -1 for invalid arguments. */
+double lddmm_order_compactness(const double *points, int64_t n, int d, const int32_t *order,
+                               double scale, double r2max);
+
 /* ---- small helpers ------------------------------------------------------ */
 
 /* Deterministic FP64 sum of x[0..n) into *out (single fixed reduction tree). */

@@ -95,6 +95,8 @@ SIGNATURES = {
     "lddmm_segments_run": (_I, [_I, _I, _I, ctypes.POINTER(Seg), _P, _I, _I64, _I64, _P, _SZ, _P]),
     "lddmm_segments_run_steps": (_I, [_I, _I, _I, ctypes.POINTER(Seg), _P, _I, _I, _I64, _I64, _P,
                                       _SZ, _P]),
+    "lddmm_spatial_order": (_I, [_P, _I64, _I, _P]),
+    "lddmm_order_compactness": (ctypes.c_double, [_P, _I64, _I, _P, _D, _D]),
     "lddmm_apply_k_workspace_bytes": (_SZ, [_I64, _I64, _I, _I]),
     "lddmm_apply_k": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _D, _I, _P, _P, _SZ, _P]),
     "lddmm_apply_diff_kt_workspace_bytes": (_SZ, [_I64, _I64, _I, _I, _I]),

@@ -36,7 +36,9 @@
 
 #include <stdlib.h>
 
+#include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "lddmm.h"
 
@@ -3061,6 +3063,80 @@ extern "C" int lddmm_sum_f64(const double *x, int64_t n, double *out, void *stre
   if (!x || !out || n < 1) return LDDMM_EINVAL;
   hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, (hipStream_t)stream, x, n, out);
   return hip_check(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Processing order (host).  Iterative bisection: each range [lo, lo + len) of
+// `order` is split at h (a multiple of 256 for ranges above a tile, of 64
+// above a quarter) along the longest side of its bounding box by nth_element;
+// ties on the coordinate break by point index, so the result is deterministic.
+// ---------------------------------------------------------------------------
+extern "C" int lddmm_spatial_order(const double *points, int64_t n, int d, int32_t *order) {
+  if (n < 0 || n > INT32_MAX || d < 1 || d > 3 || (n > 0 && (!points || !order)))
+    return LDDMM_EINVAL;
+  for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+  struct Range { int64_t lo, len; };
+  std::vector<Range> stack;
+  stack.push_back({0, n});
+  while (!stack.empty()) {
+    const Range r = stack.back();
+    stack.pop_back();
+    if (r.len <= 64) continue;
+    const int64_t unit = r.len > TS ? TS : 64;
+    int64_t h = (int64_t)llround((double)r.len / (2.0 * (double)unit)) * unit;
+    if (h < unit) h = unit;
+    if (h > r.len - 1) h = r.len - 1;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = r.lo; i < r.lo + r.len; ++i) {
+      const double *p = points + (int64_t)order[i] * d;
+      for (int c = 0; c < d; ++c) {
+        lo[c] = p[c] < lo[c] ? p[c] : lo[c];
+        hi[c] = p[c] > hi[c] ? p[c] : hi[c];
+      }
+    }
+    int ax = 0;
+    for (int c = 1; c < d; ++c)
+      if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
+    int32_t *first = order + r.lo;
+    std::nth_element(first, first + h, first + r.len, [&](int32_t a, int32_t b) {
+      const double pa = points[(int64_t)a * d + ax], pb = points[(int64_t)b * d + ax];
+      return pa < pb || (pa == pb && a < b);
+    });
+    stack.push_back({r.lo + h, r.len - h});  // popped second: the high half follows
+    stack.push_back({r.lo, h});
+  }
+  return LDDMM_OK;
+}
+
+extern "C" double lddmm_order_compactness(const double *points, int64_t n, int d,
+                                          const int32_t *order, double scale, double r2max) {
+  if (n < 1 || d < 1 || d > 3 || !points || !order) return -1.0;
+  int64_t ok = 0, total = 0;
+  for (int64_t base = 0; base < n; base += TS) {
+    const int64_t len = n - base < TS ? n - base : TS;
+    double org[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < 4; ++q) {
+      const int64_t k = 32 + 64 * q < len ? 32 + 64 * q : len - 1;
+      const double *p = points + (int64_t)order[base + k] * d;
+      for (int c = 0; c < d; ++c) org[c] += p[c];
+    }
+    for (int c = 0; c < d; ++c) org[c] *= 0.25;
+    for (int64_t q0 = 0; q0 < len; q0 += 64) {
+      double m = 0.0;
+      for (int64_t i = q0; i < len && i < q0 + 64; ++i) {
+        const double *p = points + (int64_t)order[base + i] * d;
+        double r2 = 0.0;
+        for (int c = 0; c < d; ++c) {
+          const double v = (p[c] - org[c]) * scale;
+          r2 += v * v;
+        }
+        m = r2 > m ? r2 : m;
+      }
+      ok += m <= r2max;
+      ++total;
+    }
+  }
+  return (double)ok / (double)total;
 }
 
 extern "C" int lddmm_last_hip_error(void) { return g_last_hip_error; }

@@ -85,33 +85,22 @@ QUARTER = 64    # sources per culling / exponent-form block
 
 
 def spatial_order(points) -> np.ndarray:
-    """A processing order of a point set for the pair kernels: recursive bisection along
-    the longest side of the bounding box, with split points at multiples of 256 (then 64)
-    so that every LDS tile and every 64-point quarter of it is one spatially compact
-    subtree.  Returns int32 indices, order[i] = the point processed i-th.  Deterministic
-    (the same points give the same order).  Sums over the points do not depend on it
-    beyond rounding; compact tiles let the FP32 Gaussian sums use the dot-product exponent
-    form."""
+    """A processing order of a point set for the pair kernels (lddmm_spatial_order, host code
+    of the kernel library): recursive bisection along the longest side of the bounding box,
+    with split points at multiples of 256 (then 64) so that every LDS tile and every 64-point
+    quarter of it is one spatially compact subtree.  Returns int32 indices, order[i] = the
+    point processed i-th.  Deterministic.  Sums over the points do not depend on it beyond
+    rounding; compact tiles let the FP32 Gaussian sums use the dot-product exponent form."""
     P = np.ascontiguousarray(np.asarray(points, dtype=np.float64))
+    if P.ndim != 2 or not 1 <= P.shape[1] <= 3:
+        raise ValueError("points must be an (n, d) array with d in 1..3")
     n = P.shape[0]
     if n > np.iinfo(np.int32).max:
         raise ValueError("point set too large for an int32 order")
-    out = []
-    stack = [np.arange(n, dtype=np.int64)]
-    while stack:
-        ix = stack.pop()
-        m = ix.size
-        if m <= QUARTER:
-            out.append(ix)
-            continue
-        unit = TILE if m > TILE else QUARTER
-        h = min(max(int(round(m / (2 * unit))) * unit, unit), m - 1)
-        p = P[ix]
-        ax = int(np.argmax(p.max(axis=0) - p.min(axis=0)))
-        part = np.argpartition(p[:, ax], h - 1)
-        stack.append(ix[part[h:]])   # popped second: the high half follows the low half
-        stack.append(ix[part[:h]])
-    return np.concatenate(out).astype(np.int32) if out else np.zeros(0, np.int32)
+    out = np.empty(n, dtype=np.int32)
+    _lib.check(_lib.load().lddmm_spatial_order(P.ctypes.data, n, P.shape[1], out.ctypes.data),
+               "lddmm_spatial_order")
+    return out
 
 
 def spatial_order_dev(points, device=None) -> torch.Tensor:
@@ -136,20 +125,18 @@ def compact_fraction(points, order, sigma: float) -> float:
     """Fraction of 64-point quarters of ``order`` that the FP32 Gaussian pair kernel would
     evaluate in dot-product form at kernel width sigma: every point within R'^2 <= EXP_R2MAX
     of its tile origin (the mean of the tile's points 32, 96, 160, 224, clamped), in the
-    prescaled coordinates x * sqrt(log2(e) / 2) / sigma."""
-    P = np.asarray(points, dtype=np.float64)[np.asarray(order)] * (_PRESCALE / sigma)
-    n = P.shape[0]
-    if n == 0:
+    prescaled coordinates x * sqrt(log2(e) / 2) / sigma (lddmm_order_compactness)."""
+    P = np.ascontiguousarray(np.asarray(points, dtype=np.float64))
+    o = np.ascontiguousarray(np.asarray(order, dtype=np.int32))
+    if P.shape[0] == 0:
         return 0.0
-    ok = total = 0
-    for base in range(0, n, TILE):
-        t = P[base:base + TILE]
-        org = t[[min(i, len(t) - 1) for i in (32, 96, 160, 224)]].mean(axis=0)
-        r2 = ((t - org) ** 2).sum(axis=1)
-        for q in range(0, len(t), QUARTER):
-            ok += bool(r2[q:q + QUARTER].max() <= EXP_R2MAX)
-            total += 1
-    return ok / total
+    if o.shape != (P.shape[0],):
+        raise ValueError("order must hold one index per point")
+    f = _lib.load().lddmm_order_compactness(P.ctypes.data, P.shape[0], P.shape[1], o.ctypes.data,
+                                            _PRESCALE / sigma, EXP_R2MAX)
+    if f < 0:
+        raise ValueError("invalid points or order")
+    return f
 
 
 def order_for(points, K: "DevKernel", min_fraction: float = 0.5):

