@@ -226,9 +226,11 @@ struct KEval<double, LDDMM_KERNEL_LAPLACIAN> {
 // ---------------------------------------------------------------------------
 // Tile culling.  A wave skips a staged source tile when the distance between
 // the bounding box of its target rows and the tile's bounding box makes every
-// kernel value of the tile smaller than EPS_K (2^-40 in FP32, 2^-80 in FP64),
-// below the accumulation precision of the row sums.  For the Laplacian the
-// bound uses P, Q, R <= (1+u)^4, and (1+u)^4 e^-u < EPS_K for u > U_CUT.
+// kernel value of the tile smaller than EPS_K (2^-32 in FP32, 2^-80 in FP64).
+// In FP32 a skipped term is below 1/256 of an ulp of a unit kernel value; even
+// all of a row's skipped terms together (each below 2^-32 |w|) stay within the
+// rounding error of its kept near-field terms.  For the Laplacian the bound
+// uses P, Q, R <= (1+u)^4, and (1+u)^4 e^-u < EPS_K for u > U_CUT.
 // Returns the squared cut radius in the kernel's coordinates (prescaled in
 // FP32, raw in FP64).
 // ---------------------------------------------------------------------------
@@ -236,10 +238,10 @@ __device__ inline double cull_r2(int prec, int kernel, double sigma) {
   const bool f32 = prec == LDDMM_PREC_FP32C;
   if (kernel == LDDMM_KERNEL_GAUSS) {
     // FP32: K = 2^-r'^2 with prescaled r'.  FP64: K = 2^(-h r^2), h = log2(e)/(2 sigma^2).
-    return f32 ? 40.0 : 80.0 * 2.0 * sigma * sigma / LOG2E;
+    return f32 ? 32.0 : 80.0 * 2.0 * sigma * sigma / LOG2E;
   }
-  // (1+u)^4 e^-u = 2^-40 at u~42.9, 2^-80 at u~72.7; FP32 coordinates carry log2(e)/sigma.
-  const double u_cut = f32 ? 43.0 * LOG2E : 73.0 * sigma;
+  // (1+u)^4 e^-u = 2^-32 at u~36.7, 2^-80 at u~72.7; FP32 coordinates carry log2(e)/sigma.
+  const double u_cut = f32 ? 37.0 * LOG2E : 73.0 * sigma;
   return u_cut * u_cut;
 }
 
@@ -251,7 +253,7 @@ __device__ inline double cull_r2(int prec, int kernel, double sigma) {
 // with the offsets from the origin, so a 64-source quarter takes it only when every source
 // lies within R'^2 <= EXP_R2MAX of the origin.  Rows with |x|'^2 > EXP_X2MAX drop their runs
 // over such a quarter at the fold: by the triangle inequality their r'^2 exceeds
-// (sqrt(60) - sqrt(2))^2 > 40, so every kernel value is below the 2^-40 culling bound.  For
+// (sqrt(60) - sqrt(2))^2 > 41, so every kernel value is below 2^-41, under the culling bound.  For
 // the other rows 2^(2x.y - |y|^2) <= 2^|x|^2 <= 2^60, which cannot overflow for weights below
 // 2^60; a dropped row's run may overflow, and the fold discards it.
 constexpr float EXP_R2MAX = 2.0f;

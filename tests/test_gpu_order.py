@@ -95,7 +95,7 @@ def test_ordered_flow_trajectory_against_oracle():
 def test_ordered_flow_with_outliers():
     """A dense blob with far outliers: blob quarters take the dot-product form, quarters with
     an outlier keep the direct form, and outlier rows (|x|'^2 > 60 from compact tiles) are
-    left out of the compact blocks, where all their kernel values are below 2^-40.  All rows
+    left out of the compact blocks, where all their kernel values are below 2^-41.  All rows
     against the oracle, over three steps."""
     from registration_amd import engine
     rng = np.random.default_rng(17)
@@ -115,6 +115,36 @@ def test_ordered_flow_with_outliers():
     assert np.abs(xt - ref).max() < 1e-6
     _, vn, _ = flow(X, at, gauss(0.3), None)
     assert rel_err(v, vn) < TOL
+
+
+def test_ordered_and_natural_segments_in_one_launch():
+    """A served batch mixes registrations with and without a processing order in one launch;
+    each trajectory against the oracle, and against the same trajectory flowed alone."""
+    import torch
+    from registration_amd import engine, synthetic
+    V5, _ = synthetic.icosphere(5)
+    V3, _ = synthetic.icosphere(3)
+    rng = np.random.default_rng(53)
+    a5 = 0.03 * rng.standard_normal((3,) + V5.shape)
+    a3 = 0.03 * rng.standard_normal((3,) + V3.shape)
+    K = gauss(0.4)
+    perm = engine.spatial_order_dev(V5)
+
+    def job(X, a, p):
+        xt = torch.empty((4,) + X.shape, dtype=torch.float64, device="cuda")
+        en = torch.zeros(3, dtype=torch.float64, device="cuda")
+        return engine.FlowJob(torch.as_tensor(X, device="cuda"), torch.as_tensor(a, device="cuda"),
+                              K, xt, en, perm=p)
+    print("ordered and natural segments in one launch", flush=True)
+    j5, j3 = job(V5, a5, perm), job(V3, a3, None)
+    engine.shoot([j5, j3])
+    alone = job(V3, a3, None)
+    engine.shoot([alone])
+    for j, X, a in ((j5, V5, a5), (j3, V3, a3)):
+        ref = orc.landmarkDirectEvolutionEuler(X, a, orc.Kernel('gauss', sigma=0.4))
+        assert np.abs(j.xt.cpu().numpy() - ref).max() < 1e-6
+    # A segment's result does not depend on the other segments of its launch.
+    assert np.array_equal(j3.xt.cpu().numpy(), alone.xt.cpu().numpy())
 
 
 def test_ordered_rkhs_products():
