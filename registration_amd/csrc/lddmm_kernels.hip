@@ -226,11 +226,14 @@ struct KEval<double, LDDMM_KERNEL_LAPLACIAN> {
 // ---------------------------------------------------------------------------
 // Tile culling.  A wave skips a staged source tile when the distance between
 // the bounding box of its target rows and the tile's bounding box makes every
-// kernel value of the tile smaller than EPS_K (2^-32 in FP32, 2^-80 in FP64).
-// In FP32 a skipped term is below 1/256 of an ulp of a unit kernel value; even
-// all of a row's skipped terms together (each below 2^-32 |w|) stay within the
-// rounding error of its kept near-field terms.  For the Laplacian the bound
-// uses P, Q, R <= (1+u)^4, and (1+u)^4 e^-u < EPS_K for u > U_CUT.
+// kernel value of the tile smaller than EPS_K (2^-32 in FP32, 2^-56 in FP64).
+// A skipped term is below 1/256 (FP32) or 1/8 (FP64) of an ulp of a unit
+// kernel value; even all of a row's skipped terms together (each below
+// EPS_K |w|) stay within the rounding error of its kept near-field terms, and
+// in an FP64 data-term energy of F faces they move the total by at most
+// F^2 EPS_K max|w|^2 (1.5e-6 max|w|^2 at 327,680 faces, whose self-energy alone
+// is of order F |w|^2).  For the Laplacian
+// the bound uses P, Q, R <= (1+u)^4, and (1+u)^4 e^-u < EPS_K for u > U_CUT.
 // Returns the squared cut radius in the kernel's coordinates (prescaled in
 // FP32, raw in FP64).
 // ---------------------------------------------------------------------------
@@ -238,10 +241,10 @@ __device__ inline double cull_r2(int prec, int kernel, double sigma) {
   const bool f32 = prec == LDDMM_PREC_FP32C;
   if (kernel == LDDMM_KERNEL_GAUSS) {
     // FP32: K = 2^-r'^2 with prescaled r'.  FP64: K = 2^(-h r^2), h = log2(e)/(2 sigma^2).
-    return f32 ? 32.0 : 80.0 * 2.0 * sigma * sigma / LOG2E;
+    return f32 ? 32.0 : 56.0 * 2.0 * sigma * sigma / LOG2E;
   }
-  // (1+u)^4 e^-u = 2^-32 at u~36.7, 2^-80 at u~72.7; FP32 coordinates carry log2(e)/sigma.
-  const double u_cut = f32 ? 37.0 * LOG2E : 73.0 * sigma;
+  // (1+u)^4 e^-u = 2^-32 at u~36.7, 2^-56 at u~54.9; FP32 coordinates carry log2(e)/sigma.
+  const double u_cut = f32 ? 37.0 * LOG2E : 55.0 * sigma;
   return u_cut * u_cut;
 }
 

@@ -191,7 +191,7 @@ def test_mfma_flow_steps_agree(monkeypatch):
 @pytest.mark.parametrize("name,order", [("gauss", 10), ("laplacian", 2)])
 def test_tile_culling_matches_full_sums(prec, name, order):
     """Clustered points with a small kernel width: most (wave, tile) pairs are culled because
-    every kernel value between them is below 2^-32 (FP32) / 2^-80 (FP64); the sums still
+    every kernel value between them is below 2^-32 (FP32) / 2^-56 (FP64); the sums still
     match the oracle, which evaluates every pair."""
     from registration_amd import kernelFunctions as kfun
     rng = np.random.default_rng(21)
