@@ -31,9 +31,9 @@ FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
 # HBM bytes per fused flow step at 40,962 points (pair kernel + finalize + totals), measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on `bench.py --roofline-only --roofline-reps 3` and
 # scaled by the counters' reading for 256 MiB device copies (FETCH x2.0, WRITE x1.0), in the
-# template's spatial processing order: 43.9 MB read + 27.3 MB written
+# template's spatial processing order: 43.8 MB read + 27.3 MB written
 # (profiles/flow_step_s6_fp32c.md).
-FLOW_STEP_TRAFFIC_S6 = 71.2e6
+FLOW_STEP_TRAFFIC_S6 = 71.0e6
 
 
 def _progress(msg):

@@ -155,7 +155,7 @@ typedef struct lddmm_seg {
   int32_t chunk_len;   /* sources per chunk (multiple of the LDS tile)          */
   int32_t fin_blocks;  /* finalize blocks                                       */
   int32_t pad_;
-  /* [in] optional processing order of FLOW_STEP and KDOT segments (rows = sources),
+  /* [in] optional processing order of FLOW_STEP, COVECTOR_STEP and KDOT segments (rows = sources),
    * NULL for the natural one: a permutation of 0..ny-1 in device memory, perm[i] =
    * the point taken as the i-th row and the i-th source.  Outputs stay indexed by
    * point, and the sums are the same up to rounding.  An order that keeps

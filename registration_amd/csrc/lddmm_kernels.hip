@@ -2338,8 +2338,10 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
         if (!s.x0) nx0 = s.ny;
         break;
     }
-    // A processing order: the flow step and the RKHS products (rows = set-0 sources).
-    if (s.perm && op != LDDMM_OP_FLOW_STEP && op != LDDMM_OP_KDOT) return LDDMM_EINVAL;
+    // A processing order: the flow and covector steps and the RKHS products (rows = set-0
+    // sources).
+    if (s.perm && op != LDDMM_OP_FLOW_STEP && op != LDDMM_OP_COVECTOR_STEP && op != LDDMM_OP_KDOT)
+      return LDDMM_EINVAL;
     if (nx0 < 1) return LDDMM_EINVAL;
     s.nx0 = nx0;
     if (op != LDDMM_OP_CURRENTS && op != LDDMM_OP_MEASURE && op != LDDMM_OP_VARIFOLD) s.nx1 = 0;

@@ -254,6 +254,7 @@ class CovectorJob:
     pxt: torch.Tensor           # (T+1, N, d) output
     dt: Optional[float] = None  # time step; 1/T when None
     A: Optional[torch.Tensor] = None  # (T, d, d) affine part: p_t += dt p_{t+1} A_t
+    perm: Optional[torch.Tensor] = None  # (N,) int32 processing order (spatial_order_dev)
 
     @property
     def T(self):
@@ -265,9 +266,10 @@ class CovectorJob:
 
 
 def covector_record(N: int, T: int, K: DevKernel, regweight: float,
-                    dt: Optional[float] = None) -> np.ndarray:
-    """The pointer-free part of a registration's covector-step descriptor (one record)."""
-    r = flow_record(N, T, K, dt)
+                    dt: Optional[float] = None, perm: Optional[torch.Tensor] = None) -> np.ndarray:
+    """The pointer-free part of a registration's covector-step descriptor (one record), with
+    the optional processing order of its points."""
+    r = flow_record(N, T, K, dt, perm)
     r["coef"] = regweight
     return r
 
@@ -302,7 +304,8 @@ def covector(jobs: Sequence[CovectorJob]) -> None:
     torch._foreach_copy_([j.pxt[j.T] for j in jobs], [j.px1 for j in jobs])
     for group in _groups(jobs, lambda j: (j.K.family, j.xt.shape[2])):
         d = group[0].xt.shape[2]
-        base = rt.stack([rt.raw(covector_record(j.xt.shape[1], j.T, j.K, j.regweight, j.step))
+        base = rt.stack([rt.raw(covector_record(j.xt.shape[1], j.T, j.K, j.regweight, j.step,
+                                                j.perm))
                          for j in group])
         aff = any(j.A is not None for j in group)
         covector_records(base, group[0].K.prec, d, [j.T for j in group],

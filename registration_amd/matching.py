@@ -132,7 +132,8 @@ class DeviceMatching:
         self._pxt = torch.empty_like(self.xt)
         self._pxt_tail = self._pxt[1:]
         self._pxT = self._pxt.data_ptr() + T * self._slice_bytes
-        self._cov_base = rt.raw(engine.covector_record(N, T, self.Kdiff, self.regweight))
+        self._cov_base = rt.raw(engine.covector_record(N, T, self.Kdiff, self.regweight,
+                                                       perm=self._order))
         self._def_stale = False
         if self.data_kind in SHAPE_TERMS:
             Ke = self.Kdist.energy()

@@ -108,13 +108,16 @@ def test_plan_rejects_invalid():
 
 
 def test_plan_processing_order():
-    """A processing order is accepted for the flow step and the RKHS products only."""
+    """A processing order is accepted for the flow and covector steps and the RKHS products
+    only."""
     assert _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP32C, 3, [_seg(100, perm=4096)])[0] == 0
     kd = _seg(100, perm=4096, na=1)
     kd.g[0] = 4096
     assert _plan(_lib.OP_KDOT, _lib.PREC_FP32C, 3, [kd])[0] == 0
     assert _plan(_lib.OP_APPLY_K, _lib.PREC_FP32C, 3, [_seg(100, perm=4096, m=3)])[0] == _lib.EINVAL
-    assert _plan(_lib.OP_COVECTOR_STEP, _lib.PREC_FP32C, 3, [_seg(100, perm=4096)])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_COVECTOR_STEP, _lib.PREC_FP32C, 3, [_seg(100, perm=4096)])[0] == 0
+    assert _plan(_lib.OP_CURRENTS, _lib.PREC_FP64, 3,
+                 [_seg(100, perm=4096, x1=4096, nx1=50)])[0] == _lib.EINVAL
     assert _plan(_lib.OP_KNORM, _lib.PREC_FP64, 3, [_seg(100, perm=4096, out0=0)])[0] == _lib.EINVAL
 
 

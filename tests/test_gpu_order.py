@@ -147,6 +147,31 @@ def test_ordered_and_natural_segments_in_one_launch():
     assert np.array_equal(j3.xt.cpu().numpy(), alone.xt.cpu().numpy())
 
 
+def test_ordered_covector_against_oracle():
+    """The adjoint (covector) steps in the processing order: trajectory of covectors against
+    the oracle's FP64 landmarkHamiltonianCovector and against the natural order."""
+    import torch
+    from registration_amd import engine, synthetic
+    V, _ = synthetic.icosphere(5)
+    rng = np.random.default_rng(35)
+    T = 4
+    at = 0.02 * rng.standard_normal((T,) + V.shape)
+    px1 = rng.standard_normal(V.shape)
+    K = gauss(0.3)
+    perm = engine.spatial_order_dev(V)
+    xt, _, _ = flow(V, at, K, perm)
+    dev = lambda a: torch.as_tensor(a, device="cuda")
+    outs = []
+    for p in (perm, None):
+        pxt = torch.empty((T + 1,) + V.shape, dtype=torch.float64, device="cuda")
+        engine.covector([engine.CovectorJob(dev(xt), dev(at), dev(px1), K, 0.5, pxt, perm=p)])
+        outs.append(pxt.cpu().numpy())
+    print("ordered covector, icosphere s5, T=4", flush=True)
+    ref, _ = orc.landmarkHamiltonianCovector(V, at, px1, orc.Kernel('gauss', sigma=0.3), 0.5)
+    assert rel_err(outs[0] - px1, ref - px1) < 5 * TOL
+    assert rel_err(outs[0] - px1, outs[1] - px1) < 5 * TOL
+
+
 def test_ordered_rkhs_products():
     """<g2_j[t], K(x_t) g1[t]> in the processing order against the natural order and the
     oracle (scale: the Cauchy-Schwarz bound sqrt(<g1, K g1><g2, K g2>))."""
