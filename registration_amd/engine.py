@@ -143,13 +143,19 @@ def order_for(points, K: "DevKernel", min_fraction: float = 0.5):
     """The processing order a registration of ``points`` under K should use, as a device
     int32 tensor, or None when it would not pay: K has no dot-product form, or fewer than
     ``min_fraction`` of the quarters would be compact enough for it (small point sets at
-    wide kernels, where the order only adds gathers)."""
+    wide kernels, where the order only adds gathers).  The spatial order competes with the
+    points' own order (the identity permutation): sampled curves are already compact in
+    arc order, more so than after bisection."""
     if not wants_order(K):
         return None
-    o = spatial_order(points)
-    if compact_fraction(points, o, K.sigma) < min_fraction:
+    P = np.asarray(points, dtype=np.float64)
+    ident = np.arange(P.shape[0], dtype=np.int32)
+    o = spatial_order(P)
+    f_o, f_id = compact_fraction(P, o, K.sigma), compact_fraction(P, ident, K.sigma)
+    best, f = (ident, f_id) if f_id >= f_o else (o, f_o)
+    if f < min_fraction:
         return None
-    return torch.from_numpy(o).to(rt.device())
+    return torch.from_numpy(best).to(rt.device())
 
 
 # --------------------------------------------------------------------------------------

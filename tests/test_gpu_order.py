@@ -117,6 +117,46 @@ def test_ordered_flow_with_outliers():
     assert rel_err(v, vn) < TOL
 
 
+@pytest.mark.parametrize("which", ["spatial", "identity"])
+def test_ordered_flow_on_a_curve(which):
+    """d = 2: a closed star curve (2,048 points, sigma 0.3), flowed in the spatial order (91% of
+    quarters compact) and in its own arc order passed as an identity permutation (all
+    compact), against the oracle and the unordered flow."""
+    import torch
+    from registration_amd import engine, synthetic
+    X = synthetic.star_curve(2048, 5, 0.2, noise=0.0, rng=np.random.default_rng(0))
+    rng = np.random.default_rng(22)
+    at = 0.02 * rng.standard_normal((4,) + X.shape)
+    if which == "spatial":
+        perm = engine.spatial_order_dev(X)
+    else:
+        perm = torch.arange(X.shape[0], dtype=torch.int32, device="cuda")
+    frac = quarter_fraction(np.c_[X, np.zeros(len(X))], perm.cpu().numpy(), 0.3)
+    assert frac > 0.8
+    print(f"ordered flow on a curve ({which}), dot-product quarters {frac:.2f}", flush=True)
+    K = gauss(0.3)
+    xt, v, en = flow(X, at, K, perm)
+    ref = orc.landmarkDirectEvolutionEuler(X, at, orc.Kernel('gauss', sigma=0.3))
+    assert np.abs(xt - ref).max() < 1e-6
+    assert rel_err(v[0], orc.apply_k(X, X, at[0], 0.3, 10)) < TOL
+    _, vn, _ = flow(X, at, K, None)
+    assert rel_err(v, vn) < TOL
+
+
+def test_driver_takes_the_more_compact_order():
+    """Curves keep their arc order (as an identity permutation); the large surface takes the
+    spatial order; small spheres at wide kernels take none."""
+    from registration_amd import engine, synthetic
+    X = synthetic.star_curve(2048, 5, 0.2, noise=0.0, rng=np.random.default_rng(0))
+    o = engine.order_for(X, gauss(0.3))
+    assert o is not None and np.array_equal(o.cpu().numpy(), np.arange(len(X)))
+    V6, _ = synthetic.icosphere(6)
+    o6 = engine.order_for(V6, gauss(0.25))
+    assert o6 is not None and not np.array_equal(o6.cpu().numpy(), np.arange(len(V6)))
+    V3, _ = synthetic.icosphere(3)
+    assert engine.order_for(V3, gauss(0.3)) is None
+
+
 def test_ordered_and_natural_segments_in_one_launch():
     """A served batch mixes registrations with and without a processing order in one launch;
     each trajectory against the oracle, and against the same trajectory flowed alone."""
