@@ -12,6 +12,7 @@ It restates, in Python 3 / NumPy float64, the executed reference path:
                           (currents, measure and varifold data terms)
   SurfaceMatching         py-lddmm/surfaceMatching.py:19-484 (affine='none', symmetric=False)
   CurveMatching           py-lddmm/curveMatching.py:19-352 (affine='none')
+  SecondOrderSurfaceMatching  py-lddmm/secondOrderMatching.py:43-509 (time series of targets)
   cg                      py-lddmm/conjugateGradient.py:33-217
 Python-2 idioms are restated with their Python-2 meaning (``x == None`` on arrays -> ``is None``).
 File output (VTK), logging text and the saveRate branches are left out: they do not feed
@@ -1388,3 +1389,171 @@ def cg(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=10.):
             if hasattr(opt, 'endOfIteration'):
                 opt.endOfIteration()
     return opt.history
+
+
+# --------------------------------------------------------------------------------------
+# Second-order (spline) regression through a time series of surfaces
+# --------------------------------------------------------------------------------------
+class SecondOrderSurfaceMatching:
+    """secondOrderMatching.SurfaceMatching (secondOrderMatching.py:43-509) restated without
+    file output: variables (a0, rhot, Afft); objective 1/2 <a0, K a0> + sum_t dt cw |rho_t|^2/2
+    + sum_t dt <w, Afft_t^2>/2 + data terms at times (k+1) T1; regression type 'affine' for
+    the first affBurnIn iterations, then typeRegression; Euclidean dot products."""
+
+    def __init__(self, Template, Targets, KparDiff, KparDist, sigmaError=1.0, timeStep=.1,
+                 errorType='current', maxIter=1000, typeRegression='spline2', affine='none',
+                 controlWeight=1.0, affineWeight=1.0, initialMomentum=None):
+        self.fv0 = Surface(surf=Template)
+        self.fv1 = [Surface(surf=s) for s in Targets]
+        self.nTarg = len(self.fv1)
+        self.iter = 0
+        self.dim = self.fv0.vertices.shape[1]
+        self.maxIter = maxIter
+        self.KparDiff, self.KparDist, self.sigmaError = KparDiff, KparDist, sigmaError
+        self.fun_obj0, self.fun_obj, self.fun_objGrad = _DATA_TERMS['surface'][errorType]
+        self.affB = AffineBasis(self.dim, affine)
+        self.affineDim = self.affB.affineDim
+        self.affineBasis = self.affB.basis
+        self.affineWeight = affineWeight * np.ones(self.affineDim)
+        self.controlWeight = controlWeight
+        self.typeRegression = 'affine'
+        self.typeRegressionSave = typeRegression
+        self.x0 = self.fv0.vertices
+        self.fvDef = [Surface(surf=self.fv0) for _ in range(self.nTarg)]
+        self.npt = self.x0.shape[0]
+        self.Tsize1 = int(round(1.0 / timeStep))
+        self.Tsize = self.nTarg * self.Tsize1
+        self.rhot = np.zeros([self.Tsize, self.npt, self.dim])
+        self.a0 = (np.zeros([self.npt, self.dim]) if initialMomentum is None
+                   else np.asarray(initialMomentum, dtype=np.float64))
+        self.rhotTry = np.zeros_like(self.rhot)
+        self.a0Try = np.zeros_like(self.a0)
+        self.Afft = np.zeros([self.Tsize, self.affineDim])
+        self.AfftTry = np.zeros_like(self.Afft)
+        self.obj = None
+        self.objTry = None
+        self.gradCoeff = self.npt
+        self.affBurnIn = 50
+        self.coeffAff = 1
+
+    def _affine(self, Afft):
+        dim2 = self.dim ** 2
+        A = [np.zeros([self.Tsize, self.dim, self.dim]), np.zeros([self.Tsize, self.dim])]
+        if self.affineDim > 0:
+            for t in range(self.Tsize):
+                AB = np.dot(self.affineBasis, Afft[t])
+                A[0][t] = AB[0:dim2].reshape([self.dim, self.dim])
+                A[1][t] = AB[dim2:dim2 + self.dim]
+        return A
+
+    def dataTerm(self, fvDef):
+        return sum(self.fun_obj(s, self.fv1[k], self.KparDist) for k, s in enumerate(fvDef)) \
+            / self.sigmaError ** 2
+
+    def objectiveFunDef(self, a0, rhot, Afft):
+        timeStep = 1.0 / self.Tsize
+        xt, at = secondOrderEvolution(self.x0, a0, rhot, self.KparDiff, affine=self._affine(Afft))
+        obj0 = 0.5 * (a0 * self.KparDiff.applyK(self.x0, a0)).sum()
+        obj1 = obj2 = 0
+        for t in range(self.Tsize):
+            obj1 += timeStep * self.controlWeight * (rhot[t] ** 2).sum() / 2
+            if self.affineDim > 0:
+                obj2 += timeStep * (self.affineWeight * Afft[t] ** 2).sum() / 2
+        return obj1 + obj2 + obj0, xt, at
+
+    def objectiveFun(self):
+        if self.obj is None:
+            self.obj, self.xt, self.at = self.objectiveFunDef(self.a0, self.rhot, self.Afft)
+            self.obj0 = 0
+            for k in range(self.nTarg):
+                self.obj0 += self.fun_obj0(self.fv1[k], self.KparDist) / self.sigmaError ** 2
+                self.fvDef[k].updateVertices(self.xt[(k + 1) * self.Tsize1])
+            self.obj += self.obj0 + self.dataTerm(self.fvDef)
+        return self.obj
+
+    def getVariable(self):
+        return (self.a0, self.rhot, self.Afft)
+
+    def updateTry(self, dir, eps, objRef=None):
+        a0Try = self.a0 - eps * dir.a0
+        rhotTry = self.rhot - eps * dir.rhot
+        AfftTry = self.Afft - eps * dir.aff if self.affineDim > 0 else self.Afft
+        obj, xt, _ = self.objectiveFunDef(a0Try, rhotTry, AfftTry)
+        ff = []
+        for k in range(self.nTarg):
+            ff.append(Surface(surf=self.fvDef[k]))
+            ff[k].updateVertices(xt[(k + 1) * self.Tsize1])
+        objTry = self.obj0 + obj + self.dataTerm(ff)
+        if np.isnan(objTry):
+            return 1e500
+        if objRef is None or objTry < objRef:
+            self.a0Try, self.rhotTry, self.AfftTry, self.objTry = a0Try, rhotTry, AfftTry, objTry
+        return objTry
+
+    def getGradient(self, coeff=1.0):
+        px1 = [-self.fun_objGrad(self.fvDef[k], self.fv1[k], self.KparDist) / self.sigmaError ** 2
+               for k in range(self.nTarg)]
+        pa1 = [np.zeros(self.a0.shape) for _ in range(self.nTarg)]
+        foo = secondOrderGradient(self.x0, self.a0, self.rhot, px1, pa1, self.KparDiff,
+                                  affine=self._affine(self.Afft),
+                                  times=(1 + np.array(range(self.nTarg))) * self.Tsize1,
+                                  controlWeight=self.controlWeight)
+        grd = Direction()
+        if self.typeRegression == 'spline':
+            grd.a0, grd.rhot = np.zeros(foo[0].shape), foo[1] / coeff
+        elif self.typeRegression == 'geodesic':
+            grd.a0, grd.rhot = foo[0] / coeff, np.zeros(foo[1].shape)
+        elif self.typeRegression == 'affine':
+            grd.a0, grd.rhot = np.zeros(foo[0].shape), np.zeros(foo[1].shape)
+        else:
+            grd.a0, grd.rhot = foo[0] / coeff, foo[1] / coeff
+        grd.aff = np.zeros(self.Afft.shape)
+        if self.affineDim > 0 and self.iter < self.affBurnIn:
+            dim2 = self.dim ** 2
+            dA, db = foo[2], foo[3]
+            grd.aff = np.multiply(self.affineWeight.reshape([1, self.affineDim]), self.Afft)
+            for t in range(self.Tsize):
+                dAff = np.dot(self.affineBasis.T, np.vstack([dA[t].reshape([dim2, 1]),
+                                                             db[t].reshape([self.dim, 1])]))
+                grd.aff[t] -= dAff.reshape(grd.aff[t].shape)
+            grd.aff /= (self.coeffAff * coeff * self.Tsize)
+        return grd
+
+    def addProd(self, dir1, dir2, beta):
+        d = Direction()
+        d.a0 = dir1.a0 + beta * dir2.a0
+        d.rhot = dir1.rhot + beta * dir2.rhot
+        d.aff = dir1.aff + beta * dir2.aff
+        return d
+
+    def copyDir(self, dir0):
+        d = Direction()
+        d.a0, d.rhot, d.aff = np.copy(dir0.a0), np.copy(dir0.rhot), np.copy(dir0.aff)
+        return d
+
+    def dotProduct(self, g1, g2):
+        res = np.zeros(len(g2))
+        for ll, gr in enumerate(g2):
+            res[ll] = (gr.rhot * g1.rhot).sum() / self.Tsize
+            res[ll] += (gr.a0 * g1.a0).sum()
+            res[ll] += (g1.aff * gr.aff).sum() * self.coeffAff
+        return res
+
+    def acceptVarTry(self):
+        self.obj = self.objTry
+        self.a0, self.rhot, self.Afft = np.copy(self.a0Try), np.copy(self.rhotTry), np.copy(self.AfftTry)
+
+    def endOfIteration(self):
+        self.iter += 1
+        if self.iter >= self.affBurnIn:
+            self.typeRegression = self.typeRegressionSave
+        _, self.xt, self.at = self.objectiveFunDef(self.a0, self.rhot, self.Afft)
+        for k in range(self.nTarg):
+            self.fvDef[k].updateVertices(self.xt[(k + 1) * self.Tsize1])
+
+    def optimizeMatching(self):
+        grd = self.getGradient(self.gradCoeff)
+        [grd2] = self.dotProduct(grd, [grd])
+        self.gradEps = max(0.001, np.sqrt(grd2) / 10000)
+        self.cgBurnIn = self.affBurnIn
+        return cg(self, maxIter=self.maxIter, epsInit=0.1)
