@@ -10,7 +10,7 @@ It restates, in Python 3 / NumPy float64, the executed reference path:
   landmark flow/adjoint   py-lddmm/pointEvolution.py:599-817 (second order: :882-1084)
   surfaces / curves       py-lddmm/surfaces.py:61-75,711-717,1009-1276; curves.py:58-73,539-701
                           (currents, measure and varifold data terms)
-  SurfaceMatching         py-lddmm/surfaceMatching.py:19-484 (affine='none', symmetric=False)
+  SurfaceMatching         py-lddmm/surfaceMatching.py:19-484 (SymmetricSurfaceMatching: symmetric=True)
   CurveMatching           py-lddmm/curveMatching.py:19-352 (affine='none')
   SecondOrderSurfaceMatching  py-lddmm/secondOrderMatching.py:43-509 (time series of targets)
   cg                      py-lddmm/conjugateGradient.py:33-217
@@ -1204,6 +1204,113 @@ class SurfaceMatching(_Matching):
         self.gradEps = max(self.gradLB, np.sqrt(grd2) / 10000)
         self.coeffAff = self.coeffAff1
         return cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+
+
+class SymmetricSurfaceMatching(SurfaceMatching):
+    """surfaceMatching.SurfaceMatching with symmetric=True (surfaceMatching.py:181-185,
+    232-289,297-299,325-327,373-376,383,466): the template's points x0 are a variable, the data
+    term adds |S(x0) - S0|^2/sigmaError^2 (obj0 adds |S0|^2), the x0 gradient is
+    (initPointGradient - p_0)/(coeffInitx coeff) with coeffInitx = 0.1, which also weighs the
+    x0 part of the dot product; the moved template's surface is refreshed in endOfIteration.
+    The flow takes the affine branch with zero matrices when there is no affine part, as the
+    reference's objectiveFunDef and getGradient always pass one."""
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.x0Try = np.copy(self.x0)
+        self.coeffInitx = .1
+
+    def _transforms(self, Afft):
+        if self.affineDim > 0:
+            return self.affB.getTransforms(Afft)
+        return [np.zeros([self.Tsize, self.dim, self.dim]), np.zeros([self.Tsize, self.dim])]
+
+    def dataTerm(self, fvDef, fvInit=None):
+        obj = self.fun_obj(fvDef, self.fv1, self.KparDist) / (self.sigmaError ** 2)
+        if fvInit is not None:
+            obj += self.fun_obj(fvInit, self.fv0, self.KparDist) / (self.sigmaError ** 2)
+        return obj
+
+    def objectiveFunDef(self, at, Afft=None, withTrajectory=False, x0=None):
+        saved = self.x0
+        if x0 is not None:
+            self.x0 = x0
+        try:
+            return super().objectiveFunDef(at, Afft, withTrajectory=withTrajectory)
+        finally:
+            self.x0 = saved
+
+    def objectiveFun(self):
+        if self.obj is None:
+            self.obj0 = self.fun_obj0(self.fv1, self.KparDist) / (self.sigmaError ** 2)
+            self.obj0 += self.fun_obj0(self.fv0, self.KparDist) / (self.sigmaError ** 2)
+            self.obj, self.xt = self.objectiveFunDef(self.at, self.Afft, withTrajectory=True)
+            self.fvDef.updateVertices(self.xt[-1])
+            self.fvInit.updateVertices(self.x0)
+            self.obj += self.obj0 + self.dataTerm(self.fvDef, self.fvInit)
+        return self.obj
+
+    def updateTry(self, dir, eps, objRef=None):
+        objTry = self.obj0
+        atTry = self.at - eps * dir.diff
+        AfftTry = self.Afft - eps * dir.aff if self.affineDim > 0 else self.Afft
+        x0Try = self.x0 - eps * dir.initx
+        obj, xt = self.objectiveFunDef(atTry, AfftTry, withTrajectory=True, x0=x0Try)
+        objTry += obj
+        ff = Surface(surf=self.fvDef)
+        ff.updateVertices(xt[-1])
+        ffI = Surface(surf=self.fvInit)
+        ffI.updateVertices(x0Try)
+        objTry += self.dataTerm(ff, ffI)
+        if np.isnan(objTry):
+            return 1e500
+        if (objRef is None) or (objTry < objRef):
+            self.atTry, self.objTry, self.AfftTry, self.x0Try = atTry, objTry, AfftTry, x0Try
+        return objTry
+
+    def getGradient(self, coeff=1.0):
+        px1 = -self.endPointGradient()
+        foo = landmarkHamiltonianGradient(self.x0, self.at, px1, self.KparDiff, self.regweight,
+                                          affine=self._transforms(self.Afft), getCovector=True)
+        grd = Direction()
+        grd.diff = foo[0] / (coeff * self.Tsize)
+        grd.aff = np.zeros(self.Afft.shape)
+        if self.affineDim > 0:
+            dA, db = foo[1], foo[2]
+            dim2 = self.dim ** 2
+            grd.aff = 2 * np.multiply(self.affineWeight.reshape([1, self.affineDim]), self.Afft)
+            for t in range(self.Tsize):
+                dAff = np.dot(self.affineBasis.T, np.vstack([dA[t].reshape([dim2, 1]),
+                                                             db[t].reshape([self.dim, 1])]))
+                grd.aff[t] -= dAff.reshape(grd.aff[t].shape)
+            grd.aff /= (self.coeffAff * coeff * self.Tsize)
+        initGrad = self.fun_objGrad(self.fvInit, self.fv0, self.KparDist) / self.sigmaError ** 2
+        grd.initx = (initGrad - foo[4][0]) / (self.coeffInitx * coeff)
+        return grd
+
+    def addProd(self, dir1, dir2, beta):
+        d = super().addProd(dir1, dir2, beta)
+        d.initx = dir1.initx + beta * dir2.initx
+        return d
+
+    def copyDir(self, dir0):
+        d = super().copyDir(dir0)
+        d.initx = np.copy(dir0.initx)
+        return d
+
+    def dotProduct(self, g1, g2):
+        res = super().dotProduct(g1, g2)
+        for ll, gr in enumerate(g2):
+            res[ll] += (g1.initx * gr.initx).sum() * self.coeffInitx
+        return res
+
+    def acceptVarTry(self):
+        super().acceptVarTry()
+        self.x0 = np.copy(self.x0Try)
+
+    def endOfIteration(self):
+        super().endOfIteration()
+        self.fvInit.updateVertices(self.x0)
 
 
 class CurveMatching(_Matching):
