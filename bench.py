@@ -230,11 +230,29 @@ def cpu_baseline(cfg, pair_counts, seconds_target=10.0):
     d_pairs = pair_counts.get(_lib.OP_COVECTOR_STEP, 0)
     secs = k_pairs / k_rate + d_pairs / d_rate
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / secs, "unit": "registrations/s", "cores": threads, "kind": "port",
-            "sample": f"oracle applyK and applyDiffKT(na=3), {rows} x {N} pairs each "
-                      f"({tk + td:.1f} s); extrapolated to the GPU run's {k_pairs + d_pairs:.3e} pairs",
-            "extrapolated": True, "pairs_per_s_applyk": k_rate, "pairs_per_s_diffkt": d_rate,
-            "seconds_per_registration": secs}
+    out = {"value": 1.0 / secs, "unit": "registrations/s", "cores": threads, "kind": "port",
+           "sample": f"oracle applyK and applyDiffKT(na=3), {rows} x {N} pairs each "
+                     f"({tk + td:.1f} s); extrapolated to the GPU run's {k_pairs + d_pairs:.3e} pairs",
+           "extrapolated": True, "pairs_per_s_applyk": k_rate, "pairs_per_s_diffkt": d_rate,
+           "seconds_per_registration": secs}
+    if orc.ref_available():
+        # The reference's own Fortran sums (kernelFunctions.f90, compiled by oracle/Makefile),
+        # same cores, a smaller row sample: the slower of the two CPU paths.
+        rr = min(N, 2048)
+        t0 = time.perf_counter()
+        orc.apply_k(V[:rr], V, a, cfg["sigmaKernel"], 10, sums="reference")
+        rk = rr * N / (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        orc.apply_diff_kt(V[:rr], V, a1[:, :rr], a2, cfg["sigmaKernel"], 10, sums="reference")
+        rd = rr * N / (time.perf_counter() - t0)
+        rsecs = k_pairs / rk + d_pairs / rd
+        out["reference"] = {"value": 1.0 / rsecs, "unit": "registrations/s", "cores": threads,
+                            "kind": "reference",
+                            "sample": f"the reference's Fortran applyk / applykdifft (na=3), "
+                                      f"{rr} x {N} pairs each; extrapolated likewise",
+                            "pairs_per_s_applyk": rk, "pairs_per_s_diffkt": rd,
+                            "seconds_per_registration": rsecs}
+    return out
 
 
 def _served_compare(reqs, prec, label, plain_sample=None, max_active=256):
