@@ -154,7 +154,9 @@ typedef struct lddmm_seg {
   int32_t chunks1;     /* source chunks over set 1                              */
   int32_t chunk_len;   /* sources per chunk (multiple of the LDS tile)          */
   int32_t fin_blocks;  /* finalize blocks                                       */
-  int32_t pad_;
+  int32_t stage0;      /* first staged source tile (segments with a perm):     *
+                        * their set-0 tiles are staged once per launch in the    *
+                        * workspace and read by every row tile                   */
   /* [in] optional processing order of FLOW_STEP, COVECTOR_STEP and KDOT segments (rows = sources),
    * NULL for the natural one: a permutation of 0..ny-1 in device memory, perm[i] =
    * the point taken as the i-th row and the i-th source.  Outputs stay indexed by
@@ -169,7 +171,8 @@ typedef struct lddmm_seg {
 
 /* Plan a segmented launch: fills the [plan] fields of segs[0..nseg) and
  * returns the number of pair-sum work items, finalize blocks and the
- * workspace size (descriptor head + chunk partials + reduction partials).
+ * workspace size (descriptor head + chunk partials + reduction partials + the
+ * staged source tiles of segments with a perm).
  * Host-only; touches no device memory.  Validates every segment (d, m, na,
  * kernel, order, sizes); all segments of one launch share the operation, the
  * kernel family, d, m and na. */

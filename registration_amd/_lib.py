@@ -61,7 +61,7 @@ class Seg(ctypes.Structure):
         ("item0", _I64), ("part0", _I64), ("fin0", _I64), ("red0", _I64),
         ("row_tiles", ctypes.c_int32), ("chunks0", ctypes.c_int32),
         ("chunks1", ctypes.c_int32), ("chunk_len", ctypes.c_int32),
-        ("fin_blocks", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("fin_blocks", ctypes.c_int32), ("stage0", ctypes.c_int32),
         ("perm", _P),
     ]
 

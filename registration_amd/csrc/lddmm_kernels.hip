@@ -369,14 +369,157 @@ __host__ __device__ inline int weight_ld(int op, const lddmm_seg_t *s, int d) {
 
 __device__ inline int find_segment(const lddmm_seg_t *__restrict__ segs, int nseg,
                                    int64_t key, int which) {
-  // Largest i with start(i) <= key; which: 0 -> item0, 1 -> fin0.
+  // Largest i with start(i) <= key; which: 0 -> item0, 1 -> fin0, 2 -> stage0.
   int lo = 0, hi = nseg - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
-    int64_t st = which == 0 ? segs[mid].item0 : segs[mid].fin0;
+    int64_t st = which == 0 ? segs[mid].item0 : (which == 1 ? segs[mid].fin0 : segs[mid].stage0);
     if (st <= key) lo = mid; else hi = mid - 1;
   }
   return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Source tiles in LDS form.  A tile holds TS consecutive sources of one set, in the
+// segment's processing order for set 0 when it has one.  FP32 evaluates each tile in a
+// frame centred at the tile (origin below), so the staged positions depend only on the
+// tile, not on the rows that read it.  k_pairs builds a tile in LDS itself, or, for
+// ordered segments, copies one that k_stage built once per launch in the workspace
+// (the same values: both paths call the functions below), so the gathers through the
+// order, the FP64 reads and conversions and the box reductions run once per tile instead
+// of once per row tile.
+// ---------------------------------------------------------------------------
+
+// Tile origin: FP32 mode evaluates the tile in a frame centred at its middle, so rounding
+// to FP32 costs relative precision of the local distances, not of the absolute
+// coordinates.  The origin is the mean of the tile's four quarter-middle sources (sources
+// 32, 96, 160, 224, clamped to the tile), close to the centre of the tile's extent for
+// spatially ordered sources.  FP64 keeps the raw frame (origin 0).
+template <typename R, int D>
+__device__ inline void tile_origin(const SrcView &src, const int32_t *__restrict__ sperm,
+                                   int64_t base, int64_t tlen, double scale, double org[D]) {
+#pragma unroll
+  for (int c = 0; c < D; ++c) org[c] = 0.0;
+  if constexpr (std::is_same<R, float>::value) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int64_t ls = base + (32 + 64 * q < tlen ? 32 + 64 * q : tlen - 1);
+      if (sperm) ls = sperm[ls];
+#pragma unroll
+      for (int c = 0; c < D; ++c) org[c] += src.pos[ls * D + c];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) org[c] *= 0.25 * scale;
+  }
+}
+
+// One source of a tile, as staged: tile-frame position (doubled in a compact quarter of
+// the dot-product form), signed weights, -|v|^2, and the wave-reduced box and largest
+// |v|^2 of its 64-source quarter.
+template <typename R, int D, int M, int NSW>
+struct SrcStage {
+  R v[D];
+  R w[NSW > 0 ? NSW : 1][M];
+  R n2;
+  R lo[D], hi[D];
+  R q2;
+};
+
+// Stage source l (< l1: a real source; else zero-weight padding with an empty box).  Every
+// lane of the wave calls it (wave reductions).  DOT: the dot-product form is possible for
+// this segment (its quarter is staged doubled when max |v|^2 <= EXP_R2MAX).
+template <typename R, int D, int M, int NSW>
+__device__ inline void stage_source(const SrcView &src, const int32_t *__restrict__ sperm,
+                                    int ld, int64_t l, int64_t l1, double scale,
+                                    const double org[D], bool dot,
+                                    SrcStage<R, D, M, NSW> &o) {
+  R n2 = 0;
+  if (l < l1) {
+    const int64_t lg = sperm ? (int64_t)sperm[l] : l;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      o.v[c] = (R)(src.pos[lg * D + c] * scale - org[c]);
+      o.lo[c] = o.v[c];
+      o.hi[c] = o.v[c];
+      n2 = o.v[c] * o.v[c] + n2;
+    }
+#pragma unroll
+    for (int j = 0; j < NSW; ++j)
+#pragma unroll
+      for (int c = 0; c < M; ++c) o.w[j][c] = (R)(src.sign * src.w[j][lg * ld + c]);
+  } else {
+    // Padding sources carry zero weight: their terms are exact zeros.
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      o.v[c] = (R)0;
+      o.lo[c] = (R)INFINITY;
+      o.hi[c] = (R)-INFINITY;
+    }
+#pragma unroll
+    for (int j = 0; j < NSW; ++j)
+#pragma unroll
+      for (int c = 0; c < M; ++c) o.w[j][c] = (R)0;
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    o.lo[c] = wave_min(o.lo[c]);
+    o.hi[c] = wave_max(o.hi[c]);
+  }
+  o.n2 = -n2;
+  o.q2 = 0;
+  if (dot) {
+    o.q2 = wave_max(n2);
+    if (o.q2 <= (R)EXP_R2MAX) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) o.v[c] = (R)2 * o.v[c];
+    }
+  }
+}
+
+// Staged tile record in the workspace: origin (D doubles), quarter boxes [4][2D], quarter
+// max |v|^2 [4], then positions [D][TS], weights [NSW][M][TS] and -|v|^2 [TS].
+constexpr int STAGE_HEAD = 256;
+__host__ __device__ inline size_t stage_rec_bytes(int rsize, int d, int m, int nsw) {
+  return STAGE_HEAD + (size_t)(d + nsw * m + 1) * TS * rsize;
+}
+
+template <typename R, int KT, int D, int M, bool KSUM, int NA>
+__global__ __launch_bounds__(BLOCK) void k_stage(const lddmm_seg_t *__restrict__ segs, int nseg,
+                                                int op, int prec, char *__restrict__ stage) {
+  constexpr int NSW = (NA > (KSUM ? 1 : 0)) ? NA : (KSUM ? 1 : 0);
+  constexpr bool DOT = std::is_same<R, float>::value && KT == LDDMM_KERNEL_GAUSS && KSUM && NA == 0;
+  const int tid = threadIdx.x;
+  const int64_t tile = blockIdx.x;
+  const lddmm_seg_t *sg = segs + find_segment(segs, nseg, tile, 2);
+  const SrcView src = resolve_src(op, sg, 0);
+  const double scale = coord_scale(prec, KT, sg->sigma);
+  const int64_t base = (tile - sg->stage0) * TS;
+  const int64_t tlen = src.n - base < TS ? src.n - base : TS;
+  double org[D];
+  tile_origin<R, D>(src, sg->perm, base, tlen, scale, org);
+  SrcStage<R, D, M, NSW> ss;
+  stage_source<R, D, M, NSW>(src, sg->perm, weight_ld(op, sg, D), base + tid, src.n, scale, org,
+                             DOT, ss);
+  char *rec = stage + tile * stage_rec_bytes(sizeof(R), D, M, NSW);
+  R *rp = (R *)(rec + STAGE_HEAD);
+#pragma unroll
+  for (int c = 0; c < D; ++c) rp[c * TS + tid] = ss.v[c];
+#pragma unroll
+  for (int j = 0; j < NSW; ++j)
+#pragma unroll
+    for (int c = 0; c < M; ++c) rp[(D + j * M + c) * TS + tid] = ss.w[j][c];
+  rp[(D + NSW * M) * TS + tid] = ss.n2;
+  const int lane = tid & 63, wave = tid >> 6;
+  if (lane == 0) {
+    R *box = (R *)(rec + 32);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      box[wave * 2 * D + c] = ss.lo[c];
+      box[wave * 2 * D + D + c] = ss.hi[c];
+    }
+    ((R *)(rec + 224))[wave] = ss.q2;
+  }
+  if (tid < D) ((double *)rec)[tid] = org[tid];
 }
 
 // ---------------------------------------------------------------------------
@@ -387,7 +530,8 @@ __device__ inline int find_segment(const lddmm_seg_t *__restrict__ segs, int nse
 template <typename R, int KT, int D, int M, bool KSUM, int NA>
 __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__ segs,
                                                 int nseg, int op, int prec,
-                                                double *__restrict__ part) {
+                                                double *__restrict__ part,
+                                                const char *__restrict__ stage) {
   constexpr int RPT = rows_per_thread<R>();
   constexpr int NSW = (NA > (KSUM ? 1 : 0)) ? NA : (KSUM ? 1 : 0);
   constexpr int W = (KSUM ? M : 0) + (NA > 0 ? D : 0);
@@ -436,6 +580,8 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   const int32_t *__restrict__ perm = sg->perm;
   const int32_t *__restrict__ sperm = set == 0 ? perm : nullptr;
   const bool expand_seg = EXPAND && perm != nullptr;
+  // Ordered set-0 tiles come staged when the launch has a staging area.
+  const bool staged = stage != nullptr && sperm != nullptr;
 
   KEval<R, KT> kev;
   kev.init(sg->order, sg->sigma);
@@ -524,89 +670,58 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   for (int64_t base = l0; base < l1; base += TS) {
     if (sym && base / TS < wg_blk) continue;  // workgroup-uniform: below every wave's block
     const int sym_w = !sym ? 1 : (base / TS > wave_blk ? 2 : (base / TS == wave_blk ? 1 : 0));
-    // Tile origin: FP32 mode evaluates the tile in a frame centred at its middle
-    // source, so rounding to FP32 costs relative precision of the local
-    // distances, not of the absolute coordinates (for sources ordered along a
-    // curve or a mesh, the middle one halves the largest offset in the tile).
-    // FP64 keeps the raw frame.
-    // The origin is the mean of the tile's four quarter-middle sources (sources 32, 96,
-    // 160, 224, clamped to the tile), close to the centre of the tile's extent for
-    // spatially ordered sources.
     double org[D];
     const int64_t tlen = l1 - base < TS ? l1 - base : TS;
+    if (staged) {
+      // Copy the tile k_stage built (chunks start at multiples of TS: tile base / TS).
+      const char *rec = stage + (sg->stage0 + base / TS) * stage_rec_bytes(sizeof(R), D, M, NSW);
 #pragma unroll
-    for (int c = 0; c < D; ++c) org[c] = 0.0;
-    if constexpr (KAHAN) {
+      for (int c = 0; c < D; ++c) org[c] = ((const double *)rec)[c];
+      __syncthreads();
+      const R *rp = (const R *)(rec + STAGE_HEAD);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        int64_t ls = base + (32 + 64 * q < tlen ? 32 + 64 * q : tlen - 1);
-        if (sperm) ls = sperm[ls];
+      for (int c = 0; c < D; ++c) s_pos[c][tid] = rp[c * TS + tid];
 #pragma unroll
-        for (int c = 0; c < D; ++c) org[c] += src.pos[ls * D + c];
+      for (int j = 0; j < NSW; ++j)
+#pragma unroll
+        for (int c = 0; c < M; ++c) s_w[j][c][tid] = rp[(D + j * M + c) * TS + tid];
+      if constexpr (EXPAND) {
+        if (expand_seg) {
+          s_n2[tid] = rp[(D + NSW * M) * TS + tid];
+          if (tid < TS / 64) s_r2max[tid] = ((const R *)(rec + 224))[tid];
+        }
       }
+      if (tid < (TS / 64) * 2 * D) (&s_box[0][0])[tid] = ((const R *)(rec + 32))[tid];
+    } else {
+      tile_origin<R, D>(src, sperm, base, tlen, scale, org);
+      __syncthreads();
+      SrcStage<R, D, M, NSW> ss;
+      stage_source<R, D, M, NSW>(src, sperm, ld, base + tid, l1, scale, org, expand_seg, ss);
 #pragma unroll
-      for (int c = 0; c < D; ++c) org[c] *= 0.25 * scale;
+      for (int c = 0; c < D; ++c) s_pos[c][tid] = ss.v[c];
+#pragma unroll
+      for (int j = 0; j < NSW; ++j)
+#pragma unroll
+        for (int c = 0; c < M; ++c) s_w[j][c][tid] = ss.w[j][c];
+      if constexpr (EXPAND) {
+        if (expand_seg) {
+          s_n2[tid] = ss.n2;
+          if (lane == 0) s_r2max[wave] = ss.q2;
+        }
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          s_box[wave][c] = ss.lo[c];
+          s_box[wave][D + c] = ss.hi[c];
+        }
+      }
+    }
+    if constexpr (KAHAN) {
 #pragma unroll
       for (int r = 0; r < RPT; ++r)
 #pragma unroll
         for (int c = 0; c < D; ++c) ty[r][c] = (R)(yd[r][c] - org[c]);
-    }
-    __syncthreads();
-    {
-      int64_t l = base + tid;
-      R lo[D], hi[D], v[D];
-      R n2 = 0;
-      if (l < l1) {
-        const int64_t lg = sperm ? (int64_t)sperm[l] : l;
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          v[c] = (R)(src.pos[lg * D + c] * scale - org[c]);
-          lo[c] = v[c];
-          hi[c] = v[c];
-          n2 = v[c] * v[c] + n2;
-        }
-#pragma unroll
-        for (int j = 0; j < NSW; ++j)
-#pragma unroll
-          for (int c = 0; c < M; ++c) s_w[j][c][tid] = (R)(src.sign * src.w[j][lg * ld + c]);
-      } else {
-        // Padding sources carry zero weight: their terms are exact zeros.
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          v[c] = (R)0;
-          lo[c] = (R)INFINITY;
-          hi[c] = (R)-INFINITY;
-        }
-#pragma unroll
-        for (int j = 0; j < NSW; ++j)
-#pragma unroll
-          for (int c = 0; c < M; ++c) s_w[j][c][tid] = (R)0;
-      }
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        lo[c] = wave_min(lo[c]);
-        hi[c] = wave_max(hi[c]);
-      }
-      // Each wave stages one 64-source quarter: a compact quarter (dot-product form) is
-      // staged as 2y with -|y|^2 beside it.
-      bool dbl = false;
-      if constexpr (EXPAND) {
-        if (expand_seg) {
-          const R q2 = wave_max(n2);
-          dbl = q2 <= (R)EXP_R2MAX;
-          s_n2[tid] = -n2;
-          if (lane == 0) s_r2max[wave] = q2;
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < D; ++c) s_pos[c][tid] = dbl ? (R)2 * v[c] : v[c];
-      if (lane == 0) {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          s_box[wave][c] = lo[c];
-          s_box[wave][D + c] = hi[c];
-        }
-      }
     }
     __syncthreads();
     // Which 64-source quarters of the tile hold a pair above the cut for this
@@ -2245,6 +2360,18 @@ size_t desc_bytes(int nseg) {
   return (b + 255) & ~(size_t)255;
 }
 
+// Staging area of the ordered segments' source tiles: after the descriptor head and
+// `doubles` partials, 256-byte aligned; one record of stage_rec bytes per tile.
+size_t stage_offset(int nseg, int64_t doubles) {
+  size_t b = desc_bytes(nseg) + (size_t)doubles * sizeof(double);
+  return (b + 255) & ~(size_t)255;
+}
+
+size_t stage_rec(int prec, int d, const Shape &sh) {
+  const int nsw = sh.NA > (sh.KSUM ? 1 : 0) ? sh.NA : (sh.KSUM ? 1 : 0);
+  return stage_rec_bytes(prec == LDDMM_PREC_FP32C ? 4 : 8, d, sh.M, nsw);
+}
+
 }  // namespace
 
 extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, int nseg,
@@ -2259,6 +2386,7 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
   const int nred = op_nred(op);
   const int kern = segs[0].kernel;
   int64_t item = 0, part = 0, fin = 0, redo = 0;
+  int64_t stage = 0;  // staged source tiles of the ordered segments
   for (int i = 0; i < nseg; ++i) {
     lddmm_seg_t &s = segs[i];
     if (s.kernel != kern) return LDDMM_EINVAL;  // one kernel family per launch
@@ -2365,11 +2493,14 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     fin += s.fin_blocks;
     s.red0 = redo;
     redo += (int64_t)s.fin_blocks * (nred > 0 ? nred : 0);
-    s.pad_ = 0;
+    if (stage > INT32_MAX) return LDDMM_EINVAL;
+    s.stage0 = (int32_t)stage;
+    if (s.perm) stage += ceil_div(nx0, TS);
   }
   if (n_items) *n_items = item;
   if (n_fin_blocks) *n_fin_blocks = fin;
-  if (work_bytes) *work_bytes = desc_bytes(nseg) + (size_t)(part + redo) * sizeof(double);
+  if (work_bytes)
+    *work_bytes = stage_offset(nseg, part + redo) + (size_t)stage * stage_rec(prec, d, shp);
   return LDDMM_OK;
 }
 
@@ -2385,33 +2516,57 @@ bool use_mfma() {
   return e && e[0] == '1';
 }
 
+// LDDMM_STAGE=0 has k_pairs build every tile itself (the same sums, bit for bit).
+bool use_stage() {
+  const char *e = getenv("LDDMM_STAGE");
+  return !(e && e[0] == '0');
+}
+
+// Per-launch options of the K-sum kernels: the matrix-core variant, and the staging area
+// of the ordered segments' source tiles (n_stage tiles; none when 0).
+struct PairOpts {
+  bool mfma;
+  char *stage;
+  int64_t n_stage;
+};
+
 template <typename R, int KT, int D, int M, bool KSUM, int NA>
 void launch_pairs(const lddmm_seg_t *segs, int nseg, int op, int prec, int64_t n_items,
-                  double *part, hipStream_t st, bool mfma) {
+                  double *part, hipStream_t st, const PairOpts &po) {
   if constexpr (std::is_same<R, float>::value && KSUM && NA == 0) {
-    if ((op == LDDMM_OP_APPLY_K || op == LDDMM_OP_FLOW_STEP || op == LDDMM_OP_KDOT) && mfma) {
+    if ((op == LDDMM_OP_APPLY_K || op == LDDMM_OP_FLOW_STEP || op == LDDMM_OP_KDOT) && po.mfma) {
       hipLaunchKernelGGL((k_pairs_mfma<KT, D, M>), dim3((unsigned)n_items), dim3(BLOCK), 0,
                          st, segs, nseg, op, prec, part);
       return;
     }
   }
+  const char *stage = nullptr;
+  // Ordered segments exist for the flow step and RKHS products (M = D, K-sums) and the
+  // covector step (M = D, two derivative weight pairs) only.
+  if constexpr (M == D && ((KSUM && NA == 0) || (!KSUM && NA == 2))) {
+    if (po.n_stage > 0) {
+      hipLaunchKernelGGL((k_stage<R, KT, D, M, KSUM, NA>), dim3((unsigned)po.n_stage),
+                         dim3(BLOCK), 0, st, segs, nseg, op, prec, po.stage);
+      stage = po.stage;
+    }
+  }
   hipLaunchKernelGGL((k_pairs<R, KT, D, M, KSUM, NA>), dim3((unsigned)n_items), dim3(BLOCK), 0,
-                     st, segs, nseg, op, prec, part);
+                     st, segs, nseg, op, prec, part, stage);
 }
 
 template <int D, int M, bool KSUM, int NA>
 int dispatch_pairs(int prec, int kern, const lddmm_seg_t *segs, int nseg, int op,
-                   int64_t n_items, double *part, hipStream_t st, bool mfma) {
+                   int64_t n_items, double *part, hipStream_t st, const PairOpts &po) {
   if (prec == LDDMM_PREC_FP32C) {
     if (kern == LDDMM_KERNEL_GAUSS)
-      launch_pairs<float, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, mfma);
+      launch_pairs<float, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, po);
     else
-      launch_pairs<float, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, mfma);
+      launch_pairs<float, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, po);
   } else {
     if (kern == LDDMM_KERNEL_GAUSS)
-      launch_pairs<double, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, mfma);
+      launch_pairs<double, LDDMM_KERNEL_GAUSS, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, po);
     else
-      launch_pairs<double, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, mfma);
+      launch_pairs<double, LDDMM_KERNEL_LAPLACIAN, D, M, KSUM, NA>(segs, nseg, op, prec, n_items, part, st, po);
   }
   return LDDMM_OK;
 }
@@ -2495,20 +2650,20 @@ void launch_fin(const lddmm_seg_t *segs, int nseg, int prec, int kern, int64_t n
 template <int D>
 int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, int nseg,
           int64_t n_items, int64_t n_fin, double *part, double *red, hipStream_t st,
-          bool mfma) {
+          const PairOpts &po) {
   switch (op) {
     case LDDMM_OP_APPLY_K:
       switch (sh.M) {
         case 1:
-          dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+          dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, po);
           launch_fin<LDDMM_OP_APPLY_K, D, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
           break;
         case 2:
-          dispatch_pairs<D, 2, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+          dispatch_pairs<D, 2, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, po);
           launch_fin<LDDMM_OP_APPLY_K, D, 2, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
           break;
         case 3:
-          dispatch_pairs<D, 3, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+          dispatch_pairs<D, 3, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, po);
           launch_fin<LDDMM_OP_APPLY_K, D, 3, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
           break;
         default:
@@ -2518,7 +2673,7 @@ int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, 
     case LDDMM_OP_DIFF_KT: {
 #define LDDMM_DKT(MM, NN)                                                                  \
   if (sh.M == MM && sh.NA == NN) {                                                         \
-    dispatch_pairs<D, MM, false, NN>(prec, kern, segs, nseg, op, n_items, part, st, mfma);       \
+    dispatch_pairs<D, MM, false, NN>(prec, kern, segs, nseg, op, n_items, part, st, po);       \
     launch_fin<LDDMM_OP_DIFF_KT, D, MM, false, NN>(segs, nseg, prec, kern, n_fin, part, red, st); \
     return LDDMM_OK;                                                                       \
   }
@@ -2528,36 +2683,36 @@ int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, 
       return LDDMM_EINVAL;
     }
     case LDDMM_OP_FLOW_STEP:
-      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, po);
       launch_fin<LDDMM_OP_FLOW_STEP, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       return LDDMM_OK;
     case LDDMM_OP_KDOT:
-      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, po);
       launch_fin<LDDMM_OP_KDOT, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       return LDDMM_OK;
     case LDDMM_OP_KNORM:
-      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+      dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, po);
       launch_fin<LDDMM_OP_KNORM, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       return LDDMM_OK;
     case LDDMM_OP_COVECTOR_STEP:
-      dispatch_pairs<D, D, false, 2>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+      dispatch_pairs<D, D, false, 2>(prec, kern, segs, nseg, op, n_items, part, st, po);
       launch_fin<LDDMM_OP_COVECTOR_STEP, D, D, false, 2>(segs, nseg, prec, kern, n_fin, part, red, st);
       return LDDMM_OK;
     case LDDMM_OP_CURRENTS:
       if (sh.NA > 0) {
-        dispatch_pairs<D, D, true, 1>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+        dispatch_pairs<D, D, true, 1>(prec, kern, segs, nseg, op, n_items, part, st, po);
         launch_fin<LDDMM_OP_CURRENTS, D, D, true, 1>(segs, nseg, prec, kern, n_fin, part, red, st);
       } else {
-        dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+        dispatch_pairs<D, D, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, po);
         launch_fin<LDDMM_OP_CURRENTS, D, D, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       }
       return LDDMM_OK;
     case LDDMM_OP_MEASURE:
       if (sh.NA > 0) {
-        dispatch_pairs<D, 1, true, 1>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+        dispatch_pairs<D, 1, true, 1>(prec, kern, segs, nseg, op, n_items, part, st, po);
         launch_fin<LDDMM_OP_MEASURE, D, 1, true, 1>(segs, nseg, prec, kern, n_fin, part, red, st);
       } else {
-        dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, mfma);
+        dispatch_pairs<D, 1, true, 0>(prec, kern, segs, nseg, op, n_items, part, st, po);
         launch_fin<LDDMM_OP_MEASURE, D, 1, true, 0>(segs, nseg, prec, kern, n_fin, part, red, st);
       }
       return LDDMM_OK;
@@ -2606,7 +2761,13 @@ extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *se
   const int64_t parts = last.part0 + last.ny * (int64_t)(last.chunks0 + last.chunks1) * sh.W;
   const int nred = op_nred(op);
   const int64_t reds = last.red0 + (int64_t)last.fin_blocks * nred;
-  const size_t need = desc_bytes(nseg) + (size_t)(parts + reds) * sizeof(double);
+  // Staged tiles: the plan numbers the ordered segments' tiles consecutively.
+  int64_t n_stage = 0;
+  for (int i = 0; i < nseg; ++i) {
+    if (segs_host[i].stage0 != n_stage) return LDDMM_EINVAL;  // not this order's plan
+    if (segs_host[i].perm) n_stage += ceil_div(segs_host[i].nx0, TS);
+  }
+  const size_t need = stage_offset(nseg, parts + reds) + (size_t)n_stage * stage_rec(prec, d, sh);
   if (work_bytes < need) return LDDMM_EWORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   char *wb = (char *)work;
@@ -2626,11 +2787,12 @@ extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *se
   double *red = part + parts;
   const int kern = segs_host[0].kernel;
   // The matrix-core K-sum (opt-in) has no processing-order support.
-  bool mfma = use_mfma();
-  for (int i = 0; i < nseg && mfma; ++i)
-    if (segs_host[i].perm) mfma = false;
-  rc = d == 2 ? run_d<2>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, mfma)
-              : run_d<3>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, mfma);
+  PairOpts po;
+  po.mfma = use_mfma() && n_stage == 0;
+  po.stage = wb + stage_offset(nseg, parts + reds);
+  po.n_stage = use_stage() ? n_stage : 0;
+  rc = d == 2 ? run_d<2>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, po)
+              : run_d<3>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, po);
   if (rc) return rc;
   if (nred > 0)
     hipLaunchKernelGGL(k_totals, dim3((unsigned)nseg), dim3(TOT_BLOCK), 0, st, segs, nred, red);
@@ -2650,7 +2812,8 @@ extern "C" int lddmm_segments_run_steps(int op, int prec, int d, const lddmm_seg
       if (a.ny != b.ny || a.nx0 != b.nx0 || a.nx1 != b.nx1 || a.item0 != b.item0 ||
           a.part0 != b.part0 || a.fin0 != b.fin0 || a.red0 != b.red0 ||
           a.row_tiles != b.row_tiles || a.chunks0 != b.chunks0 || a.chunks1 != b.chunks1 ||
-          a.chunk_len != b.chunk_len || a.fin_blocks != b.fin_blocks)
+          a.chunk_len != b.chunk_len || a.fin_blocks != b.fin_blocks || a.stage0 != b.stage0 ||
+          !a.perm != !b.perm)
         return LDDMM_EINVAL;
     }
     int rc = lddmm_segments_run(op, prec, d, h, segs_dev + (int64_t)t * nseg, nseg, n_items,

@@ -23,7 +23,7 @@ from ._lib import Seg
 SEG_DT = np.dtype(Seg)
 SEG_SIZE = SEG_DT.itemsize
 PLAN_FIELDS = ("nx0", "nx1", "item0", "part0", "fin0", "red0", "row_tiles", "chunks0",
-               "chunks1", "chunk_len", "fin_blocks")
+               "chunks1", "chunk_len", "fin_blocks", "stage0")
 _SEG_PTR = ctypes.POINTER(Seg)
 
 
@@ -40,7 +40,7 @@ def _byte_ranges(fields):
     return [tuple(r) for r in out]
 
 
-_PLAN_BYTES = _byte_ranges(PLAN_FIELDS + ("pad_",))
+_PLAN_BYTES = _byte_ranges(PLAN_FIELDS)
 
 _workspaces: dict = {}
 
