@@ -1870,19 +1870,31 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
 #pragma unroll
   for (int c = 0; c < D; ++c) accD[c] = 0;
   if (valid) {
-    for (int cgi = 0; cgi < nch; ++cgi) {
+    // Chunk partials in chunk order, set 0 then set 1; unrolled so that the loads of
+    // several chunks are in flight together (one HBM round trip per group, not per chunk).
+    const int nch0 = sg->chunks0;
+#pragma unroll 8
+    for (int cgi = 0; cgi < nch0; ++cgi) {
       const double *o = pp + ((int64_t)cgi * ny + kk) * W;
-      int q = 0;
       if constexpr (KSUM) {
 #pragma unroll
-        for (int c = 0; c < M; ++c) {
-          if (cgi < sg->chunks0) accK0[c] += o[q + c]; else accK1[c] += o[q + c];
-        }
-        q += M;
+        for (int c = 0; c < M; ++c) accK0[c] += o[c];
       }
       if constexpr (NA > 0) {
 #pragma unroll
-        for (int c = 0; c < D; ++c) accD[c] += o[q + c];
+        for (int c = 0; c < D; ++c) accD[c] += o[(KSUM ? M : 0) + c];
+      }
+    }
+#pragma unroll 8
+    for (int cgi = nch0; cgi < nch; ++cgi) {
+      const double *o = pp + ((int64_t)cgi * ny + kk) * W;
+      if constexpr (KSUM) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) accK1[c] += o[c];
+      }
+      if constexpr (NA > 0) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) accD[c] += o[(KSUM ? M : 0) + c];
       }
     }
   }
