@@ -28,12 +28,13 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (AMD published)
 FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64 (AMD published)
 FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
-# HBM bytes per fused flow step at 40,962 points (pair kernel + finalize + totals), measured with
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on `bench.py --roofline-only --roofline-reps 3` and
-# scaled by the counters' reading for 256 MiB device copies (FETCH x2.0, WRITE x1.0), in the
-# template's spatial processing order: 43.8 MB read + 27.3 MB written
-# (profiles/flow_step_s6_fp32c.md).
-FLOW_STEP_TRAFFIC_S6 = 71.0e6
+# HBM bytes per fused flow step at 40,962 points (tile staging + pair kernel + finalize + totals),
+# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on `bench.py --roofline-only
+# --roofline-reps 3` and scaled by the counters' reading for 256 MiB device copies (FETCH x2.0,
+# WRITE x1.0), in the template's spatial processing order: 82.5 MB read + 63.8 MB written, most
+# of it the FP64 chunk partials (54 chunks x 40,962 rows x 3), written by the pair kernel and
+# read back by the finalize (profiles/flow_step_s6_fp32c.md).
+FLOW_STEP_TRAFFIC_S6 = 146.3e6
 
 
 def _progress(msg):
