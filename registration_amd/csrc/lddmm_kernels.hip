@@ -55,6 +55,11 @@ constexpr int TOT_BLOCK = 256;   // threads of the totals kernel
 // 40,962-point flow step (4,374 items) and the 81,920-face FP64 currents energy (6,400
 // items): 8% and 9% faster than with a 2,048 target (1,863 and 2,560 items).
 constexpr int64_t ITEMS_PER_SEGMENT = 6144;
+// Energy-only sums (no per-row outputs) evaluate each unordered pair of self tiles once:
+// the workgroups of a row tile skip every source tile below it, so their items differ widely
+// in size and balance needs more of them.  The 81,920-face FP64 currents energy: 3.17 ms at
+// 6,144 items per segment, 2.94 at 12,288, 2.81 at 24,576.
+constexpr int64_t ITEMS_ENERGY = 24576;
 constexpr int MAX_NRED = 2;      // FP64 totals per segment
 
 constexpr double LOG2E = 1.4426950408889634074;
@@ -2492,7 +2497,12 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     if (op != LDDMM_OP_CURRENTS && op != LDDMM_OP_MEASURE && op != LDDMM_OP_VARIFOLD) s.nx1 = 0;
     const int64_t row_tiles = ceil_div(s.ny, rows);
     const int64_t src_tiles = ceil_div(nx0, TS) + (nx1 ? ceil_div(nx1, TS) : 0);
-    int64_t chunk_tiles = ceil_div(src_tiles * row_tiles, ITEMS_PER_SEGMENT);
+    const bool energy_only =
+        !s.rowdot && !s.out0 && shp.NA == 0 &&
+        (op == LDDMM_OP_KNORM || op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE ||
+         op == LDDMM_OP_VARIFOLD);
+    int64_t chunk_tiles =
+        ceil_div(src_tiles * row_tiles, energy_only ? ITEMS_ENERGY : ITEMS_PER_SEGMENT);
     if (chunk_tiles < 1) chunk_tiles = 1;
     const int64_t chunk_len = chunk_tiles * TS;
     const int64_t c0 = ceil_div(nx0, chunk_len), c1 = nx1 ? ceil_div(nx1, chunk_len) : 0;
