@@ -192,16 +192,83 @@ __device__ inline double exp2_nonpos(double t) {
   return __hiloint2double(__double2hiint(p) + (ni << 20), __double2loint(p));
 }
 
+// 2^(j/64), j = 0..63, each the double nearest to the exact value (generated with 200-bit
+// arithmetic; every entry lies at least 0.003 ulp from a rounding midpoint).
+__constant__ double EXP2_TAB64[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
+    0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
+    0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,
+    0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,
+    0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,
+    0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,
+    0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,
+    0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,
+    0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,
+    0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0,
+};
+
+// (ln 2 / 64)^k / k!, k = 0..5.
+struct Exp2T64Coef {
+  double c[6];
+  constexpr Exp2T64Coef() : c() {
+    const double ln2 = 0.69314718055994530942;
+    double v = 1.0;
+    for (int k = 0; k < 6; ++k) {
+      c[k] = v;
+      v = v * ln2 / 64.0 / (double)(k + 1);
+    }
+  }
+};
+constexpr Exp2T64Coef EXP2T64C{};
+
+// 2^(t64 / 64) in double for t64 <= 0, with the 64-entry table (in LDS) of 2^(j/64):
+// t64 = 64 m + j + g with rint (the 1.5 * 2^52 shift), j = 0..63 and |g| <= 1/2;
+// 2^(g/64) - 1 = q = sum_{k>=1} (g ln 2 / 64)^k / k! truncated after degree 5 (remainder
+// < 4e-17 relative: |g ln 2 / 64| < 0.0055), the result 2^m * fma(T[j], q, T[j]).  Measured
+// against 200-bit arithmetic: < 2 units of 2^-53 relative.  t64 is clamped at -64000 (2^-1000),
+// which keeps every result a normal double.  5 FMA + 1 multiply instead of exp2_nonpos's 13 FMA.
+__device__ inline double exp2_tab64(double t64, const double *__restrict__ tab) {
+  t64 = fmax(t64, -64000.0);
+  const double shifter = 6755399441055744.0;  // 1.5 * 2^52
+  const double k = t64 + shifter;
+  const double n = k - shifter;
+  const double g = t64 - n;
+  double q = EXP2T64C.c[5];
+#pragma unroll
+  for (int i = 4; i >= 1; --i) q = fma(q, g, EXP2T64C.c[i]);
+  q *= g;
+  const int ni = __double2loint(k);
+  const double tj = tab[ni & 63];
+  const double r = fma(tj, q, tj);
+  return __hiloint2double(__double2hiint(r) + ((ni >> 6) << 20), __double2loint(r));
+}
+
 template <>
 struct KEval<double, LDDMM_KERNEL_GAUSS> {
   static constexpr double R2_0 = 0.0;
-  double h, cut_r2;
+  double h, h64, cut_r2;
   __device__ void init(int, double sigma) {
     h = LOG2E / (2.0 * sigma * sigma);  // exp(-r^2/(2 sigma^2)) = 2^(-h r^2)
+    h64 = 64.0 * h;                     // exact
     cut_r2 = CUTOFF * sigma * sigma;    // (r/sigma)^2 < 1e-8  <=>  r^2 < 1e-8 sigma^2
   }
   __device__ inline void operator()(double r2, double &k0, double &k1, double &k2) const {
     double e = exp2_nonpos(-h * r2);
+    double k = r2 < cut_r2 ? 1.0 : e;
+    k0 = k;
+    k1 = k;
+    k2 = k;
+  }
+  // The same with the LDS table form of the exponential (tab: EXP2_TAB64 in LDS).
+  __device__ inline void eval_tab(double r2, const double *__restrict__ tab, double &k0,
+                                  double &k1, double &k2) const {
+    double e = exp2_tab64(-h64 * r2, tab);
     double k = r2 < cut_r2 ? 1.0 : e;
     k0 = k;
     k1 = k;
@@ -565,6 +632,9 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   __shared__ R s_box[TS / 64][2 * D];  // bounding box of each 64-source quarter
   __shared__ R s_n2[EXPAND ? TS : 1];  // -|y|^2 of each source (dot-product exponent)
   __shared__ R s_r2max[TS / 64];       // max |y|^2 of each quarter
+  // FP64 Gaussian sums evaluate the exponential with the 2^(j/64) table (exp2_tab64).
+  constexpr bool TAB = std::is_same<R, double>::value && KT == LDDMM_KERNEL_GAUSS;
+  __shared__ double s_e2[TAB ? 64 : 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -595,6 +665,10 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
 
   KEval<R, KT> kev;
   kev.init(sg->order, sg->sigma);
+  if constexpr (TAB) {
+    // Read only after the tile loop's first barrier.
+    if (tid < 64) s_e2[tid] = EXP2_TAB64[tid];
+  }
 
   // Target rows held in registers; each wave owns RPT x 64 consecutive rows.
   // Positions stay in FP64 (yd, prescaled); FP32 mode rounds them per tile,
@@ -863,7 +937,8 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
             r2 = dx[c] * dx[c] + r2;
           }
           R k0, k1, k2;
-          kev(r2, k0, k1, k2);
+          if constexpr (TAB) kev.eval_tab(r2, s_e2, k0, k1, k2);
+          else kev(r2, k0, k1, k2);
           if constexpr (KSUM) {
 #pragma unroll
             for (int c = 0; c < M; ++c) tK[r][c] = k0 * sw[0][c] + tK[r][c];
