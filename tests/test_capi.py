@@ -165,3 +165,47 @@ def test_struct_mirrors_match_the_header(tmp_path):
         assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
         for fname, *_ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_plan_numbers_staged_tiles_of_ordered_segments():
+    """Segments with a processing order get consecutive staged source tiles (stage0), one
+    256-source record each appended to the workspace; a run whose descriptors do not carry
+    that numbering is rejected on the host, before any device call."""
+    tiles = lambda n: -(-n // 256)
+    rec = 256 + (3 + 3 + 1) * 256 * 4  # FP32, d = 3: origin/boxes head, pos, weights, -|y|^2
+    rc, arr, items, fins, wb_plain = _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP32C, 3,
+                                           [_seg(1000), _seg(300), _seg(5000)])
+    assert rc == 0 and [a.stage0 for a in arr] == [0, 0, 0]
+    rc, arr, items, fins, wb = _plan(_lib.OP_FLOW_STEP, _lib.PREC_FP32C, 3,
+                                     [_seg(1000, perm=4096), _seg(300), _seg(5000, perm=4096)])
+    assert rc == 0
+    assert [a.stage0 for a in arr] == [0, tiles(1000), tiles(1000)]
+    assert wb - wb_plain >= (tiles(1000) + tiles(5000)) * rec
+    assert wb - wb_plain < (tiles(1000) + tiles(5000)) * rec + 256
+    # FP64 covector records: two weight pairs of 8-byte values.
+    rc, arr64, _, _, wb64 = _plan(_lib.OP_COVECTOR_STEP, _lib.PREC_FP64, 3, [_seg(1000, perm=4096)])
+    rc0, _, _, _, wb64_plain = _plan(_lib.OP_COVECTOR_STEP, _lib.PREC_FP64, 3, [_seg(1000)])
+    assert rc == 0 and rc0 == 0
+    assert wb64 - wb64_plain >= tiles(1000) * (256 + (3 + 6 + 1) * 256 * 8)
+    # A perm added after planning breaks the numbering: EINVAL, nothing launched.
+    lib = _lib.load()
+    arr[1].perm = 4096
+    rc = lib.lddmm_segments_run(_lib.OP_FLOW_STEP, _lib.PREC_FP32C, 3, arr, None, 3, items, fins,
+                                4096, wb, None)
+    assert rc == _lib.EINVAL
+
+
+def test_exp2_table_entries_are_correctly_rounded():
+    """The FP64 pair kernel's table of 2^(j/64), j = 0..63 (hex literals in the kernel source),
+    holds the double nearest to each exact value (200-bit arithmetic)."""
+    mpmath = pytest.importorskip("mpmath")
+    src = open(os.path.join(ROOT, "registration_amd", "csrc", "lddmm_kernels.hip")).read()
+    block = re.search(r"EXP2_TAB64\[64\]\s*=\s*\{(.*?)\};", src, flags=re.S).group(1)
+    vals = [float.fromhex(t) for t in re.findall(r"0x[0-9a-fA-Fp.+-]+", block)]
+    assert len(vals) == 64
+    mpmath.mp.prec = 200
+    for j, v in enumerate(vals):
+        exact = mpmath.power(2, mpmath.mpf(j) / 64)
+        assert v == float(exact), j
+        # and no other double is closer
+        assert abs(mpmath.mpf(v) - exact) <= mpmath.mpf(2) ** -53 * v
