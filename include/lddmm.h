@@ -154,9 +154,10 @@ typedef struct lddmm_seg {
   int32_t chunks1;     /* source chunks over set 1                              */
   int32_t chunk_len;   /* sources per chunk (multiple of the LDS tile)          */
   int32_t fin_blocks;  /* finalize blocks                                       */
-  int32_t stage0;      /* first staged source tile (segments with a perm):     *
-                        * their set-0 tiles are staged once per launch in the    *
-                        * workspace and read by every row tile                   */
+  int32_t stage0;      /* first staged source tile: segments with a perm (set   *
+                        * 0) and energy-only CURRENTS / MEASURE / KNORM sums     *
+                        * (both sets) have their tiles staged once per launch in *
+                        * the workspace and read by every row tile               */
   /* [in] optional processing order of FLOW_STEP, COVECTOR_STEP and KDOT segments (rows = sources),
    * NULL for the natural one: a permutation of 0..ny-1 in device memory, perm[i] =
    * the point taken as the i-th row and the i-th source.  Outputs stay indexed by

@@ -560,6 +560,25 @@ __host__ __device__ inline size_t stage_rec_bytes(int rsize, int d, int m, int n
   return STAGE_HEAD + (size_t)(d + nsw * m + 1) * TS * rsize;
 }
 
+// Source sets of a segment whose tiles are staged once per launch: set 0 of segments with a
+// processing order (flow and covector steps, RKHS products); both sets of the energy-only
+// sums of k_pairs (data-term energies and RKHS norms without per-row outputs), whose every
+// row tile reads every source tile; none otherwise.  na: the launch's derivative pair count.
+__host__ __device__ inline int stage_sets(int op, const lddmm_seg_t &s, int na) {
+  if (s.perm) return 1;
+  if (!s.rowdot && !s.out0 && na == 0 &&
+      (op == LDDMM_OP_KNORM || op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE))
+    return 2;
+  return 0;
+}
+
+// Staged tiles of a planned segment (nx0, nx1 as the plan sets them).
+__host__ __device__ inline int64_t stage_tiles(int op, const lddmm_seg_t &s, int na) {
+  const int sets = stage_sets(op, s, na);
+  if (sets == 0) return 0;
+  return (s.nx0 + TS - 1) / TS + (sets == 2 ? (s.nx1 + TS - 1) / TS : 0);
+}
+
 template <typename R, int KT, int D, int M, bool KSUM, int NA>
 __global__ __launch_bounds__(BLOCK) void k_stage(const lddmm_seg_t *__restrict__ segs, int nseg,
                                                 int op, int prec, char *__restrict__ stage) {
@@ -568,15 +587,20 @@ __global__ __launch_bounds__(BLOCK) void k_stage(const lddmm_seg_t *__restrict__
   const int tid = threadIdx.x;
   const int64_t tile = blockIdx.x;
   const lddmm_seg_t *sg = segs + find_segment(segs, nseg, tile, 2);
-  const SrcView src = resolve_src(op, sg, 0);
+  // Tiles of set 0, then (energy-only sums) of set 1.
+  const int64_t t = tile - sg->stage0;
+  const int64_t tiles0 = (sg->nx0 + TS - 1) / TS;
+  const int set = t < tiles0 ? 0 : 1;
+  const SrcView src = resolve_src(op, sg, set);
+  const int32_t *__restrict__ sperm = set == 0 ? sg->perm : nullptr;
   const double scale = coord_scale(prec, KT, sg->sigma);
-  const int64_t base = (tile - sg->stage0) * TS;
+  const int64_t base = (set ? t - tiles0 : t) * TS;
   const int64_t tlen = src.n - base < TS ? src.n - base : TS;
   double org[D];
-  tile_origin<R, D>(src, sg->perm, base, tlen, scale, org);
+  tile_origin<R, D>(src, sperm, base, tlen, scale, org);
   SrcStage<R, D, M, NSW> ss;
-  stage_source<R, D, M, NSW>(src, sg->perm, weight_ld(op, sg, D), base + tid, src.n, scale, org,
-                             DOT, ss);
+  stage_source<R, D, M, NSW>(src, sperm, weight_ld(op, sg, D), base + tid, src.n, scale, org,
+                             DOT && sperm != nullptr, ss);
   char *rec = stage + tile * stage_rec_bytes(sizeof(R), D, M, NSW);
   R *rp = (R *)(rec + STAGE_HEAD);
 #pragma unroll
@@ -660,8 +684,10 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   const int32_t *__restrict__ perm = sg->perm;
   const int32_t *__restrict__ sperm = set == 0 ? perm : nullptr;
   const bool expand_seg = EXPAND && perm != nullptr;
-  // Ordered set-0 tiles come staged when the launch has a staging area.
-  const bool staged = stage != nullptr && sperm != nullptr;
+  // Ordered set-0 tiles, and both sets of energy-only sums, come staged when the launch has
+  // a staging area; this set's first record.
+  const bool staged = stage != nullptr && (sperm != nullptr || stage_sets(op, *sg, NA) == 2);
+  const int64_t stage_t0 = sg->stage0 + (set ? (sg->nx0 + TS - 1) / TS : 0);
 
   KEval<R, KT> kev;
   kev.init(sg->order, sg->sigma);
@@ -758,7 +784,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
     const int64_t tlen = l1 - base < TS ? l1 - base : TS;
     if (staged) {
       // Copy the tile k_stage built (chunks start at multiples of TS: tile base / TS).
-      const char *rec = stage + (sg->stage0 + base / TS) * stage_rec_bytes(sizeof(R), D, M, NSW);
+      const char *rec = stage + (stage_t0 + base / TS) * stage_rec_bytes(sizeof(R), D, M, NSW);
 #pragma unroll
       for (int c = 0; c < D; ++c) org[c] = ((const double *)rec)[c];
       __syncthreads();
@@ -2597,7 +2623,7 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     redo += (int64_t)s.fin_blocks * (nred > 0 ? nred : 0);
     if (stage > INT32_MAX) return LDDMM_EINVAL;
     s.stage0 = (int32_t)stage;
-    if (s.perm) stage += ceil_div(nx0, TS);
+    stage += stage_tiles(op, s, shp.NA);
   }
   if (n_items) *n_items = item;
   if (n_fin_blocks) *n_fin_blocks = fin;
@@ -2643,9 +2669,9 @@ void launch_pairs(const lddmm_seg_t *segs, int nseg, int op, int prec, int64_t n
     }
   }
   const char *stage = nullptr;
-  // Ordered segments exist for the flow step and RKHS products (M = D, K-sums) and the
-  // covector step (M = D, two derivative weight pairs) only.
-  if constexpr (M == D && ((KSUM && NA == 0) || (!KSUM && NA == 2))) {
+  // Staged segments: K-sums without derivative terms (flow steps, RKHS products and norms,
+  // data-term energies) and the covector step (M = D, two derivative weight pairs).
+  if constexpr ((KSUM && NA == 0) || (M == D && !KSUM && NA == 2)) {
     if (po.n_stage > 0) {
       hipLaunchKernelGGL((k_stage<R, KT, D, M, KSUM, NA>), dim3((unsigned)po.n_stage),
                          dim3(BLOCK), 0, st, segs, nseg, op, prec, po.stage);
@@ -2867,7 +2893,7 @@ extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *se
   int64_t n_stage = 0;
   for (int i = 0; i < nseg; ++i) {
     if (segs_host[i].stage0 != n_stage) return LDDMM_EINVAL;  // not this order's plan
-    if (segs_host[i].perm) n_stage += ceil_div(segs_host[i].nx0, TS);
+    n_stage += stage_tiles(op, segs_host[i], sh.NA);
   }
   const size_t need = stage_offset(nseg, parts + reds) + (size_t)n_stage * stage_rec(prec, d, sh);
   if (work_bytes < need) return LDDMM_EWORKSPACE;
@@ -2915,7 +2941,7 @@ extern "C" int lddmm_segments_run_steps(int op, int prec, int d, const lddmm_seg
           a.part0 != b.part0 || a.fin0 != b.fin0 || a.red0 != b.red0 ||
           a.row_tiles != b.row_tiles || a.chunks0 != b.chunks0 || a.chunks1 != b.chunks1 ||
           a.chunk_len != b.chunk_len || a.fin_blocks != b.fin_blocks || a.stage0 != b.stage0 ||
-          !a.perm != !b.perm)
+          !a.perm != !b.perm || !a.rowdot != !b.rowdot || !a.out0 != !b.out0)
         return LDDMM_EINVAL;
     }
     int rc = lddmm_segments_run(op, prec, d, h, segs_dev + (int64_t)t * nseg, nseg, n_items,
