@@ -17,11 +17,14 @@
 // chunk order and applies the operation's epilogue (Euler update, covector
 // update, row products); FP64 totals are reduced in a fixed tree.  The chunk
 // length of a segment depends only on that segment's sizes, so a segment's
-// result is bit-identical alone and inside any batch.
+// result is bit-identical alone and inside any batch.  Segments with a processing
+// order and energy-only sums have their source tiles built once per launch by
+// k_stage into the workspace; their workgroups copy the tiles instead of building
+// them (same values).
 //
 // Precision.  FP64 mode evaluates every pair in double (the pair kernels' own
-// exp2_nonpos; the dense kernel-matrix kernel uses the device math library's exp)
-// with the reference's diagonal cut-off.  FP32 mode prescales positions so that
+// exp2_tab64 / exp2_nonpos; the dense kernel-matrix kernel uses the device math
+// library's exp) with the reference's diagonal cut-off.  FP32 mode prescales positions so that
 // the Gaussian is a bare v_exp_f32 of -r'^2, accumulates each run of 32 or 64
 // sources in FP32 and folds it into a Kahan-compensated (sum, comp) pair;
 // partials leave the kernel as double(sum) - double(comp).  Segments with a
