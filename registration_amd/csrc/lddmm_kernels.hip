@@ -249,7 +249,7 @@ __device__ inline double exp2_tab64(double t64, const double *__restrict__ tab) 
   const int ni = __double2loint(k);
   const double tj = tab[ni & 63];
   const double r = fma(tj, q, tj);
-  return __hiloint2double(__double2hiint(r) + ((ni >> 6) << 20), __double2loint(r));
+  return ldexp(r, ni >> 6);  // exact: r in [1, 2), the result a normal double
 }
 
 template <>
@@ -268,11 +268,13 @@ struct KEval<double, LDDMM_KERNEL_GAUSS> {
     k1 = k;
     k2 = k;
   }
-  // The same with the LDS table form of the exponential (tab: EXP2_TAB64 in LDS).
+  // The same with the LDS table form of the exponential (tab: EXP2_TAB64 in LDS).  The
+  // cut-off zeroes the high word of r^2 instead of selecting the result: r^2 then is below
+  // 2^-1042, and exp2_tab64 of -h64 times it is exactly 1 (q rounds away in fma(1, q, 1)).
   __device__ inline void eval_tab(double r2, const double *__restrict__ tab, double &k0,
                                   double &k1, double &k2) const {
-    double e = exp2_tab64(-h64 * r2, tab);
-    double k = r2 < cut_r2 ? 1.0 : e;
+    const double r2c = __hiloint2double(r2 < cut_r2 ? 0 : __double2hiint(r2), __double2loint(r2));
+    const double k = exp2_tab64(-h64 * r2c, tab);
     k0 = k;
     k1 = k;
     k2 = k;
