@@ -240,14 +240,15 @@ __device__ inline double exp2_tab64(double t64, const double *__restrict__ tab) 
   t64 = fmax(t64, -64000.0);
   const double shifter = 6755399441055744.0;  // 1.5 * 2^52
   const double k = t64 + shifter;
+  // The table read first: its LDS latency overlaps the polynomial.
+  const int ni = __double2loint(k);
+  const double tj = tab[ni & 63];
   const double n = k - shifter;
   const double g = t64 - n;
   double q = EXP2T64C.c[5];
 #pragma unroll
   for (int i = 4; i >= 1; --i) q = fma(q, g, EXP2T64C.c[i]);
   q *= g;
-  const int ni = __double2loint(k);
-  const double tj = tab[ni & 63];
   const double r = fma(tj, q, tj);
   return ldexp(r, ni >> 6);  // exact: r in [1, 2), the result a normal double
 }
