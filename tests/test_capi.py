@@ -196,16 +196,16 @@ def test_plan_numbers_staged_tiles_of_ordered_segments():
 
 
 def test_exp2_table_entries_are_correctly_rounded():
-    """The FP64 pair kernel's table of 2^(j/64), j = 0..63 (hex literals in the kernel source),
-    holds the double nearest to each exact value (200-bit arithmetic)."""
+    """The FP64 pair kernel's table of 2^(j/256), j = 0..255 (hex literals in the kernel
+    source), holds the double nearest to each exact value (200-bit arithmetic)."""
     mpmath = pytest.importorskip("mpmath")
     src = open(os.path.join(ROOT, "registration_amd", "csrc", "lddmm_kernels.hip")).read()
-    block = re.search(r"EXP2_TAB64\[64\]\s*=\s*\{(.*?)\};", src, flags=re.S).group(1)
+    block = re.search(r"EXP2_TAB256\[256\]\s*=\s*\{(.*?)\};", src, flags=re.S).group(1)
     vals = [float.fromhex(t) for t in re.findall(r"0x[0-9a-fA-Fp.+-]+", block)]
-    assert len(vals) == 64
+    assert len(vals) == 256
     mpmath.mp.prec = 200
     for j, v in enumerate(vals):
-        exact = mpmath.power(2, mpmath.mpf(j) / 64)
+        exact = mpmath.power(2, mpmath.mpf(j) / 256)
         assert v == float(exact), j
         # and no other double is closer
         assert abs(mpmath.mpf(v) - exact) <= mpmath.mpf(2) ** -53 * v
