@@ -692,9 +692,10 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
   constexpr int W = (KSUM ? M : 0) + (NA > 0 ? D : 0);
   constexpr bool KAHAN = std::is_same<R, float>::value;
   constexpr int NKAH = KAHAN ? 1 : 0;
-  // Sources per unrolled step of the pair loop (measured on gfx950: 4 is fastest for the
-  // FP32 loop, 8 for the FP64 one).
-  constexpr int PAIR_UNROLL = KAHAN ? 4 : 8;
+  // Sources per unrolled step of the pair loop, measured on gfx950: 4 for both (8 costs the
+  // FP32 flow step 4%; with the table exponential the FP64 energy is 1.8% faster at 4 than
+  // at 8).
+  constexpr int PAIR_UNROLL = 4;
   // Sources per FP32 run before its Kahan fold: 64 for the plain K-sums (one culling
   // quarter), which halves the fold cost of the 32 used elsewhere; measured on the
   // 40,962-point flow step: 2-3% faster, the step's error against FP64 from 1.6e-7 to 2.3e-7
