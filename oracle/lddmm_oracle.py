@@ -15,8 +15,9 @@ It restates, in Python 3 / NumPy float64, the executed reference path:
   SecondOrderSurfaceMatching  py-lddmm/secondOrderMatching.py:43-509 (time series of targets)
   cg                      py-lddmm/conjugateGradient.py:33-217
 Python-2 idioms are restated with their Python-2 meaning (``x == None`` on arrays -> ``is None``).
-File output (VTK), logging text and the saveRate branches are left out: they do not feed
-back into the optimisation.
+File output (VTK) and the saveRate branches are left out: they do not feed back into the
+optimisation.  cg keeps the reference's log lines (its call traces are pinned against the
+reference's own conjugateGradient.cg by tests/golden/cg_trace.npz).
 
 With ``sums='reference'`` the kernel sums call the reference's own Fortran subroutines
 compiled by oracle/Makefile into oracle/_ref/libkff_ref.so (present only where that build ran).
@@ -24,6 +25,7 @@ compiled by oracle/Makefile into oracle/_ref/libkff_ref.so (present only where t
 from __future__ import annotations
 
 import ctypes
+import logging
 import os
 
 import numpy as np
@@ -1203,7 +1205,8 @@ class SurfaceMatching(_Matching):
         [grd2] = self.dotProduct(grd, [grd])
         self.gradEps = max(self.gradLB, np.sqrt(grd2) / 10000)
         self.coeffAff = self.coeffAff1
-        return cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+        cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+        return self.history
 
 
 class SymmetricSurfaceMatching(SurfaceMatching):
@@ -1341,7 +1344,8 @@ class CurveMatching(_Matching):
         grd = self.getGradient(self.gradCoeff)
         [grd2] = self.dotProduct(grd, [grd])
         self.gradEps = max(self.gradLB, np.sqrt(grd2) / 10000)
-        return cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+        cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+        return self.history
 
 
 class PointSet:
@@ -1396,32 +1400,55 @@ class LandmarkMatching(_Matching):
         grd = self.getGradient(self.gradCoeff)
         [grd2] = self.dotProduct(grd, [grd])
         self.gradEps = max(self.gradLB, np.sqrt(grd2) / 10000)
-        return cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+        cg(self, maxIter=self.maxIter, epsInit=self.epsInit)
+        return self.history
 
 
 def cg(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=10.):
-    """conjugateGradient.cg restated (conjugateGradient.py:33-217).  Returns the per-iteration
-    objective history (the reference returns opt.getVariable(); the history is kept on
-    ``opt.history``)."""
+    """conjugateGradient.cg restated (conjugateGradient.py:33-217), with its optional hooks
+    (startOptim, startOfIteration, endOfIteration, endOptim: :40-41,72-73,211-212,214-215), the
+    gradient test (:76-92), the plain-array fallbacks when the object has no dotProduct /
+    copyDir / addProd (:95-131) and its log lines (root logger, :66,143-144,161-162,185-186,
+    198-199).  Returns opt.getVariable() as the reference does; the accepted objectives are
+    kept on ``opt.history`` and the stop on ``opt.stop_reason`` ('maxIter' when none fired)."""
+    if hasattr(opt, 'startOptim'):
+        opt.startOptim()
     gradEps = getattr(opt, 'gradEps', 1.0)
     cgBurnIn = getattr(opt, 'cgBurnIn', -1)
     gradCoeff = getattr(opt, 'gradCoeff', 1.0)
     restartRate = getattr(opt, 'restartRate', 100)
+    hasDot = hasattr(opt, 'dotProduct')
+
+    def dot(g, gs):
+        return opt.dotProduct(g, gs) if hasDot else [np.multiply(g, h).sum() for h in gs]
+
+    def copy(d):
+        return opt.copyDir(d) if hasattr(opt, 'copyDir') else np.copy(d)
+
+    def itlog(it, obj, eps, beta, grd2):
+        logging.info('iteration {0:d}: obj = {1:.5f}, eps = {2:.5f}, beta = {3:.5f}, '
+                     'gradient: {4:.5f}'.format(it + 1, obj, eps, beta, np.sqrt(grd2)))
+
     eps = epsInit
     epsMin = 1e-10
     opt.converged = False
     obj = opt.objectiveFun()
+    logging.info('iteration 0: obj = {0: .5f}'.format(obj))
     opt.history = [obj]
     opt.stop_reason = 'maxIter'
     skipCG = 0
     for it in range(maxIter):
         if it % restartRate == 0:
             skipCG = 1
+        if hasattr(opt, 'startOfIteration'):
+            opt.startOfIteration()
         grd = opt.getGradient(gradCoeff)
-        hasDot = hasattr(opt, 'dotProduct')
-        dot = (lambda g, gs: opt.dotProduct(g, gs)) if hasDot else \
-            (lambda g, gs: [np.multiply(g, h).sum() for h in gs])
-        copy = opt.copyDir if hasattr(opt, 'copyDir') else np.copy
+        if TestGradient:
+            dirfoo = opt.randomDir() if hasattr(opt, 'randomDir') else np.random.normal(size=grd.shape)
+            epsfoo = 1e-7
+            objfoo = opt.updateTry(dirfoo, epsfoo, obj - 1e10)
+            [grdfoo] = dot(grd, [dirfoo])
+            logging.info('Test Gradient: %.4f %.4f' % ((objfoo - obj) / epsfoo, -grdfoo * gradCoeff))
         if it == 0 or it == cgBurnIn:
             [grdOld2] = dot(grd, [grd])
             grd2 = grdOld2
@@ -1448,10 +1475,14 @@ def cg(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=10.):
             objTry0 = opt.updateTry(dir0, epsSmall, obj)
             if objTry0 > obj:
                 if (skipCG == 1) | (beta < 1e-10):
+                    itlog(it, obj, eps, beta, grd2)
+                    logging.info('Stopping Gradient Descent: bad direction')
                     opt.stop_reason = 'bad direction'
                     break
                 else:
+                    # the reference disables CG here only when verb is set (:151-154)
                     if verb:
+                        logging.info('Disabling CG: bad direction')
                         skipCG = 1
                         noUpdate = 1
             else:
@@ -1477,10 +1508,14 @@ def cg(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=10.):
                     contt = 0
             if np.fabs(obj - objTry) < .000001:
                 if (skipCG == 1) | (beta < 1e-10):
+                    itlog(it, obj, eps, beta, grd2)
+                    logging.info('Stopping Gradient Descent: small variation')
                     opt.converged = True
                     opt.stop_reason = 'small variation'
                     break
                 else:
+                    if verb:
+                        logging.info('Disabling CG: small variation')
                     skipCG = 1
                     eps = 1.0
             else:
@@ -1488,14 +1523,19 @@ def cg(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=10.):
             opt.acceptVarTry()
             obj = objTry
             opt.history.append(obj)
+            if verb | (it == maxIter):
+                itlog(it, obj, eps, beta, grd2)
             if np.sqrt(grd2) < gradEps:
+                logging.info('Stopping Gradient Descent: small gradient')
                 opt.converged = True
                 opt.stop_reason = 'small gradient'
                 break
             eps = 100 * eps
             if hasattr(opt, 'endOfIteration'):
                 opt.endOfIteration()
-    return opt.history
+    if hasattr(opt, 'endOptim'):
+        opt.endOptim()
+    return opt.getVariable()
 
 
 # --------------------------------------------------------------------------------------
@@ -1663,4 +1703,5 @@ class SecondOrderSurfaceMatching:
         [grd2] = self.dotProduct(grd, [grd])
         self.gradEps = max(0.001, np.sqrt(grd2) / 10000)
         self.cgBurnIn = self.affBurnIn
-        return cg(self, maxIter=self.maxIter, epsInit=0.1)
+        cg(self, maxIter=self.maxIter, epsInit=0.1)
+        return self.history
