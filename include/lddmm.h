@@ -36,7 +36,8 @@ enum {
   LDDMM_EINVAL = -1,     /* bad size, dimension, kernel or order */
   LDDMM_EWORKSPACE = -2, /* work_bytes smaller than the query returned */
   LDDMM_EHIP = -3,       /* a HIP runtime call failed (see lddmm_last_hip_error) */
-  LDDMM_EUNSUPPORTED = -4
+  LDDMM_EUNSUPPORTED = -4,
+  LDDMM_ESCRATCH = -5    /* request larger than lddmm_apply_k_noalloc's fixed scratch */
 };
 
 enum { LDDMM_KERNEL_GAUSS = 0, LDDMM_KERNEL_LAPLACIAN = 1 };
@@ -215,6 +216,25 @@ int lddmm_apply_k(const double *y, const double *x, const double *w,
                   int64_t ny, int64_t nx, int d, int m, int kernel, int order,
                   double sigma, int prec, double *out, void *work,
                   size_t work_bytes, void *stream);
+
+/* Allocation-free form of lddmm_apply_k.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory or pass a workspace (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as lddmm_apply_k, without (work, work_bytes): the
+ * scratch is one fixed array of LDDMM_NOALLOC_SCRATCH_BYTES per GPU, in the
+ * library's device data, held until the process ends; the call allocates nothing.
+ * A request whose plan needs more scratch than that returns LDDMM_ESCRATCH before
+ * launching anything.  96 MiB holds the plan of every square sum up to 4,000,000
+ * points with m <= 3 at either precision (the chunk partials level off near
+ * 92 MiB; 40,962 points need 51 MiB at FP32C, 31 MiB at FP64).
+ * Every call on one GPU shares that GPU's array and leaves its contents behind:
+ * calls that overlap on one GPU (different streams, or threads) race and must be
+ * serialised by the caller.  `stream` must belong to the current device. */
+#define LDDMM_NOALLOC_SCRATCH_BYTES ((size_t)96 << 20)
+int lddmm_apply_k_noalloc(const double *y, const double *x, const double *w,
+                          int64_t ny, int64_t nx, int d, int m, int kernel,
+                          int order, double sigma, int prec, double *out,
+                          void *stream);
 
 /* kernelFunctions.f90:56-114 applykdifft (via Kernel.applyDiffKT,
  * kernelFunctions.py:413-449): a1 (na, ny, m), a2 (na, nx, m) -> out (ny, d). */

@@ -19,6 +19,7 @@ EINVAL = -1
 EWORKSPACE = -2
 EHIP = -3
 EUNSUPPORTED = -4
+ESCRATCH = -5
 
 KERNEL_GAUSS = 0
 KERNEL_LAPLACIAN = 1
@@ -99,6 +100,7 @@ SIGNATURES = {
     "lddmm_order_compactness": (ctypes.c_double, [_P, _I64, _I, _P, _D, _D]),
     "lddmm_apply_k_workspace_bytes": (_SZ, [_I64, _I64, _I, _I]),
     "lddmm_apply_k": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _D, _I, _P, _P, _SZ, _P]),
+    "lddmm_apply_k_noalloc": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _D, _I, _P, _P]),
     "lddmm_apply_diff_kt_workspace_bytes": (_SZ, [_I64, _I64, _I, _I, _I]),
     "lddmm_apply_diff_kt": (_I, [_P, _P, _P, _P, _I64, _I64, _I, _I, _I, _I, _I, _D, _I,
                                  _P, _P, _SZ, _P]),
@@ -150,6 +152,7 @@ _ERRORS = {
     EWORKSPACE: "workspace too small",
     EHIP: "HIP runtime error",
     EUNSUPPORTED: "unsupported shape",
+    ESCRATCH: "request larger than the allocation-free entry point's fixed scratch",
 }
 
 

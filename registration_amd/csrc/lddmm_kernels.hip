@@ -40,6 +40,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -3054,10 +3055,12 @@ extern "C" size_t lddmm_apply_k_workspace_bytes(int64_t ny, int64_t nx, int d, i
   return best;
 }
 
-extern "C" int lddmm_apply_k(const double *y, const double *x, const double *w, int64_t ny,
-                             int64_t nx, int d, int m, int kernel, int order, double sigma,
-                             int prec, double *out, void *work, size_t work_bytes,
-                             void *stream) {
+namespace {
+
+// lddmm_apply_k's body, for a caller-given workspace or the allocation-free form's scratch.
+int apply_k_impl(const double *y, const double *x, const double *w, int64_t ny, int64_t nx,
+                 int d, int m, int kernel, int order, double sigma, int prec, double *out,
+                 void *work, size_t work_bytes, void *stream) {
   if (!y || !x || !w || !out || ny < 1 || nx < 1 || m < 1 || d < 2 || d > 3) return LDDMM_EINVAL;
   // Columns are processed in groups of up to 3: the weight/output row stride
   // stays m while the kernel's column count is the group width.
@@ -3090,6 +3093,81 @@ extern "C" int lddmm_apply_k(const double *y, const double *x, const double *w, 
     if (rc) return rc;
   }
   return LDDMM_OK;
+}
+
+// Workspace bytes apply_k_impl needs at this precision (the largest column group's plan).
+int apply_k_need(int64_t ny, int64_t nx, int d, int m, int kernel, int order, double sigma,
+                 int prec, size_t *need) {
+  *need = 0;
+  for (int c0 = 0; c0 < m; c0 += 3) {
+    lddmm_seg_t s = blank_seg();
+    s.y = QP; s.x0 = QP; s.w0 = QP; s.out0 = (double *)QP;
+    s.ny = ny; s.nx0 = nx; s.sigma = sigma; s.kernel = kernel; s.order = order;
+    s.m = (m - c0) < 3 ? (m - c0) : 3;
+    int64_t items, fins;
+    size_t b = 0;
+    int rc = plan_one(LDDMM_OP_APPLY_K, prec, d, &s, &items, &fins, &b);
+    if (rc) return rc;
+    if (b > *need) *need = b;
+  }
+  return LDDMM_OK;
+}
+
+// Scratch of the allocation-free entry point: one fixed array per GPU, in the code
+// object's zero-filled data, held until the process ends.  Its host-side name is a
+// stand-in, never a device address: host code uses the address hipGetSymbolAddress gives
+// for the current device, looked up once per device on that device's first call.
+constexpr size_t NOALLOC_SCRATCH = LDDMM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[MAX_DEVICES];  // device address of `scratch` on each GPU (null: not looked up)
+
+int scratch_on_current_device(void **ptr) {
+  int dev = -1;
+  int rc = hip_check(hipGetDevice(&dev));
+  if (rc) return rc;
+  if (dev < 0 || dev >= MAX_DEVICES) return LDDMM_EUNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (rc) return rc;
+    if (!p) return LDDMM_EHIP;
+    g_scratch_dev[dev] = p;
+  }
+  *ptr = g_scratch_dev[dev];
+  return LDDMM_OK;
+}
+
+}  // namespace
+
+extern "C" int lddmm_apply_k(const double *y, const double *x, const double *w, int64_t ny,
+                             int64_t nx, int d, int m, int kernel, int order, double sigma,
+                             int prec, double *out, void *work, size_t work_bytes,
+                             void *stream) {
+  return apply_k_impl(y, x, w, ny, nx, d, m, kernel, order, sigma, prec, out, work, work_bytes,
+                      stream);
+}
+
+extern "C" int lddmm_apply_k_noalloc(const double *y, const double *x, const double *w,
+                                     int64_t ny, int64_t nx, int d, int m, int kernel,
+                                     int order, double sigma, int prec, double *out,
+                                     void *stream) {
+  if (!y || !x || !w || !out || ny < 1 || nx < 1 || m < 1 || d < 2 || d > 3) return LDDMM_EINVAL;
+  // The whole request is checked against the fixed scratch before anything is launched.
+  size_t need = 0;
+  int rc = apply_k_need(ny, nx, d, m, kernel, order, sigma, prec, &need);
+  if (rc) return rc;
+  if (need > NOALLOC_SCRATCH) return LDDMM_ESCRATCH;
+  void *ptr = nullptr;
+  rc = scratch_on_current_device(&ptr);
+  if (rc) return rc;
+  // Nothing in the scratch has to start at zero: the descriptor is stamped and every
+  // chunk partial is written before it is read, by the same kernels as lddmm_apply_k's.
+  return apply_k_impl(y, x, w, ny, nx, d, m, kernel, order, sigma, prec, out, ptr,
+                      NOALLOC_SCRATCH, stream);
 }
 
 extern "C" size_t lddmm_apply_diff_kt_workspace_bytes(int64_t ny, int64_t nx, int d, int m,
