@@ -10,8 +10,12 @@ objective), every RKHS dot-product result, the gradient threshold gradEps, the a
 objective history, the stop reason and the iteration count.  The GPU test of the same
 registration in FP64 compares its own sequence against these numbers.
 
-Run (about an hour on 8 cores):  python tests/golden/make_config2_oracle.py
+Run (about an hour on 8 cores):  python tests/golden/make_config2_oracle.py [cache_dir]
+With a cache directory, every kernel sum's result is also stored there under a hash of its
+inputs, so an interrupted run replays the finished part from disk and goes on from there
+(the optimisation path is deterministic, so the replayed calls are the same calls).
 """
+import hashlib
 import os
 import sys
 import time
@@ -26,7 +30,29 @@ from registration_amd import synthetic  # noqa: E402
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "config2_oracle.npz")
 
 
+def cached(fn, cache_dir):
+    """fn(*arrays_and_scalars) -> ndarray, memoised on disk by a hash of its arguments."""
+    def wrapper(*args, **kw):
+        h = hashlib.sha256(fn.__name__.encode())
+        for a in list(args) + sorted(kw.items()):
+            a = np.ascontiguousarray(a) if isinstance(a, np.ndarray) else np.asarray(repr(a))
+            h.update(str(a.shape).encode() + str(a.dtype).encode())
+            h.update(a.tobytes())
+        path = os.path.join(cache_dir, h.hexdigest() + ".npy")
+        if os.path.exists(path):
+            return np.load(path)
+        r = fn(*args, **kw)
+        np.save(path + ".tmp.npy", r)
+        os.replace(path + ".tmp.npy", path)
+        return r
+    return wrapper
+
+
 def main():
+    if len(sys.argv) > 1:
+        os.makedirs(sys.argv[1], exist_ok=True)
+        orc.apply_k = cached(orc.apply_k, sys.argv[1])
+        orc.apply_diff_kt = cached(orc.apply_diff_kt, sys.argv[1])
     c = synthetic.surface_pair_config2()
     (F, V), (_, Y) = c["template"], c["target"]
     o = orc.SurfaceMatching(orc.Surface(FV=(F, V)), orc.Surface(FV=(F, Y)),
