@@ -59,6 +59,9 @@ def parse():
                     help="launches per data-term kernel in the data_terms leg (0 = skip)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the flow-step launches and a calibration copy (for rocprof passes)")
+    ap.add_argument("--energy-only", action="store_true",
+                    help="only the FP64 currents-energy launches of the data_terms leg (for "
+                         "rocprof passes)")
     return ap.parse_args()
 
 
@@ -137,7 +140,7 @@ def flow_step_roofline(cfg, prec, reps, natural=True):
 DATA_TERM_FLOPS = {"currents": 16, "measure": 12, "varifold": 20}
 
 
-def data_terms_leg(cfg, reps):
+def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradients=True):
     """Per-launch time of the data-term pair kernels on the workload's surface pair (deformed
     template against target, F x F pairs for each of the two source sets), timed with HIP
     events on the launch stream: FP64 energies (what every line-search trial runs) and FP32
@@ -178,7 +181,8 @@ def data_terms_leg(cfg, reps):
 
     out = {}
     pairs = 2 * nf * nf
-    for name, op in ops.items():
+    for name in names:
+        op = ops[name]
         ms = timed(rt.Plan(op, _lib.PREC_FP64, 3, [rt.seg(**base, **weights[name])]))
         tf = DATA_TERM_FLOPS[name] * pairs / (ms * 1e-3) / 1e12
         # Nominal: every one of the 2 F^2 pairs counted, although tile culling skips far
@@ -189,10 +193,14 @@ def data_terms_leg(cfg, reps):
                                       "flops_per_pair": DATA_TERM_FLOPS[name],
                                       "nominal_tflops": round(tf, 3),
                                       "fp64_peak_tflops": FP64_PEAK_TFLOPS}
+        if not gradients:
+            continue
         ms = timed(rt.Plan(op, _lib.PREC_FP32C, 3,
                            [rt.seg(**base, **weights[name], out0=z1, out1=dz)]))
         out[f"{name}_gradient_fp32c"] = {"ms": round(ms, 4), "nominal_pairs": pairs,
                                          "nominal_pairs_per_s": pairs / (ms * 1e-3)}
+    if not gradients:
+        return out
     # divergence of K(., x) a at the template's points (one Jacobian-flow step)
     x = torch.as_tensor(V, device=dev)
     a = torch.as_tensor(1e-2 * np.random.default_rng(1).standard_normal(V.shape), device=dev)
@@ -387,6 +395,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"roofline": roof, "calibration_copy_bytes": 256 << 20}))
+        return 0
+
+    if args.energy_only:
+        print(json.dumps(data_terms_leg(cfg, args.data_term_reps, names=("currents",),
+                                        gradients=False)))
         return 0
 
     for _ in range(args.warmup):

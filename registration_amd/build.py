@@ -15,9 +15,6 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # No packed FP32 instructions anywhere in the device code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # Matrix-core results in ordinary VGPRs (gfx950 allows it): otherwise the compiler may
-    # park them in AGPRs and read each back with a v_accvgpr_read (k_pairs_mfma).
-    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I", os.path.join(ROOT, "include"),
 ]
 

@@ -167,7 +167,7 @@ struct KEval<float, LDDMM_KERNEL_LAPLACIAN> {
 // t = n + f with n = rint(t) (exact: the 1.5 * 2^52 shift rounds to an integer), |f| <= 1/2;
 // 2^f = sum_k (f ln 2)^k / k! truncated after degree 13 (remainder < 4e-18 on |f| <= 1/2,
 // Horner rounding ~2 ulp); 2^n is added to the exponent field.  t is clamped at -1000
-// (2^-1000 ~ 9e-302), which keeps every result a normal double.  (Estrin's form, depth 4
+// (2^-1000 ~ 9e-302), which keeps every result a normal double; a NaN t gives NaN.  (Estrin's form, depth 4
 // instead of 13 for three more multiplies, measured 6-9% slower in the FP64 energy sums:
 // they are bound by instruction issue, not by this chain's latency.)
 struct Exp2Coef {
@@ -184,7 +184,7 @@ struct Exp2Coef {
 constexpr Exp2Coef EXP2C{};
 
 __device__ inline double exp2_nonpos(double t) {
-  t = fmax(t, -1000.0);
+  t = t < -1000.0 ? -1000.0 : t;  // a select, not fmax: a NaN exponent stays NaN
   const double shifter = 6755399441055744.0;  // 1.5 * 2^52
   const double k = t + shifter;
   const double n = k - shifter;
@@ -282,11 +282,16 @@ constexpr Exp2T256Coef EXP2T256C{};
 // 2^(t / 256) in double for t <= 0, with the 256-entry table (in LDS) of 2^(j/256):
 // t = 256 m + j + g with rint (the 1.5 * 2^52 shift), j = 0..255 and |g| <= 1/2;
 // 2^(g/256) - 1 = q = sum_{k>=1} (g ln 2 / 256)^k / k! truncated after degree 4 (remainder
-// < 4e-17 relative: |g ln 2 / 256| < 0.0014), the result ldexp(fma(T[j], q, T[j]), m).  t is
-// clamped at -256000 (2^-1000), which keeps every result a normal double.  4 FMA + 1 multiply
-// instead of exp2_nonpos's 13 FMA.
+// < 4e-17 relative: |g ln 2 / 256| < 0.0014), the result ldexp(fma(T[j], q, T[j]), m).  4 FMA
+// + 1 multiply instead of exp2_nonpos's 13 FMA.  The integer m must fit 32 bits: CLAMP
+// clamps t near -256000 (2^-1000) by selecting the high word of -256000 (the low word
+// stays, so the clamped t is -256000 within 2^-20): one select that keeps a NaN t NaN,
+// where fmax would turn it into the clamp value.  Without CLAMP the caller guarantees
+// t >= -2^30 (the result then underflows correctly through ldexp).
+template <bool CLAMP>
 __device__ inline double exp2_tab256(double t, const double *__restrict__ tab) {
-  t = fmax(t, -256000.0);
+  if constexpr (CLAMP)
+    t = __hiloint2double(t < -256000.0 ? (int)0xC10F4000 : __double2hiint(t), __double2loint(t));
   const double shifter = 6755399441055744.0;  // 1.5 * 2^52
   const double k = t + shifter;
   // The table read first: its LDS latency overlaps the polynomial.
@@ -321,10 +326,12 @@ struct KEval<double, LDDMM_KERNEL_GAUSS> {
   // The same with the LDS table form of the exponential (tab: EXP2_TAB256 in LDS).  The
   // cut-off zeroes the high word of r^2 instead of selecting the result: r^2 then is below
   // 2^-1042, and exp2_tab256 of -h256 times it is exactly 1 (q rounds away in fma(1, q, 1)).
+  // CLAMP: see exp2_tab256.
+  template <bool CLAMP>
   __device__ inline void eval_tab(double r2, const double *__restrict__ tab, double &k0,
                                   double &k1, double &k2) const {
     const double r2c = __hiloint2double(r2 < cut_r2 ? 0 : __double2hiint(r2), __double2loint(r2));
-    const double k = exp2_tab256(-h256 * r2c, tab);
+    const double k = exp2_tab256<CLAMP>(-h256 * r2c, tab);
     k0 = k;
     k1 = k;
     k2 = k;
@@ -405,6 +412,12 @@ __device__ inline R wave_max(R v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
+}
+// A wave-uniform double (every lane holds the same value, e.g. after a wave reduction) moved
+// to scalar registers, which frees the two VGPRs the compiler would otherwise keep it in.
+__device__ inline double wave_uniform(double v) {
+  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                          __builtin_amdgcn_readfirstlane(__double2loint(v)));
 }
 
 // ---------------------------------------------------------------------------
@@ -522,17 +535,26 @@ __device__ inline int find_segment(const lddmm_seg_t *__restrict__ segs, int nse
 // of once per row tile.
 // ---------------------------------------------------------------------------
 
+// FP64 Gaussian K-sums without derivative terms (applyK, flow steps, RKHS products and
+// norms, data-term energies) evaluate blocks of pairs near the tile origin with the expanded
+// exponent -h r^2 = -h|x'|^2 - h|y'|^2 + 2h <x', y'> in tile coordinates (see k_pairs).
+template <typename R, int KT, bool KSUM, int NA>
+constexpr bool xform_kernel() {
+  return std::is_same<R, double>::value && KT == LDDMM_KERNEL_GAUSS && KSUM && NA == 0;
+}
+
 // Tile origin: FP32 mode evaluates the tile in a frame centred at its middle, so rounding
 // to FP32 costs relative precision of the local distances, not of the absolute
-// coordinates.  The origin is the mean of the tile's four quarter-middle sources (sources
-// 32, 96, 160, 224, clamped to the tile), close to the centre of the tile's extent for
-// spatially ordered sources.  FP64 keeps the raw frame (origin 0).
-template <typename R, int D>
+// coordinates; so does the FP64 expanded exponent, whose rounding error grows with the
+// offsets from the origin.  The origin is the mean of the tile's four quarter-middle
+// sources (sources 32, 96, 160, 224, clamped to the tile), close to the centre of the
+// tile's extent for spatially ordered sources.  Other FP64 sums keep the raw frame (origin 0).
+template <typename R, int D, bool CENTRE>
 __device__ inline void tile_origin(const SrcView &src, const int32_t *__restrict__ sperm,
                                    int64_t base, int64_t tlen, double scale, double org[D]) {
 #pragma unroll
   for (int c = 0; c < D; ++c) org[c] = 0.0;
-  if constexpr (std::is_same<R, float>::value) {
+  if constexpr (CENTRE) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       int64_t ls = base + (32 + 64 * q < tlen ? 32 + 64 * q : tlen - 1);
@@ -559,11 +581,12 @@ struct SrcStage {
 
 // Stage source l (< l1: a real source; else zero-weight padding with an empty box).  Every
 // lane of the wave calls it (wave reductions).  DOT: the dot-product form is possible for
-// this segment (its quarter is staged doubled when max |v|^2 <= EXP_R2MAX).
+// this segment (its quarter is staged doubled when max |v|^2 <= EXP_R2MAX).  QMAX: the
+// quarter's max |v|^2 is wanted without the dot-product form (the FP64 expanded exponent).
 template <typename R, int D, int M, int NSW>
 __device__ inline void stage_source(const SrcView &src, const int32_t *__restrict__ sperm,
                                     int ld, int64_t l, int64_t l1, double scale,
-                                    const double org[D], bool dot,
+                                    const double org[D], bool dot, bool qmax,
                                     SrcStage<R, D, M, NSW> &o) {
   R n2 = 0;
   if (l < l1) {
@@ -599,9 +622,9 @@ __device__ inline void stage_source(const SrcView &src, const int32_t *__restric
   }
   o.n2 = -n2;
   o.q2 = 0;
-  if (dot) {
+  if (dot || qmax) {
     o.q2 = wave_max(n2);
-    if (o.q2 <= (R)EXP_R2MAX) {
+    if (dot && o.q2 <= (R)EXP_R2MAX) {
 #pragma unroll
       for (int c = 0; c < D; ++c) o.v[c] = (R)2 * o.v[c];
     }
@@ -639,6 +662,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(const lddmm_seg_t *__restrict__
                                                 int op, int prec, char *__restrict__ stage) {
   constexpr int NSW = (NA > (KSUM ? 1 : 0)) ? NA : (KSUM ? 1 : 0);
   constexpr bool DOT = std::is_same<R, float>::value && KT == LDDMM_KERNEL_GAUSS && KSUM && NA == 0;
+  constexpr bool XFORM = xform_kernel<R, KT, KSUM, NA>();
   const int tid = threadIdx.x;
   const int64_t tile = blockIdx.x;
   const lddmm_seg_t *sg = segs + find_segment(segs, nseg, tile, 2);
@@ -652,10 +676,10 @@ __global__ __launch_bounds__(BLOCK) void k_stage(const lddmm_seg_t *__restrict__
   const int64_t base = (set ? t - tiles0 : t) * TS;
   const int64_t tlen = src.n - base < TS ? src.n - base : TS;
   double org[D];
-  tile_origin<R, D>(src, sperm, base, tlen, scale, org);
+  tile_origin<R, D, std::is_same<R, float>::value || XFORM>(src, sperm, base, tlen, scale, org);
   SrcStage<R, D, M, NSW> ss;
   stage_source<R, D, M, NSW>(src, sperm, weight_ld(op, sg, D), base + tid, src.n, scale, org,
-                             DOT && sperm != nullptr, ss);
+                             DOT && sperm != nullptr, XFORM, ss);
   char *rec = stage + tile * stage_rec_bytes(sizeof(R), D, M, NSW);
   R *rp = (R *)(rec + STAGE_HEAD);
 #pragma unroll
@@ -684,7 +708,9 @@ __global__ __launch_bounds__(BLOCK) void k_stage(const lddmm_seg_t *__restrict__
 // row and chunk.
 // ---------------------------------------------------------------------------
 template <typename R, int KT, int D, int M, bool KSUM, int NA>
-__global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__ segs,
+__global__ __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(xform_kernel<R, KT, KSUM, NA>() ? 4 : 1)))
+void k_pairs(const lddmm_seg_t *__restrict__ segs,
                                                 int nseg, int op, int prec,
                                                 double *__restrict__ part,
                                                 const char *__restrict__ stage) {
@@ -706,11 +732,21 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
 
   // Dot-product exponent: FP32 Gaussian K-sums whose segments carry a processing order.
   constexpr bool EXPAND = KAHAN && KT == LDDMM_KERNEL_GAUSS && KSUM && NA == 0;
+  // FP64 Gaussian K-sums: the expanded exponent in tile coordinates,
+  //   t = -h256 r^2 = (-h256 |y'|^2 - h256 |x'|^2) + <2 h256 x', y'>   (one add, three FMAs
+  // in place of three differences, a square and a scale).  Its rounding error is about
+  // eps h256 (|x'| + |y'|)^2, so a (wave, quarter) block takes it only when
+  // h (|x'|max + |y'|max)^2 <= XF_HMAX: the exponent's error is then at most 64 eps bits,
+  // no more than the direct form's 2 eps h |x - y| |x| in absolute coordinates on a unit
+  // sphere, and t >= -256 XF_HMAX needs no clamp.
+  constexpr bool XFORM = xform_kernel<R, KT, KSUM, NA>();
+  constexpr double XF_HMAX = 64.0;
 
   __shared__ R s_pos[D][TS];
   __shared__ R s_w[NSW][M][TS];
   __shared__ R s_box[TS / 64][2 * D];  // bounding box of each 64-source quarter
-  __shared__ R s_n2[EXPAND ? TS : 1];  // -|y|^2 of each source (dot-product exponent)
+  // -|y|^2 of each source (FP32 dot-product exponent), -h256 |y'|^2 (FP64 expanded exponent)
+  __shared__ R s_n2[(EXPAND || XFORM) ? TS : 1];
   __shared__ R s_r2max[TS / 64];       // max |y|^2 of each quarter
   // FP64 Gaussian sums evaluate the exponential with the 2^(j/256) table (exp2_tab256).
   constexpr bool TAB = std::is_same<R, double>::value && KT == LDDMM_KERNEL_GAUSS;
@@ -794,10 +830,34 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
     double lo = yd[0][c], hi = yd[0][c];
 #pragma unroll
     for (int r = 1; r < RPT; ++r) { lo = fmin(lo, yd[r][c]); hi = fmax(hi, yd[r][c]); }
-    tlo[c] = wave_min(lo);
-    thi[c] = wave_max(hi);
+    tlo[c] = wave_uniform(wave_min(lo));
+    thi[c] = wave_uniform(wave_max(hi));
   }
   const R cut2 = (R)cull_r2(prec, KT, sg->sigma);
+  // The workgroup's box (union of its waves' boxes): a staged tile whose quarters all lie
+  // beyond the cut from it is skipped before it is copied (no loads, no barriers).
+  __shared__ double s_wgbox[BLOCK / 64][2 * D];
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      s_wgbox[wave][c] = tlo[c];
+      s_wgbox[wave][D + c] = thi[c];
+    }
+  }
+  __syncthreads();
+  double wlo[D], whi[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    wlo[c] = s_wgbox[0][c];
+    whi[c] = s_wgbox[0][D + c];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) {
+      wlo[c] = fmin(wlo[c], s_wgbox[w][c]);
+      whi[c] = fmax(whi[c], s_wgbox[w][D + c]);
+    }
+    wlo[c] = wave_uniform(wlo[c]);
+    whi[c] = wave_uniform(whi[c]);
+  }
 
   // Accumulators: running sum (s), Kahan compensation (c).
   R sK[RPT][KSUM ? M : 1], cK[RPT][KSUM ? M : 1];
@@ -844,6 +904,23 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
       const char *rec = stage + (stage_t0 + base / TS) * stage_rec_bytes(sizeof(R), D, M, NSW);
 #pragma unroll
       for (int c = 0; c < D; ++c) org[c] = ((const double *)rec)[c];
+      {
+        // Workgroup-level cull from the record's quarter boxes (tile coordinates).
+        const R *qb = (const R *)(rec + 32);
+        R d2 = 0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          R lo = qb[c], hi = qb[D + c];
+#pragma unroll
+          for (int q = 1; q < TS / 64; ++q) {
+            lo = fmin(lo, qb[q * 2 * D + c]);
+            hi = fmax(hi, qb[q * 2 * D + D + c]);
+          }
+          const R g = fmax(fmax((R)(wlo[c] - org[c]) - hi, lo - (R)(whi[c] - org[c])), (R)0);
+          d2 = g * g + d2;
+        }
+        if (d2 > cut2) continue;  // workgroup-uniform: no quarter of this tile is live
+      }
       __syncthreads();
       const R *rp = (const R *)(rec + STAGE_HEAD);
 #pragma unroll
@@ -858,12 +935,21 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
           if (tid < TS / 64) s_r2max[tid] = ((const R *)(rec + 224))[tid];
         }
       }
+      if constexpr (XFORM) {
+        s_n2[tid] = kev.h256 * rp[(D + NSW * M) * TS + tid];
+        if (tid < TS / 64) s_r2max[tid] = ((const R *)(rec + 224))[tid];
+      }
       if (tid < (TS / 64) * 2 * D) (&s_box[0][0])[tid] = ((const R *)(rec + 32))[tid];
     } else {
-      tile_origin<R, D>(src, sperm, base, tlen, scale, org);
+      tile_origin<R, D, KAHAN || XFORM>(src, sperm, base, tlen, scale, org);
       __syncthreads();
       SrcStage<R, D, M, NSW> ss;
-      stage_source<R, D, M, NSW>(src, sperm, ld, base + tid, l1, scale, org, expand_seg, ss);
+      stage_source<R, D, M, NSW>(src, sperm, ld, base + tid, l1, scale, org, expand_seg, XFORM,
+                                 ss);
+      if constexpr (XFORM) {
+        s_n2[tid] = kev.h256 * ss.n2;
+        if (lane == 0) s_r2max[wave] = ss.q2;
+      }
 #pragma unroll
       for (int c = 0; c < D; ++c) s_pos[c][tid] = ss.v[c];
 #pragma unroll
@@ -890,15 +976,40 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
 #pragma unroll
         for (int c = 0; c < D; ++c) ty[r][c] = (R)(yd[r][c] - org[c]);
     }
+    // Expanded exponent: rows in tile coordinates x', 2 h256 x' and -h256 |x'|^2; the wave's
+    // largest |x'|^2 decides which blocks take the form.
+    R xX[XFORM ? RPT : 1][D], xc[XFORM ? RPT : 1];
+    R x2max = 0;
+    if constexpr (XFORM) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        R x2 = 0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          ty[r][c] = yd[r][c] - org[c];
+          xX[r][c] = (2.0 * kev.h256) * ty[r][c];
+          x2 = ty[r][c] * ty[r][c] + x2;
+        }
+        xc[r] = -kev.h256 * x2;
+        x2max = fmax(x2max, x2);
+      }
+      x2max = wave_uniform(wave_max(x2max));
+    }
     __syncthreads();
     // Which 64-source quarters of the tile hold a pair above the cut for this
     // wave (squared distance between the target box and each quarter's box).
-    unsigned live = 0, far = 0, expq = 0;
+    unsigned live = 0, far = 0, expq = 0, xq = 0;
 #pragma unroll
     for (int q = 0; q < TS / 64; ++q) {
       if constexpr (EXPAND) {
         if (expand_seg && s_r2max[q] <= (R)EXP_R2MAX) expq |= 1u << q;
       }
+      if constexpr (XFORM) {
+        // (FP32 square roots: a threshold test, with a margin for their rounding)
+        const float xy = __builtin_sqrtf((float)x2max) + __builtin_sqrtf((float)s_r2max[q]);
+        if ((float)kev.h * xy * xy <= (float)(0.98 * XF_HMAX)) xq |= 1u << q;
+      }
+
       R d2 = 0;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
@@ -973,6 +1084,76 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
           continue;
         }
       }
+      if constexpr (XFORM) {
+        if ((xq >> (j0 / 64)) & 1u) {
+          if (mixed && ((far >> (j0 / 64)) & 1u)) {
+            // Every pair of this run has K < 2^-26 (see below): v_exp_f32 of the expanded
+            // FP64 exponent.
+#pragma unroll PAIR_UNROLL
+            for (int j = j0; j < j0 + KFP; ++j) {
+              R sx[D], sw[M];
+#pragma unroll
+              for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j];
+#pragma unroll
+              for (int c = 0; c < M; ++c) sw[c] = s_w[0][c][j];
+              const R a = s_n2[j];
+#pragma unroll
+              for (int r = 0; r < RPT; ++r) {
+                R t = a + xc[r];
+#pragma unroll
+                for (int c = 0; c < D; ++c) t = xX[r][c] * sx[c] + t;
+                const R k = (R)__builtin_amdgcn_exp2f((float)(t * (1.0 / 256.0)));
+#pragma unroll
+                for (int c = 0; c < M; ++c) tK[r][c] = k * sw[c] + tK[r][c];
+              }
+            }
+          } else {
+            // The reference's cut-off (K = 1 for r^2 < 1e-8 sigma^2) on t: zeroing its high
+            // word makes the exponential exactly 1, as for the direct form.  The loop runs
+            // exp2_tab256 in stages over XU sources (exponents, then table reads, then
+            // polynomials), so each table read's LDS latency overlaps the other sources'
+            // arithmetic; written as one chain per source, the compiler issued the chains
+            // one after another and waited on every table read.
+            const R tcut = -kev.h256 * kev.cut_r2;
+            constexpr int XU = 4;
+            static_assert(RPT == 1, "one row per thread in FP64");
+#pragma unroll 1
+            for (int j = j0; j < j0 + KFP; j += XU) {
+              R sw[XU][M], t[XU];
+#pragma unroll
+              for (int u = 0; u < XU; ++u) {
+#pragma unroll
+                for (int c = 0; c < M; ++c) sw[u][c] = s_w[0][c][j + u];
+                R e = s_n2[j + u] + xc[0];
+#pragma unroll
+                for (int c = 0; c < D; ++c) e = xX[0][c] * s_pos[c][j + u] + e;
+                t[u] = __hiloint2double(e > tcut ? 0 : __double2hiint(e), __double2loint(e));
+              }
+              const double shifter = 6755399441055744.0;  // 1.5 * 2^52, as in exp2_tab256
+              int ni[XU];
+              R tj[XU], g[XU];
+#pragma unroll
+              for (int u = 0; u < XU; ++u) {
+                const double kk = t[u] + shifter;
+                ni[u] = __double2loint(kk);
+                tj[u] = s_e2[ni[u] & 255];
+                g[u] = t[u] - (kk - shifter);
+              }
+#pragma unroll
+              for (int u = 0; u < XU; ++u) {
+                double q = EXP2T256C.c[4];
+#pragma unroll
+                for (int i = 3; i >= 1; --i) q = fma(q, g[u], EXP2T256C.c[i]);
+                q *= g[u];
+                const R k = ldexp(fma(tj[u], q, tj[u]), ni[u] >> 8);
+#pragma unroll
+                for (int c = 0; c < M; ++c) tK[0][c] = k * sw[u][c] + tK[0][c];
+              }
+            }
+          }
+          goto fold;
+        }
+      }
       if constexpr (MIXED) {
         if (mixed && ((far >> (j0 / 64)) & 1u)) {
           // Every pair of this run has K < 2^-26: v_exp_f32 of the FP64 exponent (relative
@@ -1001,42 +1182,44 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
           goto fold;
         }
       }
+      {
 #pragma unroll PAIR_UNROLL
-      for (int j = j0; j < j0 + KFP; ++j) {
-        R sx[D], sw[NSW][M];
+          for (int j = j0; j < j0 + KFP; ++j) {
+            R sx[D], sw[NSW][M];
 #pragma unroll
-        for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j];
+            for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j];
 #pragma unroll
-        for (int q = 0; q < NSW; ++q)
+            for (int q = 0; q < NSW; ++q)
 #pragma unroll
-          for (int c = 0; c < M; ++c) sw[q][c] = s_w[q][c][j];
+              for (int c = 0; c < M; ++c) sw[q][c] = s_w[q][c][j];
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-          R dx[D];
-          R r2 = KEval<R, KT>::R2_0;
+            for (int r = 0; r < RPT; ++r) {
+              R dx[D];
+              R r2 = KEval<R, KT>::R2_0;
 #pragma unroll
-          for (int c = 0; c < D; ++c) {
-            dx[c] = ty[r][c] - sx[c];
-            r2 = dx[c] * dx[c] + r2;
+              for (int c = 0; c < D; ++c) {
+                dx[c] = ty[r][c] - sx[c];
+                r2 = dx[c] * dx[c] + r2;
+              }
+              R k0, k1, k2;
+              if constexpr (TAB) kev.template eval_tab<true>(r2, s_e2, k0, k1, k2);
+              else kev(r2, k0, k1, k2);
+              if constexpr (KSUM) {
+#pragma unroll
+                for (int c = 0; c < M; ++c) tK[r][c] = k0 * sw[0][c] + tK[r][c];
+              }
+              if constexpr (NA > 0) {
+                R wsum = 0;
+#pragma unroll
+                for (int q = 0; q < NA; ++q)
+#pragma unroll
+                  for (int c = 0; c < M; ++c) wsum = tw[r][q][c] * sw[q][c] + wsum;
+                R f = k1 * wsum;
+#pragma unroll
+                for (int c = 0; c < D; ++c) tD[r][c] = f * dx[c] + tD[r][c];
+              }
+            }
           }
-          R k0, k1, k2;
-          if constexpr (TAB) kev.eval_tab(r2, s_e2, k0, k1, k2);
-          else kev(r2, k0, k1, k2);
-          if constexpr (KSUM) {
-#pragma unroll
-            for (int c = 0; c < M; ++c) tK[r][c] = k0 * sw[0][c] + tK[r][c];
-          }
-          if constexpr (NA > 0) {
-            R wsum = 0;
-#pragma unroll
-            for (int q = 0; q < NA; ++q)
-#pragma unroll
-              for (int c = 0; c < M; ++c) wsum = tw[r][q][c] * sw[q][c] + wsum;
-            R f = k1 * wsum;
-#pragma unroll
-            for (int c = 0; c < D; ++c) tD[r][c] = f * dx[c] + tD[r][c];
-          }
-        }
       }
     fold:
       // Fold the run into the compensated row sums.
@@ -1088,210 +1271,6 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(const lddmm_seg_t *__restrict__
     if constexpr (NA > 0) {
 #pragma unroll
       for (int c = 0; c < D; ++c) o[q++] = (double)sD[r][c] - (double)cD[r][c];
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// FP32 K-weighted sums with the accumulation on the matrix cores (applyK, the
-// Euler flow step, RKHS products).
-//
-// v_mfma_f32_4x4x1f32 runs 16 independent 4x4x1 products per wave: for block
-// b = lane/4, lane 4b+i supplies row i of A, lane 4b+j supplies column j of B,
-// and lane 4b+j's result register v holds D[b][v][j] (layout checked on
-// gfx950).  Each lane owns one target row per row set; per source l it computes
-// A = K(y_lane, x_l) on the VALU (3 sub, 3 FMA, 1 exp) and supplies
-// B = w_l[lane % 4] (weight column j, zero for j >= M), so one MFMA per source
-// and row set accumulates D[b][v][j] += K(y_{4b+v}, x_l) w_l[j]: the three
-// accumulation FMAs per pair leave the VALU.  Same tiles, per-tile origin and
-// quarter culling as k_pairs; two independent accumulator chains per row set
-// (sources alternate between them), summed and folded into Kahan pairs once
-// per 64-source quarter.
-// ---------------------------------------------------------------------------
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <int KT, int D, int M>
-__global__ __launch_bounds__(BLOCK) void k_pairs_mfma(const lddmm_seg_t *__restrict__ segs,
-                                                     int nseg, int op, int prec,
-                                                     double *__restrict__ part) {
-  constexpr int RPT = 2;  // row sets per wave (64 rows each)
-  constexpr int W = M;
-  static_assert(M >= 1 && M <= 4, "MFMA K-sum handles up to 4 weight columns");
-  __shared__ float s_pos[D][TS];
-  __shared__ float s_wt[4][TS];
-  __shared__ float s_box[TS / 64][2 * D];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int jcol = lane & 3;
-  const int64_t item = blockIdx.x;
-  const int si = find_segment(segs, nseg, item, 0);
-  const lddmm_seg_t *sg = segs + si;
-
-  const int64_t ny = sg->ny;
-  const int row_tiles = sg->row_tiles;
-  const int64_t it = item - sg->item0;
-  const int rt = (int)(it % row_tiles);
-  const int cg = (int)(it / row_tiles);
-  const int set = cg < sg->chunks0 ? 0 : 1;
-  const int ch = set ? cg - sg->chunks0 : cg;
-  const int ld = weight_ld(op, sg, D);
-  const SrcView src = resolve_src(op, sg, set);
-  const double scale = coord_scale(prec, KT, sg->sigma);
-  KEval<float, KT> kev;
-  kev.init(sg->order, sg->sigma);
-
-  // Rows as in k_pairs: each wave owns RPT x 64 consecutive target rows.
-  const int64_t wbase = (int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT);
-  double yd[RPT][D];
-  float ty[RPT][D];
-#pragma unroll
-  for (int r = 0; r < RPT; ++r) {
-    int64_t k = wbase + r * 64 + lane;
-    int64_t kk = k < ny ? k : ny - 1;
-#pragma unroll
-    for (int c = 0; c < D; ++c) yd[r][c] = sg->y[kk * D + c] * scale;
-  }
-  double tlo[D], thi[D];
-#pragma unroll
-  for (int c = 0; c < D; ++c) {
-    double lo = yd[0][c], hi = yd[0][c];
-#pragma unroll
-    for (int r = 1; r < RPT; ++r) { lo = fmin(lo, yd[r][c]); hi = fmax(hi, yd[r][c]); }
-    tlo[c] = wave_min(lo);
-    thi[c] = wave_max(hi);
-  }
-  const float cut2 = (float)cull_r2(prec, KT, sg->sigma);
-
-  float sK[RPT][4], cK[RPT][4];
-#pragma unroll
-  for (int r = 0; r < RPT; ++r)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) { sK[r][v] = 0.f; cK[r][v] = 0.f; }
-
-  const int64_t l0 = (int64_t)ch * sg->chunk_len;
-  int64_t l1 = l0 + sg->chunk_len;
-  if (l1 > src.n) l1 = src.n;
-
-  for (int64_t base = l0; base < l1; base += TS) {
-    double org[D];
-    const int64_t mid = base + ((l1 - base < TS ? l1 - base : TS) >> 1);
-#pragma unroll
-    for (int c = 0; c < D; ++c) org[c] = src.pos[mid * D + c] * scale;
-#pragma unroll
-    for (int r = 0; r < RPT; ++r)
-#pragma unroll
-      for (int c = 0; c < D; ++c) ty[r][c] = (float)(yd[r][c] - org[c]);
-    __syncthreads();
-    {
-      int64_t l = base + tid;
-      float lo[D], hi[D];
-      if (l < l1) {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          float v = (float)(src.pos[l * D + c] * scale - org[c]);
-          s_pos[c][tid] = v;
-          lo[c] = v;
-          hi[c] = v;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          s_wt[q][tid] = q < M ? (float)(src.sign * src.w[0][l * ld + q]) : 0.f;
-      } else {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          s_pos[c][tid] = 0.f;
-          lo[c] = INFINITY;
-          hi[c] = -INFINITY;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s_wt[q][tid] = 0.f;
-      }
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        lo[c] = wave_min(lo[c]);
-        hi[c] = wave_max(hi[c]);
-      }
-      if (lane == 0) {
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          s_box[wave][c] = lo[c];
-          s_box[wave][D + c] = hi[c];
-        }
-      }
-    }
-    __syncthreads();
-    unsigned live = 0;
-#pragma unroll
-    for (int q = 0; q < TS / 64; ++q) {
-      float d2 = 0.f;
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        float g = fmaxf(fmaxf((float)(tlo[c] - org[c]) - s_box[q][D + c],
-                              s_box[q][c] - (float)(thi[c] - org[c])), 0.f);
-        d2 += g * g;
-      }
-      if (!(d2 > cut2)) live |= 1u << q;
-    }
-    if (!live) continue;
-
-    for (int q = 0; q < TS / 64; ++q) {
-      if (!((live >> q) & 1u)) continue;
-      v4f acc[2][RPT];
-#pragma unroll
-      for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) acc[h][r] = (v4f){0.f, 0.f, 0.f, 0.f};
-      for (int j = q * 64; j < q * 64 + 64; j += 4) {
-        // Kernel values of 4 sources x RPT rows first, then their MFMAs.
-        float kv[4][RPT], wb[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          float sx[D];
-#pragma unroll
-          for (int c = 0; c < D; ++c) sx[c] = s_pos[c][j + u];
-          wb[u] = s_wt[jcol][j + u];
-#pragma unroll
-          for (int r = 0; r < RPT; ++r) {
-            float r2 = KEval<float, KT>::R2_0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-              float dx = ty[r][c] - sx[c];
-              r2 = dx * dx + r2;
-            }
-            float k1, k2;
-            kev(r2, kv[u][r], k1, k2);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int r = 0; r < RPT; ++r)
-            acc[u & 1][r] = __builtin_amdgcn_mfma_f32_4x4x1f32(kv[u][r], wb[u], acc[u & 1][r], 0, 0, 0);
-      }
-#pragma unroll
-      for (int r = 0; r < RPT; ++r)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          float y = (acc[0][r][v] + acc[1][r][v]) - cK[r][v];
-          float t = sK[r][v] + y;
-          cK[r][v] = (t - sK[r][v]) - y;
-          sK[r][v] = t;
-        }
-    }
-  }
-
-  // Lane 4b+j holds column j of rows 4b+v (v = 0..3) of each row set.
-  if (jcol < M) {
-    double *pp = part + sg->part0 + (int64_t)cg * ny * W;
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int64_t k = wbase + r * 64 + (lane & ~3) + v;
-        if (k < ny) pp[k * W + jcol] = (double)sK[r][v] - (double)cK[r][v];
-      }
     }
   }
 }
@@ -2691,26 +2670,9 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
 
 namespace {
 
-// LDDMM_MFMA=1 selects the K-sum kernel that accumulates on the matrix cores for
-// FP32 applyK, flow steps and RKHS products (same sums up to FP32 accumulation
-// rounding; energy-only sums keep k_pairs' symmetric form).  It issues 7 instead
-// of ~11 VALU instructions per pair but measured 12% slower on MI355X on the
-// 40,962-point flow step (0.61 vs 0.54 ms), so the all-VALU kernel is the default.
-bool use_mfma() {
-  const char *e = getenv("LDDMM_MFMA");
-  return e && e[0] == '1';
-}
-
-// LDDMM_STAGE=0 has k_pairs build every tile itself (the same sums, bit for bit).
-bool use_stage() {
-  const char *e = getenv("LDDMM_STAGE");
-  return !(e && e[0] == '0');
-}
-
-// Per-launch options of the K-sum kernels: the matrix-core variant, and the staging area
-// of the ordered segments' source tiles (n_stage tiles; none when 0).
+// Per-launch options of the K-sum kernels: the staging area of the source tiles built once
+// per launch (n_stage tiles; none when 0).
 struct PairOpts {
-  bool mfma;
   char *stage;
   int64_t n_stage;
 };
@@ -2718,13 +2680,6 @@ struct PairOpts {
 template <typename R, int KT, int D, int M, bool KSUM, int NA>
 void launch_pairs(const lddmm_seg_t *segs, int nseg, int op, int prec, int64_t n_items,
                   double *part, hipStream_t st, const PairOpts &po) {
-  if constexpr (std::is_same<R, float>::value && KSUM && NA == 0) {
-    if ((op == LDDMM_OP_APPLY_K || op == LDDMM_OP_FLOW_STEP || op == LDDMM_OP_KDOT) && po.mfma) {
-      hipLaunchKernelGGL((k_pairs_mfma<KT, D, M>), dim3((unsigned)n_items), dim3(BLOCK), 0,
-                         st, segs, nseg, op, prec, part);
-      return;
-    }
-  }
   const char *stage = nullptr;
   // Staged segments: K-sums without derivative terms (flow steps, RKHS products and norms,
   // data-term energies) and the covector step (M = D, two derivative weight pairs).
@@ -2971,11 +2926,9 @@ extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *se
   double *part = (double *)(wb + desc_bytes(nseg));
   double *red = part + parts;
   const int kern = segs_host[0].kernel;
-  // The matrix-core K-sum (opt-in) has no processing-order support.
   PairOpts po;
-  po.mfma = use_mfma() && n_stage == 0;
   po.stage = wb + stage_offset(nseg, parts + reds);
-  po.n_stage = use_stage() ? n_stage : 0;
+  po.n_stage = n_stage;
   rc = d == 2 ? run_d<2>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, po)
               : run_d<3>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, po);
   if (rc) return rc;

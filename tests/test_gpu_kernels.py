@@ -144,49 +144,6 @@ def test_large_rows_sampled_against_oracle():
     assert rel_err(z32, z64) < TOL["fp32c"]
 
 
-def test_mfma_and_valu_ksum_kernels_agree(monkeypatch):
-    """The opt-in FP32 K-sum with its accumulation on the matrix cores (LDDMM_MFMA=1,
-    v_mfma_f32_4x4x1f32) and the
-    default all-VALU kernel is the same sum with the same per-source order."""
-    rng = np.random.default_rng(12)
-    y, x, w = rng.standard_normal((1500, 3)), rng.standard_normal((2100, 3)), rng.standard_normal((2100, 3))
-    for order, sigma in ((10, 0.5), (2, 0.7)):
-        K = kernel(order, sigma, "fp32c")
-        a = K.applyK(x, w, firstVar=y)
-        monkeypatch.setenv("LDDMM_MFMA", "1")
-        b = K.applyK(x, w, firstVar=y)
-        monkeypatch.delenv("LDDMM_MFMA")
-        ref = orc.apply_k(y, x, w, sigma, order)
-        assert rel_err(a, ref) < TOL["fp32c"] and rel_err(b, ref) < TOL["fp32c"]
-        assert rel_err(a, b) < 1e-6
-    # d = 2, one weight column, clustered points (culled quarters)
-    centers = np.array([[0, 0], [3, 0], [0, 3]], dtype=float)
-    pts = np.concatenate([c + 0.15 * rng.standard_normal((700, 2)) for c in centers])
-    w1 = rng.standard_normal((pts.shape[0], 1))
-    K = kernel(10, 0.05, "fp32c")
-    monkeypatch.setenv("LDDMM_MFMA", "1")
-    b = K.applyK(pts, w1)
-    monkeypatch.delenv("LDDMM_MFMA")
-    assert rel_err(b, orc.apply_k(pts, pts, w1, 0.05, 10)) < TOL["fp32c"]
-
-
-def test_mfma_flow_steps_agree(monkeypatch):
-    """Euler flow with the matrix-core K-sum against the all-VALU one and the oracle."""
-    from registration_amd import pointEvolution as pe, synthetic
-    V, F = synthetic.icosphere(4)
-    rng = np.random.default_rng(3)
-    at = 0.05 * rng.standard_normal((5,) + V.shape)
-    for order, sigma in ((10, 0.3), (3, 0.25)):
-        K = kernel(order, sigma, "fp32c")
-        a = pe.landmarkDirectEvolutionEuler(V, at, K)
-        monkeypatch.setenv("LDDMM_MFMA", "1")
-        b = pe.landmarkDirectEvolutionEuler(V, at, K)
-        monkeypatch.delenv("LDDMM_MFMA")
-        ref = orc.landmarkDirectEvolutionEuler(V, at, orc.Kernel('gauss' if order > 4 else 'laplacian',
-                                                                 sigma=sigma, order=order))
-        assert np.abs(b - ref).max() < 1e-6 and np.abs(a - b).max() < 1e-6
-
-
 @pytest.mark.parametrize("prec", ["fp64", "fp32c"])
 @pytest.mark.parametrize("name,order", [("gauss", 10), ("laplacian", 2)])
 def test_tile_culling_matches_full_sums(prec, name, order):

@@ -254,12 +254,13 @@ def test_invalid_order_is_rejected_on_the_host():
 
 
 @pytest.mark.parametrize("precision", ["fp32c", "fp64"])
-def test_staged_tiles_are_bit_identical(monkeypatch, precision):
-    """Ordered segments read their source tiles from the launch's staging area (built once
-    per launch); LDDMM_STAGE=0 has every workgroup build them itself.  Both give the same
-    flows, covectors and RKHS products bit for bit, for a batch of ordered and natural
-    segments (staged tiles numbered past a natural segment; ragged last tiles; one cloud
-    smaller than a tile)."""
+def test_staged_tiles_match_the_natural_order(precision):
+    """Ordered segments read their source tiles from the launch's staging area (built once per
+    launch by k_stage); segments without an order build each tile in the pair kernel.  A batch
+    of ordered and natural segments (staged tiles numbered past a natural segment; ragged last
+    tiles; one cloud smaller than a tile) gives the same flows, covectors and RKHS products as
+    the same batch with every segment natural: to rounding in FP64, and within the FP32 bar
+    (the ordered FP32 sums use the dot-product exponent)."""
     import torch
     from registration_amd import engine, synthetic
     from registration_amd.kernelFunctions import Kernel
@@ -267,12 +268,11 @@ def test_staged_tiles_are_bit_identical(monkeypatch, precision):
     rng = np.random.default_rng(77)
     clouds = [synthetic.icosphere(4)[0], synthetic.icosphere(3)[0], synthetic.icosphere(5)[0],
               rng.standard_normal((100, 3))]
-    ordered = [True, False, True, True]
     T = 2
     ats = [0.02 * rng.standard_normal((T,) + X.shape) for X in clouds]
     dev = lambda a: torch.as_tensor(a, device="cuda")
 
-    def run():
+    def run(ordered):
         perms = [engine.spatial_order_dev(X) if o else None for X, o in zip(clouds, ordered)]
         jobs = [engine.FlowJob(dev(X), dev(a), K,
                                torch.empty((T + 1,) + X.shape, dtype=torch.float64, device="cuda"),
@@ -288,23 +288,24 @@ def test_staged_tiles_are_bit_identical(monkeypatch, precision):
                              torch.zeros((T, 2), dtype=torch.float64, device="cuda"), perm=p)
               for j, a, p in zip(jobs, ats, perms)]
         engine.kdot(kd)
-        return ([j.xt.cpu().numpy() for j in jobs] + [c.pxt.cpu().numpy() for c in cov] +
-                [k.out.cpu().numpy() for k in kd])
+        return ([j.xt.cpu().numpy() - X for j, X in zip(jobs, clouds)] +
+                [c.pxt.cpu().numpy() for c in cov] + [k.out.cpu().numpy() for k in kd])
 
-    print(f"staged tiles vs in-kernel staging, {precision}", flush=True)
-    staged = run()
-    monkeypatch.setenv("LDDMM_STAGE", "0")
-    built = run()
-    for a, b in zip(staged, built):
-        assert np.array_equal(a, b)
+    print(f"staged tiles vs natural order, {precision}", flush=True)
+    staged = run([True, False, True, True])
+    natural = run([False] * 4)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    for a, b in zip(staged, natural):
+        assert np.isfinite(a).all() and np.abs(b).max() > 0
+        assert np.abs(a - b).max() <= tol * np.abs(b).max(), np.abs(a - b).max() / np.abs(b).max()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
-def test_staged_energy_tiles_are_bit_identical(monkeypatch, precision):
+def test_staged_energy_tiles_match_the_oracle(precision):
     """Energy-only sums (currents, measure, RKHS norm: no per-row outputs) stage the tiles of
-    both source sets once per launch; LDDMM_STAGE=0 builds them in every workgroup.  Same
-    totals bit for bit, in batches with ragged sizes, a shape smaller than one tile and a
-    measure without target."""
+    both source sets once per launch and evaluate each unordered self pair once.  Totals in
+    batches with ragged sizes, a shape smaller than one tile and a measure without target
+    match the oracle's sums (relative to the sum of the magnitudes of the terms)."""
     import torch
     from registration_amd import engine
     from registration_amd.kernelFunctions import Kernel
@@ -312,27 +313,37 @@ def test_staged_energy_tiles_are_bit_identical(monkeypatch, precision):
     rng = np.random.default_rng(91)
     dev = lambda a: torch.as_tensor(a, device="cuda")
     sizes = [(3000, 2700), (200, 130), (5000, 5100)]
-    shapes = [(dev(rng.standard_normal((n1, 3))), dev(0.01 * rng.standard_normal((n1, 3))),
-               dev(rng.standard_normal((n2, 3))), dev(0.01 * rng.standard_normal((n2, 3))))
-              for n1, n2 in sizes]
+    host = [(rng.standard_normal((n1, 3)), 0.01 * rng.standard_normal((n1, 3)),
+             rng.standard_normal((n2, 3)), 0.01 * rng.standard_normal((n2, 3))) for n1, n2 in sizes]
+    shapes = [tuple(dev(a) for a in h) for h in host]
+    cj = [engine.CurrentsJob(c1, r1, c2, r2, K, torch.zeros(1, dtype=torch.float64, device="cuda"))
+          for c1, r1, c2, r2 in shapes]
+    engine.currents(cj)
+    mj = [engine.MeasureJob(c1, r1[:, 0].abs().contiguous(), c2 if i != 1 else None,
+                            r2[:, 0].abs().contiguous() if i != 1 else None, K,
+                            torch.zeros(1, dtype=torch.float64, device="cuda"))
+          for i, (c1, r1, c2, r2) in enumerate(shapes)]
+    engine.measure(mj)
+    kj = [engine.KNormJob(c1, r1, K, torch.zeros(1, dtype=torch.float64, device="cuda"))
+          for c1, r1, _, _ in shapes]
+    engine.knorm(kj)
+    print(f"staged energy tiles vs oracle, {precision}", flush=True)
 
-    def run():
-        cj = [engine.CurrentsJob(c1, r1, c2, r2, K, torch.zeros(1, dtype=torch.float64, device="cuda"))
-              for c1, r1, c2, r2 in shapes]
-        engine.currents(cj)
-        mj = [engine.MeasureJob(c1, r1[:, 0].abs().contiguous(), c2 if i != 1 else None,
-                                r2[:, 0].abs().contiguous() if i != 1 else None, K,
-                                torch.zeros(1, dtype=torch.float64, device="cuda"))
-              for i, (c1, r1, c2, r2) in enumerate(shapes)]
-        engine.measure(mj)
-        kj = [engine.KNormJob(c1, r1, K, torch.zeros(1, dtype=torch.float64, device="cuda"))
-              for c1, r1, _, _ in shapes]
-        engine.knorm(kj)
-        return [j.total.item() for j in cj + mj + kj]
+    def quad(c, w, c2=None, w2=None):
+        """<w, K(c, c2) w2> and the sum of |terms|' bound |w| . K |w2|."""
+        c2 = c if c2 is None else c2
+        w2 = w if w2 is None else w2
+        return (np.sum(w * orc.apply_k(c, c2, w2, 0.2, 10)),
+                np.sum(np.abs(w) * orc.apply_k(c, c2, np.abs(w2), 0.2, 10)))
 
-    print(f"staged energy tiles vs in-kernel staging, {precision}", flush=True)
-    staged = run()
-    monkeypatch.setenv("LDDMM_STAGE", "0")
-    built = run()
-    assert staged == built
-    assert all(np.isfinite(staged)) and all(v != 0.0 for v in staged)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    for i, (c1, r1, c2, r2) in enumerate(host):
+        s, ms = quad(c1, r1)
+        x, mx = quad(c1, r1, c2, r2)
+        assert abs(cj[i].total.item() - (s - 2 * x)) <= tol * (ms + 2 * mx)
+        a1, a2 = np.abs(r1[:, :1]), np.abs(r2[:, :1])
+        s, ms = quad(c1, a1)
+        x, mx = quad(c1, a1, c2, a2) if i != 1 else (0.0, 0.0)
+        assert abs(mj[i].total.item() - (s - 2 * x)) <= tol * (ms + 2 * mx)
+        s, ms = quad(c1, r1)
+        assert abs(kj[i].total.item() - s) <= tol * ms
