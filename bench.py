@@ -78,7 +78,7 @@ def registration(cfg, prec):
                                 sigmaError=cfg["sigmaError"], errorType='current')
     F, V = cfg["template"]
     _, Y = cfg["target"]
-    return sm.SurfaceMatching(Template=surfaces.Surface(FV=(F, V)), Target=surfaces.Surface(FV=(F, Y)),
+    return sm.SurfaceMatching(outputDir=None, Template=surfaces.Surface(FV=(F, V)), Target=surfaces.Surface(FV=(F, Y)),
                               param=p, maxIter=cfg["maxIter"], testGradient=False)
 
 

@@ -66,7 +66,7 @@ class DeviceMatching:
     def _init_device(self, x0_np, faces_np, target_centers, target_cr, KparDiff, KparDist,
                      sigmaError, Tsize, regweight, target_points=None, data_term=CURRENT,
                      affine='none', affineWeight=1.0, rotWeight=None, scaleWeight=None,
-                     transWeight=None):
+                     transWeight=None, template_geometry=None):
         """Device state.  With ``target_points`` (N, d) the data term is the landmark one,
         sum |x_T - y|^2 / sigmaError^2 over corresponding points, and the shape arguments
         are unused; otherwise it is the currents (data_term 'current'), measure ('measure') or
@@ -74,7 +74,9 @@ class DeviceMatching:
         target_cr).  ``affine`` adds the reference's global affine part A_t x + b_t to the
         velocity, with coefficients Afft (T, affineDim) on an AffineBasis, penalised by
         sum_t dt <affineWeight, Afft_t^2>; Afft and the directions' affine parts are device
-        tensors."""
+        tensors.  ``template_geometry`` (centers, surfels of the template as given) makes the
+        registration symmetric: the template's points x0 become a variable and the data term
+        adds |S(x0) - S0|^2 / sigmaError^2 (see the template-term methods below)."""
         dev = rt.device()
         self.npt, self.dim = x0_np.shape
         self.Tsize = Tsize
@@ -183,16 +185,20 @@ class DeviceMatching:
         self.nobj = 0
         self.ngrad = 0
         self.iter = 0
+        self.sym = None
+        if template_geometry is not None:
+            self.sym = _TemplateTerm(self, *template_geometry)
 
-    def _data_record(self, sh, K, gradient=False):
+    def _data_record(self, sh, K, gradient=False, target=None):
         """Descriptor of the data term of device shape ``sh`` against the target (outputs
-        unset)."""
+        unset); ``target`` = (centers, surfels, weights) replaces the registration's target."""
+        c2, cr2, a2 = (self.c2, self.cr2, getattr(self, "a2", None)) if target is None else target
         if self.data_kind == CURRENT:
-            return engine.currents_record(sh.centers, sh.surfel, self.c2, self.cr2, K, self.coef)
+            return engine.currents_record(sh.centers, sh.surfel, c2, cr2, K, self.coef)
         if self.data_kind == VARIFOLD:
-            return engine.varifold_record(sh.centers, sh.weights, sh.surfel, self.c2, self.a2,
-                                          self.cr2, K, self.coef)
-        return engine.measure_record(sh.centers, sh.weights, self.c2, self.a2, K, self.coef,
+            return engine.varifold_record(sh.centers, sh.weights, sh.surfel, c2, a2, cr2, K,
+                                          self.coef)
+        return engine.measure_record(sh.centers, sh.weights, c2, a2, K, self.coef,
                                      n1=sh.surfel if gradient else None)
 
     # ----- global affine part -----------------------------------------------------------
@@ -221,6 +227,8 @@ class DeviceMatching:
         return self.obj
 
     def getVariable(self):
+        if self.sym is not None:
+            return [self.at, self.Afft, self.x0]
         return [self.at, self.Afft]
 
     def updateTry(self, dir, eps, objRef=None):
@@ -241,6 +249,10 @@ class DeviceMatching:
         # registrations) before the next request that reads it.
         if self.fvDef_dev is not None:
             self._def_stale = True
+        if self.sym is not None:
+            # the kept trial's template points are slice 0 of its trajectory
+            self.x0 = self.xt[0].clone()
+            self.sym.stale = True
 
     def endOfIteration(self):
         self.iter += 1
@@ -251,12 +263,16 @@ class DeviceMatching:
         d = Direction()
         d.diff = dir1.diff + beta * dir2.diff
         d.aff = dir1.aff + beta * dir2.aff
+        if self.sym is not None:
+            d.initx = dir1.initx + beta * dir2.initx
         return d
 
     def copyDir(self, dir0):
         d = Direction()
         d.diff = dir0.diff.clone()
         d.aff = dir0.aff.clone() if isinstance(dir0.aff, torch.Tensor) else np.copy(dir0.aff)
+        if self.sym is not None:
+            d.initx = dir0.initx.clone()
         return d
 
     def randomDir(self):
@@ -265,12 +281,105 @@ class DeviceMatching:
                              device=self.x0.device)
         d.aff = (torch.randn((self.Tsize, self.affineDim), dtype=torch.float64, device=self.x0.device)
                  if self.affineDim > 0 else np.zeros([self.Tsize, 0]))
+        if self.sym is not None:
+            d.initx = torch.randn((self.npt, self.dim), dtype=torch.float64, device=self.x0.device)
         return d
 
     # ----- helpers ---------------------------------------------------------------------
     def deformed_vertices(self) -> torch.Tensor:
         return self.xt[-1]
 
+
+
+class _TemplateTerm:
+    """Symmetric matching's second data term (surfaceMatching.py:181-185,232-289,297-327,
+    353-383): the template's points x0 are a variable, and the objective adds
+    |S(x0) - S0|^2 / sigmaError^2, the data-term distance (currents, measure or varifold) from
+    the moved template to the template as given, whose norm |S0|^2 / sigmaError^2 joins obj0.
+    The gradient in x0 is (d|S(x0) - S0|^2/dx0 / sigmaError^2 - p_0) / (coeffInitx coeff), p_0
+    the flow's covector at t = 0, and coeffInitx also weighs the x0 part of the dot product.
+
+    Device state: the moved template S(x0) (geometry refreshed lazily after an accepted
+    trial, as the reference refreshes it in endOfIteration), a trial shape S(x0Try), the
+    original template's geometry (c0, cr0 and the measure weights), the gradient buffers and
+    the descriptors of the energy, gradient and gather launches.  A trial writes its x0Try
+    into slice 0 of its trajectory buffer, so the flow starts there and an accepted trial
+    carries its template points with it."""
+
+    coeffInitx = 0.1
+
+    def __init__(self, o, centers0, cr0):
+        if o.data_kind not in SHAPE_TERMS:
+            raise NotImplementedError("symmetric matching needs a shape data term")
+        dev = rt.device()
+        weights = o.data_kind != CURRENT
+        self.c0 = rt.as_dev(centers0)
+        self.cr0 = rt.as_dev(cr0)
+        self.a0 = self.a0_norm0 = None
+        if weights:
+            self.a0 = rt.as_dev(engine.measure_weights(cr0))
+            self.a0_norm0 = rt.as_dev(engine.measure_weights(cr0, o.measure_norm0_eps))
+        target = (self.c0, self.cr0, self.a0)
+        self.shape = o._new_shape(o.x0, weights)        # S(x0)
+        self.try_shape = o._new_shape(o.x0, weights)    # S(x0Try) of the trial in flight
+        self.stale = False
+        Ke = o.Kdist.energy()
+        self.try_rec = rt.raw(o._data_record(self.try_shape, Ke, target=target))
+        self.try_geom = rt.raw(engine.geometry_record(self.try_shape))
+        F = self.shape.centers.shape[0]
+        N, d = o.npt, o.dim
+        self.z1 = torch.empty((F, d), dtype=torch.float64, device=dev)
+        self.dz = torch.empty((F, d), dtype=torch.float64, device=dev)
+        self.gtot = torch.empty(1, dtype=torch.float64, device=dev)
+        self.px0 = torch.empty((N, d), dtype=torch.float64, device=dev)
+        g = o._data_record(self.shape, o.Kdist, gradient=True, target=target)
+        g["out0"] = self.z1.data_ptr()
+        g["out1"] = self.dz.data_ptr()
+        g["total"] = self.gtot.data_ptr()
+        self.grad_rec = rt.raw(g)
+        # +2/sigma^2 times the reference's vertex scatter: d|S(x0) - S0|^2/dx0 / sigma^2
+        self.gather = rt.raw(engine.gather_record(self.shape, self.z1, self.dz,
+                                                  2.0 / (o.sigmaError ** 2), self.px0.data_ptr()))
+        self.geom = rt.raw(engine.geometry_record(self.shape))
+
+    def norm0_job(self, o, total):
+        """The energy job of |S0|^2 into ``total`` (a 1-element device view)."""
+        Ke = o.Kdist.energy()
+        if o.data_kind == CURRENT:
+            return engine.KNormJob(self.c0, self.cr0, Ke, total)
+        if o.data_kind == MEASURE:
+            return engine.MeasureJob(self.c0, self.a0_norm0, None, None, Ke, total)
+        return engine.VarifoldJob(self.c0, self.a0, self.cr0, None, None, None, Ke, total)
+
+
+def _template_energies(opts, totals) -> None:
+    """|S(x_0) - S0|^2 of symmetric registrations at their current template points into
+    ``totals`` (1-element device views); the moved templates' geometry is refreshed."""
+    for o in opts:
+        o.sym.shape.vertices = o.x0
+        o.sym.stale = False
+    engine.geometry_records(rt.stack([o.sym.geom for o in opts], engine.GEOM_DT),
+                            [o.x0.data_ptr() for o in opts])
+    for idx in _by_key([o._cur_key for o in opts]):
+        g = [opts[i] for i in idx]
+        recs = rt.stack([rt.raw(o._data_record(o.sym.shape, o.Kdist.energy(),
+                                               target=(o.sym.c0, o.sym.cr0, o.sym.a0)))
+                         for o in g])
+        key = g[0]._cur_key
+        engine.currents_records(recs, key[1], key[2], [totals[i].data_ptr() for i in idx],
+                                op=key[3])
+
+
+def _refresh_template(opts) -> None:
+    """Face geometry of the moved templates whose points changed (one launch)."""
+    st = [o for o in opts if o.sym.stale]
+    if not st:
+        return
+    engine.geometry_records(rt.stack([o.sym.geom for o in st], engine.GEOM_DT),
+                            [o.x0.data_ptr() for o in st])
+    for o in st:
+        o.sym.shape.vertices = o.x0
+        o.sym.stale = False
 
 
 # --------------------------------------------------------------------------------------
@@ -364,8 +473,10 @@ def batch_objective(opts: Sequence[DeviceMatching]) -> list:
 def launch_objective(opts: Sequence[DeviceMatching]) -> Pending:
     """batch_objective with its answers pending."""
     dev = rt.device()
-    # per registration: T energies, the data term, the target norm, and T affine penalties
-    lens = [o.Tsize + 2 + (o.Tsize if o.affineDim > 0 else 0) for o in opts]
+    # per registration: T energies, the data term, the target norm, T affine penalties, and
+    # (symmetric) the template term and the template's norm
+    lens = [o.Tsize + 2 + (o.Tsize if o.affineDim > 0 else 0) + (2 if o.sym is not None else 0)
+            for o in opts]
     scal = torch.zeros(sum(lens), dtype=torch.float64, device=dev)
     flows, norms, offs = [], [], []
     off = 0
@@ -397,23 +508,33 @@ def launch_objective(opts: Sequence[DeviceMatching]) -> Pending:
         elif o.data_kind == VARIFOLD:
             vnorms.append(engine.VarifoldJob(o.c2, o.a2, o.cr2, None, None, None,
                                              o.Kdist.energy(), scal[off + T + 1:off + T + 2]))
+    sym = [(o, off + L - 2) for o, off, L in zip(opts, offs, lens) if o.sym is not None]
+    for o, at in sym:
+        job = o.sym.norm0_job(o, scal[at + 1:at + 2])
+        {CURRENT: norms, MEASURE: mnorms, VARIFOLD: vnorms}[o.data_kind].append(job)
     engine.knorm(norms)
     engine.measure(mnorms)
     engine.varifold(vnorms)
+    if sym:
+        _template_energies([o for o, _ in sym], [scal[at:at + 1] for _, at in sym])
 
     def finish(vals):
         out = []
-        for o, off in zip(opts, offs):
+        for o, off, L in zip(opts, offs, lens):
             T = o.Tsize
             s2 = o.sigmaError ** 2
             o.obj0 = vals[off + T + 1] / s2
+            data = vals[off + T] / s2
+            if o.sym is not None:
+                o.obj0 += vals[off + L - 1] / s2
+                data += vals[off + L - 2] / s2
             obj = 0
             dt = 1.0 / T
             for t in range(T):
                 obj = obj + o.regweight * dt * vals[off + t]
                 if o.affineDim > 0:
                     obj += vals[off + T + 2 + t]
-            o.obj = obj + (o.obj0 + vals[off + T] / s2)
+            o.obj = obj + (o.obj0 + data)
             o.nobj += 1
             out.append(o.obj)
         return out
@@ -449,11 +570,21 @@ def launch_update_try(items) -> Pending:
     for i in aff:
         pen_off[i] = end
         end += int(Ts[i])
+    # symmetric registrations: the template term of the trial's x0Try
+    sym = [i for i, o in enumerate(opts) if o.sym is not None]
+    sym_off = {}
+    for i in sym:
+        sym_off[i] = end
+        end += 1
     scal = torch.zeros(end, dtype=torch.float64, device=dev)
     sbase = np.uint64(scal.data_ptr())
     # atTry = at - eps dir for every registration (multi-tensor launches)
     ats = torch._foreach_mul([it[1].diff for it in items], [-float(it[2]) for it in items])
     torch._foreach_add_(ats, [o.at for o in opts])
+    for i in sym:
+        # x0Try = x0 - eps dir.initx into slice 0 of the trial's trajectory
+        o, dir_, eps = items[i][0], items[i][1], items[i][2]
+        torch.add(o.x0, dir_.initx, alpha=-float(eps), out=o._xt_scratch[0])
     xt = np.fromiter((o._xt_scratch.data_ptr() for o in opts), np.uint64, n)
     at = np.fromiter((a.data_ptr() for a in ats), np.uint64, n)
     slice_bytes = np.fromiter((o._slice_bytes for o in opts), np.uint64, n)
@@ -491,6 +622,14 @@ def launch_update_try(items) -> Pending:
     if lmk:
         engine.landmark_batch([opts[i]._xt_scratch[-1] for i in lmk], [opts[i].y for i in lmk],
                               [int(data_total[i]) for i in lmk])
+    if sym:
+        engine.geometry_records(rt.stack([opts[i].sym.try_geom for i in sym], engine.GEOM_DT),
+                                xt[sym])
+        for idx in _by_key([opts[i]._cur_key for i in sym]):
+            g = [sym[i] for i in idx]
+            key = opts[g[0]]._cur_key
+            engine.currents_records(rt.stack([opts[i].sym.try_rec for i in g]), key[1], key[2],
+                                    [int(sbase) + 8 * sym_off[i] for i in g], op=key[3])
 
     def finish(host):
         vals = host.tolist()
@@ -506,7 +645,10 @@ def launch_update_try(items) -> Pending:
                 if pen is not None:
                     objDef += pen[t]
             objTry += objDef
-            objTry += vals[off + T] / (o.sigmaError ** 2)
+            if i in sym_off:
+                objTry += vals[off + T] / (o.sigmaError ** 2) + vals[sym_off[i]] / (o.sigmaError ** 2)
+            else:
+                objTry += vals[off + T] / (o.sigmaError ** 2)
             o.nobj += 1
             if math.isnan(objTry):
                 out.append(float('inf'))
@@ -545,6 +687,16 @@ def batch_gradient(items) -> list:
         engine.gather_records(rt.stack([opts[i]._gather for i in cur], engine.GATHER_DT),
                               [opts[i].xt.data_ptr() + opts[i].Tsize * opts[i]._slice_bytes
                                for i in cur])
+    sym = [o for o in opts if o.sym is not None]
+    if sym:
+        # d|S(x0) - S0|^2/dx0 / sigmaError^2 into each template term's px0
+        _refresh_template(sym)
+        for idx in _by_key([o._cur_grad_key for o in sym]):
+            g = [sym[i] for i in idx]
+            key = g[0]._cur_grad_key
+            rt.launch(key[3], key[1], key[2], rt.stack([o.sym.grad_rec for o in g]))
+        engine.gather_records(rt.stack([o.sym.gather for o in sym], engine.GATHER_DT),
+                              [o.x0.data_ptr() for o in sym])
     lmk = [o for o in opts if o.data_kind == LANDMARK]
     if lmk:
         # px1 = -2 (x_T - y) / sigmaError^2
@@ -577,6 +729,8 @@ def batch_gradient(items) -> list:
         g = Direction()
         g.diff = dat
         g.aff = _affine_gradient(o, coeff) if o.affineDim > 0 else np.zeros(o.Afft.shape)
+        if o.sym is not None:
+            g.initx = (o.sym.px0 - o._pxt[0]) / (o.sym.coeffInitx * coeff)
         o.ngrad += 1
         out.append(g)
     return out
@@ -631,6 +785,16 @@ def launch_dot(items) -> Pending:
             for q, g in enumerate(items[i][2]):
                 abuf[first[i]:first[i] + Ts[i], q] = (a1 * g.aff).sum(dim=1)
         buf = torch.cat([buf, abuf], dim=1)
+    sym = [i for i, o in enumerate(opts) if o.sym is not None]
+    sym_row = {}
+    if sym:
+        # x0 parts <g1.initx, g2.initx> (surfaceMatching.py:373-376), one row per registration
+        sbuf = torch.zeros((len(sym), buf.shape[1]), dtype=torch.float64, device=dev)
+        for r, i in enumerate(sym):
+            sym_row[i] = int(Ts.sum()) + r
+            for q, g in enumerate(items[i][2]):
+                sbuf[r, q] = (items[i][1].initx * g.initx).sum()
+        buf = torch.cat([buf, sbuf], dim=0)
 
     def finish(allv):
         out = []
@@ -646,5 +810,7 @@ def launch_dot(items) -> Pending:
                 out.append(r)
             else:
                 out.append(np.cumsum(v, axis=0)[-1].copy())
+            if i in sym_row:
+                out[-1] = out[-1] + allv[sym_row[i], :na[i]] * opts[i].sym.coeffInitx
         return out
     return Pending(buf, finish)

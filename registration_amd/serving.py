@@ -89,7 +89,7 @@ def build_registration(req: Request, precision: str):
     if req.kind == 'surface':
         p = sm.SurfaceMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, KparDist=Kw,
                                     sigmaError=req.sigmaError, errorType=req.errorType)
-        return sm.SurfaceMatching(Template=surfaces.Surface(FV=req.template),
+        return sm.SurfaceMatching(outputDir=None, Template=surfaces.Surface(FV=req.template),
                                   Target=surfaces.Surface(FV=req.target), param=p,
                                   maxIter=req.maxIter, testGradient=False, affine=req.affine,
                                   affineWeight=req.affineWeight)

@@ -15,7 +15,7 @@ def surface_pair(prec, subdiv=2, T=5):
     p = sm.SurfaceMatchingParam(timeStep=1.0 / T, KparDiff=kfun.Kernel('gauss', sigma=0.5, precision=prec),
                                 KparDist=kfun.Kernel('gauss', sigma=0.3, precision=prec),
                                 sigmaError=0.1, errorType='current')
-    g = sm.SurfaceMatching(Template=sm.surfaces.Surface(FV=(F, V)), Target=sm.surfaces.Surface(FV=(F, Y)),
+    g = sm.SurfaceMatching(outputDir=None, Template=sm.surfaces.Surface(FV=(F, V)), Target=sm.surfaces.Surface(FV=(F, Y)),
                            param=p, testGradient=False)
     o = orc.SurfaceMatching(orc.Surface(FV=(F, V)), orc.Surface(FV=(F, Y)), orc.Kernel('gauss', sigma=0.5),
                             orc.Kernel('gauss', sigma=0.3), sigmaError=0.1, timeStep=1.0 / T)
@@ -218,7 +218,7 @@ def _config1(prec):
                                 KparDiff=kfun.Kernel('gauss', sigma=c["sigmaKernel"], precision=prec),
                                 KparDist=kfun.Kernel('gauss', sigma=c["sigmaDist"], precision=prec),
                                 sigmaError=c["sigmaError"], errorType='current')
-    g = sm.SurfaceMatching(Template=sm.surfaces.Surface(FV=(F, V)), Target=sm.surfaces.Surface(FV=(F, Y)),
+    g = sm.SurfaceMatching(outputDir=None, Template=sm.surfaces.Surface(FV=(F, V)), Target=sm.surfaces.Surface(FV=(F, Y)),
                            param=p, maxIter=c["maxIter"], testGradient=False)
     return g, c
 
@@ -308,7 +308,7 @@ def _affine_pair(kind, prec, affine, T=4):
         Y = 1.15 * synthetic.bump_target(V) @ R.T + 0.1
         p = sm.SurfaceMatchingParam(timeStep=1.0 / T, KparDiff=Kd, KparDist=Kw, sigmaError=0.1,
                                     errorType='current')
-        g = sm.SurfaceMatching(Template=sm.surfaces.Surface(FV=(F, V)), Target=sm.surfaces.Surface(FV=(F, Y)),
+        g = sm.SurfaceMatching(outputDir=None, Template=sm.surfaces.Surface(FV=(F, V)), Target=sm.surfaces.Surface(FV=(F, Y)),
                                param=p, testGradient=False, affine=affine, affineWeight=0.5,
                                rotWeight=2.0)
         o = orc.SurfaceMatching(orc.Surface(FV=(F, V)), orc.Surface(FV=(F, Y)), orc.Kernel('gauss', sigma=0.5),

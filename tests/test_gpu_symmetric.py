@@ -1,6 +1,8 @@
 """Symmetric surface matching (SurfaceMatching(..., symmetric=True), surfaceMatching.py:74-484)
 against the oracle's restatement: the per-iteration objective sequence of the CG run, without
 and with an affine part.  FP64 mode 1e-9 relative; FP32-compensated mode 1e-4."""
+import os
+
 import numpy as np
 import pytest
 
@@ -38,8 +40,10 @@ def test_symmetric_cg_sequence_fp64(tmp_path, affine):
     hg, ho = np.array(g.history), np.array(o.history)
     assert len(hg) == len(ho) and len(hg) >= 4, (hg, ho)
     assert np.all(np.abs(hg - ho) <= 1e-9 * np.abs(ho)), (hg, ho)
-    assert np.abs(g.x0 - o.x0).max() <= 1e-8 * np.abs(o.x0).max()
-    assert np.abs(g.x0 - g.fv0.vertices).max() > 0  # the template's points moved
+    gx0 = g.x0.cpu().numpy()
+    assert np.abs(gx0 - o.x0).max() <= 1e-8 * np.abs(o.x0).max()
+    assert np.abs(gx0 - g.fv0.vertices).max() > 0  # the template's points moved
+    assert os.path.exists(os.path.join(str(tmp_path), 'Template.vtk'))
 
 
 def test_symmetric_cg_fp32c(tmp_path):
@@ -60,5 +64,5 @@ def test_non_symmetric_stays_on_the_batched_driver(tmp_path):
                                 KparDist=kfun.Kernel('gauss', sigma=0.4), sigmaError=0.2,
                                 errorType='current')
     g = sm.SurfaceMatching(Template=surfaces.Surface(FV=(F, V)), Target=surfaces.Surface(FV=(F, 1.1 * V)),
-                           param=p, symmetric=False)
+                           param=p, symmetric=False, outputDir=None)
     assert type(g) is sm.SurfaceMatching

@@ -56,7 +56,7 @@ def _surface_pair(prec, T=5, subdiv=2):
     p = sm.SurfaceMatchingParam(timeStep=1.0 / T, KparDiff=kfun.Kernel('gauss', sigma=0.5, precision=prec),
                                 KparDist=kfun.Kernel('gauss', sigma=0.3, precision=prec),
                                 sigmaError=0.1, errorType='varifold')
-    g = sm.SurfaceMatching(Template=sm.surfaces.Surface(FV=(F, V)),
+    g = sm.SurfaceMatching(outputDir=None, Template=sm.surfaces.Surface(FV=(F, V)),
                            Target=sm.surfaces.Surface(FV=(F, Y)), param=p, testGradient=False)
     o = orc.SurfaceMatching(orc.Surface(FV=(F, V)), orc.Surface(FV=(F, Y)),
                             orc.Kernel('gauss', sigma=0.5), orc.Kernel('gauss', sigma=0.3),
