@@ -100,7 +100,7 @@ def build_registration(req: Request, precision: str):
     if req.kind == 'curve':
         p = cm.CurveMatchingParam(timeStep=1.0 / req.T, KparDiff=Kd, KparDist=Kw,
                                   sigmaError=req.sigmaError, errorType=req.errorType)
-        return cm.CurveMatching(Template=curves.Curve(FV=req.template),
+        return cm.CurveMatching(outputDir=None, Template=curves.Curve(FV=req.template),
                                 Target=curves.Curve(FV=req.target), param=p, maxIter=req.maxIter,
                                 affine=req.affine, affineWeight=req.affineWeight)
     raise ValueError(f"unknown request kind {req.kind!r}")
@@ -263,11 +263,17 @@ class Server:
         self.max_batch = max(self.max_batch, len(still))
         self.rounds += 1
 
-    def run(self, idle_sleep: float = 1e-4) -> dict:
-        """Serve until every submitted request is finished; returns {rid: Result}."""
+    def run(self, idle_sleep: float = 1e-4, progress=None, progress_every: float = 30.0) -> dict:
+        """Serve until every submitted request is finished; returns {rid: Result}.
+        ``progress(finished, active, queued)`` is called about every ``progress_every``
+        seconds of the service clock."""
         if self.t0 is None:
             self.t0 = self.clock()
+        next_report = progress_every
         while self.queue or self.cohorts[0] or self.cohorts[1]:
+            if progress is not None and self._now() >= next_report:
+                progress(len(self.results), len(self.active), len(self.queue))
+                next_report += progress_every
             self._admit()
             if not (self.cohorts[0] or self.cohorts[1]):
                 wait = self.queue[0].arrival - self._now()

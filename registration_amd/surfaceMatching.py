@@ -108,10 +108,13 @@ class SurfaceMatching(DeviceMatching):
         v1 = self.fv1.surfVolume()
         if v0 * v1 < 0:
             self.fv1.flipFaces()
-        self.fvInit = surfaces.Surface(surf=self.fv0)
-        self.fvDef = surfaces.Surface(surf=self.fvInit)
-        self.fv0ori = -1 if self.fvInit.surfVolume() < 0 else 1
-        self.fv1ori = -1 if self.fv1.surfVolume() < 0 else 1
+            v1 = -v1  # flipping every face negates each determinant
+        # copies of the template whose geometry was just computed from its vertices
+        self.fvInit = self.fv0.clone()
+        self.fvDef = self.fv0.clone()
+        # orientations of the (copied) template and of the target after the flip
+        self.fv0ori = -1 if v0 < 0 else 1
+        self.fv1ori = -1 if v1 < 0 else 1
         x0 = np.copy(self.fvInit.vertices)
         Tsize = int(round(1.0 / self.param.timeStep))
         self._init_device(x0, self.fvInit.faces, self.fv1.centers, self.fv1.surfel,
@@ -138,6 +141,97 @@ class SurfaceMatching(DeviceMatching):
     def _new_shape(self, vertices, with_weights=False):
         return surfaces.DeviceSurface(self.faces_np, vertices, with_weights=with_weights)
 
+
+    def endOfIteration(self):
+        """surfaceMatching.py:386-471: the affine scaling after the burn-in and, every
+        saveRate iterations when files are written, the snapshot (``_snapshot``)."""
+        DeviceMatching.endOfIteration(self)
+        if self.outputDir is not None and self.saveRate > 0 and self.iter % self.saveRate == 0:
+            self._snapshot()
+
+    def _snapshot(self):
+        """The reference's saveRate output: (1) without an affine part, geodesic shooting from
+        (x0, a_0) and its end point against the flow's ('EPDiff difference' = sum |x_T -
+        x_T^EPDiff|, kept in ``epdiffDifference``; saveFile + 'EPDiff.vtk'); (2) the flow with
+        its log-Jacobian J_t (kept in ``jacobian``, (T+1, N)); (3) for 'euclidean' or
+        'translation', the trajectory and target in the frame that undoes the affine flow
+        (saveFile + 'Corrected<t>.vtk', 'TargetCorrected.vtk'); (4) the deformed template at
+        every step with the fields Jacobian (e^J - 1), Jacobian_T (vertex-area ratio - 1),
+        Jacobian_N, displacement (the normal velocity integrated in time) and velocity
+        (saveFile + '<t>.vtk').  Kernel sums on the GPU; the fields in host NumPy."""
+        logging.info('Saving surfaces...')
+        K = self.param.KparDiff
+        T, N, d = self.Tsize, self.npt, self.dim
+        dt = 1.0 / T
+        out = lambda name: os.path.join(self.outputDir, name)
+        x0 = self.x0.cpu().numpy()
+        at = self.at
+        xt = self.xt.cpu().numpy()
+        # velocities K(x_t) a_t of the accepted trajectory (the reference's self.v; slice T
+        # stays zero, as there)
+        v = np.zeros((T + 1, N, d))
+        for t in range(T):
+            v[t] = kfun.apply_k_dev(self.xt[t], self.xt[t], at[t], K).cpu().numpy()
+        if self.affineDim <= 0:
+            xe, _ = evol.landmarkEPDiff(T, self.x0, at[0].contiguous(), K)
+            xe = xe.cpu().numpy()
+            self.fvDef.updateVertices(xe[-1])
+            self.fvDef.saveVTK(out(self.saveFile + 'EPDiff.vtk'))
+            self.epdiffDifference = float(np.fabs(xt[-1] - xe[-1]).sum())
+            logging.info('EPDiff difference %f' % self.epdiffDifference)
+        self.fvDef.updateVertices(xt[-1])
+        self.fvInit.updateVertices(x0)
+        if self.affineDim > 0:
+            Afft = self.Afft.cpu().numpy()
+            A = self.affB.getTransforms(Afft)
+        else:
+            Afft = np.zeros((T, 0))
+            A = [np.zeros((T, d, d)), np.zeros((T, d))]
+        xj, Jt = evol.landmarkDirectEvolutionEuler(self.x0, at, K, affine=A, withJacobian=True)
+        xj, Jt = xj.cpu().numpy(), Jt.cpu().numpy()
+        self.jacobian = Jt
+        at_np = at.cpu().numpy()
+        if self.affine in ('euclidean', 'translation'):
+            f = self.fvInit.clone()
+            X = self.affB.integrateFlow(Afft)
+            displ = np.zeros(N)
+            for t in range(T + 1):
+                U = np.linalg.inv(X[0][t])
+                yyt = (xt[t] - X[1][t]) @ U.T
+                zt = (xj[t] - X[1][t]) @ U.T
+                if t < T:
+                    vt = K.applyK(yyt, at_np[t] @ U.T, firstVar=zt)
+                f.updateVertices(yyt)
+                vf = surfaces.vtkFields()
+                vf.scalars += ['Jacobian', np.exp(Jt[t]) - 1, 'displacement', displ]
+                vf.vectors += ['velocity', vt]
+                nu = self.fv0ori * f.computeVertexNormals()
+                displ = displ + dt * (vt * nu).sum(axis=1)
+                f.saveVTK2(out(self.saveFile + 'Corrected' + str(t) + '.vtk'), vf)
+            f = self.fv1.clone()
+            f.updateVertices((f.vertices - X[1][-1]) @ U.T)
+            f.saveVTK(out('TargetCorrected.vtk'))
+        fvDef = self.fvInit.clone()
+        AV0 = fvDef.computeVertexArea()[0]
+        nu = self.fv0ori * self.fvInit.computeVertexNormals()
+        vel = v[0]
+        displ = np.zeros(N)
+        for kk in range(T + 1):
+            fvDef.updateVertices(xj[kk])
+            AV = fvDef.computeVertexArea()[0] / AV0 - 1
+            vf = surfaces.vtkFields()
+            vf.scalars += ['Jacobian', np.exp(Jt[kk]) - 1, 'Jacobian_T', AV[:, 0],
+                           'Jacobian_N', np.exp(Jt[kk]) / (AV[:, 0] + 1) - 1,
+                           'displacement', displ]
+            displ = displ + dt * (vel * nu).sum(axis=1)
+            if kk < T:
+                nu = self.fv0ori * self.fvDef.computeVertexNormals()
+                vel = v[kk]
+                kkm = kk
+            else:
+                kkm = kk - 1
+            vf.vectors += ['velocity', v[kkm]]
+            fvDef.saveVTK2(out(self.saveFile + str(kk) + '.vtk'), vf)
 
     def endOptim(self):
         self.fvDef.updateVertices(self.xt[-1].cpu().numpy())

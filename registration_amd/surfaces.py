@@ -67,9 +67,62 @@ class Surface:
         self.computeCentersAreas()
 
     def surfVolume(self):
-        """Signed volume sum_f det[x1; x2; x3] / 6 (surfaces.py:711-717)."""
+        """Signed volume sum_f det[x1; x2; x3] / 6 (surfaces.py:711-717), each determinant as
+        the scalar triple product x1 . (x2 x x3)."""
         v = self.vertices[self.faces]
-        return float(np.linalg.det(v).sum() / 6)
+        return float(np.einsum('ij,ij->i', v[:, 0], np.cross(v[:, 1], v[:, 2])).sum() / 6)
+
+    def computeVertexArea(self):
+        """(AV (nv, 1), AF (nf, 1)): face areas and the area each vertex collects from its
+        faces (surfaces.py:77-126): an obtuse face gives half its area to the obtuse vertex
+        and a quarter to each other one; a non-obtuse face gives each vertex its Voronoi part,
+        (|e1|^2 cot + |e2|^2 cot') / 8 over the two edges that meet there."""
+        V, F = self.vertices, self.faces
+        x12 = V[F[:, 1]] - V[F[:, 0]]
+        x13 = V[F[:, 2]] - V[F[:, 0]]
+        x23 = V[F[:, 2]] - V[F[:, 1]]
+        n12, n13, n23 = (np.sqrt((e ** 2).sum(axis=1)) for e in (x12, x13, x23))
+        c1 = (x12 * x13).sum(axis=1) / (n12 * n13)
+        c2 = -(x12 * x23).sum(axis=1) / (n12 * n23)
+        c3 = (x13 * x23).sum(axis=1) / (n13 * n23)
+        AF = np.sqrt((np.cross(x12, x13) ** 2).sum(axis=1)) / 2
+        share = np.empty((F.shape[0], 3))
+        ob1 = c1 < 0
+        ob2 = ~ob1 & (c2 < 0)
+        ob3 = ~ob1 & ~ob2 & (c3 < 0)
+        acute = ~(ob1 | ob2 | ob3)
+        for ob, k in ((ob1, 0), (ob2, 1), (ob3, 2)):
+            share[ob] = (AF[ob] / 4)[:, None]
+            share[ob, k] = AF[ob] / 2
+        with np.errstate(divide='ignore', invalid='ignore'):
+            cot1, cot2, cot3 = (c / np.sqrt(1 - c ** 2) for c in (c1, c2, c3))
+        q12, q13, q23 = n12 ** 2, n13 ** 2, n23 ** 2
+        share[acute, 0] = ((q12 * cot3 + q13 * cot2) / 8)[acute]
+        share[acute, 1] = ((q12 * cot3 + q23 * cot1) / 8)[acute]
+        share[acute, 2] = ((q13 * cot2 + q23 * cot1) / 8)[acute]
+        AV = np.zeros(V.shape[0])
+        for j in range(3):
+            np.add.at(AV, F[:, j], share[:, j])
+        return AV[:, None], AF[:, None]
+
+    def computeVertexNormals(self):
+        """Unit vertex normals: the normalised sum of the area-weighted normals of the faces
+        around each vertex (surfaces.py:128-139)."""
+        self.computeCentersAreas()
+        normals = np.zeros(self.vertices.shape)
+        for j in range(3):
+            np.add.at(normals, self.faces[:, j], self.surfel)
+        return normals / np.sqrt((normals ** 2).sum(axis=1))[:, None]
+
+    def clone(self):
+        """A copy whose centers and surfels are copied, not recomputed: for a surface whose
+        geometry is known to match its vertices (Surface(surf=...) recomputes it)."""
+        out = Surface.__new__(Surface)
+        out.vertices = np.copy(self.vertices)
+        out.faces = np.copy(self.faces)
+        out.centers = np.copy(self.centers)
+        out.surfel = np.copy(self.surfel)
+        return out
 
     # ---- files (surfaces.py:720-985) --------------------------------------------------
     def _read(self, reader, path):

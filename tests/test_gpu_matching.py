@@ -90,7 +90,7 @@ def curve_pair(prec):
     p = cm.CurveMatchingParam(timeStep=0.2, KparDiff=kfun.Kernel('laplacian', sigma=0.2, order=1, precision=prec),
                               KparDist=kfun.Kernel('gauss', sigma=0.1, precision=prec), sigmaError=0.1,
                               errorType='current')
-    g = cm.CurveMatching(Template=curves.Curve(pointSet=X), Target=curves.Curve(pointSet=Y), param=p,
+    g = cm.CurveMatching(outputDir=None, Template=curves.Curve(pointSet=X), Target=curves.Curve(pointSet=Y), param=p,
                          maxIter=6)
     o = orc.CurveMatching(orc.Curve(pointSet=X), orc.Curve(pointSet=Y), orc.Kernel('laplacian', sigma=0.2, order=1),
                           orc.Kernel('gauss', sigma=0.1), sigmaError=0.1, timeStep=0.2, maxIter=6)
@@ -320,7 +320,7 @@ def _affine_pair(kind, prec, affine, T=4):
         Y = 1.2 * synthetic.star_curve(150, 4, 0.2) + 0.1
         p = cm.CurveMatchingParam(timeStep=1.0 / T, KparDiff=Kd, KparDist=Kw, sigmaError=0.1,
                                   errorType='current')
-        g = cm.CurveMatching(Template=cm.curves.Curve(FV=(E, X)), Target=cm.curves.Curve(FV=(E, Y)),
+        g = cm.CurveMatching(outputDir=None, Template=cm.curves.Curve(FV=(E, X)), Target=cm.curves.Curve(FV=(E, Y)),
                              param=p, affine=affine, affineWeight=0.5)
         o = orc.CurveMatching(orc.Curve(FV=(E, X)), orc.Curve(FV=(E, Y)), orc.Kernel('gauss', sigma=0.5),
                               orc.Kernel('gauss', sigma=0.3), sigmaError=0.1, timeStep=1.0 / T,

@@ -103,7 +103,7 @@ def _curve_pair(prec, T=5, n=200):
     p = cm.CurveMatchingParam(timeStep=1.0 / T, KparDiff=kfun.Kernel('gauss', sigma=0.3, precision=prec),
                               KparDist=kfun.Kernel('gauss', sigma=0.1, precision=prec),
                               sigmaError=0.1)
-    g = cm.CurveMatching(Template=cm.curves.Curve(FV=(E, X)), Target=cm.curves.Curve(FV=(E, Y)),
+    g = cm.CurveMatching(outputDir=None, Template=cm.curves.Curve(FV=(E, X)), Target=cm.curves.Curve(FV=(E, Y)),
                          param=p)
     o = orc.CurveMatching(orc.Curve(FV=(E, X)), orc.Curve(FV=(E, Y)), orc.Kernel('gauss', sigma=0.3),
                           orc.Kernel('gauss', sigma=0.1), sigmaError=0.1, timeStep=1.0 / T,

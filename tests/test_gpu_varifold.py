@@ -99,5 +99,5 @@ def test_varifold_rejected_for_curves():
     with pytest.raises(NotImplementedError):
         p = cm.CurveMatchingParam(KparDiff=kfun.Kernel('gauss', sigma=0.3),
                                   KparDist=kfun.Kernel('gauss', sigma=0.1), errorType='varifold')
-        cm.CurveMatching(Template=cm.curves.Curve(FV=(E, X)), Target=cm.curves.Curve(FV=(E, X)),
+        cm.CurveMatching(outputDir=None, Template=cm.curves.Curve(FV=(E, X)), Target=cm.curves.Curve(FV=(E, X)),
                          param=p)
