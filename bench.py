@@ -65,14 +65,19 @@ def parse():
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the flow-step launches and a calibration copy (for rocprof passes)")
     ap.add_argument("--energy-only", action="store_true",
-                    help="only the FP64 currents-energy launches of the data_terms leg (for "
-                         "rocprof passes)")
+                    help="only the FP64 energy launches of the data_terms leg (for rocprof "
+                         "passes)")
+    ap.add_argument("--cpu-only", metavar="BENCH_JSON",
+                    help="only the CPU baseline, extrapolated with the per-operation pair "
+                         "counts of an earlier bench line (no GPU used)")
     return ap.parse_args()
 
 
 def make_workload(name):
     from registration_amd import synthetic
-    return synthetic.surface_pair_config2() if name == "surface-s6" else synthetic.surface_pair_config1()
+    cfg = synthetic.surface_pair_config2() if name == "surface-s6" else synthetic.surface_pair_config1()
+    cfg["name"] = name
+    return cfg
 
 
 def registration(cfg, prec):
@@ -143,6 +148,12 @@ def flow_step_roofline(cfg, prec, reps, natural=True):
 # kernel value 3d+1 = 10, plus 2m for the weighted sum (currents m = 3, measure m = 1) or, for
 # the varifold, g = <u_k, u_l> (5), g^2 (1), K a_l (1) and a_l K (1 + g^2) accumulated (3).
 DATA_TERM_FLOPS = {"currents": 16, "measure": 12, "varifold": 20}
+# FP64 FLOP one energy launch evaluates on the large surface pair (81,920 x 81,920 triangle
+# pairs per source set), read with rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP64 on
+# `bench.py --energy-only --data-term-reps 3` (profiles/currents_energy_s6_fp64.md).  Fewer
+# than the nominal pairs x flops: far tiles are culled, each unordered block pair of the self
+# sum is evaluated once, and far blocks take the FP32 exponential.
+EVALUATED_FP64_FLOP_S6 = {"currents": 5.813e10, "measure": 4.686e10, "varifold": 1.011e11}
 
 
 def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradients=True):
@@ -189,15 +200,17 @@ def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradien
     for name in names:
         op = ops[name]
         ms = timed(rt.Plan(op, _lib.PREC_FP64, 3, [rt.seg(**base, **weights[name])]))
-        tf = DATA_TERM_FLOPS[name] * pairs / (ms * 1e-3) / 1e12
-        # Nominal: every one of the 2 F^2 pairs counted, although tile culling skips far
-        # blocks, the self sum evaluates each unordered block pair once and FP64 far blocks
-        # use the FP32 exponential, so these rates exceed the evaluated FP64 work.
-        out[f"{name}_energy_fp64"] = {"ms": round(ms, 4), "nominal_pairs": pairs,
-                                      "nominal_pairs_per_s": pairs / (ms * 1e-3),
-                                      "flops_per_pair": DATA_TERM_FLOPS[name],
-                                      "nominal_tflops": round(tf, 3),
-                                      "fp64_peak_tflops": FP64_PEAK_TFLOPS}
+        # nominal: all 2 F^2 pairs of the two sums (target-target and target-template)
+        rec = {"ms": round(ms, 4), "nominal_pairs": pairs,
+               "nominal_pairs_per_s": pairs / (ms * 1e-3),
+               "flops_per_pair": DATA_TERM_FLOPS[name]}
+        flop = EVALUATED_FP64_FLOP_S6.get(name) if cfg.get("name") == "surface-s6" else None
+        if flop is not None:
+            tf = flop / (ms * 1e-3) / 1e12
+            rec.update({"evaluated_fp64_flop": flop, "evaluated_fp64_tflops": round(tf, 3),
+                        "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                        "frac_of_fp64_peak": round(tf / FP64_PEAK_TFLOPS, 4)})
+        out[f"{name}_energy_fp64"] = rec
         if not gradients:
             continue
         ms = timed(rt.Plan(op, _lib.PREC_FP32C, 3,
@@ -218,47 +231,79 @@ def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradien
     return out
 
 
-def cpu_baseline(cfg, pair_counts, seconds_target=10.0):
-    """FP64 CPU oracle (C/OpenMP restatement of the reference's Fortran sums) timed on a row
-    sample of the workload's two sum kinds; registrations/s extrapolated with the pair counts
-    of the GPU run."""
+def _host():
+    """The CPU this runs on: model name, logical CPUs, and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0))}
+
+
+def _omp_threads(n=None):
+    """Set (n given) and return the OpenMP thread count of the CPU sums' parallel loops."""
+    import ctypes
+    gomp = ctypes.CDLL("libgomp.so.1")
+    if n is not None:
+        gomp.omp_set_num_threads(int(n))
+    return int(gomp.omp_get_max_threads())
+
+
+def _cpu_rates(cfg, rows, sums="oracle"):
+    """Pairs/s of the two CPU sum kinds (applyK and applyDiffKT with three momentum sets) on
+    the first ``rows`` template points against all of them, and the seconds taken."""
     import numpy as np
     from oracle import lddmm_oracle as orc
-    from registration_amd import _lib
     F, V = cfg["template"]
     N = V.shape[0]
+    rows = min(N, rows)
     rng = np.random.default_rng(1)
     a = 1e-2 * rng.standard_normal(V.shape)
-    rows = min(N, 8192)
-    t0 = time.perf_counter()
-    orc.apply_k(V[:rows], V, a, cfg["sigmaKernel"], 10)
-    tk = time.perf_counter() - t0
-    k_rate = rows * N / tk
     a1 = rng.standard_normal((3, rows, 3))
     a2 = rng.standard_normal((3, N, 3))
     t0 = time.perf_counter()
-    orc.apply_diff_kt(V[:rows], V, a1, a2, cfg["sigmaKernel"], 10)
+    orc.apply_k(V[:rows], V, a, cfg["sigmaKernel"], 10, sums=sums)
+    tk = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    orc.apply_diff_kt(V[:rows], V, a1, a2, cfg["sigmaKernel"], 10, sums=sums)
     td = time.perf_counter() - t0
-    d_rate = rows * N / td
+    return rows * N / tk, rows * N / td, rows, tk + td
+
+
+def cpu_baseline(cfg, pair_counts):
+    """FP64 CPU oracle (C/OpenMP restatement of the reference's Fortran sums) timed on a row
+    sample of the workload's two sum kinds, on all usable cores and on one; registrations/s
+    extrapolated with the per-operation pair counts of the GPU run.  Where the reference's
+    own Fortran sums are built (oracle/_ref), they are timed the same way."""
+    from oracle import lddmm_oracle as orc
+    from registration_amd import _lib
+    N = cfg["template"][1].shape[0]
     k_pairs = sum(v for op, v in pair_counts.items() if op != _lib.OP_COVECTOR_STEP)
     d_pairs = pair_counts.get(_lib.OP_COVECTOR_STEP, 0)
-    secs = k_pairs / k_rate + d_pairs / d_rate
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = _omp_threads()
+    kr, dr, rows, took = _cpu_rates(cfg, 8192)
+    secs = k_pairs / kr + d_pairs / dr
     out = {"value": 1.0 / secs, "unit": "registrations/s", "cores": threads, "kind": "port",
            "sample": f"oracle applyK and applyDiffKT(na=3), {rows} x {N} pairs each "
-                     f"({tk + td:.1f} s); extrapolated to the GPU run's {k_pairs + d_pairs:.3e} pairs",
-           "extrapolated": True, "pairs_per_s_applyk": k_rate, "pairs_per_s_diffkt": d_rate,
-           "seconds_per_registration": secs}
+                     f"({took:.1f} s); extrapolated to the GPU run's {k_pairs + d_pairs:.3e} pairs",
+           "extrapolated": True, "host": _host(), "pairs_per_s_applyk": kr,
+           "pairs_per_s_diffkt": dr, "seconds_per_registration": secs}
+    _omp_threads(1)
+    kr1, dr1, rows1, took1 = _cpu_rates(cfg, 1024)
+    secs1 = k_pairs / kr1 + d_pairs / dr1
+    out["single_thread"] = {"value": 1.0 / secs1, "cores": 1,
+                            "sample": f"{rows1} x {N} pairs each ({took1:.1f} s)",
+                            "pairs_per_s_applyk": kr1, "pairs_per_s_diffkt": dr1,
+                            "seconds_per_registration": secs1}
+    _omp_threads(threads)
     if orc.ref_available():
-        # The reference's own Fortran sums (kernelFunctions.f90, compiled by oracle/Makefile),
-        # same cores, a smaller row sample: the slower of the two CPU paths.
-        rr = min(N, 2048)
-        t0 = time.perf_counter()
-        orc.apply_k(V[:rr], V, a, cfg["sigmaKernel"], 10, sums="reference")
-        rk = rr * N / (time.perf_counter() - t0)
-        t0 = time.perf_counter()
-        orc.apply_diff_kt(V[:rr], V, a1[:, :rr], a2, cfg["sigmaKernel"], 10, sums="reference")
-        rd = rr * N / (time.perf_counter() - t0)
+        rk, rd, rr, _ = _cpu_rates(cfg, 2048, sums="reference")
         rsecs = k_pairs / rk + d_pairs / rd
         out["reference"] = {"value": 1.0 / rsecs, "unit": "registrations/s", "cores": threads,
                             "kind": "reference",
@@ -267,6 +312,18 @@ def cpu_baseline(cfg, pair_counts, seconds_target=10.0):
                             "pairs_per_s_applyk": rk, "pairs_per_s_diffkt": rd,
                             "seconds_per_registration": rsecs}
     return out
+
+
+def cpu_only(path, workload):
+    """The CPU baseline alone, with the pair counts per operation of the bench line in
+    ``path`` (its "pairs_by_op"); no GPU is touched."""
+    from registration_amd import _lib
+    with open(path) as f:
+        line = json.loads(f.read().strip().splitlines()[-1])
+    names = {v: k for k, v in vars(_lib).items() if k.startswith("OP_") and isinstance(v, int)}
+    inv = {v: k for k, v in names.items()}
+    counts = {inv[k]: v for k, v in line["pairs_by_op"].items()}
+    return cpu_baseline(make_workload(workload), counts)
 
 
 def _served_run(reqs, prec, max_active=256, keep_momenta=False):
@@ -340,7 +397,7 @@ def _served_compare(reqs, prec, label, plain_sample=None, max_active=256, groups
            "plain_registrations_per_s": plain_rate, "plain_wall_s": plain_wall,
            "plain_latency_ms": _percentiles(plat),
            "speedup_served_vs_plain": (n / wall) / plain_rate,
-           "identical_results": same, "turns": srv.rounds,
+           "identical_results": same, "rerun_in_fp64": srv.promoted, "turns": srv.rounds,
            "mean_batch_per_turn": srv.answered / max(srv.rounds, 1), "max_batch": srv.max_batch,
            "served_pair_rates": _pair_rates(counts, wall)}
     if groups is not None:
@@ -395,6 +452,14 @@ def small_pairs_leg(m, prec, plain_sample):
                            plain_sample=plain_sample, max_active=m)
 
 
+class _Iterate:
+    """A final iterate (momenta, no affine part) in the form serving.fp64_twin reads."""
+    affineDim = 0
+
+    def __init__(self, at):
+        self.at = at
+
+
 def mixed_service_leg(k, prec):
     """The first k requests of the mixed-service stream (2,000 requests: 40% closed curves,
     40% surfaces s3-s7, 20% landmark sets 1k-200k points; Gaussian or Laplacian kernels,
@@ -402,6 +467,7 @@ def mixed_service_leg(k, prec):
     service: per-class latency percentiles, and each result's energy checked against the FP64
     form of the same objective at the same final momenta (per class)."""
     import collections as _c
+    import numpy as np
     import torch
     from registration_amd import serving, synthetic
     specs = synthetic.mixed_service_requests(k)
@@ -415,13 +481,17 @@ def mixed_service_leg(k, prec):
     _progress(f"mixed service: {k} requests served in {wall:.1f} s")
     errs = _c.defaultdict(list)
     lat = _c.defaultdict(list)
+    err32 = _c.defaultdict(list)
+    promoted = _c.Counter()
     for r in reqs:
-        m64 = serving.build_registration(r, "fp64")
-        m64.at = res[r.rid].momenta.clone()
-        e64 = m64.objectiveFun()
-        errs[r.kind].append(abs(res[r.rid].energy - e64) / abs(e64))
-        lat[r.kind].append(res[r.rid].latency * 1e3)
-        del m64
+        got = res[r.rid]
+        # an FP64 evaluation of the returned iterate, independent of the service's own
+        e64 = serving.fp64_twin(r, _Iterate(got.momenta)).objectiveFun()
+        errs[r.kind].append(abs(got.energy - e64) / abs(e64))
+        lat[r.kind].append(got.latency * 1e3)
+        if got.fp32c_error == got.fp32c_error:
+            err32[r.kind].append(got.fp32c_error)
+        promoted[r.kind] += got.precision == "fp64" and prec == "fp32c"
     torch.cuda.synchronize()
     all_lat = [res[r.rid].latency * 1e3 for r in reqs]
     return {"workload": "first %d requests of the mixed-service stream (curves 256-8192 pts, "
@@ -430,8 +500,12 @@ def mixed_service_leg(k, prec):
             "requests": k, "served_registrations_per_s": k / wall, "served_wall_s": wall,
             "served_latency_ms": _percentiles(all_lat),
             "classes": {c: {"count": len(v), "max_energy_rel_err_vs_fp64": float(max(v)),
-                            "latency_ms": _percentiles(lat[c])}
+                            "latency_ms": _percentiles(lat[c]),
+                            "rerun_in_fp64": int(promoted[c]),
+                            "fp32c_run_error_max": float(max(err32[c])) if err32[c] else None,
+                            "fp32c_run_error_p50": float(np.median(err32[c])) if err32[c] else None}
                         for c, v in sorted(errs.items())},
+            "precision_guard": {"certify_tol": srv.certify_tol, "rerun_in_fp64": srv.promoted},
             "sizes": [[r.kind, int(r.template[1].shape[0]), r.T, r.typeKernel] for r in reqs],
             "turns": srv.rounds, "mean_batch_per_turn": srv.answered / max(srv.rounds, 1),
             "max_batch": srv.max_batch, "served_pair_rates": _pair_rates(counts, wall)}
@@ -439,6 +513,9 @@ def mixed_service_leg(k, prec):
 
 def main():
     args = parse()
+    if args.cpu_only:
+        print(json.dumps({"cpu_baseline": cpu_only(args.cpu_only, args.workload)}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (registrations "
               "are independent, so several GPUs would each serve their own requests with "
@@ -463,8 +540,7 @@ def main():
         return 0
 
     if args.energy_only:
-        print(json.dumps(data_terms_leg(cfg, args.data_term_reps, names=("currents",),
-                                        gradients=False)))
+        print(json.dumps(data_terms_leg(cfg, args.data_term_reps, gradients=False)))
         return 0
 
     for _ in range(args.warmup):
@@ -508,6 +584,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, {k: v / args.steps for k, v in counts.items()})
         _progress("cpu baseline done")
+    op_names = {v: k for k, v in vars(_lib).items() if k.startswith("OP_") and isinstance(v, int)}
 
     served = {}
     if args.served > 0:
@@ -545,6 +622,7 @@ def main():
         "final_energy": final_obj,
         "energy_rel_err_vs_fp64": energy_err,
         "pairs_per_registration": pairs_per_reg,
+        "pairs_by_op": {op_names[k]: v / args.steps for k, v in sorted(counts.items())},
         "pair_rate_overall": pairs_per_reg / (wall / args.steps),
         "roofline": roof,
         "data_terms": data_terms,

@@ -14,6 +14,16 @@ refilled continuously.
 A registration's result does not depend on its batch neighbours: every segment keeps its own
 chunking and reduction order, so its kernel sums, and therefore its optimiser path, are the
 same alone and in any mix.
+
+Precision guard.  FP32-compensated kernel sums carry per-term rounding (kernel values and
+64-term FP32 runs) that is small against the sum of |terms|; a registration whose momenta
+are large and oscillating makes the velocity a small difference of large sums, and its
+energy error can exceed 1e-6 (a closed curve with 931 points, Laplacian kernel of width
+0.48 and momenta up to 1.5: 1.15e-5).  When the optimiser of an FP32-compensated
+registration stops, its objective is evaluated once more in FP64 at the final iterate; if
+the two differ by more than ``certify_tol`` relative, the registration is run again from
+the start in FP64 and that run is its result.  Both paths (served and plain) apply the same
+rule, so results stay identical alone and in the mix.
 """
 from __future__ import annotations
 
@@ -73,6 +83,10 @@ class Result:
     finished: float
     history: list = field(default_factory=list)
     momenta: Optional[torch.Tensor] = None   # final at (T, N, d) when the server keeps it
+    precision: str = 'fp32c'                 # arithmetic of the run that gave the result
+    energy_fp64: float = float('nan')        # the FP64 objective at the final iterate
+    fp32c_error: float = float('nan')        # |energy - FP64 objective| / |FP64 objective|
+                                             # of the FP32-compensated run (before any re-run)
 
     @property
     def latency(self) -> float:
@@ -120,6 +134,22 @@ def footprint_bytes(req: Request) -> int:
 
 
 _HOST_ONLY = (cg.ACCEPT, cg.END_ITER)
+CERTIFY = 'certify'   # an FP64 objective at the final iterate of an FP32-compensated run
+DEFAULT_CERTIFY_TOL = 5e-7
+
+
+def fp64_twin(req: Request, opt):
+    """An FP64 registration of ``req`` holding ``opt``'s current iterate (momenta and affine
+    coefficients), for evaluating its objective in FP64."""
+    twin = build_registration(req, 'fp64')
+    twin.at = opt.at.clone()
+    if getattr(opt, 'affineDim', 0) > 0:
+        twin.Afft = opt.Afft.clone()
+    return twin
+
+
+def _rel(a, b):
+    return abs(a - b) / abs(b) if b != 0 else abs(a - b)
 
 # Errors of a request's own inputs (shapes, indices, options outside the GPU path), raised
 # by build_registration before any device work; device errors are not caught.
@@ -127,16 +157,22 @@ _REJECT = (ValueError, NotImplementedError, IndexError, TypeError)
 
 
 class _Active:
-    __slots__ = ("req", "opt", "gen", "pending", "admitted")
+    __slots__ = ("req", "opt", "gen", "pending", "admitted", "precision", "twin", "err32")
 
-    def __init__(self, req, opt, admitted):
+    def __init__(self, req, opt, admitted, precision):
         self.req = req
+        self.admitted = admitted
+        self.twin = None
+        self.err32 = float('nan')
+        self.start(opt, precision)
+
+    def start(self, opt, precision):
         self.opt = opt
+        self.precision = precision
         self.gen = opt.cg_steps()
         self.pending = next(self.gen)
         while self.pending[0] in _HOST_ONLY:
             self.pending = self.gen.send(getattr(opt, self.pending[0])())
-        self.admitted = admitted
 
 
 _LAUNCH = {
@@ -145,6 +181,7 @@ _LAUNCH = {
     cg.GRADIENT: lambda regs: matching.launch_gradient([(a.opt, a.pending[1]) for a in regs]),
     cg.DOT: lambda regs: matching.launch_dot([(a.opt, a.pending[1], a.pending[2]) for a in regs]),
     cg.OBJECTIVE: lambda regs: matching.launch_objective([a.opt for a in regs]),
+    CERTIFY: lambda regs: matching.launch_objective([a.twin for a in regs]),
 }
 
 
@@ -153,7 +190,9 @@ class Server:
 
     hbm_cap_bytes: admission cap on the summed footprint of active registrations;
     max_active: cap on concurrently optimised registrations; precision: 'fp32c' or 'fp64';
-    keep_momenta: results carry the final momenta (device tensors) for checking.
+    keep_momenta: results carry the final momenta (device tensors) for checking;
+    certify_tol: the precision guard's bound on the FP32-compensated energy error (None:
+    no guard; FP64 service: unused).
 
     Active registrations are split into two cohorts that take turns: a turn collects the
     answers of one cohort's batched requests (launched on its previous turn), advances its
@@ -168,8 +207,11 @@ class Server:
 
     def __init__(self, hbm_cap_bytes: float = 200e9, max_active: int = 256,
                  precision: str = 'fp32c', clock: Callable[[], float] = time.perf_counter,
-                 keep_momenta: bool = False):
+                 keep_momenta: bool = False,
+                 certify_tol: Optional[float] = DEFAULT_CERTIFY_TOL):
         self.keep_momenta = bool(keep_momenta)
+        self.certify_tol = certify_tol if precision == 'fp32c' else None
+        self.promoted = 0      # registrations re-run in FP64 by the precision guard
         self.cap = float(hbm_cap_bytes)
         self.max_active = int(max_active)
         self.precision = precision
@@ -208,7 +250,8 @@ class Server:
                 break
             self.queue.popleft()
             try:
-                active = _Active(req, build_registration(req, self.precision), now)
+                active = _Active(req, build_registration(req, self.precision), now,
+                                 self.precision)
             except _REJECT as exc:
                 # Invalid inputs fail this request only; the others keep being served.
                 self.results[req.rid] = Result(req.rid, float('nan'), 0,
@@ -219,12 +262,37 @@ class Server:
             c = 0 if len(self.cohorts[0]) <= len(self.cohorts[1]) else 1
             self.cohorts[c].append(active)
 
-    def _finish(self, a: _Active, value) -> None:
+    def _finish(self, a: _Active, e64=None) -> None:
         o = a.opt
+        if e64 is None:
+            e64 = float(o.obj) if a.precision == 'fp64' else float('nan')
         self.results[a.req.rid] = Result(a.req.rid, float(o.obj), len(o.history) - 1, o.stop_reason,
                                          a.req.arrival, a.admitted, self._now(), list(o.history),
-                                         o.at if self.keep_momenta else None)
+                                         o.at if self.keep_momenta else None, a.precision,
+                                         float(e64), a.err32)
         self.used -= footprint_bytes(a.req)
+
+    def _stopped(self, a: _Active) -> bool:
+        """The optimiser of ``a`` stopped: True when ``a`` is done; False when it stays
+        active (its FP64 certification is launched next, or its FP64 re-run started)."""
+        if a.precision == 'fp32c' and self.certify_tol is not None:
+            a.twin = fp64_twin(a.req, a.opt)
+            a.pending = (CERTIFY,)
+            return False
+        self._finish(a)
+        return True
+
+    def _certified(self, a: _Active, e64: float) -> bool:
+        """The FP64 objective at the final iterate of ``a``'s FP32-compensated run is in:
+        finish within the bound, else re-run in FP64 (True when ``a`` is done)."""
+        a.twin = None
+        a.err32 = _rel(float(a.opt.obj), e64)
+        if a.err32 <= self.certify_tol:
+            self._finish(a, e64)
+            return True
+        self.promoted += 1
+        a.start(build_registration(a.req, 'fp64'), 'fp64')
+        return False
 
     def step(self) -> None:
         """One turn of the next cohort: answer its requests launched on its previous turn,
@@ -242,21 +310,26 @@ class Server:
             if id(a) not in answers:          # admitted since the last turn: nothing to answer
                 still.append(a)
                 continue
+            if a.pending[0] == CERTIFY:
+                if not self._certified(a, answers[id(a)]):
+                    still.append(a)
+                continue
             try:
                 a.pending = a.gen.send(answers[id(a)])
                 # Requests without device work are answered at once, not in a later turn.
                 while a.pending[0] in _HOST_ONLY:
                     a.pending = a.gen.send(getattr(a.opt, a.pending[0])())
                 still.append(a)
-            except StopIteration as stop:
-                self._finish(a, stop.value)
+            except StopIteration:
+                if not self._stopped(a):
+                    still.append(a)
         self.cohorts[c] = still
         if not still:
             return
         groups = collections.defaultdict(list)
         for a in still:
             groups[a.pending[0]].append(a)
-        for kind in (cg.TRY, cg.GRADIENT, cg.DOT, cg.OBJECTIVE):
+        for kind in (cg.TRY, cg.GRADIENT, cg.DOT, cg.OBJECTIVE, CERTIFY):
             regs = groups.get(kind)
             if regs:
                 self.inflight[c].append((regs, _LAUNCH[kind](regs)))
@@ -296,12 +369,25 @@ def poisson_arrivals(n: int, rate: float, seed: int = 0, burst_factor: float = 1
     return np.cumsum(gaps)
 
 
-def plain_run(req: Request, precision: str) -> Result:
-    """One request through the plain single-registration path (reference call pattern)."""
+def plain_run(req: Request, precision: str,
+              certify_tol: Optional[float] = DEFAULT_CERTIFY_TOL) -> Result:
+    """One request through the plain single-registration path (reference call pattern), with
+    the service's precision guard (FP32-compensated runs only; None: no guard)."""
     t = time.perf_counter()
     opt = build_registration(req, precision)
     opt.optimizeMatching()
+    used, e64, err32 = precision, float('nan'), float('nan')
+    if precision == 'fp64':
+        e64 = float(opt.obj)
+    elif certify_tol is not None:
+        e64 = fp64_twin(req, opt).objectiveFun()
+        err32 = _rel(float(opt.obj), e64)
+        if err32 > certify_tol:
+            used = 'fp64'
+            opt = build_registration(req, 'fp64')
+            opt.optimizeMatching()
+            e64 = float(opt.obj)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     return Result(req.rid, float(opt.obj), len(opt.history) - 1, opt.stop_reason, 0.0, 0.0,
-                  t1 - t, list(opt.history))
+                  t1 - t, list(opt.history), None, used, e64, err32)
