@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--energy-only", action="store_true",
                     help="only the FP64 energy launches of the data_terms leg (for rocprof "
                          "passes)")
+    ap.add_argument("--graph-trial", action="store_true",
+                    help="only the HIP-graph trial comparison (graph_trial_leg)")
     ap.add_argument("--cpu-only", metavar="BENCH_JSON",
                     help="only the CPU baseline, extrapolated with the per-operation pair "
                          "counts of an earlier bench line (no GPU used)")
@@ -90,6 +92,36 @@ def registration(cfg, prec):
     _, Y = cfg["target"]
     return sm.SurfaceMatching(outputDir=None, Template=surfaces.Surface(FV=(F, V)), Target=surfaces.Surface(FV=(F, Y)),
                               param=p, maxIter=cfg["maxIter"], testGradient=False)
+
+
+# The FP64 oracle's whole registration of the large surface pair (the reference's
+# optimizeMatching path restated on the CPU), written by tests/golden/make_config2_oracle.py.
+CONFIG2_ORACLE = os.path.join(ROOT, "tests", "golden", "config2_oracle.npz")
+
+
+def fp64_form_leg(cfg, final_fp32c):
+    """One registration of the workload in the FP64 form (every kernel sum in FP64), timed
+    like the headline ones, and the FP32-compensated run's final energy against its final
+    energy and, where its record is present, the oracle's."""
+    import numpy as np
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m = registration(cfg, "fp64")
+    m.optimizeMatching()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    out = {"seconds": secs, "registrations_per_s": 1.0 / secs, "final_energy": float(m.obj),
+           "cg_iterations": len(m.history) - 1, "stop_reason": m.stop_reason,
+           "fp32c_final_energy_rel_err": abs(final_fp32c - m.obj) / abs(m.obj)}
+    if cfg.get("name") == "surface-s6" and os.path.exists(CONFIG2_ORACLE):
+        ref = np.load(CONFIG2_ORACLE)
+        e = float(ref["history"][-1])
+        out.update({"oracle_final_energy": e,
+                    "fp64_final_energy_rel_err_vs_oracle": abs(m.obj - e) / abs(e),
+                    "fp32c_final_energy_rel_err_vs_oracle": abs(final_fp32c - e) / abs(e),
+                    "oracle_stop_reason": str(ref["stop_reason"])})
+    return out
 
 
 def flow_step_roofline(cfg, prec, reps, natural=True):
@@ -511,6 +543,162 @@ def mixed_service_leg(k, prec):
             "max_batch": srv.max_batch, "served_pair_rates": _pair_rates(counts, wall)}
 
 
+class _HipGraph:
+    """Stream capture into a HIP graph through the HIP runtime's C API (libamdhip64)."""
+
+    def __init__(self):
+        import ctypes
+        self.c = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.graph = ctypes.c_void_p()
+        self.exec = ctypes.c_void_p()
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: hipError {rc}")
+
+    def begin(self, stream):
+        self._ok(self.hip.hipStreamBeginCapture(self.c.c_void_p(stream), 1), "hipStreamBeginCapture")
+
+    def end(self, stream):
+        c = self.c
+        self._ok(self.hip.hipStreamEndCapture(c.c_void_p(stream), c.byref(self.graph)), "hipStreamEndCapture")
+        self._ok(self.hip.hipGraphInstantiate(c.byref(self.exec), self.graph, None, None, c.c_size_t(0)),
+                 "hipGraphInstantiate")
+
+    def launch(self, stream):
+        self._ok(self.hip.hipGraphLaunch(self.exec, self.c.c_void_p(stream)), "hipGraphLaunch")
+
+    def nodes(self):
+        n = self.c.c_size_t(0)
+        self._ok(self.hip.hipGraphGetNodes(self.graph, None, self.c.byref(n)), "hipGraphGetNodes")
+        return int(n.value)
+
+    def close(self):
+        if self.exec:
+            self.hip.hipGraphExecDestroy(self.exec)
+        if self.graph:
+            self.hip.hipGraphDestroy(self.graph)
+
+
+def graph_trial_leg(trials=200):
+    """One line-search trial of a small registration (icosphere s3: 642 vertices, 1,280
+    faces; Gaussian deformation kernel sigma 0.3, T 10, FP32-compensated; FP64 currents
+    energy, width 0.3): T fused flow steps, the deformed template's face geometry, its
+    currents energy against the target and the target's norm.  Issued three ways, each timed
+    over ``trials`` trials by the wall clock with one synchronisation at the end:
+    (a) the registration path (matching.launch_objective: descriptors built and uploaded per
+    call, the T steps through lddmm_segments_run_steps); (b) the same kernels from descriptors
+    planned once, one library call per launch with the descriptors passed by value
+    (segs_dev NULL); (c) (b) captured once into a HIP graph, one hipGraphLaunch per trial.
+    Every replay's energies are compared bit for bit with (b)'s, with eager launches between
+    replays."""
+    import ctypes
+    import numpy as np
+    import torch
+    from registration_amd import _lib, engine, matching, runtime as rt, synthetic
+    lib = _lib.load()
+    cfg = dict(synthetic.surface_pair_config1(), sigmaKernel=0.3, T=10, maxIter=1)
+    cfg["name"] = "surface-s3"
+    m = registration(cfg, "fp32c")
+    T, N = m.Tsize, m.npt
+    rng = np.random.default_rng(3)
+    m.at = torch.as_tensor(1e-2 * rng.standard_normal((T, N, 3)), device="cuda")
+    # (a): the registration path
+    matching.launch_objective([m]).result()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(trials):
+        matching.launch_objective([m])
+    torch.cuda.synchronize()
+    a_ms = (time.perf_counter() - t0) / trials * 1e3
+
+    # (b): pre-planned descriptors on a capture-capable stream
+    dev = "cuda"
+    K = m.Kdiff
+    Kd = m.Kdist.energy()
+    F = torch.as_tensor(np.ascontiguousarray(cfg["template"][0]), dtype=torch.int32, device=dev)
+    xt = torch.zeros((T + 1, N, 3), dtype=torch.float64, device=dev)
+    xt[0] = m.x0
+    at = m.at.clone()
+    en = torch.zeros(T + 2, dtype=torch.float64, device=dev)   # T flow energies, data, norm
+    nf = F.shape[0]
+    c1 = torch.empty((nf, 3), dtype=torch.float64, device=dev)
+    cr1 = torch.empty_like(c1)
+    base = engine.flow_record(N, T, K, perm=m._order)
+    steps = np.repeat(base[None, :], T, axis=0).reshape(T, 1)
+    for t in range(T):
+        steps["y"][t, 0] = xt[t].data_ptr()
+        steps["w0"][t, 0] = at[t].data_ptr()
+        steps["out0"][t, 0] = xt[t + 1].data_ptr()
+        steps["total"][t, 0] = en[t].data_ptr()
+    fplan = rt.Plan(_lib.OP_FLOW_STEP, K.prec, 3, steps[0])
+    raw = steps.view(np.uint8).reshape(T, 1, rt.SEG_SIZE)
+    for lo, hi in rt._PLAN_BYTES:
+        raw[1:, :, lo:hi] = fplan.recs.view(np.uint8).reshape(1, rt.SEG_SIZE)[None, :, lo:hi]
+    raw[0] = fplan.recs.view(np.uint8).reshape(1, rt.SEG_SIZE)
+    crec = engine.currents_record(c1, cr1, m.c2, m.cr2, Kd, m.coef)
+    crec["total"] = en[T].data_ptr()
+    cplan = rt.Plan(_lib.OP_CURRENTS, Kd.prec, 3, crec)
+    nrec = rt.seg(y=m.c2, w0=m.cr2, ny=m.c2.shape[0], sigma=Kd.sigma, kernel=Kd.kernel,
+                  order=Kd.order, total=en[T + 1])
+    nplan = rt.Plan(_lib.OP_KNORM, Kd.prec, 3, nrec)
+    works = [torch.empty(p.work_bytes, dtype=torch.uint8, device=dev) for p in (fplan, cplan, nplan)]
+    SP = rt._SEG_PTR
+
+    def issue(st):
+        for t in range(T):
+            _lib.check(lib.lddmm_segments_run(_lib.OP_FLOW_STEP, K.prec, 3, steps[t].ctypes.data_as(SP),
+                                              None, 1, fplan.n_items, fplan.n_fin, works[0].data_ptr(),
+                                              works[0].numel(), st), "flow step")
+        _lib.check(lib.lddmm_surface_geometry(xt[T].data_ptr(), F.data_ptr(), nf, c1.data_ptr(),
+                                              cr1.data_ptr(), st), "geometry")
+        for p, w in ((cplan, works[1]), (nplan, works[2])):
+            _lib.check(lib.lddmm_segments_run(p.op, p.prec, 3, p.recs.ctypes.data_as(SP), None, 1,
+                                              p.n_items, p.n_fin, w.data_ptr(), w.numel(), st), "energy")
+
+    s = torch.cuda.Stream()
+    sh = s.cuda_stream
+    with torch.cuda.stream(s):
+        issue(sh)
+        torch.cuda.synchronize()
+        ref = en.clone()
+        t0 = time.perf_counter()
+        for _ in range(trials):
+            issue(sh)
+        torch.cuda.synchronize()
+        b_ms = (time.perf_counter() - t0) / trials * 1e3
+        # (c): capture (b) once, replay
+        g = _HipGraph()
+        try:
+            g.begin(sh)
+            issue(sh)
+            g.end(sh)
+            nodes = g.nodes()
+            same = True
+            for k in range(3):
+                en.zero_()
+                g.launch(sh)
+                torch.cuda.synchronize()
+                same = same and bool(torch.equal(en, ref))
+                issue(sh)                      # eager work between replays
+                torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(trials):
+                g.launch(sh)
+            torch.cuda.synchronize()
+            c_ms = (time.perf_counter() - t0) / trials * 1e3
+            same = same and bool(torch.equal(en, ref))
+        finally:
+            g.close()
+    return {"workload": "one line-search trial of an icosphere-s3 registration (642 vertices, "
+                        "1,280 faces; sigma 0.3, T 10; FP32-compensated flow, FP64 currents "
+                        "energy)",
+            "trials": trials, "graph_nodes": nodes,
+            "registration_path_ms": round(a_ms, 4), "preplanned_calls_ms": round(b_ms, 4),
+            "hip_graph_replay_ms": round(c_ms, 4), "graph_replays_identical": same}
+
+
 def main():
     args = parse()
     if args.cpu_only:
@@ -537,6 +725,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"roofline": roof, "calibration_copy_bytes": 256 << 20}))
+        return 0
+
+    if args.graph_trial:
+        print(json.dumps({"graph_trial": graph_trial_leg()}))
         return 0
 
     if args.energy_only:
@@ -570,12 +762,13 @@ def main():
     pairs_per_reg = sum(counts.values()) / args.steps
 
     # Energy error of the final state against the FP64 form of the same objective.
-    from registration_amd import kernelFunctions as kfun
     final_obj = m.obj
     m64 = registration(cfg, "fp64")
     m64.at = m.at.clone()
     obj64 = m64.objectiveFun()
     energy_err = abs(final_obj - obj64) / abs(obj64)
+    del m64
+    fp64_form = fp64_form_leg(cfg, final_obj) if args.precision == "fp32c" else None
 
     roof = flow_step_roofline(cfg, args.precision, args.roofline_reps)
     data_terms = data_terms_leg(cfg, args.data_term_reps) if args.data_term_reps > 0 else None
@@ -621,6 +814,7 @@ def main():
         "stop_reason": m.stop_reason,
         "final_energy": final_obj,
         "energy_rel_err_vs_fp64": energy_err,
+        "fp64_form": fp64_form,
         "pairs_per_registration": pairs_per_reg,
         "pairs_by_op": {op_names[k]: v / args.steps for k, v in sorted(counts.items())},
         "pair_rate_overall": pairs_per_reg / (wall / args.steps),

@@ -394,6 +394,27 @@ int lddmm_spatial_order(const double *points, int64_t n, int d, int32_t *order);
 double lddmm_order_compactness(const double *points, int64_t n, int d, const int32_t *order,
                                double scale, double r2max);
 
+/* ---- chunking targets (no counterpart in the reference: its sums are one
+ * OpenMP loop over rows, kernelFunctions.f90:1-54) ------------------------ */
+
+/* lddmm_segments_plan cuts a segment of ny rows into chunks of source tiles so
+ * that (row tiles) x (chunks) is about a target number of work items.  The
+ * target is kept per operation, precision, energy-only form (a data-term sum
+ * with no per-row output) and size class of ny; items <= 0 restores the
+ * built-in value (6,144; 24,576 for energy-only sums).  A setting applies to
+ * plans made after it, in this process; set targets before planning, not
+ * concurrently with it.  A segment's chunking still depends only on its own
+ * sizes, so results stay the same alone and in any batch.
+ * lddmm_apply_k_noalloc's scratch bound holds for the built-in targets of
+ * LDDMM_OP_APPLY_K; a larger one can make it return LDDMM_ESCRATCH. */
+#define LDDMM_SIZE_CLASSES 6
+/* Size class of a segment of ny rows: 0 below 1,024 rows, 1 below 4,096, 2
+ * below 16,384, 3 below 65,536, 4 below 262,144, 5 from 262,144 on. */
+int lddmm_size_class(int64_t ny);
+int lddmm_set_items_target(int op, int prec, int energy_only, int size_class, int64_t items);
+/* The target in force (built-in when unset), or LDDMM_EINVAL. */
+int64_t lddmm_get_items_target(int op, int prec, int energy_only, int size_class);
+
 /* ---- small helpers ------------------------------------------------------ */
 
 /* Deterministic FP64 sum of x[0..n) into *out (single fixed reduction tree). */

@@ -13,6 +13,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblddmm_hip.so")
+# Chunking targets chosen by registration_amd.autotune on an MI355X, applied when the
+# library loads; LDDMM_TUNING names another file, or "off" for the built-in targets.
+TUNING_PATH = os.path.join(_HERE, "tuning_gfx950.json")
 
 OK = 0
 EINVAL = -1
@@ -39,6 +42,8 @@ OP_MEASURE = 8
 OP_VARIFOLD = 9
 OP_DIVERGENCE = 10
 OP_DIFF_K = 11
+OP_IDS = {k[3:]: v for k, v in dict(globals()).items() if k.startswith("OP_")}
+PRECS = {"fp64": PREC_FP64, "fp32c": PREC_FP32C}
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -118,6 +123,9 @@ SIGNATURES = {
     "lddmm_vertex_gradient_batch": (_I, [ctypes.POINTER(Gather), _P, _I, _P]),
     "lddmm_landmark_batch": (_I, [ctypes.POINTER(Landmark), _P, _I, _P]),
     "lddmm_sum_f64": (_I, [_P, _I64, _P, _P]),
+    "lddmm_size_class": (_I, [_I64]),
+    "lddmm_set_items_target": (_I, [_I, _I, _I, _I, _I64]),
+    "lddmm_get_items_target": (_I64, [_I, _I, _I, _I]),
     "lddmm_last_hip_error": (_I, []),
     "lddmm_version": (ctypes.c_char_p, []),
 }
@@ -143,8 +151,28 @@ def load():
             fn = getattr(lib, name)
             fn.restype = res
             fn.argtypes = args
+        apply_tuning(lib)
         _lib = lib
     return _lib
+
+
+def apply_tuning(lib, path=None) -> int:
+    """Set the chunking targets recorded in a tuning file (default: LDDMM_TUNING, else
+    TUNING_PATH when present); returns the number of targets set."""
+    import json
+    path = path or os.environ.get("LDDMM_TUNING", TUNING_PATH)
+    if path == "off" or not os.path.exists(path):
+        return 0
+    with open(path) as f:
+        table = json.load(f)
+    n = 0
+    for e in table["entries"]:
+        rc = lib.lddmm_set_items_target(OP_IDS[e["op"]], PRECS[e["prec"]], int(e["energy_only"]),
+                                        int(e["size_class"]), int(e["items"]))
+        if rc != OK:
+            raise RuntimeError(f"tuning entry rejected by the library: {e}")
+        n += 1
+    return n
 
 
 _ERRORS = {

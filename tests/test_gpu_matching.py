@@ -56,8 +56,9 @@ def test_surface_cg_sequence_fp64():
 def test_surface_cg_fp32c_energies_and_path():
     """FP32-compensated run: (a) the energy of every accepted iterate equals the FP64 energy of
     the same momenta within 1e-6 relative (the precision claim); (b) the iterate path stays
-    within 1e-4 of the FP64 oracle's (FP32 gradients change the search directions slightly,
-    so the paths are not identical)."""
+    within 5e-5 of the FP64 oracle's.  FP32 gradients change the search directions slightly,
+    so the paths are not identical: the gap grows with the iterations as the line search
+    amplifies them (measured: 2.9e-7 at the first accepted iterate, 9.1e-6 at the eighth)."""
     import torch
     g, o = surface_pair("fp32c", subdiv=2, T=5)
     g.maxIter = o.maxIter = 8
@@ -79,7 +80,7 @@ def test_surface_cg_fp32c_energies_and_path():
     hg, ho = np.array(g.history), np.array(o.history)
     n = min(len(hg), len(ho))
     assert n >= 5
-    assert np.all(np.abs(hg[:n] - ho[:n]) <= 1e-4 * np.abs(ho[:n])), (hg[:n], ho[:n])
+    assert np.all(np.abs(hg[:n] - ho[:n]) <= 5e-5 * np.abs(ho[:n])), (hg[:n], ho[:n])
 
 
 def curve_pair(prec):
@@ -107,6 +108,7 @@ def test_curve_cg_sequence_fp64():
 
 
 def test_curve_cg_fp32c_energies_and_path():
+    """As for surfaces; the curve path stays within 5e-6 of the oracle's (measured 6.3e-7)."""
     g, o = curve_pair("fp32c")
     stash = []
     accept = g.acceptVarTry
@@ -126,7 +128,7 @@ def test_curve_cg_fp32c_energies_and_path():
     hg, ho = np.array(g.history), np.array(o.history)
     n = min(len(hg), len(ho))
     assert n >= 4
-    assert np.all(np.abs(hg[:n] - ho[:n]) <= 1e-4 * np.abs(ho[:n])), (hg[:n], ho[:n])
+    assert np.all(np.abs(hg[:n] - ho[:n]) <= 5e-6 * np.abs(ho[:n])), (hg[:n], ho[:n])
 
 
 def test_nan_trial_is_rejected():
@@ -242,7 +244,11 @@ def test_config1_full_run_fp64_matches_oracle():
 
 def test_config1_full_run_fp32c_energies():
     """The FP32-compensated run of the same case: every accepted iterate's energy within 1e-6
-    of the FP64 energy of the same momenta, and the final energy within 1e-4 of the FP64 run's."""
+    of the FP64 energy of the same momenta (measured worst 2.9e-8), and the final energy within
+    5e-5 of the FP64 run's.  Both runs take 22 iterations and stop on a small gradient; the
+    end-to-end gap is the optimiser's, not the arithmetic's: the two CG paths separate by
+    about 2e-7 per iteration early on and 1e-6 to 2e-5 from the seventh iteration on
+    (measured final gap 1.39e-5).  The large surface pair, 3 iterations, ends 2.0e-8 apart."""
     g, _ = _config1("fp32c")
     stash = []
     accept = g.acceptVarTry
@@ -262,7 +268,8 @@ def test_config1_full_run_fp32c_energies():
     assert worst <= 1e-6, worst
     f64, _ = _config1("fp64")
     f64.optimizeMatching()
-    assert abs(g.obj - f64.obj) <= 1e-4 * abs(f64.obj), (g.obj, f64.obj)
+    assert g.stop_reason == f64.stop_reason
+    assert abs(g.obj - f64.obj) <= 5e-5 * abs(f64.obj), (g.obj, f64.obj)
 
 
 def test_bad_direction_branch_follows_reference():
