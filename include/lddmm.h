@@ -415,6 +415,20 @@ int lddmm_set_items_target(int op, int prec, int energy_only, int size_class, in
 /* The target in force (built-in when unset), or LDDMM_EINVAL. */
 int64_t lddmm_get_items_target(int op, int prec, int energy_only, int size_class);
 
+/* ---- template decimation ------------------------------------------------ */
+
+/* surfaces.Surface.Simplify (py-lddmm/surfaces.py:244-259, used by SurfaceMatching's
+ * subsampleTargetSize, surfaceMatching.py:131-133): decimate a triangle mesh to at most
+ * `target` vertices by quadric-error edge collapse (Garland & Heckbert 1997; the
+ * reference calls VTK's vtkQuadricDecimation, which implements the same method).  Host
+ * code, no device.  out_vertices holds nv x 3 and out_faces nf x 3 entries (the result is
+ * never larger); *nv_out / *nf_out receive the sizes kept.  Collapses that would make the
+ * mesh non-manifold or fold a face are skipped, so a mesh may stop above `target`.
+ * Deterministic.  LDDMM_EINVAL for face indices out of range or non-finite vertices. */
+int lddmm_decimate_surface(const double *vertices, int64_t nv, const int32_t *faces,
+                           int64_t nf, int64_t target, double *out_vertices,
+                           int32_t *out_faces, int64_t *nv_out, int64_t *nf_out);
+
 /* ---- small helpers ------------------------------------------------------ */
 
 /* Deterministic FP64 sum of x[0..n) into *out (single fixed reduction tree). */

@@ -123,6 +123,8 @@ SIGNATURES = {
     "lddmm_vertex_gradient_batch": (_I, [ctypes.POINTER(Gather), _P, _I, _P]),
     "lddmm_landmark_batch": (_I, [ctypes.POINTER(Landmark), _P, _I, _P]),
     "lddmm_sum_f64": (_I, [_P, _I64, _P, _P]),
+    "lddmm_decimate_surface": (_I, [_P, _I64, _P, _I64, _I64, _P, _P, ctypes.POINTER(_I64),
+                                    ctypes.POINTER(_I64)]),
     "lddmm_size_class": (_I, [_I64]),
     "lddmm_set_items_target": (_I, [_I, _I, _I, _I, _I64]),
     "lddmm_get_items_target": (_I64, [_I, _I, _I, _I]),

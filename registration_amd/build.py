@@ -8,6 +8,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "lddmm_kernels.hip")
+# host-only C++ (mesh decimation), linked into the same library
+HOST_SRCS = [os.path.join(HERE, "csrc", "decimate.cpp")]
 OUT = os.path.join(HERE, "liblddmm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -21,7 +23,7 @@ FLAGS = [
 
 def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, SRC, "-o", tmp]
+    cmd = [HIPCC, *FLAGS, SRC, *HOST_SRCS, "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

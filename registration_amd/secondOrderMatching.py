@@ -148,21 +148,26 @@ class SurfaceMatching:
     |rho|^2; affine / affineWeight / rotWeight / scaleWeight / transWeight as in
     surfaceMatching.  Files: Template.vtk and Target<k>.vtk at construction and, every saveRate
     iterations, the deformed template at every time step with its Jacobian
-    (saveFile<t>.vtk); ``outputDir=None`` writes nothing.  subsampleTargetSize > 0 (template
-    decimation) is not supported."""
+    (saveFile<t>.vtk); ``outputDir=None`` writes nothing.  subsampleTargetSize > 0 decimates
+    the template to that many vertices first (Surface.Simplify) and orients the targets
+    against it (secondOrderMatching.py:112-120)."""
 
     def __init__(self, Template=None, Targets=None, fileTempl=None, fileTarg=None, param=None,
                  initialMomentum=None, maxIter=1000, regWeight=1.0, verb=True,
                  typeRegression='spline2', affine='none', controlWeight=1.0, affineWeight=1.0,
                  rotWeight=None, scaleWeight=None, transWeight=None, subsampleTargetSize=-1,
                  testGradient=True, saveFile='evolution', outputDir='.'):
-        if subsampleTargetSize > 0:
-            raise NotImplementedError("template decimation (subsampleTargetSize) is not supported")
         self.fv0 = _surface(Template, fileTempl, 'template surface')
         if Targets is None and fileTarg is None:
             raise ValueError('Please provide a list of target surfaces')
         self.fv1 = ([surfaces.Surface(surf=s) for s in Targets] if Targets is not None
                     else [surfaces.Surface(filename=f) for f in fileTarg])
+        if subsampleTargetSize > 0:
+            self.fv0.Simplify(subsampleTargetSize)
+            v0 = self.fv0.surfVolume()
+            for s in self.fv1:
+                if v0 * s.surfVolume() < 0:
+                    s.flipFaces()
         self.param = SurfaceMatchingParam() if param is None else param
         if self.param.errorType not in ('current', 'measure', 'varifold'):
             raise NotImplementedError("the GPU path implements the currents, measure and "

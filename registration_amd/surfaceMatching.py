@@ -8,7 +8,8 @@ currents, measure and varifold data terms (errorType 'current', 'measure' (the d
 'varifold'), the global affine part (affine 'affine', 'similitude', 'euclidean',
 'translation' or 'none', with affineWeight / rotWeight / scaleWeight / transWeight),
 symmetric matching (the template's points a variable too), templates and targets given as
-surfaces or shape files; subsampleTargetSize=-1 only.
+surfaces or shape files, and template decimation (subsampleTargetSize > 0, by quadric-error
+edge collapse in the kernel library's host code).
 """
 from __future__ import annotations
 
@@ -90,8 +91,6 @@ class SurfaceMatching(DeviceMatching):
         ``outputDir=None`` (an addition) writes nothing."""
         self.fv0 = _shape_arg(Template, fileTempl, 'template')
         self.fv1 = _shape_arg(Target, fileTarg, 'target')
-        if subsampleTargetSize > 0:
-            raise NotImplementedError("template decimation (subsampleTargetSize) is not supported")
         self.param = SurfaceMatchingParam() if param is None else param
         if self.param.errorType not in ('current', 'measure', 'varifold'):
             raise NotImplementedError("the GPU path implements the currents, measure and "
@@ -109,10 +108,16 @@ class SurfaceMatching(DeviceMatching):
         if v0 * v1 < 0:
             self.fv1.flipFaces()
             v1 = -v1  # flipping every face negates each determinant
-        # copies of the template whose geometry was just computed from its vertices
+        # copies of the template whose geometry was just computed from its vertices; with
+        # subsampleTargetSize > 0 the registered template is the decimated copy
+        # (surfaceMatching.py:131-133; the target was oriented against the full template)
         self.fvInit = self.fv0.clone()
-        self.fvDef = self.fv0.clone()
-        # orientations of the (copied) template and of the target after the flip
+        if subsampleTargetSize > 0:
+            self.fvInit.Simplify(subsampleTargetSize)
+            logging.info('simplified template %d' % self.fvInit.vertices.shape[0])
+            v0 = self.fvInit.surfVolume()
+        self.fvDef = self.fvInit.clone()
+        # orientations of the registered template and of the target after the flip
         self.fv0ori = -1 if v0 < 0 else 1
         self.fv1ori = -1 if v1 < 0 else 1
         x0 = np.copy(self.fvInit.vertices)

@@ -114,6 +114,32 @@ class Surface:
             np.add.at(normals, self.faces[:, j], self.surfel)
         return normals / np.sqrt((normals ** 2).sum(axis=1))[:, None]
 
+    def Simplify(self, target=1000.0):
+        """Decimate to at most ``target`` vertices (surfaces.py:244-259): quadric-error edge
+        collapse in the kernel library's host code (lddmm_decimate_surface; the reference
+        calls VTK's vtkQuadricDecimation, the same method, and raises without VTK), then, as
+        there, the faces are flipped if the signed volume is positive."""
+        from . import _lib
+        import ctypes
+        lib = _lib.load()
+        V = np.ascontiguousarray(self.vertices, dtype=np.float64)
+        F = np.ascontiguousarray(self.faces, dtype=np.int32)
+        if V.ndim != 2 or V.shape[1] != 3 or F.ndim != 2 or F.shape[1] != 3:
+            raise ValueError("Simplify needs a triangulated surface in 3D")
+        Vo = np.empty_like(V)
+        Fo = np.empty_like(F)
+        nv, nf = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(lib.lddmm_decimate_surface(V.ctypes.data, V.shape[0], F.ctypes.data,
+                                              F.shape[0], int(max(3, int(target))),
+                                              Vo.ctypes.data, Fo.ctypes.data, ctypes.byref(nv),
+                                              ctypes.byref(nf)), "lddmm_decimate_surface")
+        self.vertices = Vo[:nv.value].copy()
+        self.faces = np.int_(Fo[:nf.value])
+        self.computeCentersAreas()
+        z = self.surfVolume()
+        if z > 0:
+            self.flipFaces()
+
     def clone(self):
         """A copy whose centers and surfels are copied, not recomputed: for a surface whose
         geometry is known to match its vertices (Surface(surf=...) recomputes it)."""
