@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--energy-only", action="store_true",
                     help="only the FP64 energy launches of the data_terms leg (for rocprof "
                          "passes)")
+    ap.add_argument("--no-fp64-form", action="store_true",
+                    help="skip the FP64-form registration (profiled runs)")
     ap.add_argument("--graph-trial", action="store_true",
                     help="only the HIP-graph trial comparison (graph_trial_leg)")
     ap.add_argument("--cpu-only", metavar="BENCH_JSON",
@@ -768,7 +770,8 @@ def main():
     obj64 = m64.objectiveFun()
     energy_err = abs(final_obj - obj64) / abs(obj64)
     del m64
-    fp64_form = fp64_form_leg(cfg, final_obj) if args.precision == "fp32c" else None
+    fp64_form = (fp64_form_leg(cfg, final_obj)
+                 if args.precision == "fp32c" and not args.no_fp64_form else None)
 
     roof = flow_step_roofline(cfg, args.precision, args.roofline_reps)
     data_terms = data_terms_leg(cfg, args.data_term_reps) if args.data_term_reps > 0 else None
