@@ -381,8 +381,8 @@ struct KEval<double, LDDMM_KERNEL_LAPLACIAN> {
 // ---------------------------------------------------------------------------
 // Tile culling.  A wave skips a staged source tile when the distance between
 // the bounding box of its target rows and the tile's bounding box makes every
-// kernel value of the tile smaller than EPS_K (2^-32 in FP32, 2^-56 in FP64).
-// A skipped term is below 1/256 (FP32) or 1/8 (FP64) of an ulp of a unit
+// kernel value of the tile smaller than EPS_K (2^-24 in FP32, 2^-56 in FP64).
+// A skipped term is below half (FP32) or 1/8 (FP64) of an ulp of a unit
 // kernel value; even all of a row's skipped terms together (each below
 // EPS_K |w|) stay within the rounding error of its kept near-field terms, and
 // in an FP64 data-term energy of F faces they move the total by at most
@@ -396,10 +396,13 @@ __device__ inline double cull_r2(int prec, int kernel, double sigma) {
   const bool f32 = prec == LDDMM_PREC_FP32C;
   if (kernel == LDDMM_KERNEL_GAUSS) {
     // FP32: K = 2^-r'^2 with prescaled r'.  FP64: K = 2^(-h r^2), h = log2(e)/(2 sigma^2).
-    return f32 ? 32.0 : 56.0 * 2.0 * sigma * sigma / LOG2E;
+    // The FP32 bound, 2^-24, is FP32's unit roundoff: a source beyond it adds less than half
+    // an ulp of its own weight (on the large surface pair the flow step runs 13% faster than
+    // at 2^-32 for the same energy error, 2.4e-8).
+    return f32 ? 24.0 : 56.0 * 2.0 * sigma * sigma / LOG2E;
   }
-  // (1+u)^4 e^-u = 2^-32 at u~36.7, 2^-56 at u~54.9; FP32 coordinates carry log2(e)/sigma.
-  const double u_cut = f32 ? 37.0 * LOG2E : 55.0 * sigma;
+  // (1+u)^4 e^-u = 2^-24 at u~30.4, 2^-56 at u~54.9; FP32 coordinates carry log2(e)/sigma.
+  const double u_cut = f32 ? 30.5 * LOG2E : 55.0 * sigma;
   return u_cut * u_cut;
 }
 
@@ -922,8 +925,8 @@ void k_pairs(const lddmm_seg_t *__restrict__ segs,
       for (int c = 0; c < D; ++c) org[c] = ((const double *)rec)[c];
       if constexpr (!KAHAN) {
         // Workgroup-level cull from the record's quarter boxes (tile coordinates); FP64
-        // only: the FP32 flow step measured 1.3% slower with it (few of its tiles are
-        // wholly culled at the 2^-32 cut).
+        // only: the FP32 flow step measured 1.3% slower with it (few of its tiles were
+        // wholly culled, measured at the earlier 2^-32 cut).
         const R *qb = (const R *)(rec + 32);
         R d2 = 0;
 #pragma unroll
