@@ -743,6 +743,59 @@ class Surface:
         return z
 
 
+def vertex_area(V, F):
+    """surfaces.Surface.computeVertexArea (surfaces.py:77-126), face by face: (AV (nv, 1),
+    AF (nf, 1)); an obtuse face gives half its area to the obtuse vertex and a quarter to the
+    others, a non-obtuse face each vertex's Voronoi part."""
+    nv, nf = V.shape[0], F.shape[0]
+    AF = np.zeros([nf, 1])
+    AV = np.zeros([nv, 1])
+    for k in range(nf):
+        i, j, l = F[k]
+        x12, x13, x23 = V[j] - V[i], V[l] - V[i], V[l] - V[j]
+        n12, n13, n23 = (np.sqrt((e ** 2).sum()) for e in (x12, x13, x23))
+        c1 = (x12 * x13).sum() / (n12 * n13)
+        c2 = -(x12 * x23).sum() / (n12 * n23)
+        c3 = (x13 * x23).sum() / (n13 * n23)
+        AF[k] = np.sqrt((np.cross(x12, x13) ** 2).sum()) / 2
+        if c1 < 0:
+            AV[i] += AF[k] / 2; AV[j] += AF[k] / 4; AV[l] += AF[k] / 4
+        elif c2 < 0:
+            AV[i] += AF[k] / 4; AV[j] += AF[k] / 2; AV[l] += AF[k] / 4
+        elif c3 < 0:
+            AV[i] += AF[k] / 4; AV[j] += AF[k] / 4; AV[l] += AF[k] / 2
+        else:
+            cot1, cot2, cot3 = (c / np.sqrt(1 - c ** 2) for c in (c1, c2, c3))
+            AV[i] += ((x12 ** 2).sum() * cot3 + (x13 ** 2).sum() * cot2) / 8
+            AV[j] += ((x12 ** 2).sum() * cot3 + (x23 ** 2).sum() * cot1) / 8
+            AV[l] += ((x13 ** 2).sum() * cot2 + (x23 ** 2).sum() * cot1) / 8
+    return AV, AF
+
+
+def vertex_normals(V, F):
+    """surfaces.Surface.computeVertexNormals (surfaces.py:128-139): the normalised sum of the
+    surfels of each vertex's faces, face by face."""
+    normals = np.zeros(V.shape)
+    for k in range(F.shape[0]):
+        s = np.cross(V[F[k, 1]] - V[F[k, 0]], V[F[k, 2]] - V[F[k, 0]])
+        for c in range(3):
+            normals[F[k, c]] += s
+    return normals / np.sqrt((normals ** 2).sum(axis=1)).reshape(-1, 1)
+
+
+def grid_lines(n, m):
+    """grid.Grid's edges for an (n, m) lattice (grid.py:18-30): along each row, then along
+    each column, vertices numbered row-major."""
+    out = []
+    for k in range(n):
+        for l in range(m - 1):
+            out.append((k * m + l, k * m + l + 1))
+    for k in range(n - 1):
+        for l in range(m):
+            out.append((k * m + l, (k + 1) * m + l))
+    return np.array(out, dtype=np.int64)
+
+
 def surf_currentNorm0(fv1, K):
     return np.multiply(fv1.surfel, K.applyK(fv1.centers, fv1.surfel)).sum()
 
