@@ -31,10 +31,10 @@ FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
 # HBM bytes per fused flow step at 40,962 points (tile staging + pair kernel + finalize + totals),
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on `bench.py --roofline-only
 # --roofline-reps 3` and scaled by the counters' reading for 256 MiB device copies (FETCH x2.0,
-# WRITE x1.0), in the template's spatial processing order: 82.5 MB read + 63.8 MB written, most
+# WRITE x1.0), in the template's spatial processing order: 83.0 MB read + 64.0 MB written, most
 # of it the FP64 chunk partials (54 chunks x 40,962 rows x 3), written by the pair kernel and
 # read back by the finalize (profiles/flow_step_s6_fp32c.md).
-FLOW_STEP_TRAFFIC_S6 = 146.3e6
+FLOW_STEP_TRAFFIC_S6 = 147.0e6
 
 
 def _progress(msg):
