@@ -58,3 +58,62 @@ def test_submit_rejects_requests_above_the_admission_cap():
     srv = serving.Server(hbm_cap_bytes=1e5)
     with pytest.raises(ValueError, match="admission cap"):
         srv.submit(_request(0))
+
+
+class _FakeOpt:
+    """A registration stand-in: its optimiser asks for one objective and stops."""
+
+    def __init__(self, obj):
+        self.obj = obj
+        self.history = [obj]
+        self.stop_reason = 'small gradient'
+        self.at = None
+        self.affineDim = 0
+
+    def cg_steps(self):
+        yield (serving.cg.OBJECTIVE,)
+
+
+def _active(srv, obj, precision):
+    a = serving._Active.__new__(serving._Active)
+    a.req = _request(7)
+    a.admitted = 0.0
+    a.twin = None
+    a.err32 = float('nan')
+    a.start(_FakeOpt(obj), precision)
+    srv.used += serving.footprint_bytes(a.req)
+    return a
+
+
+def test_precision_guard_rule(monkeypatch):
+    """The guard's decisions without a device: within certify_tol the FP32-compensated result
+    is kept with its error and the FP64 energy; beyond it the registration restarts in FP64
+    (a new FP64 registration, its optimiser from the start); an FP64 service, or
+    certify_tol=None, never certifies."""
+    t = [0.0]
+    srv = serving.Server(clock=lambda: t[0], precision='fp32c')
+    srv.t0 = 0.0
+    assert srv.certify_tol == serving.DEFAULT_CERTIFY_TOL == 5e-7
+    built = []
+    monkeypatch.setattr(serving, 'build_registration',
+                        lambda req, prec: built.append(prec) or _FakeOpt(2.0))
+    monkeypatch.setattr(serving, 'fp64_twin', lambda req, opt: 'twin')
+    a = _active(srv, 1.0, 'fp32c')
+    assert srv._stopped(a) is False and a.pending == (serving.CERTIFY,) and a.twin == 'twin'
+    assert srv._certified(a, 1.0 + 4e-7) is True
+    r = srv.results[7]
+    assert r.precision == 'fp32c' and r.energy == 1.0 and r.energy_fp64 == 1.0 + 4e-7
+    assert abs(r.fp32c_error - 4e-7 / (1 + 4e-7)) < 1e-15 and srv.promoted == 0 and built == []
+    b = _active(srv, 1.0, 'fp32c')
+    srv._stopped(b)
+    assert srv._certified(b, 1.0 + 1e-5) is False
+    assert srv.promoted == 1 and built == ['fp64'] and b.precision == 'fp64'
+    assert b.pending == (serving.cg.OBJECTIVE,) and b.opt.obj == 2.0 and b.err32 > 5e-7
+    assert srv._stopped(b) is True                       # an FP64 run finishes without a check
+    r = srv.results[7]
+    assert r.precision == 'fp64' and r.energy == r.energy_fp64 == 2.0 and r.fp32c_error > 5e-7
+    assert serving.Server(precision='fp64').certify_tol is None
+    off = serving.Server(clock=lambda: t[0], precision='fp32c', certify_tol=None)
+    off.t0 = 0.0
+    c = _active(off, 1.0, 'fp32c')
+    assert off._stopped(c) is True and off.results[7].precision == 'fp32c'
