@@ -37,6 +37,17 @@ FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
 FLOW_STEP_TRAFFIC_S6 = 147.0e6
 
 
+def _warm(plan, seconds=0.25):
+    """Run a planned launch until ``seconds`` have passed: the GPU's clocks ramp up over the
+    first ~100 ms of work, and a leg that starts cold would time the ramp."""
+    import torch
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            plan.run()
+        torch.cuda.synchronize()
+
+
 def _progress(msg):
     """One progress line per finished leg on stderr (the JSON result stays alone on stdout)."""
     print(f"bench: {msg}", file=sys.stderr, flush=True)
@@ -150,10 +161,13 @@ def flow_step_roofline(cfg, prec, reps, natural=True):
                       kernel=K.kernel_code, order=K.order)
         if perm is not None:
             fields["perm"] = perm
-        plan = rt.Plan(_lib.OP_FLOW_STEP, K.prec_code, 3, [rt.seg(**fields)])
-        for _ in range(reps):  # untimed: the clocks settle over the first launches
-            plan.run()
-        torch.cuda.synchronize()
+        rec = rt.seg(**fields)
+        # the dispatch order a registration's flow steps carry (estimated from the template)
+        items = engine.item_order(_lib.OP_FLOW_STEP, K.prec_code, 3, rec, V,
+                                  perm=None if perm is None else perm.cpu().numpy())
+        engine.attach_item_order(rec, items)
+        plan = rt.Plan(_lib.OP_FLOW_STEP, K.prec_code, 3, [rec])
+        _warm(plan)  # untimed: the clocks settle over the first launches
         st = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(st)
@@ -197,7 +211,7 @@ def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradien
     gradient forms, plus the divergence of the Jacobian flow at the template's points."""
     import numpy as np
     import torch
-    from registration_amd import _lib, runtime as rt, surfaces
+    from registration_amd import _lib, engine, runtime as rt, surfaces
     F, V = cfg["template"]
     _, Y = cfg["target"]
     sig = cfg["sigmaDist"]
@@ -217,9 +231,18 @@ def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradien
     }
     ops = {"currents": _lib.OP_CURRENTS, "measure": _lib.OP_MEASURE, "varifold": _lib.OP_VARIFOLD}
 
+    c1_np, c2_np = S.centers.cpu().numpy(), T.centers.cpu().numpy()
+    orders = []
+
+    def ordered(op, prec, rec):
+        """rec with the dispatch order a registration's data-term launches carry."""
+        o = engine.item_order(op, prec, 3, rec, c1_np, x1=c2_np)
+        engine.attach_item_order(rec, o)
+        orders.append(o)
+        return rec
+
     def timed(plan):
-        plan.run()
-        torch.cuda.synchronize()
+        _warm(plan)
         st = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(st)
@@ -233,7 +256,8 @@ def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradien
     pairs = 2 * nf * nf
     for name in names:
         op = ops[name]
-        ms = timed(rt.Plan(op, _lib.PREC_FP64, 3, [rt.seg(**base, **weights[name])]))
+        ms = timed(rt.Plan(op, _lib.PREC_FP64, 3,
+                           [ordered(op, _lib.PREC_FP64, rt.seg(**base, **weights[name]))]))
         # nominal: all 2 F^2 pairs of the two sums (target-target and target-template)
         rec = {"ms": round(ms, 4), "nominal_pairs": pairs,
                "nominal_pairs_per_s": pairs / (ms * 1e-3),
@@ -248,7 +272,8 @@ def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradien
         if not gradients:
             continue
         ms = timed(rt.Plan(op, _lib.PREC_FP32C, 3,
-                           [rt.seg(**base, **weights[name], out0=z1, out1=dz)]))
+                           [ordered(op, _lib.PREC_FP32C,
+                                    rt.seg(**base, **weights[name], out0=z1, out1=dz))]))
         out[f"{name}_gradient_fp32c"] = {"ms": round(ms, 4), "nominal_pairs": pairs,
                                          "nominal_pairs_per_s": pairs / (ms * 1e-3)}
     if not gradients:

@@ -169,6 +169,17 @@ typedef struct lddmm_seg {
    * pair).  The permutation itself is not checked (it is not read on the host).
    * No counterpart in the reference, whose sums run in point order. */
   const int32_t *perm;
+  /* [in] optional dispatch order of this segment's work items, NULL for item order:
+   * a permutation of 0..n_order-1 in device memory, item_order[j] = the work item
+   * (row tile, source chunk) that the segment's j-th workgroup runs.  n_order must be
+   * the planned item count (row_tiles x (chunks0 + chunks1)); lddmm_segments_plan
+   * rejects any other.  Every item writes its own partials, so the results are the
+   * same bit for bit in any order; an order that starts with the items holding the
+   * most unculled pairs (lddmm_item_order) shortens the launch's tail, where the
+   * last workgroups run while the rest of the chip idles.  No counterpart in the
+   * reference. */
+  const int32_t *item_order;
+  int64_t n_order;
 } lddmm_seg_t;
 
 /* Plan a segmented launch: fills the [plan] fields of segs[0..nseg) and
@@ -393,6 +404,25 @@ int lddmm_spatial_order(const double *points, int64_t n, int d, int32_t *order);
  * Returns -1 for invalid arguments. */
 double lddmm_order_compactness(const double *points, int64_t n, int d, const int32_t *order,
                                double scale, double r2max);
+
+/* Dispatch order of a segment's work items, longest first (the seg_t field
+ * item_order).  seg: the segment as it will be launched (its sizes, kernel, sigma,
+ * perm and output flags; device pointers may still be NULL, they are not read);
+ * y: host copy of its rows (ny, d), which are also its set-0 sources; x1: host copy
+ * of its set-1 sources (nx1, d) for CURRENTS and MEASURE segments with nx1 > 0,
+ * else NULL; perm: host copy of its processing order (ny), NULL when it has none.
+ * Plans the segment, estimates each work item's cost from bounding boxes (the
+ * (wave, 64-source quarter) blocks that survive the pair kernel's cull, the
+ * symmetric energy sums' skipped tiles, FP64 energies' near blocks counted double)
+ * and writes into order[0..n) the items sorted by decreasing cost, ties in item
+ * order; n must be the planned item count.  With order NULL nothing is written and
+ * the planned item count is returned (y, x1, perm, n unused).  Host-only,
+ * deterministic.  Supported for FLOW_STEP, COVECTOR_STEP, KDOT, KNORM, CURRENTS
+ * and MEASURE (others return LDDMM_EUNSUPPORTED).  The geometry only steers the schedule: an order estimated
+ * from earlier positions (a registration's template, reused for every step and
+ * trial) leaves every result unchanged.  No counterpart in the reference. */
+int lddmm_item_order(int op, int prec, int d, const lddmm_seg_t *seg, const double *y,
+                     const double *x1, const int32_t *perm, int32_t *order, int64_t n);
 
 /* ---- chunking targets (no counterpart in the reference: its sums are one
  * OpenMP loop over rows, kernelFunctions.f90:1-54) ------------------------ */

@@ -68,7 +68,7 @@ class Seg(ctypes.Structure):
         ("row_tiles", ctypes.c_int32), ("chunks0", ctypes.c_int32),
         ("chunks1", ctypes.c_int32), ("chunk_len", ctypes.c_int32),
         ("fin_blocks", ctypes.c_int32), ("stage0", ctypes.c_int32),
-        ("perm", _P),
+        ("perm", _P), ("item_order", _P), ("n_order", _I64),
     ]
 
 
@@ -103,6 +103,7 @@ SIGNATURES = {
                                       _SZ, _P]),
     "lddmm_spatial_order": (_I, [_P, _I64, _I, _P]),
     "lddmm_order_compactness": (ctypes.c_double, [_P, _I64, _I, _P, _D, _D]),
+    "lddmm_item_order": (_I, [_I, _I, _I, ctypes.POINTER(Seg), _P, _P, _P, _P, _I64]),
     "lddmm_apply_k_workspace_bytes": (_SZ, [_I64, _I64, _I, _I]),
     "lddmm_apply_k": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _D, _I, _P, _P, _SZ, _P]),
     "lddmm_apply_k_noalloc": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _D, _I, _P, _P]),

@@ -392,7 +392,7 @@ struct KEval<double, LDDMM_KERNEL_LAPLACIAN> {
 // Returns the squared cut radius in the kernel's coordinates (prescaled in
 // FP32, raw in FP64).
 // ---------------------------------------------------------------------------
-__device__ inline double cull_r2(int prec, int kernel, double sigma) {
+__host__ __device__ inline double cull_r2(int prec, int kernel, double sigma) {
   const bool f32 = prec == LDDMM_PREC_FP32C;
   if (kernel == LDDMM_KERNEL_GAUSS) {
     // FP32: K = 2^-r'^2 with prescaled r'.  FP64: K = 2^(-h r^2), h = log2(e)/(2 sigma^2).
@@ -541,6 +541,13 @@ __device__ inline int find_segment(const lddmm_seg_t *__restrict__ segs, int nse
     if (st <= key) lo = mid; else hi = mid - 1;
   }
   return lo;
+}
+
+// The segment-local work item (row tile, source chunk) that workgroup `item` of the launch
+// runs: its own position in the segment, or what the segment's dispatch order puts there.
+__device__ inline int64_t local_item(const lddmm_seg_t *sg, int64_t item) {
+  const int64_t j = item - sg->item0;
+  return sg->item_order ? (int64_t)sg->item_order[j] : j;
 }
 
 // ---------------------------------------------------------------------------
@@ -780,7 +787,7 @@ void k_pairs(const lddmm_seg_t *__restrict__ segs,
 
   const int64_t ny = sg->ny;
   const int row_tiles = sg->row_tiles;
-  const int64_t it = item - sg->item0;
+  const int64_t it = local_item(sg, item);
   const int rt = (int)(it % row_tiles);
   const int cg = (int)(it / row_tiles);
   const int set = cg < sg->chunks0 ? 0 : 1;
@@ -1347,7 +1354,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_ddk(const lddmm_seg_t *__restri
   const lddmm_seg_t *sg = segs + si;
   const int64_t ny = sg->ny;
   const int row_tiles = sg->row_tiles;
-  const int64_t it = item - sg->item0;
+  const int64_t it = local_item(sg, item);
   const int rt = (int)(it % row_tiles);
   const int cg = (int)(it / row_tiles);
   const double scale = coord_scale(prec, KT, sg->sigma);
@@ -1491,7 +1498,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
 
   const int64_t ny = sg->ny;
   const int row_tiles = sg->row_tiles;
-  const int64_t it = item - sg->item0;
+  const int64_t it = local_item(sg, item);
   const int rt = (int)(it % row_tiles);
   const int cg = (int)(it / row_tiles);
   const int set = cg < sg->chunks0 ? 0 : 1;
@@ -1768,7 +1775,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_dk(const lddmm_seg_t *__restric
   const lddmm_seg_t *sg = segs + si;
   const int64_t ny = sg->ny;
   const int row_tiles = sg->row_tiles;
-  const int64_t it = item - sg->item0;
+  const int64_t it = local_item(sg, item);
   const int rt = (int)(it % row_tiles);
   const int cg = (int)(it / row_tiles);
   const double scale = coord_scale(prec, KT, sg->sigma);
@@ -1901,7 +1908,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_div(const lddmm_seg_t *__restri
   const lddmm_seg_t *sg = segs + si;
   const int64_t ny = sg->ny;
   const int row_tiles = sg->row_tiles;
-  const int64_t it = item - sg->item0;
+  const int64_t it = local_item(sg, item);
   const int rt = (int)(it % row_tiles);
   const int cg = (int)(it / row_tiles);
   const double *xs = sg->x0 ? sg->x0 : sg->y;
@@ -2665,6 +2672,8 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
     const int64_t chunk_len = chunk_tiles * TS;
     const int64_t c0 = ceil_div(nx0, chunk_len), c1 = nx1 ? ceil_div(nx1, chunk_len) : 0;
     if (row_tiles > INT32_MAX || chunk_len > INT32_MAX) return LDDMM_EINVAL;
+    // A dispatch order covers exactly this plan's items.
+    if (s.item_order && s.n_order != row_tiles * (c0 + c1)) return LDDMM_EINVAL;
     s.row_tiles = (int32_t)row_tiles;
     s.chunk_len = (int32_t)chunk_len;
     s.chunks0 = (int32_t)c0;
@@ -2993,7 +3002,8 @@ extern "C" int lddmm_segments_run_steps(int op, int prec, int d, const lddmm_seg
           a.part0 != b.part0 || a.fin0 != b.fin0 || a.red0 != b.red0 ||
           a.row_tiles != b.row_tiles || a.chunks0 != b.chunks0 || a.chunks1 != b.chunks1 ||
           a.chunk_len != b.chunk_len || a.fin_blocks != b.fin_blocks || a.stage0 != b.stage0 ||
-          !a.perm != !b.perm || !a.rowdot != !b.rowdot || !a.out0 != !b.out0)
+          !a.perm != !b.perm || !a.rowdot != !b.rowdot || !a.out0 != !b.out0 ||
+          !a.item_order != !b.item_order || a.n_order != b.n_order)
         return LDDMM_EINVAL;
     }
     int rc = lddmm_segments_run(op, prec, d, h, segs_dev + (int64_t)t * nseg, nseg, n_items,
@@ -3564,6 +3574,163 @@ extern "C" double lddmm_order_compactness(const double *points, int64_t n, int d
     }
   }
   return (double)ok / (double)total;
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch order of a segment's work items (host).  The hardware hands workgroups to
+// free slots in launch order; with items of very different cost (culled blocks, the
+// symmetric sums' skipped tiles) the last expensive ones keep a few CUs busy while the
+// rest idle.  Running the items in decreasing estimated cost leaves only cheap items for
+// the end (longest-processing-time-first list scheduling).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Boxes {
+  std::vector<double> lo, hi;  // (n, 3)
+};
+
+// Bounding boxes of consecutive runs of `run` points, of p(i) for i in [0, n).
+template <typename P>
+Boxes run_boxes(int64_t n, int64_t run, int d, P p) {
+  const int64_t nb = (n + run - 1) / run;
+  Boxes b;
+  b.lo.assign(nb * 3, 0.0);
+  b.hi.assign(nb * 3, 0.0);
+  for (int64_t k = 0; k < nb; ++k) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = k * run; i < n && i < (k + 1) * run; ++i) {
+      const double *v = p(i);
+      for (int c = 0; c < d; ++c) {
+        lo[c] = v[c] < lo[c] ? v[c] : lo[c];
+        hi[c] = v[c] > hi[c] ? v[c] : hi[c];
+      }
+    }
+    for (int c = 0; c < 3; ++c) {
+      b.lo[k * 3 + c] = c < d ? lo[c] : 0.0;
+      b.hi[k * 3 + c] = c < d ? hi[c] : 0.0;
+    }
+  }
+  return b;
+}
+
+double box_gap2(const Boxes &a, int64_t i, const Boxes &b, int64_t j) {
+  double d2 = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const double g = std::max(std::max(a.lo[i * 3 + c] - b.hi[j * 3 + c],
+                                       b.lo[j * 3 + c] - a.hi[i * 3 + c]), 0.0);
+    d2 += g * g;
+  }
+  return d2;
+}
+
+}  // namespace
+
+extern "C" int lddmm_item_order(int op, int prec, int d, const lddmm_seg_t *seg, const double *y,
+                                const double *x1, const int32_t *perm, int32_t *order,
+                                int64_t n) {
+  if (!seg || (d != 2 && d != 3)) return LDDMM_EINVAL;
+  if (op != LDDMM_OP_FLOW_STEP && op != LDDMM_OP_COVECTOR_STEP && op != LDDMM_OP_KDOT &&
+      op != LDDMM_OP_KNORM && op != LDDMM_OP_CURRENTS && op != LDDMM_OP_MEASURE)
+    return LDDMM_EUNSUPPORTED;
+  // Plan a copy.  Device pointers the plan only checks for presence get stand-ins; the
+  // output pointers and rowdot keep the caller's values (they decide the energy-only form).
+  lddmm_seg_t s = *seg;
+  s.item_order = nullptr;
+  s.n_order = 0;
+  if (!s.y) s.y = QP;
+  if (!s.w0) s.w0 = QP;
+  switch (op) {
+    case LDDMM_OP_FLOW_STEP:
+      if (!s.out0) s.out0 = (double *)QP;
+      break;
+    case LDDMM_OP_COVECTOR_STEP:
+      if (!s.w1) s.w1 = QP;
+      if (!s.out0) s.out0 = (double *)QP;
+      break;
+    case LDDMM_OP_KDOT:
+      if (s.na < 1) s.na = 1;
+      for (int j = 0; j < s.na && j < 2; ++j)
+        if (!s.g[j]) s.g[j] = QP;
+      break;
+    case LDDMM_OP_CURRENTS:
+    case LDDMM_OP_MEASURE:
+      if (s.nx1 > 0 && !s.x1) s.x1 = QP;
+      if (s.nx1 > 0 && !s.w1) s.w1 = QP;
+      if (op == LDDMM_OP_MEASURE && s.out0 && !s.w2) s.w2 = QP;
+      break;
+  }
+  int64_t n_items = 0, n_fin = 0;
+  size_t wb = 0;
+  int rc = lddmm_segments_plan(op, prec, d, &s, 1, &n_items, &n_fin, &wb);
+  if (rc) return rc;
+  if (!order) return n_items > INT32_MAX ? LDDMM_EINVAL : (int)n_items;  // size query
+  if (!y || n != n_items) return LDDMM_EINVAL;
+  const bool two_sets = (op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE) && s.nx1 > 0;
+  if (two_sets && !x1) return LDDMM_EINVAL;
+  Shape sh;
+  rc = op_shape(op, d, s, &sh);
+  if (rc) return rc;
+
+  const bool f32 = prec == LDDMM_PREC_FP32C;
+  const int64_t wave_rows = 64 * (f32 ? rows_per_thread<float>() : rows_per_thread<double>());
+  const int64_t tile_rows = (BLOCK / 64) * wave_rows;
+  const double scale = coord_scale(prec, s.kernel, s.sigma);
+  const double cut2 = cull_r2(prec, s.kernel, s.sigma);
+  // The pair kernel's savings that only energy-only sums take: each unordered tile pair of
+  // the self sum once, and (FP64 Gaussian) the FP32 exponential on far blocks, whose pairs
+  // cost about half of a near block's.
+  const bool energy_only = !s.rowdot && !s.out0 && sh.NA == 0 &&
+                           (op == LDDMM_OP_KNORM || op == LDDMM_OP_CURRENTS ||
+                            op == LDDMM_OP_MEASURE);
+  const bool mixed = energy_only && !f32 && s.kernel == LDDMM_KERNEL_GAUSS;
+  const double far2 = 26.0 * 2.0 * s.sigma * s.sigma / LOG2E;
+
+  std::vector<double> q0pts((size_t)s.ny * 3), q1pts;
+  for (int64_t i = 0; i < s.ny; ++i) {
+    const int64_t g = perm ? (int64_t)perm[i] : i;
+    if (g < 0 || g >= s.ny) return LDDMM_EINVAL;
+    for (int c = 0; c < d; ++c) q0pts[i * 3 + c] = y[g * d + c] * scale;
+  }
+  if (two_sets) {
+    q1pts.resize((size_t)s.nx1 * 3);
+    for (int64_t i = 0; i < s.nx1; ++i)
+      for (int c = 0; c < d; ++c) q1pts[i * 3 + c] = x1[i * d + c] * scale;
+  }
+  const Boxes waves = run_boxes(s.ny, wave_rows, d, [&](int64_t i) { return &q0pts[i * 3]; });
+  const Boxes quarters0 = run_boxes(s.ny, 64, d, [&](int64_t i) { return &q0pts[i * 3]; });
+  Boxes quarters1;
+  if (two_sets)
+    quarters1 = run_boxes(s.nx1, 64, d, [&](int64_t i) { return &q1pts[i * 3]; });
+  const int64_t nwaves = (int64_t)waves.lo.size() / 3;
+
+  std::vector<int64_t> cost((size_t)n_items);
+  for (int64_t it = 0; it < n_items; ++it) {
+    const int64_t rt = it % s.row_tiles, cg = it / s.row_tiles;
+    const int set = cg < s.chunks0 ? 0 : 1;
+    const int64_t ch = set ? cg - s.chunks0 : cg;
+    const int64_t nsrc = set ? s.nx1 : s.nx0;
+    const int64_t l0 = ch * s.chunk_len;
+    const int64_t l1 = std::min(l0 + (int64_t)s.chunk_len, nsrc);
+    const Boxes &qb = set ? quarters1 : quarters0;
+    int64_t c = 1;  // the item's fixed part: rows, boxes, partial stores
+    for (int w = 0; w < BLOCK / 64; ++w) {
+      const int64_t wi = rt * (BLOCK / 64) + w;
+      if (wi >= nwaves) break;
+      const int64_t wave_blk = (rt * tile_rows + w * wave_rows) / TS;
+      for (int64_t q = l0; q < l1; q += 64) {
+        if (energy_only && set == 0 && q / TS < wave_blk) continue;  // symmetric: skipped
+        const double g2 = box_gap2(waves, wi, qb, q / 64);
+        if (g2 > cut2) continue;
+        c += (mixed && !(g2 > far2)) ? 2 : 1;
+      }
+    }
+    cost[it] = c;
+  }
+  std::vector<int32_t> idx((size_t)n_items);
+  for (int64_t i = 0; i < n_items; ++i) idx[i] = (int32_t)i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+  std::copy(idx.begin(), idx.end(), order);
+  return LDDMM_OK;
 }
 
 extern "C" int lddmm_last_hip_error(void) { return g_last_hip_error; }

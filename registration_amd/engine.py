@@ -139,6 +139,52 @@ def compact_fraction(points, order, sigma: float) -> float:
     return f
 
 
+# Segments with fewer work items than this fit in about one round of workgroups on the chip
+# (256 CUs hold 1,024-1,280 pair-kernel workgroups at once): no dispatch order shortens them.
+MIN_ORDERED_ITEMS = 1024
+
+
+def item_order(op: int, prec: int, d: int, rec: np.ndarray, y, x1=None, perm=None,
+               min_items: Optional[int] = None) -> Optional[torch.Tensor]:
+    """Dispatch order of one segment's work items, the items with the most unculled pairs
+    first (lddmm_item_order), as a device int32 tensor for ``attach_item_order``; None when
+    the segment plans fewer than ``min_items`` (MIN_ORDERED_ITEMS) items or its operation
+    takes no order (lddmm_item_order lists those that do).  ``rec``: the
+    (1,) descriptor as it will be launched (device pointers may still be unset); ``y``: host
+    copy of its rows (ny, d); ``x1``: host copy of a data-term segment's target points;
+    ``perm``: host copy of its processing order.  The estimate only steers the schedule, so
+    an order made from a registration's template serves every later step and trial without
+    changing any result."""
+    lib = _lib.load()
+    r = np.ascontiguousarray(np.array(rec, copy=True).reshape(1))
+    r["item_order"] = 0
+    r["n_order"] = 0
+    sp = r.ctypes.data_as(ctypes.POINTER(_lib.Seg))
+    n = lib.lddmm_item_order(op, prec, d, sp, None, None, None, None, 0)
+    if n == _lib.EUNSUPPORTED:   # an operation whose launches run in item order only
+        return None
+    if n < 0:
+        _lib.check(n, "lddmm_item_order")
+    if n < (MIN_ORDERED_ITEMS if min_items is None else min_items):
+        return None
+    Y = np.ascontiguousarray(np.asarray(y, dtype=np.float64))
+    X1 = None if x1 is None else np.ascontiguousarray(np.asarray(x1, dtype=np.float64))
+    P = None if perm is None else np.ascontiguousarray(np.asarray(perm, dtype=np.int32))
+    out = np.empty(n, dtype=np.int32)
+    _lib.check(lib.lddmm_item_order(op, prec, d, sp, Y.ctypes.data,
+                                    None if X1 is None else X1.ctypes.data,
+                                    None if P is None else P.ctypes.data, out.ctypes.data, n),
+               "lddmm_item_order")
+    return torch.from_numpy(out).to(rt.device())
+
+
+def attach_item_order(rec: np.ndarray, order: Optional[torch.Tensor]) -> None:
+    """Point descriptor(s) ``rec`` at a dispatch order (None clears it); the caller keeps
+    the tensor alive while the descriptors are launched."""
+    rec["item_order"] = 0 if order is None else order.data_ptr()
+    rec["n_order"] = 0 if order is None else order.numel()
+
+
 def order_for(points, K: "DevKernel", min_fraction: float = 0.5):
     """The processing order a registration of ``points`` under K should use, as a device
     int32 tensor, or None when it would not pay: K has no dot-product form, or fewer than
@@ -345,6 +391,7 @@ def kdot_records(base, prec, d, T, xt, g1, g2, na, out) -> None:
     first = np.concatenate([[0], np.cumsum(T)[:-1]])
     tt = (np.arange(int(T.sum())) - first[rep]).astype(np.uint64)
     r = base[rep].copy()
+    attach_item_order(r, None)   # a flow step's dispatch order does not fit the KDOT plan
     r["dt"] = 0.0
     r["na"] = np.asarray(na, dtype=np.int32)[rep]
     st = stride[rep]

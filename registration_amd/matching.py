@@ -39,6 +39,8 @@ VARIFOLD = 'varifold'
 LANDMARK = 'landmark'
 SHAPE_TERMS = (CURRENT, MEASURE, VARIFOLD)
 _DATA_OP = {CURRENT: _lib.OP_CURRENTS, MEASURE: _lib.OP_MEASURE, VARIFOLD: _lib.OP_VARIFOLD}
+# operations whose descriptors take a dispatch order (lddmm_item_order)
+_ORDERED_OPS = (_lib.OP_FLOW_STEP, _lib.OP_COVECTOR_STEP, _lib.OP_CURRENTS, _lib.OP_MEASURE)
 
 
 class Direction:
@@ -125,7 +127,13 @@ class DeviceMatching:
         # Pointer-free descriptor parts of this registration's launches, built once; the
         # batched requests fill in the buffer addresses of each call with NumPy.
         # Kept as raw bytes (runtime.raw / runtime.stack).
-        self._flow_base = rt.raw(engine.flow_record(N, T, self.Kdiff, perm=self._order))
+        # Dispatch orders of the large launches (work items with the most unculled pairs
+        # first), estimated once from the template and the target; they only schedule.
+        self._order_np = None if self._order is None else self._order.cpu().numpy()
+        self._item_orders = []
+        flow = engine.flow_record(N, T, self.Kdiff, perm=self._order)
+        self._attach_item_order(flow, _lib.OP_FLOW_STEP, self.Kdiff.prec, x0_np)
+        self._flow_base = rt.raw(flow)
         self._flow_key = (self.Kdiff.kernel, self.Kdiff.prec, d)
         self._slice_bytes = N * d * 8
         # Gradient buffers, reused by every gradient request: the covector (T+1, N, d) with
@@ -134,13 +142,19 @@ class DeviceMatching:
         self._pxt = torch.empty_like(self.xt)
         self._pxt_tail = self._pxt[1:]
         self._pxT = self._pxt.data_ptr() + T * self._slice_bytes
-        self._cov_base = rt.raw(engine.covector_record(N, T, self.Kdiff, self.regweight,
-                                                       perm=self._order))
+        cov = engine.covector_record(N, T, self.Kdiff, self.regweight, perm=self._order)
+        self._attach_item_order(cov, _lib.OP_COVECTOR_STEP, self.Kdiff.prec, x0_np)
+        self._cov_base = rt.raw(cov)
         self._def_stale = False
         if self.data_kind in SHAPE_TERMS:
             Ke = self.Kdist.energy()
             op = _DATA_OP[self.data_kind]
-            self._cur_try = rt.raw(self._data_record(self._try_shape, Ke))
+            # template and target points of the data term, for its dispatch orders
+            c1_np = np.asarray(x0_np, dtype=np.float64)[self.faces_np].mean(axis=1)
+            c2_np = rt.as_dev(target_centers).cpu().numpy()
+            rec = self._data_record(self._try_shape, Ke)
+            self._attach_item_order(rec, op, Ke.prec, c1_np, c2_np, perm=False)
+            self._cur_try = rt.raw(rec)
             self._cur_key = (Ke.kernel, Ke.prec, d, op)
             self._geom_try = rt.raw(engine.geometry_record(self._try_shape))
             sh = self.fvDef_dev
@@ -152,6 +166,7 @@ class DeviceMatching:
             g["out0"] = self._z1.data_ptr()
             g["out1"] = self._dz.data_ptr()
             g["total"] = self._gtot.data_ptr()
+            self._attach_item_order(g, op, self.Kdist.prec, c1_np, c2_np, perm=False)
             self._cur_grad = rt.raw(g)
             self._cur_grad_key = (self.Kdist.kernel, self.Kdist.prec, d, op)
             self._gather = rt.raw(engine.gather_record(sh, self._z1, self._dz,
@@ -188,6 +203,18 @@ class DeviceMatching:
         self.sym = None
         if template_geometry is not None:
             self.sym = _TemplateTerm(self, *template_geometry)
+
+    def _attach_item_order(self, rec, op, prec, y, x1=None, perm=True):
+        """Give descriptor ``rec`` (op ``op``) a dispatch order estimated from host points
+        ``y`` (its rows) and ``x1`` (its target set), through the registration's processing
+        order when ``perm``; kept alive in self._item_orders."""
+        if op not in _ORDERED_OPS:
+            return
+        o = engine.item_order(op, prec, self.dim, rec, y, x1=x1,
+                              perm=self._order_np if perm else None)
+        if o is not None:
+            self._item_orders.append(o)
+            engine.attach_item_order(rec, o)
 
     def _data_record(self, sh, K, gradient=False, target=None):
         """Descriptor of the data term of device shape ``sh`` against the target (outputs

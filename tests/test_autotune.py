@@ -9,15 +9,21 @@ import pytest
 from registration_amd import _lib, runtime as rt
 
 
-@pytest.fixture()
-def lib():
-    lib = _lib.load()
-    yield lib
+def _builtin_targets(lib):
     for op in _lib.OP_IDS.values():
         for prec in (0, 1):
             for eo in (0, 1):
                 for c in range(6):
                     lib.lddmm_set_items_target(op, prec, eo, c, 0)
+
+
+@pytest.fixture()
+def lib():
+    lib = _lib.load()
+    _builtin_targets(lib)
+    yield lib
+    _builtin_targets(lib)
+    _lib.apply_tuning(lib)  # the targets every later test of the process plans with
 
 
 def test_size_classes(lib):

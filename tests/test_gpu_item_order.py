@@ -1,0 +1,114 @@
+"""Dispatch orders of work items on the GPU: a launch with a segment's items in
+lddmm_item_order's order gives the same results bit for bit as in item order (flow step,
+covector step, FP64 currents energy, FP32 currents gradient), and a registration whose
+launches carry orders follows the same optimisation path to the last bit."""
+import numpy as np
+import pytest
+import torch
+
+from registration_amd import _lib, engine, runtime as rt, synthetic
+from registration_amd import kernelFunctions as kfun, surfaceMatching as sm, surfaces
+
+pytestmark = pytest.mark.gpu
+
+
+def _with_and_without(op, prec, rec_fields, y_np, x1_np=None, perm_np=None, outs=()):
+    """Run one segment twice, in item order and in its dispatch order; return both results
+    (the named output tensors, cloned) and the order."""
+    res = []
+    order = None
+    for ordered in (False, True):
+        rec = rt.seg(**rec_fields)
+        if ordered:
+            order = engine.item_order(op, prec, 3, rec, y_np, x1=x1_np, perm=perm_np, min_items=1)
+            assert order is not None
+            engine.attach_item_order(rec, order)
+        p = rt.Plan(op, prec, 3, [rec])
+        p.run()
+        torch.cuda.synchronize()
+        res.append([t.clone() for t in outs])
+    return res, order
+
+
+def test_flow_and_covector_steps_identical_in_any_item_order():
+    V, _ = synthetic.icosphere(5)
+    N = V.shape[0]
+    rng = np.random.default_rng(0)
+    x = torch.as_tensor(V, device="cuda")
+    a = torch.as_tensor(1e-2 * rng.standard_normal(V.shape), device="cuda")
+    p1 = torch.as_tensor(rng.standard_normal(V.shape), device="cuda")
+    perm_np = engine.spatial_order(V)
+    perm = torch.as_tensor(perm_np, device="cuda")
+    tot = torch.empty(1, dtype=torch.float64, device="cuda")
+    for prec, sigma in ((_lib.PREC_FP32C, 0.25), (_lib.PREC_FP64, 0.25), (_lib.PREC_FP32C, 0.1)):
+        out = torch.empty_like(x)
+        fields = dict(y=x, w0=a, out0=out, total=tot, ny=N, sigma=sigma, dt=0.05,
+                      kernel=_lib.KERNEL_GAUSS, order=10)
+        if prec == _lib.PREC_FP32C:
+            fields["perm"] = perm
+        (r0, r1), order = _with_and_without(_lib.OP_FLOW_STEP, prec, fields, V,
+                                            perm_np=perm_np if "perm" in fields else None,
+                                            outs=(out, tot))
+        if prec == _lib.PREC_FP32C:   # some items of the ordered sphere are culled
+            assert not np.array_equal(order.cpu().numpy(), np.arange(order.numel()))
+        assert all(torch.equal(u, v) for u, v in zip(r0, r1)), (prec, sigma)
+        cov = torch.empty_like(x)
+        fields = dict(y=x, w0=a, w1=p1, out0=cov, ny=N, sigma=sigma, dt=0.05, coef=1.0,
+                      kernel=_lib.KERNEL_GAUSS, order=10)
+        if prec == _lib.PREC_FP32C:
+            fields["perm"] = perm
+        (r0, r1), _ = _with_and_without(_lib.OP_COVECTOR_STEP, prec, fields, V,
+                                        perm_np=perm_np if "perm" in fields else None,
+                                        outs=(cov,))
+        assert torch.equal(r0[0], r1[0]), (prec, sigma)
+
+
+def test_currents_energy_and_gradient_identical_in_any_item_order():
+    cfg = synthetic.surface_pair_config1()
+    F, V = cfg["template"]
+    _, Y = cfg["target"]
+    V5, F5 = synthetic.icosphere(5)
+    Y5 = synthetic.bump_target(V5)
+    for (FF, VV, YY) in ((F, V, Y), (F5, V5, Y5)):
+        S = surfaces.DeviceSurface(FF, torch.as_tensor(VV, device="cuda"))
+        T = surfaces.DeviceSurface(FF, torch.as_tensor(YY, device="cuda"))
+        nf = FF.shape[0]
+        c1, c2 = S.centers.cpu().numpy(), T.centers.cpu().numpy()
+        tot = torch.empty(1, dtype=torch.float64, device="cuda")
+        base = dict(y=S.centers, w0=S.surfel, x1=T.centers, w1=T.surfel, ny=nf, nx1=nf,
+                    sigma=0.1, coef=1.0 / 3.0, kernel=_lib.KERNEL_GAUSS, order=10, total=tot)
+        (r0, r1), _ = _with_and_without(_lib.OP_CURRENTS, _lib.PREC_FP64, base, c1, x1_np=c2,
+                                        outs=(tot,))
+        assert torch.equal(r0[0], r1[0])
+        z1 = torch.empty((nf, 3), dtype=torch.float64, device="cuda")
+        dz = torch.empty_like(z1)
+        (r0, r1), _ = _with_and_without(_lib.OP_CURRENTS, _lib.PREC_FP32C,
+                                        dict(base, out0=z1, out1=dz), c1, x1_np=c2,
+                                        outs=(z1, dz, tot))
+        assert all(torch.equal(u, v) for u, v in zip(r0, r1))
+
+
+def test_registration_path_identical_with_dispatch_orders(monkeypatch):
+    """Orders attach only to launches of at least MIN_ORDERED_ITEMS items; with the bound at
+    1 every flow, covector and data-term launch of this registration carries one."""
+    V, F = synthetic.icosphere(4)
+    Y = synthetic.bump_target(V)
+
+    def run():
+        p = sm.SurfaceMatchingParam(timeStep=0.1,
+                                    KparDiff=kfun.Kernel('gauss', sigma=0.3, precision='fp32c'),
+                                    KparDist=kfun.Kernel('gauss', sigma=0.2, precision='fp32c'),
+                                    sigmaError=0.1, errorType='current')
+        m = sm.SurfaceMatching(outputDir=None, Template=surfaces.Surface(FV=(F, V)),
+                               Target=surfaces.Surface(FV=(F, Y)), param=p, maxIter=4,
+                               testGradient=False)
+        m.optimizeMatching()
+        return m
+
+    plain = run()
+    assert not plain._item_orders
+    monkeypatch.setattr(engine, "MIN_ORDERED_ITEMS", 1)
+    ordered = run()
+    assert len(ordered._item_orders) == 4   # flow, covector, FP64 energy, FP32 gradient
+    assert np.array_equal(np.asarray(plain.history), np.asarray(ordered.history))
+    assert torch.equal(plain.at, ordered.at)
