@@ -19,7 +19,10 @@ Representative problems, one per size class: icosphere vertices (subdivisions 3-
 2,562, 10,242, 40,962 and 163,842 points) and the face centres of subdivision 7 (327,680)
 for the flow and covector steps, with a Gaussian deformation kernel of width 0.25 on the
 unit sphere, in the spatial processing order the registrations use; icosphere faces
-(subdivisions 2-7: 320 to 327,680) for the currents sums, width 0.1.
+(subdivisions 2-7: 320 to 327,680) for the currents sums, width 0.1.  Each launch carries the
+dispatch order of its work items that a registration's launch of the same plan carries
+(engine.item_order): with the longest items first the tail of a launch is short, which
+moves the best target towards fewer, larger items.
 """
 from __future__ import annotations
 
@@ -33,7 +36,7 @@ import torch
 from . import _lib, engine, runtime as rt, surfaces, synthetic
 from .kernelFunctions import Kernel
 
-CANDIDATES = (768, 1536, 3072, 6144, 12288, 24576, 49152)
+CANDIDATES = (768, 1536, 3072, 4096, 6144, 8192, 12288, 16384, 24576, 49152)
 # Icosphere subdivision whose vertices (flow, covector) represent each size class; 8 stands
 # for the 327,680 face centres of subdivision 7.
 CLASS_SUBDIV = (3, 4, 5, 6, 7, 8)
@@ -69,6 +72,7 @@ def _runner(op, prec, energy_only, cls):
         fields = dict(y=S.centers, ny=nf, x1=T.centers, nx1=nf, w0=S.surfel, w1=T.surfel,
                       sigma=0.1, coef=1.0 / 3.0, kernel=_lib.KERNEL_GAUSS, order=10, total=tot)
         keep = [S, T, tot]
+        hosts = (S.centers.cpu().numpy(), T.centers.cpu().numpy(), None)
         if not energy_only:
             z1 = torch.empty((nf, 3), dtype=torch.float64, device=dev)
             dz = torch.empty((nf, 3), dtype=torch.float64, device=dev)
@@ -95,6 +99,7 @@ def _runner(op, prec, energy_only, cls):
         if perm is not None:
             fields["perm"] = perm
         keep = [x, a, out, tot, perm] + ([keep_px] if op != "FLOW_STEP" else [])
+        hosts = (V, None, None if perm is None else perm.cpu().numpy())
         rows = N
     rec = rt.seg(**fields)
     code = _lib.OP_IDS[op]
@@ -102,8 +107,12 @@ def _runner(op, prec, energy_only, cls):
 
     def run(reps):
         # reps launches from one library call (lddmm_segments_run_steps), so that small
-        # problems are timed on the GPU, not on the host's launch loop
-        steps = np.repeat(np.asarray(rec).reshape(1, 1), reps, axis=0)
+        # problems are timed on the GPU, not on the host's launch loop; with the dispatch
+        # order a registration's launch of this plan carries (the plan follows the target)
+        r = np.array(rec, copy=True)
+        order = engine.item_order(code, pc, 3, r, hosts[0], x1=hosts[1], perm=hosts[2])
+        engine.attach_item_order(r, order)
+        steps = np.repeat(np.asarray(r).reshape(1, 1), reps, axis=0)
         rt.run_steps(code, pc, 3, steps[:2])
         torch.cuda.synchronize()
         st = torch.cuda.current_stream()
@@ -112,7 +121,7 @@ def _runner(op, prec, energy_only, cls):
         rt.run_steps(code, pc, 3, steps)
         e1.record(st)
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps, rt.Plan(code, pc, 3, [rec]).n_items
+        return e0.elapsed_time(e1) / reps, rt.Plan(code, pc, 3, [r]).n_items
     run.keep = keep
     return run, rows
 
