@@ -29,6 +29,7 @@ KERNEL_LAPLACIAN = 1
 
 PREC_FP64 = 0
 PREC_FP32C = 1
+PREC_FP64E = 2   # data-term energies of FP32C registrations (energy-only segments)
 
 OP_APPLY_K = 0
 OP_DIFF_KT = 1

@@ -399,11 +399,25 @@ __host__ __device__ inline double cull_r2(int prec, int kernel, double sigma) {
     // The FP32 bound, 2^-24, is FP32's unit roundoff: a source beyond it adds less than half
     // an ulp of its own weight (on the large surface pair the flow step runs 13% faster than
     // at 2^-32 for the same energy error, 2.4e-8).
-    return f32 ? 24.0 : 56.0 * 2.0 * sigma * sigma / LOG2E;
+    // FP64E (the energies of FP32C registrations): 2^-40; see far_r2.
+    if (f32) return 24.0;
+    return (prec == LDDMM_PREC_FP64E ? 40.0 : 56.0) * 2.0 * sigma * sigma / LOG2E;
   }
-  // (1+u)^4 e^-u = 2^-24 at u~30.4, 2^-56 at u~54.9; FP32 coordinates carry log2(e)/sigma.
-  const double u_cut = f32 ? 30.5 * LOG2E : 55.0 * sigma;
+  // (1+u)^4 e^-u = 2^-24 at u~30.4, 2^-40 at u~42.8, 2^-56 at u~54.9; FP32 coordinates carry
+  // log2(e)/sigma.
+  const double u_cut =
+      f32 ? 30.5 * LOG2E : (prec == LDDMM_PREC_FP64E ? 43.0 : 55.0) * sigma;
   return u_cut * u_cut;
+}
+
+// Mixed precision by magnitude (FP64 Gaussian energy sums): blocks whose box distance puts
+// every kernel value below 2^-26 (FP64) or 2^-16 (FP64E) take the FP32 exponential of the
+// FP64 exponent, a relative error of at most ~2e-6 per term.  FP64E is FP32C's energy form:
+// with the cull at 2^-40 it keeps the data term of the 81,920-face surface pair within
+// 2.2e-10 of the FP64 value (measured at a data term 6.4e-7 of the single sums, where the
+// culled terms of the shape's self sum and of the cross sum cancel), for 24% less time.
+__host__ __device__ inline double far_r2(int prec, double sigma) {
+  return (prec == LDDMM_PREC_FP64E ? 16.0 : 26.0) * 2.0 * sigma * sigma / LOG2E;
 }
 
 // Dot-product exponent (FP32 Gaussian K-sums of segments with a processing order).
@@ -914,11 +928,12 @@ void k_pairs(const lddmm_seg_t *__restrict__ segs,
   const int64_t wg_blk = ((int64_t)rt * (BLOCK * RPT)) / TS;
   // Mixed precision by magnitude (FP64 Gaussian energy sums only: segments without per-row
   // outputs, both source sets): runs of 64 sources whose bounding box is so far from the
-  // wave's rows that every kernel value is below 2^-26 are evaluated with an FP32 exponential.
+  // wave's rows that every kernel value is below far_r2's bound are evaluated with an FP32
+  // exponential.
   constexpr bool MIXED = std::is_same<R, double>::value && KT == LDDMM_KERNEL_GAUSS && KSUM &&
                          NA == 0;
   const bool mixed = MIXED && energy_only;
-  const R far2 = MIXED ? (R)(26.0 * 2.0 * sg->sigma * sg->sigma / LOG2E) : (R)0;
+  const R far2 = MIXED ? (R)far_r2(prec, sg->sigma) : (R)0;
 
   for (int64_t base = l0; base < l1; base += TS) {
     if (sym && base / TS < wg_blk) continue;  // workgroup-uniform: below every wave's block
@@ -1115,8 +1130,8 @@ void k_pairs(const lddmm_seg_t *__restrict__ segs,
       if constexpr (XFORM) {
         if ((xq >> (j0 / 64)) & 1u) {
           if (mixed && ((far >> (j0 / 64)) & 1u)) {
-            // Every pair of this run has K < 2^-26 (see below): v_exp_f32 of the expanded
-            // FP64 exponent.
+            // Every pair of this run has K below far_r2's bound (see below): v_exp_f32 of
+            // the expanded FP64 exponent.
 #pragma unroll PAIR_UNROLL
             for (int j = j0; j < j0 + KFP; ++j) {
               R sx[D], sw[M];
@@ -1184,9 +1199,10 @@ void k_pairs(const lddmm_seg_t *__restrict__ segs,
       }
       if constexpr (MIXED) {
         if (mixed && ((far >> (j0 / 64)) & 1u)) {
-          // Every pair of this run has K < 2^-26: v_exp_f32 of the FP64 exponent (relative
-          // error ~4e-6 from rounding an exponent of magnitude < 80 to FP32) changes each
-          // term by less than 1e-13 of a unit kernel value; the products accumulate in FP64.
+          // Every pair of this run has K below far_r2's bound (2^-26 in FP64): v_exp_f32 of
+          // the FP64 exponent (relative error ~4e-6 from rounding an exponent of magnitude
+          // < 80 to FP32) changes each term by less than 1e-13 of a unit kernel value (FP64;
+          // 2e-11 in FP64E); the products accumulate in FP64.
 #pragma unroll 8
           for (int j = j0; j < j0 + KFP; ++j) {
             R sx[D], sw[M];
@@ -1558,11 +1574,11 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_varifold(const lddmm_seg_t *__r
   // Energy-only self set: each unordered pair of TS blocks once (see k_pairs).
   const bool energy_only = !GRAD && !sg->rowdot && !sg->out0;
   const bool sym = energy_only && set == 0;
-  // FP64 Gaussian energies: 64-source runs whose every kernel value is below 2^-26 take the
-  // FP32 exponential of the FP64 exponent, as in k_pairs.
+  // FP64 Gaussian energies: 64-source runs whose every kernel value is below far_r2's bound
+  // take the FP32 exponential of the FP64 exponent, as in k_pairs.
   constexpr bool MIXED = std::is_same<R, double>::value && KT == LDDMM_KERNEL_GAUSS && !GRAD;
   const bool mixed = MIXED && energy_only;
-  const R far2 = MIXED ? (R)(26.0 * 2.0 * sg->sigma * sg->sigma / LOG2E) : (R)0;
+  const R far2 = MIXED ? (R)far_r2(prec, sg->sigma) : (R)0;
   const int64_t wave_blk = ((int64_t)rt * (BLOCK * RPT) + wave * (64 * RPT)) / TS;
   const int64_t wg_blk = ((int64_t)rt * (BLOCK * RPT)) / TS;
 
@@ -2565,7 +2581,8 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
                                    int64_t *n_items, int64_t *n_fin_blocks,
                                    size_t *work_bytes) {
   if (!op_valid(op) || nseg < 1 || !segs || (d != 2 && d != 3)) return LDDMM_EINVAL;
-  if (prec != LDDMM_PREC_FP64 && prec != LDDMM_PREC_FP32C) return LDDMM_EINVAL;
+  if (prec != LDDMM_PREC_FP64 && prec != LDDMM_PREC_FP32C && prec != LDDMM_PREC_FP64E)
+    return LDDMM_EINVAL;
   Shape shp;
   int rc = op_shape(op, d, segs[0], &shp);
   if (rc) return rc;
@@ -2666,6 +2683,7 @@ extern "C" int lddmm_segments_plan(int op, int prec, int d, lddmm_seg_t *segs, i
         !s.rowdot && !s.out0 && shp.NA == 0 &&
         (op == LDDMM_OP_KNORM || op == LDDMM_OP_CURRENTS || op == LDDMM_OP_MEASURE ||
          op == LDDMM_OP_VARIFOLD);
+    if (prec == LDDMM_PREC_FP64E && !energy_only) return LDDMM_EINVAL;  // energies only
     int64_t chunk_tiles =
         ceil_div(src_tiles * row_tiles, items_target(op, prec, energy_only, s.ny));
     if (chunk_tiles < 1) chunk_tiles = 1;
@@ -2942,7 +2960,8 @@ extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *se
                                   int64_t n_fin_blocks, void *work, size_t work_bytes,
                                   void *stream) {
   if (!op_valid(op) || nseg < 1 || !segs_host || (d != 2 && d != 3) || !work) return LDDMM_EINVAL;
-  if (prec != LDDMM_PREC_FP64 && prec != LDDMM_PREC_FP32C) return LDDMM_EINVAL;
+  if (prec != LDDMM_PREC_FP64 && prec != LDDMM_PREC_FP32C && prec != LDDMM_PREC_FP64E)
+    return LDDMM_EINVAL;
   if (n_items < 1 || n_items > INT32_MAX || n_fin_blocks < 1 || n_fin_blocks > INT32_MAX)
     return LDDMM_EINVAL;
   const lddmm_seg_t &last = segs_host[nseg - 1];
@@ -3683,7 +3702,7 @@ extern "C" int lddmm_item_order(int op, int prec, int d, const lddmm_seg_t *seg,
                            (op == LDDMM_OP_KNORM || op == LDDMM_OP_CURRENTS ||
                             op == LDDMM_OP_MEASURE);
   const bool mixed = energy_only && !f32 && s.kernel == LDDMM_KERNEL_GAUSS;
-  const double far2 = 26.0 * 2.0 * s.sigma * s.sigma / LOG2E;
+  const double far2 = far_r2(prec, s.sigma);
 
   std::vector<double> q0pts((size_t)s.ny * 3), q1pts;
   for (int64_t i = 0; i < s.ny; ++i) {

@@ -51,8 +51,12 @@ class DevKernel:
         """The form used for data-term energies: always FP64 pairwise arithmetic.  A currents
         norm is a small difference of large sums (|S|^2 - 2<S,T> + |T|^2), and per-pair FP32
         rounding leaves an error near 1e-8 of the large sums, i.e. far above 1e-6 of the
-        objective once the shapes match."""
-        return DevKernel(self.kernel, self.order, self.sigma, _lib.PREC_FP64)
+        objective once the shapes match.  FP32-compensated kernels take PREC_FP64E (FP64
+        arithmetic, pairs below 2^-40 culled and the FP32 exponential below 2^-16), which
+        keeps the data term within ~2e-10 of FP64 where the shapes nearly match; FP64 kernels
+        keep the full FP64 thresholds."""
+        prec = _lib.PREC_FP64 if self.prec == _lib.PREC_FP64 else _lib.PREC_FP64E
+        return DevKernel(self.kernel, self.order, self.sigma, prec)
 
 
 def _groups(jobs, key):

@@ -107,6 +107,26 @@ def test_plan_rejects_invalid():
                  [_seg(10, out1=4096, kernel=_lib.KERNEL_LAPLACIAN, order=0)])[0] == _lib.EINVAL
 
 
+def test_fp64e_plans_energy_only_segments_alone():
+    """FP64E (the energies of FP32C registrations) plans like FP64 and only for sums whose
+    total alone is read: KNORM, and CURRENTS / MEASURE / VARIFOLD without per-row outputs."""
+    E = _lib.PREC_FP64E
+    assert _plan(_lib.OP_KNORM, E, 3, [_seg(10, out0=0)])[0] == 0
+    assert _plan(_lib.OP_KNORM, E, 3, [_seg(10, out0=0, rowdot=4096)])[0] == _lib.EINVAL
+    cur = _seg(300, out0=0, nx1=7, x1=4096)
+    rc, arr, items, fins, wb = _plan(_lib.OP_CURRENTS, E, 3, [cur])
+    rc64, arr64, items64, fins64, wb64 = _plan(_lib.OP_CURRENTS, _lib.PREC_FP64, 3, [cur])
+    assert rc == 0 and rc64 == 0 and (items, fins, wb) == (items64, fins64, wb64)
+    assert arr[0].row_tiles == arr64[0].row_tiles == 2    # one FP64 row per thread
+    assert _plan(_lib.OP_CURRENTS, E, 3, [_seg(300, nx1=7, x1=4096, out1=4096)])[0] == _lib.EINVAL
+    assert _plan(_lib.OP_MEASURE, E, 3, [_seg(10, w2=0, out0=0)])[0] == 0
+    assert _plan(_lib.OP_VARIFOLD, E, 3, [_seg(10, out0=0)])[0] == 0
+    assert _plan(_lib.OP_VARIFOLD, E, 3, [_seg(10, out1=4096)])[0] == _lib.EINVAL
+    for op in (_lib.OP_FLOW_STEP, _lib.OP_APPLY_K, _lib.OP_KDOT, _lib.OP_DIVERGENCE):
+        assert _plan(op, E, 3, [_seg(10, m=3, na=1, g=(4096, 0))])[0] == _lib.EINVAL, op
+    assert _plan(_lib.OP_KNORM, 3, 3, [_seg(10, out0=0)])[0] == _lib.EINVAL
+
+
 def test_plan_processing_order():
     """A processing order is accepted for the flow and covector steps and the RKHS products
     only."""
