@@ -269,6 +269,14 @@ def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradien
                         "fp64_peak_tflops": FP64_PEAK_TFLOPS,
                         "frac_of_fp64_peak": round(tf / FP64_PEAK_TFLOPS, 4)})
         out[f"{name}_energy_fp64"] = rec
+        # the same energy in FP64E, the form every line-search trial of an FP32-compensated
+        # registration runs (FP64 arithmetic, cull 2^-40, FP32 exponential below 2^-16)
+        tot64 = tot.item()
+        ms = timed(rt.Plan(op, _lib.PREC_FP64E, 3,
+                           [ordered(op, _lib.PREC_FP64E, rt.seg(**base, **weights[name]))]))
+        out[f"{name}_energy_fp64e"] = {"ms": round(ms, 4), "nominal_pairs": pairs,
+                                       "nominal_pairs_per_s": pairs / (ms * 1e-3),
+                                       "total_minus_fp64": tot.item() - tot64}
         if not gradients:
             continue
         ms = timed(rt.Plan(op, _lib.PREC_FP32C, 3,
