@@ -31,10 +31,10 @@ FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
 # HBM bytes per fused flow step at 40,962 points (tile staging + pair kernel + finalize + totals),
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on `bench.py --roofline-only
 # --roofline-reps 3` and scaled by the counters' reading for 256 MiB device copies (FETCH x2.0,
-# WRITE x1.0), in the template's spatial processing order: 83.0 MB read + 64.0 MB written, most
-# of it the FP64 chunk partials (54 chunks x 40,962 rows x 3), written by the pair kernel and
-# read back by the finalize (profiles/flow_step_s6_fp32c.md).
-FLOW_STEP_TRAFFIC_S6 = 147.0e6
+# WRITE x1.0), in the template's spatial processing order with the items longest first: 98.8 MB
+# read + 64.4 MB written, most of it the FP64 chunk partials (54 chunks x 40,962 rows x 3),
+# written by the pair kernel and read back by the finalize (profiles/flow_step_s6_fp32c.md).
+FLOW_STEP_TRAFFIC_S6 = 163.2e6
 
 
 def _warm(plan, seconds=0.25):
@@ -201,7 +201,10 @@ DATA_TERM_FLOPS = {"currents": 16, "measure": 12, "varifold": 20}
 # `bench.py --energy-only --data-term-reps 3` (profiles/currents_energy_s6_fp64.md).  Fewer
 # than the nominal pairs x flops: far tiles are culled, each unordered block pair of the self
 # sum is evaluated once, and far blocks take the FP32 exponential.
-EVALUATED_FP64_FLOP_S6 = {"currents": 5.813e10, "measure": 4.686e10, "varifold": 1.011e11}
+EVALUATED_FP64_FLOP_S6 = {"currents": 5.781e10, "measure": 4.686e10, "varifold": 1.011e11}
+# The same counter on the FP64E energies (cull 2^-40, FP32 exponential below 2^-16) of that
+# run, which the line-search trials of FP32-compensated registrations evaluate.
+EVALUATED_FP64E_FLOP_S6 = {"currents": 4.197e10, "measure": 3.379e10, "varifold": 7.444e10}
 
 
 def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradients=True):
@@ -274,9 +277,15 @@ def data_terms_leg(cfg, reps, names=("currents", "measure", "varifold"), gradien
         tot64 = tot.item()
         ms = timed(rt.Plan(op, _lib.PREC_FP64E, 3,
                            [ordered(op, _lib.PREC_FP64E, rt.seg(**base, **weights[name]))]))
-        out[f"{name}_energy_fp64e"] = {"ms": round(ms, 4), "nominal_pairs": pairs,
-                                       "nominal_pairs_per_s": pairs / (ms * 1e-3),
-                                       "total_minus_fp64": tot.item() - tot64}
+        rec = {"ms": round(ms, 4), "nominal_pairs": pairs,
+               "nominal_pairs_per_s": pairs / (ms * 1e-3), "total_minus_fp64": tot.item() - tot64}
+        flop = EVALUATED_FP64E_FLOP_S6.get(name) if cfg.get("name") == "surface-s6" else None
+        if flop is not None:
+            tf = flop / (ms * 1e-3) / 1e12
+            rec.update({"evaluated_fp64_flop": flop, "evaluated_fp64_tflops": round(tf, 3),
+                        "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                        "frac_of_fp64_peak": round(tf / FP64_PEAK_TFLOPS, 4)})
+        out[f"{name}_energy_fp64e"] = rec
         if not gradients:
             continue
         ms = timed(rt.Plan(op, _lib.PREC_FP32C, 3,
@@ -806,7 +815,9 @@ def main():
     fp64_form = (fp64_form_leg(cfg, final_obj)
                  if args.precision == "fp32c" and not args.no_fp64_form else None)
 
-    roof = flow_step_roofline(cfg, args.precision, args.roofline_reps)
+    # (--roofline-reps 0 leaves the roofline leg out: profiled runs of the registration alone)
+    roof = (flow_step_roofline(cfg, args.precision, args.roofline_reps)
+            if args.roofline_reps > 0 else None)
     data_terms = data_terms_leg(cfg, args.data_term_reps) if args.data_term_reps > 0 else None
     _progress("energy check, roofline and data-term legs done")
     cpu = None
