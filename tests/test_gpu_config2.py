@@ -77,6 +77,7 @@ def test_config2_fp32c_energy_error():
     worst = 0.0
     for at, e in stash:
         m64.at = at
+        m64.obj = None          # objectiveFun caches the objective of the current momenta
         e64 = m64.objectiveFun()
         worst = max(worst, abs(e - e64) / abs(e64))
     assert worst <= 1e-6, worst
