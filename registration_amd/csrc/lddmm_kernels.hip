@@ -759,10 +759,11 @@ void k_pairs(const lddmm_seg_t *__restrict__ segs,
   constexpr int W = (KSUM ? M : 0) + (NA > 0 ? D : 0);
   constexpr bool KAHAN = std::is_same<R, float>::value;
   constexpr int NKAH = KAHAN ? 1 : 0;
-  // Sources per unrolled step of the pair loop, measured on gfx950: 4 for both (8 costs the
-  // FP32 flow step 4%; with the table exponential the FP64 energy is 1.8% faster at 4 than
-  // at 8).
-  constexpr int PAIR_UNROLL = 4;
+  // Sources per unrolled step of the pair loop, measured on gfx950: 4 (8 costs the FP32 flow
+  // step 4%, 2 costs it 8%; with the table exponential the FP64 energy is 1.8% faster at 4
+  // than at 8), and 2 for sums with two or more derivative weight pairs (the covector step:
+  // 126 instead of 154 VGPRs, 4 waves per SIMD instead of 3, 1.2% faster).
+  constexpr int PAIR_UNROLL = NA >= 2 ? 2 : 4;
   // Sources per FP32 run before its Kahan fold: 64 for the plain K-sums (one culling
   // quarter), which halves the fold cost of the 32 used elsewhere; measured on the
   // 40,962-point flow step: 2-3% faster, the step's error against FP64 from 1.6e-7 to 2.3e-7
@@ -3642,6 +3643,16 @@ double box_gap2(const Boxes &a, int64_t i, const Boxes &b, int64_t j) {
   return d2;
 }
 
+// Largest squared distance between a point of box i of a and a point of box j of b.
+double box_far2(const Boxes &a, int64_t i, const Boxes &b, int64_t j) {
+  double d2 = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const double g = std::max(a.hi[i * 3 + c] - b.lo[j * 3 + c], b.hi[j * 3 + c] - a.lo[i * 3 + c]);
+    d2 += g * g;
+  }
+  return d2;
+}
+
 }  // namespace
 
 extern "C" int lddmm_item_order(int op, int prec, int d, const lddmm_seg_t *seg, const double *y,
@@ -3716,11 +3727,59 @@ extern "C" int lddmm_item_order(int op, int prec, int d, const lddmm_seg_t *seg,
       for (int c = 0; c < d; ++c) q1pts[i * 3 + c] = x1[i * d + c] * scale;
   }
   const Boxes waves = run_boxes(s.ny, wave_rows, d, [&](int64_t i) { return &q0pts[i * 3]; });
+  const Boxes rtiles = run_boxes(s.ny, tile_rows, d, [&](int64_t i) { return &q0pts[i * 3]; });
   const Boxes quarters0 = run_boxes(s.ny, 64, d, [&](int64_t i) { return &q0pts[i * 3]; });
-  Boxes quarters1;
-  if (two_sets)
+  const Boxes tiles0 = run_boxes(s.ny, TS, d, [&](int64_t i) { return &q0pts[i * 3]; });
+  Boxes quarters1, tiles1;
+  if (two_sets) {
     quarters1 = run_boxes(s.nx1, 64, d, [&](int64_t i) { return &q1pts[i * 3]; });
+    tiles1 = run_boxes(s.nx1, TS, d, [&](int64_t i) { return &q1pts[i * 3]; });
+  }
   const int64_t nwaves = (int64_t)waves.lo.size() / 3;
+  const int wpt = BLOCK / 64;  // waves per row tile
+
+  // Cost of one (row tile, source tile) pair: its live (wave, quarter) blocks, FP64 near
+  // blocks counted twice.  The pair's bounding boxes decide most pairs at once (all blocks
+  // culled, or all within the cut / the near radius); the rest are tested block by block.
+  auto tile_cost = [&](int64_t rt, int set, int64_t ti) -> int64_t {
+    const Boxes &tb = set ? tiles1 : tiles0;
+    const Boxes &qb = set ? quarters1 : quarters0;
+    const int64_t nsrc = set ? s.nx1 : s.nx0;
+    const int64_t q0 = ti * (TS / 64), q1 = std::min((ti + 1) * (TS / 64), (nsrc + 63) / 64);
+    const double g2 = box_gap2(rtiles, rt, tb, ti);
+    if (g2 > cut2) return 0;
+    const double m2 = box_far2(rtiles, rt, tb, ti);
+    int uniform = 0;  // cost of every block when the pair decides them all
+    if (!mixed && m2 <= cut2) uniform = 1;
+    if (mixed && m2 <= far2) uniform = 2;
+    if (mixed && g2 > far2 && m2 <= cut2) uniform = 1;
+    int64_t c = 0;
+    for (int w = 0; w < wpt; ++w) {
+      const int64_t wi = rt * wpt + w;
+      if (wi >= nwaves) break;
+      const int64_t wave_blk = (rt * tile_rows + w * wave_rows) / TS;
+      if (energy_only && set == 0 && ti < wave_blk) continue;  // symmetric: skipped
+      int wu = uniform;  // the same decision for this wave's box alone
+      if (!wu) {
+        const double wg2 = box_gap2(waves, wi, tb, ti);
+        if (wg2 > cut2) continue;
+        const double wm2 = box_far2(waves, wi, tb, ti);
+        if (!mixed && wm2 <= cut2) wu = 1;
+        if (mixed && wm2 <= far2) wu = 2;
+        if (mixed && wg2 > far2 && wm2 <= cut2) wu = 1;
+      }
+      if (wu) {
+        c += wu * (q1 - q0);
+        continue;
+      }
+      for (int64_t q = q0; q < q1; ++q) {
+        const double b2 = box_gap2(waves, wi, qb, q);
+        if (b2 > cut2) continue;
+        c += (mixed && !(b2 > far2)) ? 2 : 1;
+      }
+    }
+    return c;
+  };
 
   std::vector<int64_t> cost((size_t)n_items);
   for (int64_t it = 0; it < n_items; ++it) {
@@ -3728,21 +3787,10 @@ extern "C" int lddmm_item_order(int op, int prec, int d, const lddmm_seg_t *seg,
     const int set = cg < s.chunks0 ? 0 : 1;
     const int64_t ch = set ? cg - s.chunks0 : cg;
     const int64_t nsrc = set ? s.nx1 : s.nx0;
-    const int64_t l0 = ch * s.chunk_len;
-    const int64_t l1 = std::min(l0 + (int64_t)s.chunk_len, nsrc);
-    const Boxes &qb = set ? quarters1 : quarters0;
+    const int64_t t0 = ch * s.chunk_len / TS;
+    const int64_t t1 = std::min((ch + 1) * (int64_t)s.chunk_len, nsrc + TS - 1) / TS;
     int64_t c = 1;  // the item's fixed part: rows, boxes, partial stores
-    for (int w = 0; w < BLOCK / 64; ++w) {
-      const int64_t wi = rt * (BLOCK / 64) + w;
-      if (wi >= nwaves) break;
-      const int64_t wave_blk = (rt * tile_rows + w * wave_rows) / TS;
-      for (int64_t q = l0; q < l1; q += 64) {
-        if (energy_only && set == 0 && q / TS < wave_blk) continue;  // symmetric: skipped
-        const double g2 = box_gap2(waves, wi, qb, q / 64);
-        if (g2 > cut2) continue;
-        c += (mixed && !(g2 > far2)) ? 2 : 1;
-      }
-    }
+    for (int64_t ti = t0; ti < t1 && ti * TS < nsrc; ++ti) c += tile_cost(rt, set, ti);
     cost[it] = c;
   }
   std::vector<int32_t> idx((size_t)n_items);

@@ -174,12 +174,14 @@ def item_order(op: int, prec: int, d: int, rec: np.ndarray, y, x1=None, perm=Non
     Y = np.ascontiguousarray(np.asarray(y, dtype=np.float64))
     X1 = None if x1 is None else np.ascontiguousarray(np.asarray(x1, dtype=np.float64))
     P = None if perm is None else np.ascontiguousarray(np.asarray(perm, dtype=np.int32))
-    out = np.empty(n, dtype=np.int32)
+    host = torch.empty(n, dtype=torch.int32, pin_memory=True)
+    out = host.numpy()
     _lib.check(lib.lddmm_item_order(op, prec, d, sp, Y.ctypes.data,
                                     None if X1 is None else X1.ctypes.data,
                                     None if P is None else P.ctypes.data, out.ctypes.data, n),
                "lddmm_item_order")
-    return torch.from_numpy(out).to(rt.device())
+    # through pinned memory, stream-ordered: no wait on the work already queued
+    return host.to(rt.device(), non_blocking=True)
 
 
 def attach_item_order(rec: np.ndarray, order: Optional[torch.Tensor]) -> None:
@@ -189,9 +191,9 @@ def attach_item_order(rec: np.ndarray, order: Optional[torch.Tensor]) -> None:
     rec["n_order"] = 0 if order is None else order.numel()
 
 
-def order_for(points, K: "DevKernel", min_fraction: float = 0.5):
-    """The processing order a registration of ``points`` under K should use, as a device
-    int32 tensor, or None when it would not pay: K has no dot-product form, or fewer than
+def order_choice(points, K: "DevKernel", min_fraction: float = 0.5) -> Optional[np.ndarray]:
+    """The processing order a registration of ``points`` under K should use, as host int32
+    indices, or None when it would not pay: K has no dot-product form, or fewer than
     ``min_fraction`` of the quarters would be compact enough for it (small point sets at
     wide kernels, where the order only adds gathers).  The spatial order competes with the
     points' own order (the identity permutation): sampled curves are already compact in
@@ -205,7 +207,13 @@ def order_for(points, K: "DevKernel", min_fraction: float = 0.5):
     best, f = (ident, f_id) if f_id >= f_o else (o, f_o)
     if f < min_fraction:
         return None
-    return torch.from_numpy(best).to(rt.device())
+    return best
+
+
+def order_for(points, K: "DevKernel", min_fraction: float = 0.5):
+    """order_choice as a device int32 tensor (the ``perm`` of a pair-sum descriptor)."""
+    o = order_choice(points, K, min_fraction)
+    return None if o is None else torch.from_numpy(o).to(rt.device())
 
 
 # --------------------------------------------------------------------------------------

@@ -88,7 +88,9 @@ class DeviceMatching:
         self.x0 = rt.as_dev(x0_np)
         # Processing order of the template's points for the flow and RKHS-product sums
         # (spatially compact tiles; the trajectory keeps the points' own order).
-        self._order = engine.order_for(x0_np, self.Kdiff)
+        self._order_np = engine.order_choice(x0_np, self.Kdiff)
+        self._order = (None if self._order_np is None
+                       else torch.from_numpy(self._order_np).pin_memory().to(dev, non_blocking=True))
         T, N, d = Tsize, self.npt, self.dim
         if target_points is not None:
             self.data_kind = LANDMARK
@@ -128,12 +130,14 @@ class DeviceMatching:
         # batched requests fill in the buffer addresses of each call with NumPy.
         # Kept as raw bytes (runtime.raw / runtime.stack).
         # Dispatch orders of the large launches (work items with the most unculled pairs
-        # first), estimated once from the template and the target; they only schedule.
-        self._order_np = None if self._order is None else self._order.cpu().numpy()
+        # first) are estimated from the template and the target when the registration
+        # launches its second line-search trial (_count_trial): they only schedule, and a
+        # registration that launches a few sums only (the precision guard's FP64 twin)
+        # would spend more host time on them than they save.
         self._item_orders = []
-        flow = engine.flow_record(N, T, self.Kdiff, perm=self._order)
-        self._attach_item_order(flow, _lib.OP_FLOW_STEP, self.Kdiff.prec, x0_np)
-        self._flow_base = rt.raw(flow)
+        self._trials = 0
+        self._order_pts = np.array(x0_np, dtype=np.float64)
+        self._flow_base = rt.raw(engine.flow_record(N, T, self.Kdiff, perm=self._order))
         self._flow_key = (self.Kdiff.kernel, self.Kdiff.prec, d)
         self._slice_bytes = N * d * 8
         # Gradient buffers, reused by every gradient request: the covector (T+1, N, d) with
@@ -142,19 +146,16 @@ class DeviceMatching:
         self._pxt = torch.empty_like(self.xt)
         self._pxt_tail = self._pxt[1:]
         self._pxT = self._pxt.data_ptr() + T * self._slice_bytes
-        cov = engine.covector_record(N, T, self.Kdiff, self.regweight, perm=self._order)
-        self._attach_item_order(cov, _lib.OP_COVECTOR_STEP, self.Kdiff.prec, x0_np)
-        self._cov_base = rt.raw(cov)
+        self._cov_base = rt.raw(engine.covector_record(N, T, self.Kdiff, self.regweight,
+                                                       perm=self._order))
         self._def_stale = False
         if self.data_kind in SHAPE_TERMS:
             Ke = self.Kdist.energy()
             op = _DATA_OP[self.data_kind]
-            # template and target points of the data term, for its dispatch orders
-            c1_np = np.asarray(x0_np, dtype=np.float64)[self.faces_np].mean(axis=1)
-            c2_np = rt.as_dev(target_centers).cpu().numpy()
-            rec = self._data_record(self._try_shape, Ke)
-            self._attach_item_order(rec, op, Ke.prec, c1_np, c2_np, perm=False)
-            self._cur_try = rt.raw(rec)
+            # the target's points, for the data term's dispatch order
+            self._order_tgt = (target_centers.cpu().numpy() if torch.is_tensor(target_centers)
+                               else np.array(target_centers, dtype=np.float64))
+            self._cur_try = rt.raw(self._data_record(self._try_shape, Ke))
             self._cur_key = (Ke.kernel, Ke.prec, d, op)
             self._geom_try = rt.raw(engine.geometry_record(self._try_shape))
             sh = self.fvDef_dev
@@ -166,7 +167,8 @@ class DeviceMatching:
             g["out0"] = self._z1.data_ptr()
             g["out1"] = self._dz.data_ptr()
             g["total"] = self._gtot.data_ptr()
-            self._attach_item_order(g, op, self.Kdist.prec, c1_np, c2_np, perm=False)
+            # (no dispatch order: a registration runs this launch once per gradient, a few
+            # times in all, and its order would cost more host time than it saves)
             self._cur_grad = rt.raw(g)
             self._cur_grad_key = (self.Kdist.kernel, self.Kdist.prec, d, op)
             self._gather = rt.raw(engine.gather_record(sh, self._z1, self._dz,
@@ -204,17 +206,34 @@ class DeviceMatching:
         if template_geometry is not None:
             self.sym = _TemplateTerm(self, *template_geometry)
 
-    def _attach_item_order(self, rec, op, prec, y, x1=None, perm=True):
-        """Give descriptor ``rec`` (op ``op``) a dispatch order estimated from host points
-        ``y`` (its rows) and ``x1`` (its target set), through the registration's processing
-        order when ``perm``; kept alive in self._item_orders."""
+    def _attach_item_order(self, base, op, prec, y, x1=None, perm=True):
+        """Give the cached descriptor bytes ``base`` (op ``op``) a dispatch order estimated
+        from host points ``y`` (its rows) and ``x1`` (its target set), through the
+        registration's processing order when ``perm``; kept alive in self._item_orders."""
         if op not in _ORDERED_OPS:
             return
+        rec = base.view(rt.SEG_DT)          # the same bytes: the cached descriptor changes
         o = engine.item_order(op, prec, self.dim, rec, y, x1=x1,
                               perm=self._order_np if perm else None)
         if o is not None:
             self._item_orders.append(o)
             engine.attach_item_order(rec, o)
+
+    def _count_trial(self):
+        """Called for every line-search trial launch; the second one attaches the dispatch
+        orders of the flow, covector and data-term energy launches."""
+        self._trials += 1
+        if self._trials != 2:
+            return
+        self._attach_item_order(self._flow_base, _lib.OP_FLOW_STEP, self.Kdiff.prec,
+                                self._order_pts)
+        self._attach_item_order(self._cov_base, _lib.OP_COVECTOR_STEP, self.Kdiff.prec,
+                                self._order_pts)
+        if self.data_kind in SHAPE_TERMS:
+            c1 = self._order_pts[self.faces_np].mean(axis=1)
+            Ke = self.Kdist.energy()
+            self._attach_item_order(self._cur_try, _DATA_OP[self.data_kind], Ke.prec, c1,
+                                    self._order_tgt, perm=False)
 
     def _data_record(self, sh, K, gradient=False, target=None):
         """Descriptor of the data term of device shape ``sh`` against the target (outputs
@@ -587,6 +606,8 @@ def launch_update_try(items) -> Pending:
     dev = rt.device()
     n = len(items)
     opts = [it[0] for it in items]
+    for o in opts:
+        o._count_trial()
     Ts = np.fromiter((o.Tsize for o in opts), np.int64, n)
     offs = np.zeros(n, np.int64)
     np.cumsum(Ts[:-1] + 1, out=offs[1:])

@@ -90,7 +90,7 @@ def test_currents_energy_and_gradient_identical_in_any_item_order():
 
 def test_registration_path_identical_with_dispatch_orders(monkeypatch):
     """Orders attach only to launches of at least MIN_ORDERED_ITEMS items; with the bound at
-    1 every flow, covector and data-term launch of this registration carries one."""
+    1 every flow, covector and data-term energy launch of this registration carries one."""
     V, F = synthetic.icosphere(4)
     Y = synthetic.bump_target(V)
 
@@ -109,6 +109,6 @@ def test_registration_path_identical_with_dispatch_orders(monkeypatch):
     assert not plain._item_orders
     monkeypatch.setattr(engine, "MIN_ORDERED_ITEMS", 1)
     ordered = run()
-    assert len(ordered._item_orders) == 4   # flow, covector, FP64 energy, FP32 gradient
+    assert len(ordered._item_orders) == 3   # flow, covector, FP64E energy
     assert np.array_equal(np.asarray(plain.history), np.asarray(ordered.history))
     assert torch.equal(plain.at, ordered.at)
