@@ -363,7 +363,9 @@ def cpu_baseline(cfg, pair_counts):
     k_pairs = sum(v for op, v in pair_counts.items() if op != _lib.OP_COVECTOR_STEP)
     d_pairs = pair_counts.get(_lib.OP_COVECTOR_STEP, 0)
     threads = _omp_threads()
-    kr, dr, rows, took = _cpu_rates(cfg, 8192)
+    # Samples of about 10 s of CPU work in all: every row of both sums on all cores (about
+    # 3 s at the large pair on 16 cores), 8,192 rows on one core (about 6 s).
+    kr, dr, rows, took = _cpu_rates(cfg, N)
     secs = k_pairs / kr + d_pairs / dr
     out = {"value": 1.0 / secs, "unit": "registrations/s", "cores": threads, "kind": "port",
            "sample": f"oracle applyK and applyDiffKT(na=3), {rows} x {N} pairs each "
@@ -371,7 +373,7 @@ def cpu_baseline(cfg, pair_counts):
            "extrapolated": True, "host": _host(), "pairs_per_s_applyk": kr,
            "pairs_per_s_diffkt": dr, "seconds_per_registration": secs}
     _omp_threads(1)
-    kr1, dr1, rows1, took1 = _cpu_rates(cfg, 1024)
+    kr1, dr1, rows1, took1 = _cpu_rates(cfg, 8192)
     secs1 = k_pairs / kr1 + d_pairs / dr1
     out["single_thread"] = {"value": 1.0 / secs1, "cores": 1,
                             "sample": f"{rows1} x {N} pairs each ({took1:.1f} s)",
