@@ -416,6 +416,16 @@ int lddmm_spatial_order(const double *points, int64_t n, int d, int32_t *order);
 double lddmm_order_compactness(const double *points, int64_t n, int d, const int32_t *order,
                                double scale, double r2max);
 
+/* Fraction of the (row block, 64-point quarter) blocks of a self sum over the
+ * points in `order` whose bounding boxes lie within squared distance cut2 of each
+ * other: the blocks a pair kernel evaluates rather than culls.  A row block is
+ * `rows` consecutive points of the order (a wave's rows: 128 in FP32C, 64 in
+ * FP64; a multiple of 64), rows past the end repeating the last point.  Beyond
+ * max_pairs block pairs every k-th row block is taken.  Host code; returns -1 for
+ * invalid arguments. */
+double lddmm_order_live_fraction(const double *points, int64_t n, int d, const int32_t *order,
+                                 int rows, double cut2, int64_t max_pairs);
+
 /* Dispatch order of a segment's work items, longest first (the seg_t field
  * item_order).  seg: the segment as it will be launched (its sizes, kernel, sigma,
  * perm and output flags; device pointers may still be NULL, they are not read);

@@ -3596,6 +3596,63 @@ extern "C" double lddmm_order_compactness(const double *points, int64_t n, int d
   return (double)ok / (double)total;
 }
 
+extern "C" double lddmm_order_live_fraction(const double *points, int64_t n, int d,
+                                            const int32_t *order, int rows, double cut2,
+                                            int64_t max_pairs) {
+  if (n < 1 || d < 1 || d > 3 || !points || !order || rows < 64 || rows % 64 ||
+      !(cut2 >= 0.0) || max_pairs < 1)
+    return -1.0;
+  // Boxes of the 64-point quarters; a row block is `rows / 64` consecutive quarters.  Rows
+  // past the end repeat the last point, which the last quarter's box already holds.
+  const int64_t nq = (n + 63) / 64;
+  std::vector<double> lo((size_t)nq * d), hi((size_t)nq * d);
+  for (int64_t q = 0; q < nq; ++q) {
+    for (int c = 0; c < d; ++c) {
+      lo[q * d + c] = INFINITY;
+      hi[q * d + c] = -INFINITY;
+    }
+    for (int64_t i = q * 64; i < n && i < q * 64 + 64; ++i) {
+      const int64_t k = order[i];
+      if (k < 0 || k >= n) return -1.0;
+      for (int c = 0; c < d; ++c) {
+        const double v = points[k * d + c];
+        lo[q * d + c] = v < lo[q * d + c] ? v : lo[q * d + c];
+        hi[q * d + c] = v > hi[q * d + c] ? v : hi[q * d + c];
+      }
+    }
+  }
+  const int per = rows / 64;
+  const int64_t nr = (nq + per - 1) / per;
+  int64_t step = (nr * nq + max_pairs - 1) / max_pairs;
+  if (step < 1) step = 1;
+  int64_t live = 0, total = 0;
+  for (int64_t r = 0; r < nr; r += step) {
+    double rl[3], rh[3];
+    for (int c = 0; c < d; ++c) {
+      rl[c] = INFINITY;
+      rh[c] = -INFINITY;
+    }
+    for (int64_t q = r * per; q < nq && q < (r + 1) * per; ++q)
+      for (int c = 0; c < d; ++c) {
+        rl[c] = lo[q * d + c] < rl[c] ? lo[q * d + c] : rl[c];
+        rh[c] = hi[q * d + c] > rh[c] ? hi[q * d + c] : rh[c];
+      }
+    for (int64_t q = 0; q < nq; ++q) {
+      double d2 = 0.0;
+      for (int c = 0; c < d; ++c) {
+        double g = lo[q * d + c] - rh[c];
+        const double g2 = rl[c] - hi[q * d + c];
+        g = g2 > g ? g2 : g;
+        g = g > 0.0 ? g : 0.0;
+        d2 += g * g;
+      }
+      live += d2 <= cut2;
+    }
+    total += nq;
+  }
+  return (double)live / (double)total;
+}
+
 // ---------------------------------------------------------------------------
 // Dispatch order of a segment's work items (host).  The hardware hands workgroups to
 // free slots in launch order; with items of very different cost (culled blocks, the

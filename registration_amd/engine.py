@@ -114,8 +114,9 @@ def spatial_order_dev(points, device=None) -> torch.Tensor:
 
 
 def wants_order(K: "DevKernel") -> bool:
-    """Whether a processing order can pay for K: the dot-product exponent form exists for
-    the FP32 Gaussian K-sums only (the gathers of other forms would only add loads)."""
+    """Whether K's sums have the dot-product exponent form, which only a processing order
+    enables (the FP32 Gaussian K-sums).  Every kernel's sums cull by tile and stage their
+    source tiles once per launch when ordered, so order_choice weighs an order for all."""
     return K.kernel == _lib.KERNEL_GAUSS and K.prec == _lib.PREC_FP32C
 
 
@@ -138,6 +139,37 @@ def compact_fraction(points, order, sigma: float) -> float:
         raise ValueError("order must hold one index per point")
     f = _lib.load().lddmm_order_compactness(P.ctypes.data, P.shape[0], P.shape[1], o.ctypes.data,
                                             _PRESCALE / sigma, EXP_R2MAX)
+    if f < 0:
+        raise ValueError("invalid points or order")
+    return f
+
+
+def cull_radius2(K: "DevKernel") -> float:
+    """Squared distance (in the points' own units) beyond which the pair kernels drop a
+    (wave, quarter) block of K's sums: the Gaussian below 2^-24 (FP32) or 2^-56 (FP64) of
+    its peak, the Laplacian beyond 30.5 sigma (FP32) or 55 sigma (FP64), as cull_r2 of the
+    kernel source bounds them."""
+    s2 = K.sigma * K.sigma
+    if K.kernel == _lib.KERNEL_GAUSS:
+        return (24.0 if K.prec == _lib.PREC_FP32C else 56.0) * 2.0 * s2 / np.log2(np.e)
+    return (30.5 if K.prec == _lib.PREC_FP32C else 55.0) ** 2 * s2
+
+
+def live_fraction(points, order, K: "DevKernel", max_pairs: int = 1 << 22) -> float:
+    """Fraction of a K-sum's (wave rows, 64-source quarter) blocks of the self sum over
+    ``points`` in ``order`` that the pair kernels evaluate: blocks whose bounding boxes lie
+    within cull_radius2 of each other (a wave holds 128 consecutive rows in FP32, 64 in
+    FP64; lddmm_order_live_fraction).  Beyond ``max_pairs`` block pairs, every k-th row
+    block is taken."""
+    P = np.ascontiguousarray(np.asarray(points, dtype=np.float64))
+    o = np.ascontiguousarray(np.asarray(order, dtype=np.int32))
+    if P.shape[0] == 0:
+        return 0.0
+    if o.shape != (P.shape[0],):
+        raise ValueError("order must hold one index per point")
+    rows = 128 if K.prec == _lib.PREC_FP32C else 64
+    f = _lib.load().lddmm_order_live_fraction(P.ctypes.data, P.shape[0], P.shape[1], o.ctypes.data,
+                                              rows, cull_radius2(K), max_pairs)
     if f < 0:
         raise ValueError("invalid points or order")
     return f
@@ -191,28 +223,47 @@ def attach_item_order(rec: np.ndarray, order: Optional[torch.Tensor]) -> None:
     rec["n_order"] = 0 if order is None else order.numel()
 
 
-def order_choice(points, K: "DevKernel", min_fraction: float = 0.5) -> Optional[np.ndarray]:
+# Smallest point sets whose sums take a processing order.  An ordered launch stages its
+# source tiles in a launch of its own (about 3.5 us on gfx950), which costs more than the
+# order saves on small sets (flow steps over icospheres of 642 and 2,562 points ran 10-15%
+# slower ordered, at every kernel width); from 10,242 points the Gaussian sums gain up to
+# 31% (FP64, sigma 0.05), at 40,962 points 19-48%.  Laplacian sums gain only through culling
+# and only from about 40,000 points (13-19% at sigma 0.05, 0-1% at wider kernels).
+MIN_ORDERED_POINTS = 8192
+MIN_ORDERED_POINTS_LAPLACIAN = 32768
+# Relative cost of a pair in the dot-product exponent form against the direct form (three
+# FMAs and the exponential against three differences, three FMAs and the exponential).
+DOT_FORM_COST = 0.78
+
+
+def order_cost(points, order, K: "DevKernel") -> float:
+    """Estimated work of K's self sums over ``points`` in ``order``, per nominal block pair:
+    the evaluated fraction of blocks, the FP32 Gaussian's compact quarters at DOT_FORM_COST."""
+    c = live_fraction(points, order, K)
+    if wants_order(K):
+        c *= 1.0 - (1.0 - DOT_FORM_COST) * compact_fraction(points, order, K.sigma)
+    return c
+
+
+def order_choice(points, K: "DevKernel") -> Optional[np.ndarray]:
     """The processing order a registration of ``points`` under K should use, as host int32
-    indices, or None when it would not pay: K has no dot-product form, or fewer than
-    ``min_fraction`` of the quarters would be compact enough for it (small point sets at
-    wide kernels, where the order only adds gathers).  The spatial order competes with the
-    points' own order (the identity permutation): sampled curves are already compact in
-    arc order, more so than after bisection."""
-    if not wants_order(K):
-        return None
+    indices, or None when no order pays (fewer than MIN_ORDERED_POINTS points, or
+    MIN_ORDERED_POINTS_LAPLACIAN for the Laplacian).  The spatial order competes with the
+    points' own order (the identity permutation) on order_cost: sampled curves are already
+    compact in arc order, more so than after bisection, and the identity order still stages
+    the source tiles once per launch."""
     P = np.asarray(points, dtype=np.float64)
-    ident = np.arange(P.shape[0], dtype=np.int32)
-    o = spatial_order(P)
-    f_o, f_id = compact_fraction(P, o, K.sigma), compact_fraction(P, ident, K.sigma)
-    best, f = (ident, f_id) if f_id >= f_o else (o, f_o)
-    if f < min_fraction:
+    n = P.shape[0]
+    if n < (MIN_ORDERED_POINTS if K.kernel == _lib.KERNEL_GAUSS else MIN_ORDERED_POINTS_LAPLACIAN):
         return None
-    return best
+    ident = np.arange(n, dtype=np.int32)
+    o = spatial_order(P)
+    return ident if order_cost(P, ident, K) <= order_cost(P, o, K) else o
 
 
-def order_for(points, K: "DevKernel", min_fraction: float = 0.5):
+def order_for(points, K: "DevKernel"):
     """order_choice as a device int32 tensor (the ``perm`` of a pair-sum descriptor)."""
-    o = order_choice(points, K, min_fraction)
+    o = order_choice(points, K)
     return None if o is None else torch.from_numpy(o).to(rt.device())
 
 

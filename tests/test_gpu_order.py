@@ -145,9 +145,9 @@ def test_ordered_flow_on_a_curve(which):
 
 def test_driver_takes_the_more_compact_order():
     """Curves keep their arc order (as an identity permutation); the large surface takes the
-    spatial order; small spheres at wide kernels take none."""
+    spatial order; small spheres take none."""
     from registration_amd import engine, synthetic
-    X = synthetic.star_curve(2048, 5, 0.2, noise=0.0, rng=np.random.default_rng(0))
+    X = synthetic.star_curve(16384, 5, 0.2, noise=0.0, rng=np.random.default_rng(0))
     o = engine.order_for(X, gauss(0.3))
     assert o is not None and np.array_equal(o.cpu().numpy(), np.arange(len(X)))
     V6, _ = synthetic.icosphere(6)

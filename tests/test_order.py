@@ -1,6 +1,7 @@
 """Processing order of a point set (engine.spatial_order over lddmm_spatial_order, host code of
 the kernel library): a permutation, deterministic, and spatially compact at the pair kernels'
-tile and quarter sizes; lddmm_order_compactness against a NumPy restatement.  Host only."""
+tile and quarter sizes; lddmm_order_compactness and lddmm_order_live_fraction against NumPy
+restatements; the driver's choice of an order.  Host only."""
 import numpy as np
 import pytest
 
@@ -9,19 +10,80 @@ from registration_amd import engine, synthetic
 PRESCALE = np.sqrt(np.log2(np.e) / 2)  # FP32 Gaussian coordinates: x * PRESCALE / sigma
 
 
-def test_order_is_taken_where_it_pays():
-    """The driver takes the order for the large surface at sigma 0.25 (every quarter compact)
-    and not for the 642-point sphere at sigma 0.3 (no quarter compact: only gathers), nor for
-    kernels without the dot-product form."""
+def _K(kind, sigma, prec='fp32c'):
     from registration_amd.kernelFunctions import Kernel
+    k = (Kernel('laplacian', sigma=sigma, order=3, precision=prec) if kind == 'lap'
+         else Kernel('gauss', sigma=sigma, precision=prec))
+    return engine.DevKernel.of(k)
+
+
+def test_order_is_taken_where_it_pays():
+    """Measured on gfx950 (flow and covector steps over icospheres): an order costs a staging
+    launch that small sets do not repay, so none below MIN_ORDERED_POINTS (Laplacian:
+    MIN_ORDERED_POINTS_LAPLACIAN); above, the spatial order where it culls more or takes
+    more dot-product quarters (the large surface at every width, FP32C and FP64), the
+    identity where it does not lose (FP64 at a width that culls nothing)."""
     V6, _ = synthetic.icosphere(6)
-    V3, _ = synthetic.icosphere(3)
-    g = lambda s, p='fp32c': engine.DevKernel.of(Kernel('gauss', sigma=s, precision=p))
-    assert engine.compact_fraction(V6, engine.spatial_order(V6), 0.25) > 0.99
-    assert engine.compact_fraction(V3, engine.spatial_order(V3), 0.3) < 0.1
-    assert engine.wants_order(g(0.25)) and not engine.wants_order(g(0.25, 'fp64'))
-    assert not engine.wants_order(engine.DevKernel.of(Kernel('laplacian', sigma=0.25, order=2,
-                                                            precision='fp32c')))
+    V5, _ = synthetic.icosphere(5)
+    V4, _ = synthetic.icosphere(4)
+    ident6 = np.arange(len(V6), dtype=np.int32)
+    for K in (_K('gauss', 0.25), _K('gauss', 0.5), _K('gauss', 0.1), _K('gauss', 0.1, 'fp64'),
+              _K('lap', 0.05)):
+        o = engine.order_choice(V6, K)
+        assert o is not None and not np.array_equal(o, ident6), (K.kernel, K.sigma, K.prec)
+    assert np.array_equal(engine.order_choice(V6, _K('gauss', 0.5, 'fp64')), ident6)
+    assert engine.order_choice(V4, _K('gauss', 0.5)) is None       # 2,562 points
+    assert engine.order_choice(V5, _K('lap', 0.05)) is None        # Laplacian, 10,242 points
+    assert engine.order_choice(V5, _K('gauss', 0.05)) is not None
+    assert engine.wants_order(_K('gauss', 0.25)) and not engine.wants_order(_K('gauss', 0.25, 'fp64'))
+    assert not engine.wants_order(_K('lap', 0.25))
+
+
+def test_curves_keep_their_arc_order():
+    """A sampled curve is compact in arc order: the choice is the identity permutation."""
+    X = synthetic.star_curve(16384, 5, 0.2, noise=0.0, rng=np.random.default_rng(0))
+    for K in (_K('gauss', 0.3), _K('gauss', 0.05), _K('gauss', 0.1, 'fp64')):
+        assert np.array_equal(engine.order_choice(X, K), np.arange(len(X)))
+
+
+def live_fraction_ref(P, order, rows, cut2):
+    """Restatement of lddmm_order_live_fraction: boxes of 64-point quarters of the order
+    (rows past the end repeat the last point), row blocks of rows / 64 quarters, the
+    fraction of (row block, quarter) box pairs within squared distance cut2."""
+    idx = np.asarray(order, dtype=np.int64)
+    nq = -(-len(idx) // 64)
+    idx = np.concatenate([idx, np.full(nq * 64 - len(idx), idx[-1])])
+    Q = P[idx].reshape(nq, 64, P.shape[1])
+    lo, hi = Q.min(axis=1), Q.max(axis=1)
+    per = rows // 64
+    nr = -(-nq // per)
+    rl = np.array([lo[r * per:(r + 1) * per].min(axis=0) for r in range(nr)])
+    rh = np.array([hi[r * per:(r + 1) * per].max(axis=0) for r in range(nr)])
+    g = np.maximum(np.maximum(lo[None] - rh[:, None], rl[:, None] - hi[None]), 0.0)
+    return float(((g * g).sum(axis=2) <= cut2).mean())
+
+
+@pytest.mark.parametrize("n,d,sigma,prec", [(1, 3, 0.3, 'fp32c'), (65, 2, 0.3, 'fp32c'),
+                                            (700, 3, 0.1, 'fp64'), (5000, 3, 0.2, 'fp32c'),
+                                            (40962, 3, 0.1, 'fp64')])
+def test_live_fraction_matches_a_numpy_restatement(n, d, sigma, prec):
+    P = (synthetic.icosphere(6)[0] if n == 40962
+         else np.random.default_rng(n).standard_normal((n, d)))
+    K = _K('gauss', sigma, prec)
+    rows = 128 if prec == 'fp32c' else 64
+    for o in (engine.spatial_order(P), np.arange(n, dtype=np.int32)):
+        ref = live_fraction_ref(P, o, rows, engine.cull_radius2(K))
+        assert engine.live_fraction(P, o, K) == pytest.approx(ref, abs=1e-12)
+
+
+def test_cull_radius_restates_the_kernel_bounds():
+    """The Gaussian at the cull radius is 2^-24 (FP32C) or 2^-56 (FP64) of its peak; the
+    Laplacian radius is 30.5 sigma (FP32C) or 55 sigma (FP64)."""
+    for prec, bits in (('fp32c', 24), ('fp64', 56)):
+        r2 = engine.cull_radius2(_K('gauss', 0.2, prec))
+        assert np.exp(-r2 / (2 * 0.2 ** 2)) == pytest.approx(2.0 ** -bits, rel=1e-12)
+    assert engine.cull_radius2(_K('lap', 0.2)) == pytest.approx((30.5 * 0.2) ** 2)
+    assert engine.cull_radius2(_K('lap', 0.2, 'fp64')) == pytest.approx((55 * 0.2) ** 2)
 
 
 def quarter_r2(P, order, sigma):
@@ -85,6 +147,10 @@ def test_order_rejects_bad_input():
         engine.spatial_order(np.zeros((10, 4)))
     with pytest.raises(ValueError):
         engine.compact_fraction(np.zeros((10, 3)), np.arange(9), 0.3)
+    with pytest.raises(ValueError):
+        engine.live_fraction(np.zeros((10, 3)), np.arange(9), _K('gauss', 0.3))
+    with pytest.raises(ValueError):
+        engine.live_fraction(np.zeros((10, 3)), np.arange(1, 11), _K('gauss', 0.3))
 
 
 def test_order_ignores_duplicates_and_flat_sets():
