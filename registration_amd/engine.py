@@ -180,40 +180,66 @@ def live_fraction(points, order, K: "DevKernel", max_pairs: int = 1 << 22) -> fl
 MIN_ORDERED_ITEMS = 1024
 
 
-def item_order(op: int, prec: int, d: int, rec: np.ndarray, y, x1=None, perm=None,
-               min_items: Optional[int] = None) -> Optional[torch.Tensor]:
-    """Dispatch order of one segment's work items, the items with the most unculled pairs
-    first (lddmm_item_order), as a device int32 tensor for ``attach_item_order``; None when
-    the segment plans fewer than ``min_items`` (MIN_ORDERED_ITEMS) items or its operation
-    takes no order (lddmm_item_order lists those that do).  ``rec``: the
-    (1,) descriptor as it will be launched (device pointers may still be unset); ``y``: host
-    copy of its rows (ny, d); ``x1``: host copy of a data-term segment's target points;
-    ``perm``: host copy of its processing order.  The estimate only steers the schedule, so
-    an order made from a registration's template serves every later step and trial without
-    changing any result."""
-    lib = _lib.load()
+def item_count(op: int, prec: int, d: int, rec: np.ndarray) -> int:
+    """Work items the plan of one segment ``rec`` has, or -1 when its operation takes no
+    dispatch order (lddmm_item_order with no output)."""
     r = np.ascontiguousarray(np.array(rec, copy=True).reshape(1))
     r["item_order"] = 0
     r["n_order"] = 0
-    sp = r.ctypes.data_as(ctypes.POINTER(_lib.Seg))
-    n = lib.lddmm_item_order(op, prec, d, sp, None, None, None, None, 0)
+    n = _lib.load().lddmm_item_order(op, prec, d, r.ctypes.data_as(ctypes.POINTER(_lib.Seg)),
+                                     None, None, None, None, 0)
     if n == _lib.EUNSUPPORTED:   # an operation whose launches run in item order only
-        return None
+        return -1
     if n < 0:
         _lib.check(n, "lddmm_item_order")
+    return int(n)
+
+
+def item_order_host(op: int, prec: int, d: int, rec: np.ndarray, y, x1=None, perm=None,
+                    min_items: Optional[int] = None) -> Optional[np.ndarray]:
+    """Dispatch order of one segment's work items, the items with the most unculled pairs
+    first (lddmm_item_order), as host int32 indices; None when the segment plans fewer than
+    ``min_items`` (MIN_ORDERED_ITEMS) items or its operation takes no order (lddmm_item_order
+    lists those that do).  ``rec``: the (1,) descriptor as it will be launched (device
+    pointers may still be unset); ``y``: host copy of its rows (ny, d); ``x1``: host copy of a
+    data-term segment's target points; ``perm``: host copy of its processing order.  Host
+    work only (the library call releases the interpreter lock), so it may run on a worker
+    thread.  The estimate only steers the schedule, so an order made from a registration's
+    template serves every later step and trial without changing any result."""
+    n = item_count(op, prec, d, rec)
     if n < (MIN_ORDERED_ITEMS if min_items is None else min_items):
         return None
+    r = np.ascontiguousarray(np.array(rec, copy=True).reshape(1))
+    r["item_order"] = 0
+    r["n_order"] = 0
     Y = np.ascontiguousarray(np.asarray(y, dtype=np.float64))
     X1 = None if x1 is None else np.ascontiguousarray(np.asarray(x1, dtype=np.float64))
     P = None if perm is None else np.ascontiguousarray(np.asarray(perm, dtype=np.int32))
-    host = torch.empty(n, dtype=torch.int32, pin_memory=True)
-    out = host.numpy()
-    _lib.check(lib.lddmm_item_order(op, prec, d, sp, Y.ctypes.data,
-                                    None if X1 is None else X1.ctypes.data,
-                                    None if P is None else P.ctypes.data, out.ctypes.data, n),
+    out = np.empty(n, dtype=np.int32)
+    _lib.check(_lib.load().lddmm_item_order(op, prec, d, r.ctypes.data_as(ctypes.POINTER(_lib.Seg)),
+                                            Y.ctypes.data,
+                                            None if X1 is None else X1.ctypes.data,
+                                            None if P is None else P.ctypes.data,
+                                            out.ctypes.data, n),
                "lddmm_item_order")
-    # through pinned memory, stream-ordered: no wait on the work already queued
+    return out
+
+
+def upload_order(order: Optional[np.ndarray]) -> Optional[torch.Tensor]:
+    """A host dispatch order as a device int32 tensor, through pinned memory and
+    stream-ordered (no wait on the work already queued)."""
+    if order is None:
+        return None
+    host = torch.empty(order.shape[0], dtype=torch.int32, pin_memory=True)
+    host.numpy()[:] = order
     return host.to(rt.device(), non_blocking=True)
+
+
+def item_order(op: int, prec: int, d: int, rec: np.ndarray, y, x1=None, perm=None,
+               min_items: Optional[int] = None) -> Optional[torch.Tensor]:
+    """item_order_host as a device int32 tensor for ``attach_item_order``."""
+    return upload_order(item_order_host(op, prec, d, rec, y, x1=x1, perm=perm,
+                                        min_items=min_items))
 
 
 def attach_item_order(rec: np.ndarray, order: Optional[torch.Tensor]) -> None:

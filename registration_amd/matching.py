@@ -130,12 +130,13 @@ class DeviceMatching:
         # batched requests fill in the buffer addresses of each call with NumPy.
         # Kept as raw bytes (runtime.raw / runtime.stack).
         # Dispatch orders of the large launches (work items with the most unculled pairs
-        # first) are estimated from the template and the target when the registration
-        # launches its second line-search trial (_count_trial): they only schedule, and a
-        # registration that launches a few sums only (the precision guard's FP64 twin)
-        # would spend more host time on them than they save.
+        # first) are estimated from the template and the target on a worker thread from the
+        # registration's first line-search trial on, and attached when ready (_count_trial):
+        # they only schedule, and a registration that launches a few sums only (the
+        # precision guard's FP64 twin) never asks for them.
         self._item_orders = []
         self._trials = 0
+        self._order_future = None
         self._order_pts = np.array(x0_np, dtype=np.float64)
         self._flow_base = rt.raw(engine.flow_record(N, T, self.Kdiff, perm=self._order))
         self._flow_key = (self.Kdiff.kernel, self.Kdiff.prec, d)
@@ -206,34 +207,54 @@ class DeviceMatching:
         if template_geometry is not None:
             self.sym = _TemplateTerm(self, *template_geometry)
 
-    def _attach_item_order(self, base, op, prec, y, x1=None, perm=True):
-        """Give the cached descriptor bytes ``base`` (op ``op``) a dispatch order estimated
-        from host points ``y`` (its rows) and ``x1`` (its target set), through the
-        registration's processing order when ``perm``; kept alive in self._item_orders."""
-        if op not in _ORDERED_OPS:
-            return
-        rec = base.view(rt.SEG_DT)          # the same bytes: the cached descriptor changes
-        o = engine.item_order(op, prec, self.dim, rec, y, x1=x1,
-                              perm=self._order_np if perm else None)
-        if o is not None:
-            self._item_orders.append(o)
-            engine.attach_item_order(rec, o)
+    def _order_launches(self):
+        """(descriptor bytes, op, prec) of the launches that take a dispatch order: the flow
+        and covector steps and the data-term energy."""
+        out = [(self._flow_base, _lib.OP_FLOW_STEP, self.Kdiff.prec),
+               (self._cov_base, _lib.OP_COVECTOR_STEP, self.Kdiff.prec)]
+        if self.data_kind in SHAPE_TERMS:
+            out.append((self._cur_try, _DATA_OP[self.data_kind], self.Kdist.energy().prec))
+        return [x for x in out if x[1] in _ORDERED_OPS]
+
+    def _estimate_orders(self, launches):
+        """Host estimates of the dispatch orders of ``launches`` [(descriptor copy, op, prec)]
+        from the template's points (no device work: it runs on a worker thread)."""
+        out = []
+        for rec, op, prec in launches:
+            if op in (_lib.OP_FLOW_STEP, _lib.OP_COVECTOR_STEP):
+                out.append(engine.item_order_host(op, prec, self.dim, rec, self._order_pts,
+                                                  perm=self._order_np))
+            else:
+                c1 = self._order_pts[self.faces_np].mean(axis=1)
+                out.append(engine.item_order_host(op, prec, self.dim, rec, c1,
+                                                  x1=self._order_tgt))
+        return out
 
     def _count_trial(self):
-        """Called for every line-search trial launch; the second one attaches the dispatch
-        orders of the flow, covector and data-term energy launches."""
+        """Called for every line-search trial launch.  The first one hands the estimate of
+        the dispatch orders (host work, ~10 ms at 81,920 faces) to a worker thread; the first
+        trial after it is done attaches them to the cached descriptors.  Orders only steer
+        the schedule, so results do not depend on when they arrive."""
         self._trials += 1
-        if self._trials != 2:
+        if self._trials == 1:
+            launches = [(base, op, prec) for base, op, prec in self._order_launches()
+                        if engine.item_count(op, prec, self.dim, base.view(rt.SEG_DT))
+                        >= engine.MIN_ORDERED_ITEMS]
+            if launches:
+                copies = [(np.array(base, copy=True).view(rt.SEG_DT), op, prec)
+                          for base, op, prec in launches]
+                self._order_future = _order_pool().submit(self._estimate_orders, copies)
+                self._order_bases = [base for base, _, _ in launches]
             return
-        self._attach_item_order(self._flow_base, _lib.OP_FLOW_STEP, self.Kdiff.prec,
-                                self._order_pts)
-        self._attach_item_order(self._cov_base, _lib.OP_COVECTOR_STEP, self.Kdiff.prec,
-                                self._order_pts)
-        if self.data_kind in SHAPE_TERMS:
-            c1 = self._order_pts[self.faces_np].mean(axis=1)
-            Ke = self.Kdist.energy()
-            self._attach_item_order(self._cur_try, _DATA_OP[self.data_kind], Ke.prec, c1,
-                                    self._order_tgt, perm=False)
+        f = self._order_future
+        if f is None or not f.done():
+            return
+        self._order_future = None
+        for base, order in zip(self._order_bases, f.result()):
+            o = engine.upload_order(order)
+            if o is not None:
+                self._item_orders.append(o)
+                engine.attach_item_order(base.view(rt.SEG_DT), o)
 
     def _data_record(self, sh, K, gradient=False, target=None):
         """Descriptor of the data term of device shape ``sh`` against the target (outputs
@@ -396,6 +417,19 @@ class _TemplateTerm:
         if o.data_kind == MEASURE:
             return engine.MeasureJob(self.c0, self.a0_norm0, None, None, Ke, total)
         return engine.VarifoldJob(self.c0, self.a0, self.cr0, None, None, None, Ke, total)
+
+
+_ORDER_POOL = None
+
+
+def _order_pool():
+    """Worker threads for the host estimates of dispatch orders."""
+    global _ORDER_POOL
+    if _ORDER_POOL is None:
+        import concurrent.futures
+        _ORDER_POOL = concurrent.futures.ThreadPoolExecutor(max_workers=2,
+                                                            thread_name_prefix="lddmm-orders")
+    return _ORDER_POOL
 
 
 def _template_energies(opts, totals) -> None:

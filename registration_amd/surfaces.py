@@ -52,11 +52,22 @@ class Surface:
             self.computeCentersAreas()
 
     def computeCentersAreas(self):
-        xDef1 = self.vertices[self.faces[:, 0], :]
-        xDef2 = self.vertices[self.faces[:, 1], :]
-        xDef3 = self.vertices[self.faces[:, 2], :]
-        self.centers = (xDef1 + xDef2 + xDef3) / 3
-        self.surfel = np.cross(xDef2 - xDef1, xDef3 - xDef1)
+        """centers = (x1 + x2 + x3) / 3 and surfel = (x2 - x1) x (x3 - x1) per face
+        (surfaces.py:62-70), the same elementwise operations as np.cross, evaluated on
+        coordinate rows (faster than gathering (nf, 3) blocks; the bits are the same)."""
+        if self.faces.shape[0] == 0:
+            self.centers = np.zeros((0, self.vertices.shape[1]))
+            self.surfel = np.zeros((0, self.vertices.shape[1]))
+            return
+        Vt = np.ascontiguousarray(self.vertices.T)
+        x1, x2, x3 = Vt[:, self.faces[:, 0]], Vt[:, self.faces[:, 1]], Vt[:, self.faces[:, 2]]
+        self.centers = np.ascontiguousarray(((x1 + x2 + x3) / 3).T)
+        a, b = x2 - x1, x3 - x1
+        sf = np.empty((a.shape[1], 3))
+        sf[:, 0] = a[1] * b[2] - a[2] * b[1]
+        sf[:, 1] = a[2] * b[0] - a[0] * b[2]
+        sf[:, 2] = a[0] * b[1] - a[1] * b[0]
+        self.surfel = sf
 
     def updateVertices(self, x0):
         self.vertices = np.copy(x0)
@@ -68,9 +79,12 @@ class Surface:
 
     def surfVolume(self):
         """Signed volume sum_f det[x1; x2; x3] / 6 (surfaces.py:711-717), each determinant as
-        the scalar triple product x1 . (x2 x x3)."""
-        v = self.vertices[self.faces]
-        return float(np.einsum('ij,ij->i', v[:, 0], np.cross(v[:, 1], v[:, 2])).sum() / 6)
+        the scalar triple product x1 . (x2 x x3), evaluated on coordinate rows."""
+        Vt = np.ascontiguousarray(self.vertices.T)
+        x1, x2, x3 = Vt[:, self.faces[:, 0]], Vt[:, self.faces[:, 1]], Vt[:, self.faces[:, 2]]
+        det = (x1[0] * (x2[1] * x3[2] - x2[2] * x3[1]) + x1[1] * (x2[2] * x3[0] - x2[0] * x3[2])
+               + x1[2] * (x2[0] * x3[1] - x2[1] * x3[0]))
+        return float(det.sum() / 6)
 
     def computeVertexArea(self):
         """(AV (nv, 1), AF (nf, 1)): face areas and the area each vertex collects from its

@@ -88,11 +88,27 @@ def test_currents_energy_and_gradient_identical_in_any_item_order():
         assert all(torch.equal(u, v) for u, v in zip(r0, r1))
 
 
-def test_registration_path_identical_with_dispatch_orders(monkeypatch):
+class _Now:
+    """An executor that runs each job at submission (the orders arrive at trial 2)."""
+
+    def submit(self, fn, *args):
+        import concurrent.futures
+        f = concurrent.futures.Future()
+        f.set_result(fn(*args))
+        return f
+
+
+@pytest.mark.parametrize("worker", ["thread", "inline"])
+def test_registration_path_identical_with_dispatch_orders(monkeypatch, worker):
     """Orders attach only to launches of at least MIN_ORDERED_ITEMS items; with the bound at
-    1 every flow, covector and data-term energy launch of this registration carries one."""
+    1 every flow, covector and data-term energy launch of this registration carries one,
+    estimated on the worker thread (attached when ready) or at submission (attached at the
+    second trial)."""
+    from registration_amd import matching
     V, F = synthetic.icosphere(4)
     Y = synthetic.bump_target(V)
+    if worker == "inline":
+        monkeypatch.setattr(matching, "_order_pool", lambda: _Now())
 
     def run():
         p = sm.SurfaceMatchingParam(timeStep=0.1,
@@ -109,6 +125,10 @@ def test_registration_path_identical_with_dispatch_orders(monkeypatch):
     assert not plain._item_orders
     monkeypatch.setattr(engine, "MIN_ORDERED_ITEMS", 1)
     ordered = run()
-    assert len(ordered._item_orders) == 3   # flow, covector, FP64E energy
+    if worker == "thread" and ordered._order_future is not None:
+        ordered._order_future.result()      # a short run may end before the estimate
+    else:
+        assert len(ordered._item_orders) == 3   # flow, covector, FP64E energy
+        assert ordered._order_future is None
     assert np.array_equal(np.asarray(plain.history), np.asarray(ordered.history))
     assert torch.equal(plain.at, ordered.at)

@@ -57,3 +57,24 @@ def test_deformation_grid_layout():
     assert pts.shape == (n * m, 2)
     assert np.allclose(pts[:m, 0], xs[0]) and np.allclose(pts[:m, 1], ys)   # row-major, x outer
     assert np.array_equal(edges, orc.grid_lines(n, m))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_centers_and_surfels_are_the_numpy_expressions(seed):
+    """Surface.computeCentersAreas on coordinate rows gives the bits of the reference's
+    expressions (surfaces.py:62-70: gathered vertex blocks, np.cross), and surfVolume agrees
+    with det[x1; x2; x3] / 6 summed over the faces; closed form: the unit cube's volume."""
+    rng = np.random.default_rng(seed)
+    V = rng.standard_normal((500, 3)) * 10.0 ** rng.integers(-3, 4)
+    F = rng.integers(0, 500, size=(997, 3))
+    S = surfaces.Surface(FV=(F, V))
+    x1, x2, x3 = V[F[:, 0], :], V[F[:, 1], :], V[F[:, 2], :]
+    assert np.array_equal(S.centers, (x1 + x2 + x3) / 3)
+    assert np.array_equal(S.surfel, np.cross(x2 - x1, x3 - x1))
+    det = np.linalg.det(np.stack([x1, x2, x3], axis=1))
+    assert S.surfVolume() == pytest.approx(det.sum() / 6, rel=1e-9, abs=1e-12 * np.abs(det).sum())
+    cube_v = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                       [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=float)
+    cube_f = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                       [2, 3, 7], [2, 7, 6], [1, 2, 6], [1, 6, 5], [0, 4, 7], [0, 7, 3]])
+    assert surfaces.Surface(FV=(cube_f, cube_v)).surfVolume() == pytest.approx(1.0, abs=1e-15)
