@@ -205,9 +205,7 @@ def run_steps(op: int, prec: int, d: int, steps: np.ndarray) -> None:
     dev = upload(steps)
     lib = _lib.load()
     ws = workspace(plan.work_bytes)
-    if COUNT_PAIRS:
-        for t in range(S):
-            _count(op, steps[t])
+    _count(op, steps)
     _lib.check(lib.lddmm_segments_run_steps(op, prec, d, steps.ctypes.data_as(_SEG_PTR),
                                             dev.data_ptr(), S, n, plan.n_items, plan.n_fin,
                                             ws.data_ptr(), ws.numel(), stream_handle()),
