@@ -87,10 +87,10 @@ class DeviceMatching:
         self.Kdiff = engine.DevKernel.of(KparDiff)
         self.x0 = rt.as_dev(x0_np)
         # Processing order of the template's points for the flow and RKHS-product sums
-        # (spatially compact tiles; the trajectory keeps the points' own order).
-        self._order_np = engine.order_choice(x0_np, self.Kdiff)
-        self._order = (None if self._order_np is None
-                       else torch.from_numpy(self._order_np).pin_memory().to(dev, non_blocking=True))
+        # (spatially compact tiles; the trajectory keeps the points' own order): host work,
+        # chosen on a worker thread while the device state below is set up.
+        order_job = _order_pool().submit(engine.order_choice, np.array(x0_np, dtype=np.float64),
+                                         self.Kdiff)
         T, N, d = Tsize, self.npt, self.dim
         if target_points is not None:
             self.data_kind = LANDMARK
@@ -134,6 +134,9 @@ class DeviceMatching:
         # registration's first line-search trial on, and attached when ready (_count_trial):
         # they only schedule, and a registration that launches a few sums only (the
         # precision guard's FP64 twin) never asks for them.
+        self._order_np = order_job.result()
+        self._order = (None if self._order_np is None
+                       else torch.from_numpy(self._order_np).pin_memory().to(dev, non_blocking=True))
         self._item_orders = []
         self._trials = 0
         self._order_future = None
