@@ -164,15 +164,43 @@ def write(table: dict, path: str) -> None:
         json.dump(table, f, indent=1)
 
 
+def merge(old: dict, new: dict) -> dict:
+    """``old``'s table with the entries of the forms and classes ``new`` measured replaced."""
+    key = lambda e: (e["op"], e["prec"], bool(e["energy_only"]), int(e["size_class"]))
+    fresh = {key(e): e for e in new["entries"]}
+    kept = [e for e in old["entries"] if key(e) not in fresh]
+    out = dict(new)
+    out["entries"] = sorted(kept + list(fresh.values()),
+                            key=lambda e: (next(i for i, f in enumerate(FORMS)
+                                                if f == (e["op"], e["prec"], bool(e["energy_only"]))),
+                                           e["size_class"]))
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=_lib.TUNING_PATH)
     ap.add_argument("--classes", default="0,1,2,3,4,5")
+    ap.add_argument("--forms", default="",
+                    help="op:prec[:energy],... of FORMS to measure (default all); the other "
+                         "entries of an existing --out table are kept")
     args = ap.parse_args(argv)
     import os
     os.environ["LDDMM_TUNING"] = "off"   # measure from the built-in targets
+    forms = FORMS
+    if args.forms:
+        want = set()
+        for f in args.forms.split(","):
+            parts = f.split(":")
+            want.add((parts[0], parts[1], len(parts) > 2 and parts[2] == "energy"))
+        forms = tuple(f for f in FORMS if f in want)
+        if len(forms) != len(want):
+            raise SystemExit(f"unknown form in {args.forms!r}; forms: {FORMS}")
     torch.cuda.set_device(0)
-    table = tune(classes=[int(c) for c in args.classes.split(",")])
+    table = tune(forms=forms, classes=[int(c) for c in args.classes.split(",")])
+    if args.forms and os.path.exists(args.out):
+        with open(args.out) as f:
+            table = merge(json.load(f), table)
     write(table, args.out)
     print(f"wrote {len(table['entries'])} targets to {args.out}", flush=True)
     return 0

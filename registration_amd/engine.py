@@ -260,14 +260,19 @@ MIN_ORDERED_POINTS_LAPLACIAN = 32768
 # Relative cost of a pair in the dot-product exponent form against the direct form (three
 # FMAs and the exponential against three differences, three FMAs and the exponential).
 DOT_FORM_COST = 0.78
+# FP64 Gaussian sums take the expanded exponent on blocks close to their tile origin: where
+# two orders cull alike, the more compact one was 2% faster (40,962 points, sigma 0.25).
+XFORM_TIE = 0.02
 
 
 def order_cost(points, order, K: "DevKernel") -> float:
     """Estimated work of K's self sums over ``points`` in ``order``, per nominal block pair:
-    the evaluated fraction of blocks, the FP32 Gaussian's compact quarters at DOT_FORM_COST."""
+    the evaluated fraction of blocks, the FP32 Gaussian's compact quarters at DOT_FORM_COST,
+    and for the FP64 Gaussian a small preference for compact quarters (XFORM_TIE)."""
     c = live_fraction(points, order, K)
-    if wants_order(K):
-        c *= 1.0 - (1.0 - DOT_FORM_COST) * compact_fraction(points, order, K.sigma)
+    if K.kernel == _lib.KERNEL_GAUSS:
+        w = (1.0 - DOT_FORM_COST) if wants_order(K) else XFORM_TIE
+        c *= 1.0 - w * compact_fraction(points, order, K.sigma)
     return c
 
 

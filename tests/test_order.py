@@ -21,8 +21,8 @@ def test_order_is_taken_where_it_pays():
     """Measured on gfx950 (flow and covector steps over icospheres): an order costs a staging
     launch that small sets do not repay, so none below MIN_ORDERED_POINTS (Laplacian:
     MIN_ORDERED_POINTS_LAPLACIAN); above, the spatial order where it culls more or takes
-    more dot-product quarters (the large surface at every width, FP32C and FP64), the
-    identity where it does not lose (FP64 at a width that culls nothing)."""
+    more dot-product quarters (the large surface at every width, FP32C and FP64, and FP64 at
+    a width that culls nothing, where the more compact order wins the tie)."""
     V6, _ = synthetic.icosphere(6)
     V5, _ = synthetic.icosphere(5)
     V4, _ = synthetic.icosphere(4)
@@ -31,7 +31,9 @@ def test_order_is_taken_where_it_pays():
               _K('lap', 0.05)):
         o = engine.order_choice(V6, K)
         assert o is not None and not np.array_equal(o, ident6), (K.kernel, K.sigma, K.prec)
-    assert np.array_equal(engine.order_choice(V6, _K('gauss', 0.5, 'fp64')), ident6)
+    o = engine.order_choice(V6, _K('gauss', 0.5, 'fp64'))
+    assert engine.live_fraction(V6, o, _K('gauss', 0.5, 'fp64')) == 1.0
+    assert not np.array_equal(o, ident6)
     assert engine.order_choice(V4, _K('gauss', 0.5)) is None       # 2,562 points
     assert engine.order_choice(V5, _K('lap', 0.05)) is None        # Laplacian, 10,242 points
     assert engine.order_choice(V5, _K('gauss', 0.05)) is not None
