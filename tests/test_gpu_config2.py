@@ -5,7 +5,8 @@ registered end to end against the FP64 oracle's own run of the reference's optim
 tests/golden/config2_oracle.npz by tests/golden/make_config2_oracle.py.
 
 FP64 form: every line-search trial (step and objective), the gradient bound, the accepted
-objectives and the stop reason equal the oracle's, objectives to 1e-9 relative.
+objectives and the stop reason equal the oracle's, steps to 1e-12 and objectives to 1e-9
+relative.
 FP32-compensated form (the headline): every accepted iterate's energy within 1e-6 of the FP64
 energy of the same momenta, the same stop reason and iteration count, and the final energy
 within 1e-6 of the oracle's."""
@@ -48,12 +49,16 @@ def recorded(m):
 def test_config2_fp64_follows_the_oracle():
     ref = np.load(FIXTURE)
     m = registration("fp64")
-    trials = np.array(recorded(m))
+    log = recorded(m)
     print("config 2, FP64 form", flush=True)
     m.optimizeMatching()
+    trials = np.array(log)
     rt = ref["trials"]
     assert trials.shape == rt.shape, (trials.shape, rt.shape)
-    assert np.array_equal(trials[:, 0], rt[:, 0])                     # the same step lengths
+    # the same step lengths: most are powers of the line search's factors, exactly equal; a
+    # step derived from dot products (the restarts) carries their summation rounding
+    # (measured: 2 of 67 steps 1.4e-15 and 2.1e-14 apart)
+    assert np.all(np.abs(trials[:, 0] - rt[:, 0]) <= 1e-12 * np.abs(rt[:, 0]))
     assert np.all(np.abs(trials[:, 1] - rt[:, 1]) <= 1e-9 * np.abs(rt[:, 1]))
     h, rh = np.array(m.history), ref["history"]
     assert len(h) == len(rh) and np.all(np.abs(h - rh) <= 1e-9 * np.abs(rh)), (h, rh)
