@@ -697,9 +697,41 @@ __host__ __device__ inline int64_t stage_tiles(int op, const lddmm_seg_t &s, int
   return (s.nx0 + TS - 1) / TS + (sets == 2 ? (s.nx1 + TS - 1) / TS : 0);
 }
 
+// Sum of one segment's per-finalize-block partial reductions (`red`, nred values each) into
+// its totals: one workgroup of TOT_BLOCK threads, in a fixed order.
+__device__ inline void segment_totals(const lddmm_seg_t *sg, int nred,
+                                      const double *__restrict__ red) {
+  __shared__ double sh[TOT_BLOCK];
+  const int tid = threadIdx.x;
+  for (int j = 0; j < nred; ++j) {
+    double v = 0.0;
+    for (int b = tid; b < sg->fin_blocks; b += TOT_BLOCK) v += red[sg->red0 + (int64_t)j * sg->fin_blocks + b];
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = TOT_BLOCK / 2; s > 0; s >>= 1) {
+      if (tid < s) sh[tid] += sh[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0 && sg->total) sg->total[j] = sh[0];
+    __syncthreads();
+  }
+}
+
+// Staged source tiles, one workgroup per tile (n_stage of them).  In a stepped sequence the
+// launch also carries the previous step's totals (tot_segs, one more workgroup per segment,
+// after the tiles): they read the reductions that step's finalize left, before this step's
+// finalize overwrites them, and spare the sequence a totals launch per step.
 template <typename R, int KT, int D, int M, bool KSUM, int NA>
 __global__ __launch_bounds__(BLOCK) void k_stage(const lddmm_seg_t *__restrict__ segs, int nseg,
-                                                int op, int prec, char *__restrict__ stage) {
+                                                int op, int prec, char *__restrict__ stage,
+                                                int64_t n_stage,
+                                                const lddmm_seg_t *__restrict__ tot_segs,
+                                                int nred, const double *__restrict__ red) {
+  static_assert(BLOCK == TOT_BLOCK, "the totals workgroups run with the staging ones");
+  if ((int64_t)blockIdx.x >= n_stage) {
+    segment_totals(tot_segs + (blockIdx.x - n_stage), nred, red);
+    return;
+  }
   constexpr int NSW = (NA > (KSUM ? 1 : 0)) ? NA : (KSUM ? 1 : 0);
   constexpr bool DOT = std::is_same<R, float>::value && KT == LDDMM_KERNEL_GAUSS && KSUM && NA == 0;
   constexpr bool XFORM = xform_kernel<R, KT, KSUM, NA>();
@@ -2258,21 +2290,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const lddmm_seg_t *__res
 // Per-segment FP64 totals from the finalize blocks' partials (fixed tree).
 __global__ __launch_bounds__(TOT_BLOCK) void k_totals(const lddmm_seg_t *__restrict__ segs,
                                                      int nred, const double *__restrict__ red) {
-  __shared__ double sh[TOT_BLOCK];
-  const lddmm_seg_t *sg = segs + blockIdx.x;
-  const int tid = threadIdx.x;
-  for (int j = 0; j < nred; ++j) {
-    double v = 0.0;
-    for (int b = tid; b < sg->fin_blocks; b += TOT_BLOCK) v += red[sg->red0 + (int64_t)j * sg->fin_blocks + b];
-    sh[tid] = v;
-    __syncthreads();
-    for (int s = TOT_BLOCK / 2; s > 0; s >>= 1) {
-      if (tid < s) sh[tid] += sh[tid + s];
-      __syncthreads();
-    }
-    if (tid == 0 && sg->total) sg->total[j] = sh[0];
-    __syncthreads();
-  }
+  segment_totals(segs + blockIdx.x, nred, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -2745,6 +2763,11 @@ namespace {
 struct PairOpts {
   char *stage;
   int64_t n_stage;
+  // Totals the staging launch also reduces: the previous step's segments (nseg of them),
+  // or none.
+  const lddmm_seg_t *tot_segs;
+  int nred;
+  const double *red;
 };
 
 template <typename R, int KT, int D, int M, bool KSUM, int NA>
@@ -2755,8 +2778,10 @@ void launch_pairs(const lddmm_seg_t *segs, int nseg, int op, int prec, int64_t n
   // data-term energies) and the covector step (M = D, two derivative weight pairs).
   if constexpr ((KSUM && NA == 0) || (M == D && !KSUM && NA == 2)) {
     if (po.n_stage > 0) {
-      hipLaunchKernelGGL((k_stage<R, KT, D, M, KSUM, NA>), dim3((unsigned)po.n_stage),
-                         dim3(BLOCK), 0, st, segs, nseg, op, prec, po.stage);
+      const int64_t extra = po.tot_segs ? nseg : 0;
+      hipLaunchKernelGGL((k_stage<R, KT, D, M, KSUM, NA>), dim3((unsigned)(po.n_stage + extra)),
+                         dim3(BLOCK), 0, st, segs, nseg, op, prec, po.stage, po.n_stage,
+                         po.tot_segs, po.nred, po.red);
       stage = po.stage;
     }
   }
@@ -2956,10 +2981,15 @@ int run_d(int op, int prec, int kern, const Shape &sh, const lddmm_seg_t *segs, 
 
 }  // namespace
 
-extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *segs_host,
-                                  const lddmm_seg_t *segs_dev, int nseg, int64_t n_items,
-                                  int64_t n_fin_blocks, void *work, size_t work_bytes,
-                                  void *stream) {
+namespace {
+
+// One launch of planned segments.  tot_prev: device descriptors of the previous launch of
+// a stepped sequence whose totals the staging launch reduces (NULL: none); skip_totals: the
+// next launch of the sequence reduces this one's totals.
+int segments_run_impl(int op, int prec, int d, const lddmm_seg_t *segs_host,
+                      const lddmm_seg_t *segs_dev, int nseg, int64_t n_items,
+                      int64_t n_fin_blocks, void *work, size_t work_bytes, void *stream,
+                      const lddmm_seg_t *tot_prev, bool skip_totals) {
   if (!op_valid(op) || nseg < 1 || !segs_host || (d != 2 && d != 3) || !work) return LDDMM_EINVAL;
   if (prec != LDDMM_PREC_FP64 && prec != LDDMM_PREC_FP32C && prec != LDDMM_PREC_FP64E)
     return LDDMM_EINVAL;
@@ -3000,12 +3030,25 @@ extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *se
   PairOpts po;
   po.stage = wb + stage_offset(nseg, parts + reds);
   po.n_stage = n_stage;
+  po.tot_segs = (n_stage > 0 && nred > 0) ? tot_prev : nullptr;
+  po.nred = nred;
+  po.red = red;
   rc = d == 2 ? run_d<2>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, po)
               : run_d<3>(op, prec, kern, sh, segs, nseg, n_items, n_fin_blocks, part, red, st, po);
   if (rc) return rc;
-  if (nred > 0)
+  if (nred > 0 && !skip_totals)
     hipLaunchKernelGGL(k_totals, dim3((unsigned)nseg), dim3(TOT_BLOCK), 0, st, segs, nred, red);
   return hip_check(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *segs_host,
+                                  const lddmm_seg_t *segs_dev, int nseg, int64_t n_items,
+                                  int64_t n_fin_blocks, void *work, size_t work_bytes,
+                                  void *stream) {
+  return segments_run_impl(op, prec, d, segs_host, segs_dev, nseg, n_items, n_fin_blocks, work,
+                           work_bytes, stream, nullptr, false);
 }
 
 extern "C" int lddmm_segments_run_steps(int op, int prec, int d, const lddmm_seg_t *segs_host,
@@ -3013,6 +3056,14 @@ extern "C" int lddmm_segments_run_steps(int op, int prec, int d, const lddmm_seg
                                         int64_t n_items, int64_t n_fin_blocks, void *work,
                                         size_t work_bytes, void *stream) {
   if (n_steps < 1 || nseg < 1 || !segs_host || !segs_dev) return LDDMM_EINVAL;
+  if (!op_valid(op) || (d != 2 && d != 3)) return LDDMM_EINVAL;
+  // Flow steps that stage tiles (ordered segments) reduce the previous step's totals in
+  // their staging launch (launch_pairs), sparing a totals launch per step.
+  Shape sh;
+  if (op_shape(op, d, segs_host[0], &sh)) return LDDMM_EINVAL;
+  int64_t n_stage = 0;
+  for (int i = 0; i < nseg; ++i) n_stage += stage_tiles(op, segs_host[i], sh.NA);
+  const bool fold = op == LDDMM_OP_FLOW_STEP && n_stage > 0;
   for (int t = 0; t < n_steps; ++t) {
     const lddmm_seg_t *h = segs_host + (int64_t)t * nseg;
     // Steps share step 0's plan: same sizes and scheduling fields, segment by segment.
@@ -3026,8 +3077,10 @@ extern "C" int lddmm_segments_run_steps(int op, int prec, int d, const lddmm_seg
           !a.item_order != !b.item_order || a.n_order != b.n_order)
         return LDDMM_EINVAL;
     }
-    int rc = lddmm_segments_run(op, prec, d, h, segs_dev + (int64_t)t * nseg, nseg, n_items,
-                                n_fin_blocks, work, work_bytes, stream);
+    int rc = segments_run_impl(op, prec, d, h, segs_dev + (int64_t)t * nseg, nseg, n_items,
+                               n_fin_blocks, work, work_bytes, stream,
+                               fold && t > 0 ? segs_dev + (int64_t)(t - 1) * nseg : nullptr,
+                               fold && t < n_steps - 1);
     if (rc) return rc;
   }
   return LDDMM_OK;

@@ -347,3 +347,48 @@ def test_staged_energy_tiles_match_the_oracle(precision):
         assert abs(mj[i].total.item() - (s - 2 * x)) <= tol * (ms + 2 * mx)
         s, ms = quad(c1, r1)
         assert abs(kj[i].total.item() - s) <= tol * ms
+
+
+@pytest.mark.parametrize("prec", ["fp32c", "fp64"])
+def test_stepped_totals_folded_into_staging_equal_single_steps(prec):
+    """An ordered flow run as one stepped sequence reduces each step's energy in the next
+    step's staging launch (the last step in its own totals launch); every step's energy,
+    position and velocity equals the same steps launched one at a time, bit for bit, and
+    a served-style batch with an ordered and an unordered segment likewise."""
+    import torch
+    from registration_amd import _lib, engine, runtime as rt, synthetic
+    from registration_amd.kernelFunctions import Kernel
+    K = engine.DevKernel.of(Kernel('gauss', sigma=0.3, precision=prec))
+    V5, _ = synthetic.icosphere(5)
+    V3, _ = synthetic.icosphere(3)
+    rng = np.random.default_rng(71)
+    T = 5
+    jobs, ref = [], []
+    for X, ordered in ((V5, True), (V3, False), (V5, False)):
+        at = torch.as_tensor(0.03 * rng.standard_normal((T,) + X.shape), device="cuda")
+        perm = engine.spatial_order_dev(X) if ordered else None
+        x0 = torch.as_tensor(X, device="cuda")
+        xt = torch.empty((T + 1,) + X.shape, dtype=torch.float64, device="cuda")
+        v = torch.empty((T,) + X.shape, dtype=torch.float64, device="cuda")
+        en = torch.full((T,), float("nan"), dtype=torch.float64, device="cuda")
+        jobs.append(engine.FlowJob(x0, at, K, xt, en, v=v, perm=perm))
+        # the same steps one launch at a time
+        yt = torch.empty_like(xt)
+        yt[0] = x0
+        w = torch.empty_like(v)
+        e1 = torch.full((T,), float("nan"), dtype=torch.float64, device="cuda")
+        for t in range(T):
+            f = dict(y=yt[t], w0=at[t], out0=yt[t + 1], out1=w[t], total=e1[t:t + 1],
+                     ny=X.shape[0], sigma=K.sigma, dt=1.0 / T, kernel=K.kernel, order=K.order)
+            if perm is not None:
+                f["perm"] = perm
+            rt.Plan(_lib.OP_FLOW_STEP, K.prec, 3, [rt.seg(**f)]).run()
+        ref.append((yt, w, e1))
+    print(f"stepped flows ({prec}): one ordered batch of 3 and 3 single-step references",
+          flush=True)
+    engine.shoot(jobs)
+    engine.shoot(jobs[:1])                      # the ordered one alone
+    torch.cuda.synchronize()
+    for j, (yt, w, e1) in zip(jobs, ref):
+        assert torch.equal(j.xt, yt) and torch.equal(j.v, w)
+        assert torch.equal(j.energy, e1) and not torch.isnan(e1).any()
