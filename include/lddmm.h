@@ -221,8 +221,12 @@ int lddmm_segments_run(int op, int prec, int d, const lddmm_seg_t *segs_host,
  * Every step must have the sizes of step 0, which was planned (n_items,
  * n_fin_blocks, work_bytes as returned for it) and whose scheduling fields
  * every step carries.  Used for the T time steps of Euler flows and adjoints,
- * where step t reads what step t-1 wrote.  No counterpart in the reference
- * (its time loop calls the kernel sum once per step, pointEvolution.py:691-697). */
+ * where step t reads what step t-1 wrote.  Flow steps with staged tiles (segments
+ * with a perm) reduce step t's totals in step t+1's staging launch and launch the
+ * totals kernel once, after the last step; every total is written when the
+ * call's work is complete, as with one call per step.  No counterpart in the
+ * reference (its time loop calls the kernel sum once per step,
+ * pointEvolution.py:691-697). */
 int lddmm_segments_run_steps(int op, int prec, int d, const lddmm_seg_t *segs_host,
                              const lddmm_seg_t *segs_dev, int n_steps, int nseg,
                              int64_t n_items, int64_t n_fin_blocks, void *work,

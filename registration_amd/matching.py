@@ -426,7 +426,9 @@ _ORDER_POOL = None
 
 
 def _order_pool():
-    """Worker threads for the host estimates of dispatch orders."""
+    """Worker threads for a registration's host-side order work: the choice of its
+    processing order during set-up and the estimate of its dispatch orders.  Both call host
+    code of the kernel library, which releases the interpreter lock."""
     global _ORDER_POOL
     if _ORDER_POOL is None:
         import concurrent.futures
