@@ -404,6 +404,16 @@ int lddmm_landmark_batch(const lddmm_lmk_t *host, const lddmm_lmk_t *dev, int n,
  * No counterpart in the reference (its sums run in point order).  They build
  * and check the optional lddmm_seg_t.perm of flow steps and RKHS products. */
 
+/* Host geometry of a triangulated surface (row-major (nv, 3) vertices, (nf, 3)
+ * int64 faces; a negative index counts from the end): face centers
+ * ((x1 + x2) + x3) / 3, surfels (x2 - x1) x (x3 - x1) and the triple products
+ * x1 . (x2 x x3), each in the reference's NumPy evaluation order and without
+ * contraction, so bit for bit the values of surfaces.py:62-70 (computeCentersAreas)
+ * and of the signed volume's terms (surfaces.py:711-717).  Any output may be NULL.
+ * Replaces the NumPy expressions on the host side of Surface construction. */
+int lddmm_surface_geometry_host(const double *v, int64_t nv, const int64_t *f, int64_t nf,
+                                double *centers, double *surfel, double *det);
+
 /* Spatial processing order of n points (row-major (n, d) host array, d = 1..3):
  * recursive bisection along the longest side of the bounding box, with split
  * points at multiples of 256 (then 64), so that every 256-point tile and every

@@ -3649,6 +3649,45 @@ extern "C" double lddmm_order_compactness(const double *points, int64_t n, int d
   return (double)ok / (double)total;
 }
 
+// Host geometry of a triangulated surface in the reference's NumPy expression order
+// (surfaces.py:62-70 and the volume's triple products), FMA contraction off, so that the
+// results are those of the NumPy expressions bit for bit: centers = ((x1 + x2) + x3) / 3,
+// surfel = (x2 - x1) x (x3 - x1) with each component a product minus a product, and
+// det = x1 . (x2 x x3) summed as ((p0 + p1) + p2).  Face indices below zero count from the
+// end, as NumPy indexing does; others out of range are an error.
+extern "C" int lddmm_surface_geometry_host(const double *v, int64_t nv, const int64_t *f,
+                                           int64_t nf, double *centers, double *surfel,
+                                           double *det) {
+#pragma clang fp contract(off)
+  if (nv < 1 || nf < 0 || !v || (nf > 0 && !f)) return LDDMM_EINVAL;
+  for (int64_t i = 0; i < nf; ++i) {
+    const double *x[3];
+    for (int k = 0; k < 3; ++k) {
+      int64_t j = f[3 * i + k];
+      if (j < 0) j += nv;
+      if (j < 0 || j >= nv) return LDDMM_EINVAL;
+      x[k] = v + 3 * j;
+    }
+    if (centers)
+      for (int c = 0; c < 3; ++c) centers[3 * i + c] = (x[0][c] + x[1][c] + x[2][c]) / 3.0;
+    if (surfel) {
+      const double a[3] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2]};
+      const double b[3] = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
+      surfel[3 * i + 0] = a[1] * b[2] - a[2] * b[1];
+      surfel[3 * i + 1] = a[2] * b[0] - a[0] * b[2];
+      surfel[3 * i + 2] = a[0] * b[1] - a[1] * b[0];
+    }
+    if (det) {
+      const double *p = x[0], *q = x[1], *r = x[2];
+      const double p0 = p[0] * (q[1] * r[2] - q[2] * r[1]);
+      const double p1 = p[1] * (q[2] * r[0] - q[0] * r[2]);
+      const double p2 = p[2] * (q[0] * r[1] - q[1] * r[0]);
+      det[i] = p0 + p1 + p2;
+    }
+  }
+  return LDDMM_OK;
+}
+
 extern "C" double lddmm_order_live_fraction(const double *points, int64_t n, int d,
                                             const int32_t *order, int rows, double cut2,
                                             int64_t max_pairs) {

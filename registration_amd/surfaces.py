@@ -28,6 +28,32 @@ from . import runtime as rt
 from .fileio import vtkFields  # noqa: F401  (part of the module's API)
 
 
+def _host_triangles(V, F) -> bool:
+    """Whether (V, F) is an FP64 triangle mesh in 3-D (the library's host geometry)."""
+    return (V.ndim == 2 and V.shape[1] == 3 and V.dtype == np.float64 and F.ndim == 2
+            and F.shape[1] == 3 and np.issubdtype(F.dtype, np.integer))
+
+
+def _host_geometry(V, F, centers=True, surfel=True, det=True):
+    """(centers, surfels, triple products) of an FP64 triangle mesh from
+    lddmm_surface_geometry_host (None for what is not asked); IndexError for a face index
+    out of range, as NumPy indexing raises."""
+    from . import _lib
+    Vc = np.ascontiguousarray(V, dtype=np.float64)
+    Fc = np.ascontiguousarray(F, dtype=np.int64)
+    nf = Fc.shape[0]
+    cen = np.empty((nf, 3)) if centers else None
+    sf = np.empty((nf, 3)) if surfel else None
+    dt = np.empty(nf) if det else None
+    rc = _lib.load().lddmm_surface_geometry_host(
+        Vc.ctypes.data, Vc.shape[0], Fc.ctypes.data, nf,
+        None if cen is None else cen.ctypes.data, None if sf is None else sf.ctypes.data,
+        None if dt is None else dt.ctypes.data)
+    if rc != _lib.OK:
+        raise IndexError("face index out of range for %d vertices" % Vc.shape[0])
+    return cen, sf, dt
+
+
 class Surface:
     def __init__(self, surf=None, filename=None, FV=None):
         if surf is None:
@@ -53,11 +79,16 @@ class Surface:
 
     def computeCentersAreas(self):
         """centers = (x1 + x2 + x3) / 3 and surfel = (x2 - x1) x (x3 - x1) per face
-        (surfaces.py:62-70), the same elementwise operations as np.cross, evaluated on
-        coordinate rows (faster than gathering (nf, 3) blocks; the bits are the same)."""
+        (surfaces.py:62-70).  Triangles in FP64 go through the library's host routine
+        (lddmm_surface_geometry_host: the same operations in the same order, uncontracted,
+        so the bits of the NumPy expressions); other shapes evaluate the expressions on
+        coordinate rows."""
         if self.faces.shape[0] == 0:
             self.centers = np.zeros((0, self.vertices.shape[1]))
             self.surfel = np.zeros((0, self.vertices.shape[1]))
+            return
+        if _host_triangles(self.vertices, self.faces):
+            self.centers, self.surfel, _ = _host_geometry(self.vertices, self.faces, det=False)
             return
         Vt = np.ascontiguousarray(self.vertices.T)
         x1, x2, x3 = Vt[:, self.faces[:, 0]], Vt[:, self.faces[:, 1]], Vt[:, self.faces[:, 2]]
@@ -79,7 +110,10 @@ class Surface:
 
     def surfVolume(self):
         """Signed volume sum_f det[x1; x2; x3] / 6 (surfaces.py:711-717), each determinant as
-        the scalar triple product x1 . (x2 x x3), evaluated on coordinate rows."""
+        the scalar triple product x1 . (x2 x x3)."""
+        if self.faces.shape[0] and _host_triangles(self.vertices, self.faces):
+            _, _, det = _host_geometry(self.vertices, self.faces, centers=False, surfel=False)
+            return float(det.sum() / 6)
         Vt = np.ascontiguousarray(self.vertices.T)
         x1, x2, x3 = Vt[:, self.faces[:, 0]], Vt[:, self.faces[:, 1]], Vt[:, self.faces[:, 2]]
         det = (x1[0] * (x2[1] * x3[2] - x2[2] * x3[1]) + x1[1] * (x2[2] * x3[0] - x2[0] * x3[2])

@@ -78,3 +78,21 @@ def test_centers_and_surfels_are_the_numpy_expressions(seed):
     cube_f = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
                        [2, 3, 7], [2, 7, 6], [1, 2, 6], [1, 6, 5], [0, 4, 7], [0, 7, 3]])
     assert surfaces.Surface(FV=(cube_f, cube_v)).surfVolume() == pytest.approx(1.0, abs=1e-15)
+
+
+def test_host_geometry_indexes_as_numpy_does():
+    """A negative face index counts from the end (NumPy indexing); one past the vertices
+    raises IndexError; non-FP64 vertices take the NumPy expressions with the same results."""
+    V = np.random.default_rng(3).standard_normal((10, 3))
+    S = surfaces.Surface(FV=(np.array([[0, 1, -1], [2, -3, 4]]), V))
+    F = np.array([[0, 1, 9], [2, 7, 4]])
+    x1, x2, x3 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    assert np.array_equal(S.centers, (x1 + x2 + x3) / 3)
+    assert np.array_equal(S.surfel, np.cross(x2 - x1, x3 - x1))
+    with pytest.raises(IndexError):
+        surfaces.Surface(FV=(np.array([[0, 1, 10]]), V))
+    S32 = surfaces.Surface(FV=(F, V))
+    S32.vertices = V.astype(np.float32)
+    S32.computeCentersAreas()
+    v = V.astype(np.float32)
+    assert np.array_equal(S32.centers, (v[F[:, 0]] + v[F[:, 1]] + v[F[:, 2]]) / 3)
