@@ -43,6 +43,8 @@
 #include <mutex>
 #include <type_traits>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 #include "lddmm.h"
 
@@ -3581,40 +3583,94 @@ extern "C" int lddmm_sum_f64(const double *x, int64_t n, double *out, void *stre
 // above a quarter) along the longest side of its bounding box by nth_element;
 // ties on the coordinate break by point index, so the result is deterministic.
 // ---------------------------------------------------------------------------
+namespace {
+
+struct BisectRange {
+  int64_t lo, len;
+};
+
+// One split of a range of the processing order: the points order[lo, lo + len) are
+// partitioned along the longest side of their bounding box at a multiple of 256 (then 64)
+// near the middle; false for a leaf (64 points or fewer).  The partition depends on the
+// range's contents only (ties broken by index), so ranges may be split in any order.
+bool bisect_split(const double *points, int d, int32_t *order, const BisectRange &r,
+                  BisectRange *lo_half, BisectRange *hi_half) {
+  if (r.len <= 64) return false;
+  const int64_t unit = r.len > TS ? TS : 64;
+  int64_t h = (int64_t)llround((double)r.len / (2.0 * (double)unit)) * unit;
+  if (h < unit) h = unit;
+  if (h > r.len - 1) h = r.len - 1;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = r.lo; i < r.lo + r.len; ++i) {
+    const double *p = points + (int64_t)order[i] * d;
+    for (int c = 0; c < d; ++c) {
+      lo[c] = p[c] < lo[c] ? p[c] : lo[c];
+      hi[c] = p[c] > hi[c] ? p[c] : hi[c];
+    }
+  }
+  int ax = 0;
+  for (int c = 1; c < d; ++c)
+    if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
+  int32_t *first = order + r.lo;
+  std::nth_element(first, first + h, first + r.len, [&](int32_t a, int32_t b) {
+    const double pa = points[(int64_t)a * d + ax], pb = points[(int64_t)b * d + ax];
+    return pa < pb || (pa == pb && a < b);
+  });
+  *lo_half = {r.lo, h};
+  *hi_half = {r.lo + h, r.len - h};
+  return true;
+}
+
+void bisect_all(const double *points, int d, int32_t *order, BisectRange r0) {
+  std::vector<BisectRange> stack{r0};
+  while (!stack.empty()) {
+    const BisectRange r = stack.back();
+    stack.pop_back();
+    BisectRange a, b;
+    if (!bisect_split(points, d, order, r, &a, &b)) continue;
+    stack.push_back(b);
+    stack.push_back(a);
+  }
+}
+
+}  // namespace
+
 extern "C" int lddmm_spatial_order(const double *points, int64_t n, int d, int32_t *order) {
   if (n < 0 || n > INT32_MAX || d < 1 || d > 3 || (n > 0 && (!points || !order)))
     return LDDMM_EINVAL;
   for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
-  struct Range { int64_t lo, len; };
-  std::vector<Range> stack;
-  stack.push_back({0, n});
-  while (!stack.empty()) {
-    const Range r = stack.back();
-    stack.pop_back();
-    if (r.len <= 64) continue;
-    const int64_t unit = r.len > TS ? TS : 64;
-    int64_t h = (int64_t)llround((double)r.len / (2.0 * (double)unit)) * unit;
-    if (h < unit) h = unit;
-    if (h > r.len - 1) h = r.len - 1;
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int64_t i = r.lo; i < r.lo + r.len; ++i) {
-      const double *p = points + (int64_t)order[i] * d;
-      for (int c = 0; c < d; ++c) {
-        lo[c] = p[c] < lo[c] ? p[c] : lo[c];
-        hi[c] = p[c] > hi[c] ? p[c] : hi[c];
+  // Large sets: split the top of the tree breadth-first, then finish the subtrees on host
+  // threads (disjoint ranges; the same permutation as one thread).
+  std::vector<BisectRange> front{{0, n}};
+  const int workers = n >= 32768 ? 8 : 1;
+  while ((int)front.size() < 2 * workers) {
+    std::vector<BisectRange> next;
+    bool split = false;
+    for (const BisectRange &r : front) {
+      BisectRange a, b;
+      if (bisect_split(points, d, order, r, &a, &b)) {
+        next.push_back(a);
+        next.push_back(b);
+        split = true;
+      } else {
+        next.push_back(r);
       }
     }
-    int ax = 0;
-    for (int c = 1; c < d; ++c)
-      if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
-    int32_t *first = order + r.lo;
-    std::nth_element(first, first + h, first + r.len, [&](int32_t a, int32_t b) {
-      const double pa = points[(int64_t)a * d + ax], pb = points[(int64_t)b * d + ax];
-      return pa < pb || (pa == pb && a < b);
-    });
-    stack.push_back({r.lo + h, r.len - h});  // popped second: the high half follows
-    stack.push_back({r.lo, h});
+    front.swap(next);
+    if (!split) break;
   }
+  if (workers == 1) {
+    for (const BisectRange &r : front) bisect_all(points, d, order, r);
+    return LDDMM_OK;
+  }
+  std::vector<std::thread> pool;
+  std::atomic<size_t> next_range{0};
+  for (int w = 0; w < workers; ++w)
+    pool.emplace_back([&]() {
+      for (size_t i = next_range++; i < front.size(); i = next_range++)
+        bisect_all(points, d, order, front[i]);
+    });
+  for (std::thread &t : pool) t.join();
   return LDDMM_OK;
 }
 

@@ -71,10 +71,10 @@ class Surface:
                 self.faces = np.int_(FV[0])
                 self.computeCentersAreas()
         else:
+            # (the reference also copies surf's centers and surfels, which the recomputation
+            # replaces at once)
             self.vertices = np.copy(surf.vertices)
             self.faces = np.copy(surf.faces)
-            self.surfel = np.copy(surf.surfel)
-            self.centers = np.copy(surf.centers)
             self.computeCentersAreas()
 
     def computeCentersAreas(self):

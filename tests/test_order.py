@@ -162,3 +162,34 @@ def test_order_ignores_duplicates_and_flat_sets():
     assert np.array_equal(np.sort(o), np.arange(300))
     o1 = engine.spatial_order(np.ones((130, 3)))
     assert np.array_equal(np.sort(o1), np.arange(130))
+
+
+def _bisect_leaves_ref(P, idx, out):
+    """Restatement of the bisection: a range of more than 64 points splits at a multiple of
+    256 (64 below 256 points) near its middle, along the longest side of its bounding box,
+    the low part being the smallest points by (coordinate, index); the leaves as sets."""
+    n = len(idx)
+    if n <= 64:
+        out.append(frozenset(idx.tolist()))
+        return
+    unit = 256 if n > 256 else 64
+    h = int(np.floor(n / (2.0 * unit) + 0.5)) * unit
+    h = min(max(h, unit), n - 1)
+    Q = P[idx]
+    ax = int(np.argmax(Q.max(axis=0) - Q.min(axis=0)))
+    srt = idx[np.lexsort((idx, Q[:, ax]))]
+    _bisect_leaves_ref(P, srt[:h], out)
+    _bisect_leaves_ref(P, srt[h:], out)
+
+
+@pytest.mark.parametrize("n,d", [(40000, 3), (33000, 2)])
+def test_threaded_bisection_matches_a_restatement(n, d):
+    """Sets of 32,768 points and more finish their subtrees on host threads; the order's
+    64-point leaves equal those of a sequential NumPy restatement, and two calls agree."""
+    P = np.random.default_rng(n).standard_normal((n, d))
+    o = engine.spatial_order(P)
+    assert np.array_equal(o, engine.spatial_order(P.copy()))
+    got = [frozenset(o[i:i + 64].tolist()) for i in range(0, n, 64)]
+    ref = []
+    _bisect_leaves_ref(P, np.arange(n), ref)
+    assert got == ref
