@@ -404,6 +404,14 @@ int lddmm_landmark_batch(const lddmm_lmk_t *host, const lddmm_lmk_t *dev, int n,
  * No counterpart in the reference (its sums run in point order).  They build
  * and check the optional lddmm_seg_t.perm of flow steps and RKHS products. */
 
+/* out[i][k] = a[i][k] + (s[i] * d[i][k]) for k < len[i], i < n (device FP64 arrays; the
+ * product rounded before the sum, as NumPy's a + s * d and two elementwise passes
+ * round it; out may be a).  The line search's trial momenta of a batch of
+ * registrations, at - eps dir (surfaceMatching.py:255-289), in one launch per 32
+ * items.  Host arrays of pointers and sizes; enqueued on `stream`. */
+int lddmm_axpy_batch(int n, double *const *out, const double *const *a, const double *const *d,
+                     const double *s, const int64_t *len, void *stream);
+
 /* Host geometry of a triangulated surface (row-major (nv, 3) vertices, (nf, 3)
  * int64 faces; a negative index counts from the end): face centers
  * ((x1 + x2) + x3) / 3, surfels (x2 - x1) x (x3 - x1) and the triple products

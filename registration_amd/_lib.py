@@ -106,6 +106,7 @@ SIGNATURES = {
     "lddmm_order_compactness": (ctypes.c_double, [_P, _I64, _I, _P, _D, _D]),
     "lddmm_order_live_fraction": (ctypes.c_double, [_P, _I64, _I, _P, _I, _D, _I64]),
     "lddmm_surface_geometry_host": (_I, [_P, _I64, _P, _I64, _P, _P, _P]),
+    "lddmm_axpy_batch": (_I, [_I, _P, _P, _P, _P, _P, _P]),
     "lddmm_item_order": (_I, [_I, _I, _I, ctypes.POINTER(Seg), _P, _P, _P, _P, _I64]),
     "lddmm_apply_k_workspace_bytes": (_SZ, [_I64, _I64, _I, _I]),
     "lddmm_apply_k": (_I, [_P, _P, _P, _I64, _I64, _I, _I, _I, _I, _D, _I, _P, _P, _SZ, _P]),

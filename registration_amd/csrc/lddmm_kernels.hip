@@ -2494,6 +2494,32 @@ __global__ void k_stamp(SegPack pack, int n, lddmm_seg_t *__restrict__ dst) {
   if (i < n) dst[i] = pack.s[i];
 }
 
+// out_i = a_i + (s_i d_i) for up to AXPY_PACK arrays per launch (the line search's trial
+// momenta at - eps dir of a whole batch of registrations), item by blockIdx.y, elements by
+// a grid-stride loop; the product is rounded before the sum, as two elementwise passes do.
+constexpr int AXPY_PACK = 32;
+struct AxpyPack {
+  double *out[AXPY_PACK];
+  const double *a[AXPY_PACK];
+  const double *d[AXPY_PACK];
+  double s[AXPY_PACK];
+  int64_t len[AXPY_PACK];
+};
+__global__ __launch_bounds__(256) void k_axpy_batch(AxpyPack pk) {
+#pragma clang fp contract(off)
+  const int j = blockIdx.y;
+  const int64_t n = pk.len[j];
+  double *__restrict__ out = pk.out[j];
+  const double *__restrict__ a = pk.a[j];
+  const double *__restrict__ d = pk.d[j];
+  const double s = pk.s[j];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const double p = s * d[k];
+    out[k] = a[k] + p;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side: planning, dispatch.
 // ---------------------------------------------------------------------------
@@ -3711,6 +3737,38 @@ extern "C" double lddmm_order_compactness(const double *points, int64_t n, int d
 // surfel = (x2 - x1) x (x3 - x1) with each component a product minus a product, and
 // det = x1 . (x2 x x3) summed as ((p0 + p1) + p2).  Face indices below zero count from the
 // end, as NumPy indexing does; others out of range are an error.
+extern "C" int lddmm_axpy_batch(int n, double *const *out, const double *const *a,
+                                const double *const *d, const double *s, const int64_t *len,
+                                void *stream) {
+  if (n < 0 || (n > 0 && (!out || !a || !d || !s || !len))) return LDDMM_EINVAL;
+  for (int i = 0; i < n; ++i)   // every item checked before anything is enqueued
+    if (len[i] < 0 || (len[i] > 0 && (!out[i] || !a[i] || !d[i]))) return LDDMM_EINVAL;
+  bool launched = false;
+  for (int i0 = 0; i0 < n; i0 += AXPY_PACK) {
+    AxpyPack pk;
+    memset(&pk, 0, sizeof(pk));
+    const int m = n - i0 < AXPY_PACK ? n - i0 : AXPY_PACK;
+    int64_t longest = 0;
+    for (int j = 0; j < m; ++j) {
+      const int i = i0 + j;
+      pk.out[j] = out[i];
+      pk.a[j] = a[i];
+      pk.d[j] = d[i];
+      pk.s[j] = s[i];
+      pk.len[j] = len[i];
+      longest = len[i] > longest ? len[i] : longest;
+    }
+    if (longest == 0) continue;
+    // enough workgroups for one element per thread up to ~1 M elements, then grid-stride
+    int64_t blocks = (longest + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_axpy_batch, dim3((unsigned)blocks, (unsigned)m), dim3(256), 0,
+                       (hipStream_t)stream, pk);
+    launched = true;
+  }
+  return launched ? hip_check(hipGetLastError()) : LDDMM_OK;
+}
+
 extern "C" int lddmm_surface_geometry_host(const double *v, int64_t nv, const int64_t *f,
                                            int64_t nf, double *centers, double *surfel,
                                            double *det) {

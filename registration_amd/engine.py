@@ -298,6 +298,30 @@ def order_for(points, K: "DevKernel"):
     return None if o is None else torch.from_numpy(o).to(rt.device())
 
 
+def axpy_batch(out, a, d, s) -> None:
+    """out[i] = a[i] + s[i] * d[i] for lists of contiguous FP64 device tensors of matching
+    sizes, the product rounded before the sum (at - eps dir as the reference's NumPy
+    evaluates it, with s = -eps); one launch per 32 items (lddmm_axpy_batch)."""
+    n = len(out)
+    if n == 0:
+        return
+    if not (len(a) == len(d) == len(s) == n):
+        raise ValueError("axpy_batch: lists of different lengths")
+    for o, x, y in zip(out, a, d):
+        if not (o.numel() == x.numel() == y.numel() and o.dtype == x.dtype == y.dtype == torch.float64
+                and o.is_contiguous() and x.is_contiguous() and y.is_contiguous()
+                and o.is_cuda and x.is_cuda and y.is_cuda):
+            raise ValueError("axpy_batch: contiguous FP64 device tensors of one size per item")
+    po = np.fromiter((t.data_ptr() for t in out), np.uint64, n)
+    pa = np.fromiter((t.data_ptr() for t in a), np.uint64, n)
+    pd = np.fromiter((t.data_ptr() for t in d), np.uint64, n)
+    ps = np.asarray(s, dtype=np.float64)
+    pl = np.fromiter((t.numel() for t in out), np.int64, n)
+    _lib.check(_lib.load().lddmm_axpy_batch(n, po.ctypes.data, pa.ctypes.data, pd.ctypes.data,
+                                            ps.ctypes.data, pl.ctypes.data, rt.stream_handle()),
+               "lddmm_axpy_batch")
+
+
 # --------------------------------------------------------------------------------------
 # Shooting (flow + energy)
 # --------------------------------------------------------------------------------------

@@ -665,9 +665,10 @@ def launch_update_try(items) -> Pending:
         end += 1
     scal = torch.zeros(end, dtype=torch.float64, device=dev)
     sbase = np.uint64(scal.data_ptr())
-    # atTry = at - eps dir for every registration (multi-tensor launches)
-    ats = torch._foreach_mul([it[1].diff for it in items], [-float(it[2]) for it in items])
-    torch._foreach_add_(ats, [o.at for o in opts])
+    # atTry = at - eps dir for every registration (one launch per 32 registrations)
+    ats = [torch.empty_like(o.at) for o in opts]
+    engine.axpy_batch(ats, [o.at for o in opts], [it[1].diff for it in items],
+                      [-float(it[2]) for it in items])
     for i in sym:
         # x0Try = x0 - eps dir.initx into slice 0 of the trial's trajectory
         o, dir_, eps = items[i][0], items[i][1], items[i][2]

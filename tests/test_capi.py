@@ -229,3 +229,20 @@ def test_exp2_table_entries_are_correctly_rounded():
         assert v == float(exact), j
         # and no other double is closer
         assert abs(mpmath.mpf(v) - exact) <= mpmath.mpf(2) ** -53 * v
+
+
+def test_axpy_batch_rejects_bad_arguments_without_launching():
+    import ctypes
+    import numpy as np
+    from registration_amd import _lib
+    lib = _lib.load()
+    p = np.zeros(1, np.uint64)
+    s = np.zeros(1)
+    bad = np.array([-1], np.int64)
+    assert lib.lddmm_axpy_batch(1, p.ctypes.data, p.ctypes.data, p.ctypes.data, s.ctypes.data,
+                                bad.ctypes.data, None) == _lib.EINVAL
+    one = np.array([5], np.int64)   # a non-empty item with null pointers
+    assert lib.lddmm_axpy_batch(1, p.ctypes.data, p.ctypes.data, p.ctypes.data, s.ctypes.data,
+                                one.ctypes.data, None) == _lib.EINVAL
+    assert lib.lddmm_axpy_batch(-1, None, None, None, None, None, None) == _lib.EINVAL
+    assert lib.lddmm_axpy_batch(0, None, None, None, None, None, None) == _lib.OK
