@@ -392,3 +392,33 @@ def test_stepped_totals_folded_into_staging_equal_single_steps(prec):
     for j, (yt, w, e1) in zip(jobs, ref):
         assert torch.equal(j.xt, yt) and torch.equal(j.v, w)
         assert torch.equal(j.energy, e1) and not torch.isnan(e1).any()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32c", TOL), ("fp64", 1e-12)])
+def test_laplacian_flow_in_spatial_order(prec, tol):
+    """Laplacian sums take an order from 32,768 points, the spatial one where it culls more
+    (FP32C at sigma 0.05 on the unit sphere; in FP64 the cull radius, 55 sigma, exceeds the
+    sphere and the points' own order is kept): a flow step over the 40,962-point sphere in
+    the spatial order equals the natural order's and the oracle's velocities on sampled
+    rows."""
+    import torch
+    from registration_amd import engine, synthetic
+    from registration_amd.kernelFunctions import Kernel
+    V, _ = synthetic.icosphere(6)
+    K = engine.DevKernel.of(Kernel('laplacian', sigma=0.05, order=3, precision=prec))
+    o = engine.order_choice(V, K)
+    if prec == "fp32c":
+        assert o is not None and not np.array_equal(o, np.arange(len(V)))
+    else:
+        assert np.array_equal(o, np.arange(len(V)))
+        o = engine.spatial_order(V)
+    at = 0.03 * np.random.default_rng(97).standard_normal((1,) + V.shape)
+    perm = torch.as_tensor(o, device="cuda")
+    print(f"Laplacian flow step in the spatial order ({prec})", flush=True)
+    _, v, e = flow(V, at, K, perm)
+    _, vn, en = flow(V, at, K, None)
+    rows = np.random.default_rng(5).choice(len(V), 256, replace=False)
+    ref = orc.apply_k(V[rows], V, at[0], 0.05, 3)
+    assert rel_err(v[0][rows], ref) < tol
+    assert rel_err(v, vn) < tol
+    assert abs(e[0] - en[0]) <= 1e-6 * abs(en[0])
