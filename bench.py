@@ -31,10 +31,11 @@ FLOPS_PER_PAIR = {"apply_k_gauss_d3_m3": 16}
 # HBM bytes per fused flow step at 40,962 points (tile staging + pair kernel + finalize + totals),
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on `bench.py --roofline-only
 # --roofline-reps 3` and scaled by the counters' reading for 256 MiB device copies (FETCH x2.0,
-# WRITE x1.0), in the template's spatial processing order with the items longest first: 98.8 MB
-# read + 64.4 MB written, most of it the FP64 chunk partials (54 chunks x 40,962 rows x 3),
-# written by the pair kernel and read back by the finalize (profiles/flow_step_s6_fp32c.md).
-FLOW_STEP_TRAFFIC_S6 = 163.2e6
+# WRITE x1.0), in the template's spatial processing order with the items longest first, at
+# the 8,192-item plan (6,561 items): 138.4 MB read + 95.7 MB written, most of it the FP64 chunk
+# partials (81 chunks x 40,962 rows x 3), written by the pair kernel and read back by the
+# finalize (profiles/flow_step_s6_fp32c.md).
+FLOW_STEP_TRAFFIC_S6 = 234.1e6
 
 
 def _warm(plan, seconds=0.25):
