@@ -96,3 +96,20 @@ def test_committed_tuning_file_is_well_formed():
         assert e["op"] in _lib.OP_IDS and e["prec"] in _lib.PRECS and 0 <= e["size_class"] < 6
         assert str(e["items"]) in e["ms"] and e["ms"][str(e["items"])] <= e["ms"][str(e["builtin"])]
         assert e["op"] != "APPLY_K"     # the allocation-free entry point's scratch bound
+
+
+def test_merge_replaces_only_the_measured_forms():
+    """autotune.merge: entries of the forms and classes a partial run measured replace the
+    table's; the rest are kept, in FORMS order."""
+    from registration_amd import autotune
+    e = lambda op, prec, eo, cls, items: {"op": op, "prec": prec, "energy_only": eo,
+                                          "size_class": cls, "items": items}
+    old = {"entries": [e("FLOW_STEP", "fp32c", False, 3, 6144), e("FLOW_STEP", "fp64", False, 3, 6144),
+                       e("CURRENTS", "fp64", True, 4, 4096)]}
+    new = {"device": "x", "made": "now", "candidates": [], "min_gain": 0.02,
+           "entries": [e("FLOW_STEP", "fp64", False, 3, 12288)]}
+    m = autotune.merge(old, new)
+    got = [(x["op"], x["prec"], x["size_class"], x["items"]) for x in m["entries"]]
+    assert got == [("FLOW_STEP", "fp32c", 3, 6144), ("FLOW_STEP", "fp64", 3, 12288),
+                   ("CURRENTS", "fp64", 4, 4096)]
+    assert m["made"] == "now"
